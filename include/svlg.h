@@ -1,0 +1,132 @@
+/*
+ * svlg.h -- C ABI of the MI355X-native SeismoVLAB dynamic-FEM hot path (libsvlg.so).
+ *
+ * Plain pointers and sizes only. Host arrays are copied to HBM at model creation;
+ * every state vector afterwards lives on the device and is only copied back when a
+ * caller asks for it. Return value: 0 = ok, nonzero = stop (the meaning of SVL's
+ * `bool stop`); svlg_last_error() gives the message. One HIP stream per model; a model
+ * handle is not thread-safe.
+ *
+ * The entry points replace, method for method, the SVL C++ interfaces that sit on the
+ * hot path (citations are relative to the SVL source tree):
+ *   Assembler    02-Run_Process/07-Assembler/Assembler.hpp:50-181
+ *   Integrator   02-Run_Process/10-Integrators/Integrator.hpp:53-144
+ *                (CentralDifference.hpp:63, ExtendedNewmarkBeta.hpp)
+ *   LinearSystem 02-Run_Process/11-Solvers/LinearSystem.hpp:44-66
+ */
+#ifndef SVLG_H
+#define SVLG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVLG_ELEM_QUAD4 0 /* lin2DQuad4  (04-Elements/06-Quadrilateral/lin2DQuad4.cpp) */
+#define SVLG_ELEM_HEXA8 1 /* lin3DHexa8  (04-Elements/10-Hexahedron/lin3DHexa8.cpp) */
+
+#define SVLG_MAT_ELASTIC 0 /* Elastic2DPlaneStrain / Elastic3DLinear */
+#define SVLG_MAT_J2 1      /* PlasticPlaneStrainJ2 / Plastic3DJ2 */
+
+#define SVLG_PRECOND_JACOBI 0
+#define SVLG_PRECOND_LEARNED 1
+
+/* Model description, in SVL's ascending-tag order (mesh nodes, then continuum
+ * elements). All arrays are host memory, read only during svlg_model_create. */
+typedef struct svlg_mesh_desc {
+    int32_t ndim;             /* 2 (Quad4, plane strain) or 3 (Hexa8) */
+    int32_t elem_type;        /* SVLG_ELEM_QUAD4 or SVLG_ELEM_HEXA8 */
+    int32_t nnodes;           /* mesh nodes */
+    int32_t nelem;            /* continuum elements */
+    const double *coords;     /* [nnodes*ndim] */
+    const int32_t *free_dof;  /* [nnodes*ndim] Plain-scheme free index, -1 restrained */
+    int32_t nfree;
+    const int32_t *conn;      /* [nelem*nen] 0-based mesh node index, SVL node order */
+    const int32_t *mat_type;  /* [nelem] SVLG_MAT_* (NULL: all elastic) */
+    const double *E;          /* [nelem] Young's modulus */
+    const double *nu;         /* [nelem] Poisson ratio */
+    const double *rho;        /* [nelem] density */
+    const double *j2;         /* [nelem*5] K, G, sigma_y, H, beta (NULL if no J2) */
+    double thickness;         /* lin2DQuad4 `th` */
+    int32_t lumped;           /* Global massform: 1 LUMPED, 0 CONSISTENT */
+    const double *ray_am;     /* [nelem] Rayleigh am (NULL: Free damping) */
+    const double *ray_ak;     /* [nelem] Rayleigh ak (NULL: Free damping) */
+    /* ZeroLength1D + Viscous1DLinear dashpots to a fully restrained partner node
+     * (01-Pre_Process/Method/Builder.py:1086-1131), in element-tag order. */
+    int32_t ndash;
+    const int32_t *dash_node; /* [ndash] mesh node */
+    const int32_t *dash_dir;  /* [ndash] 0-based direction */
+    const double *dash_eta;   /* [ndash] viscous coefficient eta */
+    /* POINTLOAD_CONCENTRATED_DYNAMIC loads of the active load combination
+     * (Assembler.cpp:334-350 + :485; Load.cpp:69-71). */
+    int32_t nload;
+    int32_t load_nt;          /* samples per series */
+    const int32_t *load_node; /* [nload] */
+    const double *load_dir;   /* [nload*ndim] direction vector */
+    const double *load_factor;/* [nload] combination factor */
+    const double *load_series;/* [nload*load_nt] amplitude time series */
+} svlg_mesh_desc;
+
+typedef struct svlg_model svlg_model;
+
+/* Error string of the last failing call on this thread. */
+const char *svlg_last_error(void);
+/* Library version string (also forces the HIP runtime to load). */
+const char *svlg_version(void);
+/* Number of HIP devices visible (0 on a machine without a GPU). */
+int svlg_device_count(void);
+
+/* Uploads a model (Mesh::Initialize + Element::SetDomain, 06-Mesh/Mesh.cpp:18-49),
+ * builds the element coloring and node tiles, and assembles the lumped mass and the
+ * dashpot damping diagonals on the device (Assembler.cpp:48-158). */
+int svlg_model_create(const svlg_mesh_desc *desc, svlg_model **out);
+int svlg_model_destroy(svlg_model *m);
+/* Uses `stream` (a hipStream_t) for every later launch of this model; NULL restores
+ * the model's own stream. Returns the stream in use through *current if non-NULL. */
+int svlg_model_set_stream(svlg_model *m, void *stream);
+void *svlg_model_stream(svlg_model *m);
+/* info[0]=ncolors info[1]=nelem info[2]=nnodes info[3]=nfree info[4]=explicit kernel
+ * variant (0 colored, 1 tiled) info[5]=ntiles info[6]=affine (1 if all elements are
+ * parallelepipeds) info[7]=reserved */
+int svlg_model_info(svlg_model *m, int64_t *info8);
+/* Element color of every element (ascending-tag greedy coloring; no two elements of
+ * one color share a node). */
+int svlg_element_colors(svlg_model *m, int32_t *colors /*[nelem]*/);
+
+/* ---- Assembler (Assembler.hpp:50-181) ---------------------------------------- */
+/* Lumped mass diagonal per free DOF (Assembler::ComputeMassMatrix, row-sum lumping
+ * lin2DQuad4.cpp:272-282 / lin3DHexa8.cpp:272-282). */
+int svlg_mass_diagonal(svlg_model *m, double *mdiag /*[nfree]*/);
+/* Dashpot damping diagonal per free DOF (Assembler::ComputeDampingMatrix restricted
+ * to ZeroLength1D dashpots, ZeroLength1D.cpp:211-234). */
+int svlg_damping_diagonal(svlg_model *m, double *cdiag /*[nfree]*/);
+/* Fint(U) in free DOFs: Assembler::ComputeInternalForceVector (Assembler.cpp:240-269)
+ * with the element stress state taken from U (lin*::UpdateState then
+ * ComputeInternalForces). `ftol` is the |Fint_e(j)| > ftol filter of Assembler.cpp:262. */
+int svlg_internal_force(svlg_model *m, const double *U /*[nfree]*/, double ftol,
+                        double *F /*[nfree]*/);
+
+/* ---- CentralDifference (10-Integrators/02-CentralDifference/CentralDifference.cpp) */
+/* Initialize (:36-71): zero state, Keff = M/dt^2 + C/(2dt) (diagonal), ftol filter. */
+int svlg_cd_init(svlg_model *m, double dt, double ftol);
+/* Advances nsteps steps starting at step index k0 (ComputeNewStep :124-152 via
+ * Linear::ComputeNewIncrement, with DynamicAnalysis' index convention: step k uses
+ * load sample k). If nrec > 0, U at the free DOFs rec_dofs[] after every step is
+ * written to hist[s*nrec + r] (the NODE recorder's row k-1). */
+int svlg_cd_steps(svlg_model *m, int32_t k0, int32_t nsteps, int32_t nrec,
+                  const int32_t *rec_dofs, double *hist);
+/* Same as svlg_cd_steps but leaves results on the device and returns immediately
+ * after enqueueing (no recording, no host synchronisation). */
+int svlg_cd_steps_async(svlg_model *m, int32_t k0, int32_t nsteps);
+/* Current U, V, A in free DOFs (CentralDifference::GetDisplacements/Velocities/
+ * Accelerations). Any pointer may be NULL. */
+int svlg_cd_get_state(svlg_model *m, double *U, double *V, double *A);
+/* Blocks until all work queued on the model's stream is done. */
+int svlg_sync(svlg_model *m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVLG_H */

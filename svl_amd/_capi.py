@@ -1,0 +1,162 @@
+"""ctypes binding of libsvlg.so (the C ABI declared in include/svlg.h).
+
+The library is loaded from this package directory only; if it is missing the import
+of the device API raises, so no code path silently falls back to the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from .mesh import Model
+
+LIB_PATH = Path(__file__).resolve().parent / "libsvlg.so"
+
+i32p = C.POINTER(C.c_int32)
+f64p = C.POINTER(C.c_double)
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int32),
+        ("elem_type", C.c_int32),
+        ("nnodes", C.c_int32),
+        ("nelem", C.c_int32),
+        ("coords", f64p),
+        ("free_dof", i32p),
+        ("nfree", C.c_int32),
+        ("conn", i32p),
+        ("mat_type", i32p),
+        ("E", f64p),
+        ("nu", f64p),
+        ("rho", f64p),
+        ("j2", f64p),
+        ("thickness", C.c_double),
+        ("lumped", C.c_int32),
+        ("ray_am", f64p),
+        ("ray_ak", f64p),
+        ("ndash", C.c_int32),
+        ("dash_node", i32p),
+        ("dash_dir", i32p),
+        ("dash_eta", f64p),
+        ("nload", C.c_int32),
+        ("load_nt", C.c_int32),
+        ("load_node", i32p),
+        ("load_dir", f64p),
+        ("load_factor", f64p),
+        ("load_series", f64p),
+    ]
+
+
+def _p(a, t):
+    if a is None:
+        return C.cast(None, t)
+    return a.ctypes.data_as(t)
+
+
+def make_desc(m: Model):
+    """Builds an svlg_mesh_desc that views the model's arrays (returns desc, keepalive)."""
+    keep = []
+
+    def arr(x, dt):
+        a = np.ascontiguousarray(x, dtype=dt)
+        keep.append(a)
+        return a
+
+    free = arr(m.free_dof().reshape(-1), np.int32)
+    d = MeshDesc()
+    d.ndim = m.ndim
+    d.elem_type = m.elem_type
+    d.nnodes = m.nnodes
+    d.nelem = m.nelem
+    d.coords = _p(arr(m.coords, np.float64), f64p)
+    d.free_dof = _p(free, i32p)
+    d.nfree = m.nfree
+    d.conn = _p(arr(m.conn, np.int32), i32p)
+    d.mat_type = _p(arr(m.mat_type, np.int32), i32p)
+    d.E = _p(arr(m.E, np.float64), f64p)
+    d.nu = _p(arr(m.nu, np.float64), f64p)
+    d.rho = _p(arr(m.rho, np.float64), f64p)
+    d.j2 = _p(arr(m.j2, np.float64), f64p)
+    d.thickness = float(m.thickness)
+    d.lumped = 1 if m.massform_lumped else 0
+    d.ray_am = _p(arr(m.ray_am, np.float64), f64p)
+    d.ray_ak = _p(arr(m.ray_ak, np.float64), f64p)
+    d.ndash = int(m.dash_node.size)
+    d.dash_node = _p(arr(m.dash_node, np.int32), i32p)
+    d.dash_dir = _p(arr(m.dash_dir, np.int32), i32p)
+    d.dash_eta = _p(arr(m.dash_eta, np.float64), f64p)
+    d.nload = int(m.load_node.size)
+    d.load_nt = int(m.load_series.shape[1]) if m.load_node.size else 0
+    d.load_node = _p(arr(m.load_node, np.int32), i32p)
+    d.load_dir = _p(arr(m.load_dir, np.float64), f64p)
+    d.load_factor = _p(arr(m.load_factor, np.float64), f64p)
+    d.load_series = _p(arr(m.load_series, np.float64), f64p)
+    return d, keep
+
+
+# Every exported symbol with its signature; also used by the CPU test that checks the
+# library exports exactly what the header declares.
+SIGNATURES = {
+    "svlg_last_error": (C.c_char_p, []),
+    "svlg_version": (C.c_char_p, []),
+    "svlg_device_count": (C.c_int, []),
+    "svlg_model_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
+    "svlg_model_destroy": (C.c_int, [C.c_void_p]),
+    "svlg_model_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "svlg_model_stream": (C.c_void_p, [C.c_void_p]),
+    "svlg_model_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "svlg_element_colors": (C.c_int, [C.c_void_p, i32p]),
+    "svlg_mass_diagonal": (C.c_int, [C.c_void_p, f64p]),
+    "svlg_damping_diagonal": (C.c_int, [C.c_void_p, f64p]),
+    "svlg_internal_force": (C.c_int, [C.c_void_p, f64p, C.c_double, f64p]),
+    "svlg_cd_init": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "svlg_cd_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, i32p, f64p]),
+    "svlg_cd_steps_async": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "svlg_cd_get_state": (C.c_int, [C.c_void_p, f64p, f64p, f64p]),
+    "svlg_sync": (C.c_int, [C.c_void_p]),
+}
+
+_LIB = None
+
+
+class SvlgError(RuntimeError):
+    """A libsvlg entry point returned nonzero (SVL's `stop == true`)."""
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m svl_amd.build` (no CPU fallback exists)")
+    L = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().svlg_last_error()
+        raise SvlgError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(f64p)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(i32p)
+
+
+def device_count() -> int:
+    return int(lib().svlg_device_count())

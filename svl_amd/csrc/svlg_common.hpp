@@ -1,0 +1,127 @@
+// svlg_common.hpp -- shared host/device declarations of libsvlg (MI355X, gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/svlg.h"
+
+namespace svlg {
+
+void set_error(const std::string &msg);
+
+#define SVLG_HIP(call)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            ::svlg::set_error(std::string(#call) + ": " + hipGetErrorString(e_));   \
+            return 10;                                                              \
+        }                                                                           \
+    } while (0)
+
+#define SVLG_CHECK(cond, msg)                                                       \
+    do {                                                                            \
+        if (!(cond)) {                                                              \
+            ::svlg::set_error(msg);                                                 \
+            return 11;                                                              \
+        }                                                                           \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc((void **)&p, count * sizeof(T));
+    }
+    hipError_t upload(const T *h, size_t count, hipStream_t s) {
+        hipError_t e = alloc(count);
+        if (e != hipSuccess || count == 0) return e;
+        return hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+// Node tile used by the fused explicit kernel: one workgroup owns the free DOFs of
+// `nown` nodes and recomputes every element touching them from an LDS-staged copy of
+// the displacements of `nstage` nodes.
+struct TileSet {
+    int ntiles = 0;
+    int max_stage = 0, max_elem = 0, max_own = 0, ncolor_max = 0;
+    DevBuf<int32_t> stage_ptr, stage_node;   // [ntiles+1], global node of staged slot
+    DevBuf<int32_t> elem_ptr;                // [ntiles+1]
+    DevBuf<uint16_t> elem_lconn;             // [sum elems * nen] local staged slot
+    DevBuf<int32_t> elem_id;                 // [sum elems] element (tag order index)
+    DevBuf<int32_t> color_ptr;               // [ntiles * (ncolor_max+1)] offsets into tile elems
+    DevBuf<int32_t> own_ptr;                 // [ntiles+1]
+    DevBuf<uint16_t> own_slot;               // [sum own] staged slot of owned node
+};
+
+struct Model {
+    int ndim = 0, nen = 0, nnodes = 0, nelem = 0, nfree = 0, ndofn = 0;  // ndofn = nnodes*ndim
+    double th = 1.0;
+    bool lumped = true;
+    bool affine = false;
+    bool has_ak = false;
+    hipStream_t own_stream = nullptr, stream = nullptr;
+
+    // Host copies needed by later set-up calls.
+    std::vector<int32_t> h_free;      // [ndofn]
+    std::vector<int32_t> h_color;     // [nelem]
+    std::vector<int32_t> color_off;   // [ncolors+1] offsets into color-sorted order
+    int ncolors = 0;
+
+    // Geometry and materials (device).
+    DevBuf<double> X;                 // [nnodes*ndim]
+    DevBuf<int32_t> conn;             // [nelem*nen] tag order
+    DevBuf<int32_t> conn_c;           // [nelem*nen] color-sorted order
+    DevBuf<int32_t> eid_c;            // [nelem] tag index of color-sorted slot
+    DevBuf<double> lam, mu, rho;      // [nelem] tag order
+    DevBuf<double> am, ak;            // [nelem] Rayleigh
+    DevBuf<int32_t> fdof;             // [ndofn] free index or -1
+    DevBuf<int32_t> n2e_ptr, n2e;     // node -> (element*nen + local) in ascending element
+
+    // Diagonals (node-DOF layout).
+    DevBuf<double> mdiag, cdiag;      // lumped mass, damping
+    DevBuf<double> scratch;           // [nelem*nen*ndim] per-element lumped masses
+
+    // Loads.
+    int nload = 0, load_nt = 0;
+    DevBuf<int32_t> load_node;
+    DevBuf<double> load_dir, load_factor, load_series;
+    std::vector<int32_t> h_load_dofs; // distinct loaded node-DOFs
+    DevBuf<int32_t> load_dofs;
+
+    // Central difference state.
+    bool cd_ready = false;
+    double dt = 0.0, ftol = 1e-12;
+    int ring = 0;                     // slot of U_k (current)
+    long steps_done = 0;
+    DevBuf<double> U[3];              // ring: U_k, U_{k-1}, U_{k-2}
+    DevBuf<double> F;                 // internal-force accumulator
+    DevBuf<double> Fext;              // dense external force (loaded DOFs only written)
+    DevBuf<double> keff, kforce;      // diag(M/dt^2 + C/2dt), diag(M/dt^2 - C/2dt)
+    DevBuf<double> kinv_plain;        // unused placeholder
+    DevBuf<double> hist;              // recorder buffer
+    DevBuf<int32_t> rec;              // recorded node-DOFs
+
+    TileSet tiles;
+    int variant = 0;                  // 0 colored, 1 tiled
+};
+
+}  // namespace svlg
+
+struct svlg_model {
+    svlg::Model m;
+};

@@ -1,0 +1,214 @@
+// svlg_explicit.hip -- CentralDifference with lumped mass, matrix-free, on the device.
+//
+// Replaces CentralDifference::Initialize / ComputeEffectiveForce / ComputeEffectiveStiffness
+// / ComputeNewStep (RP/10-Integrators/02-CentralDifference/CentralDifference.cpp:36-71,
+// 124-152, 206-231) driven by Linear::ComputeNewIncrement (RP/09-Algorithms/01-Linear/
+// Linear.cpp:22-56) inside DynamicAnalysis::Analyze (RP/08-Analysis/02-Dynamic/
+// DynamicAnalysis.cpp:25-64). With lumped mass and dashpot-only damping Keff is
+// diagonal, so EigenSolver's LDL^T solve is the division dU_i = Feff_i / Keff_ii.
+//
+// Per step k (load sample k, displacements U_{k-1}, U_{k-2}):
+//   Feff = (Fext(k) - Fint(U_{k-1})) + (M/dt^2 - C/2dt)(U_{k-1} - U_{k-2})
+//   U_k  = U_{k-1} + Feff / (M/dt^2 + C/2dt)
+// The three displacement vectors rotate through a ring so V and A (which need
+// U_{k-2}) can be formed on demand with SVL's expressions.
+#include <algorithm>
+#include <cstring>
+
+#include "svlg_common.hpp"
+#include "svlg_elem.hpp"
+
+namespace svlg {
+
+int launch_fint_colored(Model &m, const double *U, double ftol, double *F);
+int gather_free(Model &m, const double *dnode, double *dfree);
+
+__global__ void k_cd_coeffs(int n, double dt, const double *__restrict__ M, const double *__restrict__ C,
+                            const int32_t *__restrict__ fdof, double *__restrict__ keff,
+                            double *__restrict__ kforce, int *__restrict__ singular) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (fdof[i] < 0) {
+        keff[i] = 0.0;
+        kforce[i] = 0.0;
+        return;
+    }
+    double k = 1.0 / dt / dt * M[i] + 1.0 / 2.0 / dt * C[i];
+    keff[i] = k;
+    kforce[i] = 1.0 / dt / dt * M[i] - 1.0 / 2.0 / dt * C[i];
+    if (!(k != 0.0)) atomicOr(singular, 1);
+}
+
+// Fext(k) = sum_l factor_l * (amp_l[k] * dir_l) by assignment per load, then summed in
+// load order (Assembler.cpp:334-350, 485). One thread keeps that order.
+__global__ void k_loads(int k, int nload, int nt, int ndim, int nldofs, const int32_t *__restrict__ ldofs,
+                        const int32_t *__restrict__ node, const double *__restrict__ dir,
+                        const double *__restrict__ factor, const double *__restrict__ series,
+                        double *__restrict__ Fext) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int i = 0; i < nldofs; ++i) Fext[ldofs[i]] = 0.0;
+    for (int l = 0; l < nload; ++l) {
+        double amp = (k >= 0 && k < nt) ? series[size_t(l) * nt + k] : 0.0;
+        for (int c = 0; c < ndim; ++c) Fext[size_t(node[l]) * ndim + c] += factor[l] * (amp * dir[size_t(l) * ndim + c]);
+    }
+}
+
+// U_next = U + (((Fext - F) + kforce (U - Up)) / keff); resets F for the next step.
+__global__ void k_cd_update(int n, const double *__restrict__ U, const double *__restrict__ Up,
+                            double *__restrict__ Unext, double *__restrict__ F, const double *__restrict__ Fext,
+                            const double *__restrict__ keff, const double *__restrict__ kforce) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double k = keff[i];
+    double fi = F[i];
+    F[i] = 0.0;
+    if (k == 0.0) {
+        Unext[i] = 0.0;
+        return;
+    }
+    double u = U[i];
+    double feff = (Fext[i] - fi) + kforce[i] * (u - Up[i]);
+    Unext[i] = u + feff / k;
+}
+
+__global__ void k_record(int nrec, const int32_t *__restrict__ rec, const double *__restrict__ U,
+                         double *__restrict__ out) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrec) out[r] = U[rec[r]];
+}
+
+int build_tiles(Model &m) {
+    m.variant = 0;
+    m.tiles.ntiles = 0;
+    return 0;
+}
+
+static int cd_one_step(Model &m, int k) {
+    double *Ucur = m.U[m.ring].p;
+    double *Uprev = m.U[(m.ring + 2) % 3].p;
+    double *Unext = m.U[(m.ring + 1) % 3].p;
+    if (m.nload > 0)
+        hipLaunchKernelGGL(k_loads, dim3(1), dim3(64), 0, m.stream, k, m.nload, m.load_nt, m.ndim,
+                           (int)m.h_load_dofs.size(), m.load_dofs.p, m.load_node.p, m.load_dir.p, m.load_factor.p,
+                           m.load_series.p, m.Fext.p);
+    if (int rc = launch_fint_colored(m, Ucur, m.ftol, m.F.p)) return rc;
+    int nb = (m.ndofn + 255) / 256;
+    hipLaunchKernelGGL(k_cd_update, dim3(nb), dim3(256), 0, m.stream, m.ndofn, Ucur, Uprev, Unext, m.F.p, m.Fext.p,
+                       m.keff.p, m.kforce.p);
+    m.ring = (m.ring + 1) % 3;
+    m.steps_done++;
+    return 0;
+}
+
+}  // namespace svlg
+
+using namespace svlg;
+
+extern "C" {
+
+int svlg_cd_init(svlg_model *h, double dt, double ftol) {
+    SVLG_CHECK(h, "null model");
+    Model &m = h->m;
+    SVLG_CHECK(dt > 0.0, "dt must be positive");
+    SVLG_CHECK(m.lumped, "CentralDifference on the device needs MassFormulation = LUMPED");
+    SVLG_CHECK(!m.has_ak, "CentralDifference on the device needs a diagonal C (Rayleigh ak must be 0)");
+    m.dt = dt;
+    m.ftol = ftol;
+    for (auto &b : m.U) {
+        SVLG_HIP(b.alloc(m.ndofn));
+        SVLG_HIP(hipMemsetAsync(b.p, 0, sizeof(double) * m.ndofn, m.stream));
+    }
+    SVLG_HIP(m.F.alloc(m.ndofn));
+    SVLG_HIP(hipMemsetAsync(m.F.p, 0, sizeof(double) * m.ndofn, m.stream));
+    SVLG_HIP(m.Fext.alloc(m.ndofn));
+    SVLG_HIP(hipMemsetAsync(m.Fext.p, 0, sizeof(double) * m.ndofn, m.stream));
+    SVLG_HIP(m.keff.alloc(m.ndofn));
+    SVLG_HIP(m.kforce.alloc(m.ndofn));
+    DevBuf<int> sing;
+    SVLG_HIP(sing.alloc(1));
+    SVLG_HIP(hipMemsetAsync(sing.p, 0, sizeof(int), m.stream));
+    int nb = (m.ndofn + 255) / 256;
+    hipLaunchKernelGGL(k_cd_coeffs, dim3(nb), dim3(256), 0, m.stream, m.ndofn, dt, m.mdiag.p, m.cdiag.p, m.fdof.p,
+                       m.keff.p, m.kforce.p, sing.p);
+    SVLG_HIP(hipGetLastError());
+    int hs = 0;
+    SVLG_HIP(hipMemcpyAsync(&hs, sing.p, sizeof(int), hipMemcpyDeviceToHost, m.stream));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    // EigenSolver::SolveSystem reports failure when the factorization breaks down
+    // (EigenSolver.cpp:52-55): a free DOF with no mass and no damping.
+    SVLG_CHECK(hs == 0, "EigenSolver::SolveSystem() : Solution for the linear system failed (zero Keff diagonal)");
+    m.ring = 0;
+    m.steps_done = 0;
+    m.cd_ready = true;
+    return 0;
+}
+
+int svlg_cd_steps_async(svlg_model *h, int32_t k0, int32_t nsteps) {
+    SVLG_CHECK(h, "null model");
+    Model &m = h->m;
+    SVLG_CHECK(m.cd_ready, "svlg_cd_init has not been called");
+    for (int s = 0; s < nsteps; ++s)
+        if (int rc = cd_one_step(m, k0 + s)) return rc;
+    SVLG_HIP(hipGetLastError());
+    return 0;
+}
+
+int svlg_cd_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const int32_t *rec_dofs, double *hist) {
+    SVLG_CHECK(h, "null model");
+    Model &m = h->m;
+    SVLG_CHECK(m.cd_ready, "svlg_cd_init has not been called");
+    SVLG_CHECK(nsteps >= 0 && nrec >= 0, "negative count");
+    if (nrec > 0) {
+        SVLG_CHECK(rec_dofs && hist, "recorder arrays missing");
+        // Free DOF -> node-DOF position.
+        std::vector<int32_t> f2n(m.nfree, -1);
+        for (int i = 0; i < m.ndofn; ++i)
+            if (m.h_free[i] >= 0) f2n[m.h_free[i]] = i;
+        std::vector<int32_t> rn(nrec);
+        for (int r = 0; r < nrec; ++r) {
+            SVLG_CHECK(rec_dofs[r] >= 0 && rec_dofs[r] < m.nfree, "recorded DOF is not a free DOF");
+            rn[r] = f2n[rec_dofs[r]];
+        }
+        SVLG_HIP(m.rec.upload(rn.data(), rn.size(), m.stream));
+        SVLG_HIP(m.hist.alloc(size_t(nrec) * std::max(nsteps, 1)));
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        if (int rc = cd_one_step(m, k0 + s)) return rc;
+        if (nrec > 0)
+            hipLaunchKernelGGL(k_record, dim3((nrec + 63) / 64), dim3(64), 0, m.stream, nrec, m.rec.p, m.U[m.ring].p,
+                               m.hist.p + size_t(s) * nrec);
+    }
+    SVLG_HIP(hipGetLastError());
+    if (nrec > 0 && nsteps > 0)
+        SVLG_HIP(hipMemcpyAsync(hist, m.hist.p, sizeof(double) * size_t(nrec) * nsteps, hipMemcpyDeviceToHost,
+                                m.stream));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+int svlg_cd_get_state(svlg_model *h, double *U, double *V, double *A) {
+    SVLG_CHECK(h, "null model");
+    Model &m = h->m;
+    SVLG_CHECK(m.cd_ready, "svlg_cd_init has not been called");
+    std::vector<double> u0(m.ndofn), u1(m.ndofn), u2(m.ndofn);
+    SVLG_HIP(hipMemcpyAsync(u0.data(), m.U[m.ring].p, sizeof(double) * m.ndofn, hipMemcpyDeviceToHost, m.stream));
+    SVLG_HIP(hipMemcpyAsync(u1.data(), m.U[(m.ring + 2) % 3].p, sizeof(double) * m.ndofn, hipMemcpyDeviceToHost,
+                            m.stream));
+    SVLG_HIP(hipMemcpyAsync(u2.data(), m.U[(m.ring + 1) % 3].p, sizeof(double) * m.ndofn, hipMemcpyDeviceToHost,
+                            m.stream));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    const double dt = m.dt;
+    for (int i = 0; i < m.ndofn; ++i) {
+        int f = m.h_free[i];
+        if (f < 0) continue;
+        if (U) U[f] = u0[i];
+        // CentralDifference::ComputeNewStep (:141-145) with U = U_{k-1}, Up = U_{k-2},
+        // dU = U_k - U_{k-1}.
+        double dU = u0[i] - u1[i];
+        if (V) V[f] = m.steps_done > 0 ? 1.0 / 2.0 / dt * (u0[i] - u2[i]) : 0.0;
+        if (A) A[f] = m.steps_done > 0 ? 1.0 / dt / dt * ((dU - u1[i]) + u2[i]) : 0.0;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,144 @@
+"""Host-side mirror of SVL's Integrator / Assembler / DynamicAnalysis interfaces.
+
+Method names, argument meaning, the step-index convention and the boolean stop
+semantics follow SVL (RP = 02-Run_Process):
+  * Assembler            RP/07-Assembler/Assembler.hpp:50-181
+  * Integrator           RP/10-Integrators/Integrator.hpp:53-144
+  * CentralDifference    RP/10-Integrators/02-CentralDifference/CentralDifference.hpp:63
+  * DynamicAnalysis      RP/08-Analysis/02-Dynamic/DynamicAnalysis.cpp:25-64
+Every numerical call goes through libsvlg (HIP on gfx950); a failing call raises
+:class:`SvlgError` or, where SVL returns ``bool stop``, returns ``True``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as capi
+from .mesh import Model
+
+
+class DeviceModel:
+    """A model resident in HBM (Mesh::Initialize + Element::SetDomain)."""
+
+    def __init__(self, model: Model):
+        self.model = model
+        self._desc, self._keep = capi.make_desc(model)
+        h = C.c_void_p()
+        capi.check(capi.lib().svlg_model_create(C.byref(self._desc), C.byref(h)), "svlg_model_create")
+        self.h = h
+        self.nfree = model.nfree
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            capi.lib().svlg_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        a = (C.c_int64 * 8)()
+        capi.check(capi.lib().svlg_model_info(self.h, a), "svlg_model_info")
+        keys = ["ncolors", "nelem", "nnodes", "nfree", "variant", "ntiles", "affine", "reserved"]
+        return dict(zip(keys, [int(v) for v in a]))
+
+    def element_colors(self) -> np.ndarray:
+        out = np.zeros(self.model.nelem, np.int32)
+        capi.check(capi.lib().svlg_element_colors(self.h, capi.iptr(out)), "svlg_element_colors")
+        return out
+
+    def set_stream(self, stream_handle: int | None):
+        capi.check(capi.lib().svlg_model_set_stream(self.h, C.c_void_p(stream_handle or 0)), "svlg_model_set_stream")
+
+    def stream(self) -> int:
+        return int(capi.lib().svlg_model_stream(self.h) or 0)
+
+    # ---------------------------------------------------------------- Assembler
+    def ComputeMassMatrix(self) -> np.ndarray:
+        """Lumped mass diagonal in free DOFs (Assembler::ComputeMassMatrix)."""
+        out = np.zeros(self.nfree)
+        capi.check(capi.lib().svlg_mass_diagonal(self.h, capi.fptr(out)), "svlg_mass_diagonal")
+        return out
+
+    def ComputeDampingMatrix(self) -> np.ndarray:
+        """Diagonal damping (dashpots + Rayleigh am*M) in free DOFs."""
+        out = np.zeros(self.nfree)
+        capi.check(capi.lib().svlg_damping_diagonal(self.h, capi.fptr(out)), "svlg_damping_diagonal")
+        return out
+
+    def ComputeInternalForceVector(self, U: np.ndarray, ftol: float = 1e-12) -> np.ndarray:
+        """Fint(U) in free DOFs (Assembler::ComputeInternalForceVector)."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == (self.nfree,)
+        out = np.zeros(self.nfree)
+        capi.check(capi.lib().svlg_internal_force(self.h, capi.fptr(U), float(ftol), capi.fptr(out)),
+                   "svlg_internal_force")
+        return out
+
+    def sync(self):
+        capi.check(capi.lib().svlg_sync(self.h), "svlg_sync")
+
+
+class CentralDifference:
+    """CentralDifference(mesh, TimeStep, mtol, ktol, ftol) on the device."""
+
+    def __init__(self, mesh: DeviceModel, TimeStep: float, mtol: float = 1e-12, ktol: float = 1e-12,
+                 ftol: float = 1e-12):
+        self.mesh = mesh
+        self.dt = float(TimeStep)
+        self.ftol = float(ftol)
+
+    def Initialize(self):
+        capi.check(capi.lib().svlg_cd_init(self.mesh.h, self.dt, self.ftol), "svlg_cd_init")
+
+    def ComputeNewStep(self, k: int) -> bool:
+        """One step with load sample k; returns SVL's stop flag."""
+        rc = capi.lib().svlg_cd_steps(self.mesh.h, int(k), 1, 0, None, None)
+        return rc != 0
+
+    def advance(self, k0: int, nsteps: int, rec_dofs=None) -> np.ndarray | None:
+        """nsteps steps from step index k0; returns U history [nsteps, nrec] if recording."""
+        if rec_dofs is None or len(rec_dofs) == 0:
+            capi.check(capi.lib().svlg_cd_steps(self.mesh.h, int(k0), int(nsteps), 0, None, None), "svlg_cd_steps")
+            return None
+        rec = np.ascontiguousarray(rec_dofs, dtype=np.int32)
+        hist = np.zeros((nsteps, rec.size))
+        capi.check(capi.lib().svlg_cd_steps(self.mesh.h, int(k0), int(nsteps), int(rec.size), capi.iptr(rec),
+                                            capi.fptr(hist)), "svlg_cd_steps")
+        return hist
+
+    def advance_async(self, k0: int, nsteps: int):
+        capi.check(capi.lib().svlg_cd_steps_async(self.mesh.h, int(k0), int(nsteps)), "svlg_cd_steps_async")
+
+    def _state(self):
+        n = self.mesh.nfree
+        U, V, A = np.zeros(n), np.zeros(n), np.zeros(n)
+        capi.check(capi.lib().svlg_cd_get_state(self.mesh.h, capi.fptr(U), capi.fptr(V), capi.fptr(A)),
+                   "svlg_cd_get_state")
+        return U, V, A
+
+    def GetDisplacements(self) -> np.ndarray:
+        return self._state()[0]
+
+    def GetVelocities(self) -> np.ndarray:
+        return self._state()[1]
+
+    def GetAccelerations(self) -> np.ndarray:
+        return self._state()[2]
+
+
+class DynamicAnalysis:
+    """DynamicAnalysis(mesh, integrator, nt): k = 1..nt-1, recorder row k-1."""
+
+    def __init__(self, integrator, nt: int):
+        self.integrator = integrator
+        self.nt = int(nt)
+
+    def Analyze(self, rec_dofs=None):
+        self.integrator.Initialize()
+        return self.integrator.advance(1, self.nt - 1, rec_dofs)

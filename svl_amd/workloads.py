@@ -1,0 +1,106 @@
+"""Synthetic workloads of the dynamic hot path, built in SVL's conventions.
+
+* ``layer2d``  -- 2D plane-strain elastic layer, lin2DQuad4 nx x ny elements of h m,
+  rho = 2000, Vs = 200, nu = 0.25, Lysmer dashpots on left/right/bottom, vertical
+  Ricker point load (velocity form, f0 = 5 Hz, t0 = 0.25 s, Ap = 1e5 N) at the
+  surface centre node, CentralDifference with dt = 0.5 h / Vp. At 200 x 100 this is
+  the 40,602-DOF, 2000-step layer.
+* ``halfspace3d`` -- 3D elastic half-space of lin3DHexa8 elements of h m with six
+  layers Vs 150 .. 800 m/s increasing with depth and a per-element lognormal
+  perturbation exp(sigma z - sigma^2/2), z ~ N(0,1), sigma = 0.1, drawn with
+  ``numpy.random.default_rng(seed)`` in element-tag order; nu = 0.3, rho = 2000;
+  Lysmer dashpots on the bottom and four sides with per-node coefficients from the
+  adjacent element's rho * V * h^2; vertical Ricker f0 = 3 Hz, t0 = 0.4 s at the
+  surface centre; dt = 0.5 min_e(h / Vp_e). At 100 x 100 x 50 elements of 2 m this is
+  the 1,560,753-DOF half-space.
+
+Lysmer convention (Builder.py:1086-1131, per-node generalisation): a boundary node's
+dashpot on the boundary-normal direction takes rho Vp A, the tangential ones rho Vs A,
+with A = h * th in 2D (the convention of SVL's 2D plane-wave examples) and h^2 in 3D;
+corners are counted once per boundary they lie on, as SVL does.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .mesh import (ELEM_HEXA8, ELEM_QUAD4, Model, lysmer_dashpots, make_domain_area, make_domain_volume,
+                   normal_dir, ricker_velocity, tags_for)
+
+
+def layer2d(nx: int = 200, ny: int = 100, h: float = 1.0, nt: int = 2001, rho: float = 2000.0,
+            vs: float = 200.0, nu: float = 0.25, f0: float = 5.0, t0: float = 0.25, Ap: float = 1.0e5,
+            cfl: float = 0.5, lysmer: bool = True):
+    """Returns (model, dt, nt)."""
+    Lx, Ly = nx * h, ny * h
+    P0 = (-Lx / 2.0, -Ly)
+    coords, conn, bnd = make_domain_area(nx, ny, P0, (Lx / 2.0, -Ly), (-Lx / 2.0, 0.0))
+    G = rho * vs * vs
+    E = 2.0 * G * (1.0 + nu)
+    vp = vs * math.sqrt(2.0 * (1.0 - nu) / (1.0 - 2.0 * nu))
+    th = 1.0
+    m = Model(ndim=2, coords=coords, node_tags=tags_for(coords.shape[0]), elem_type=ELEM_QUAD4, conn=conn,
+              E=E, nu=nu, rho=rho, thickness=th, name=f"layer2d_{nx}x{ny}")
+    if lysmer:
+        eta_s, eta_p = rho * vs * h * th, rho * vp * h * th
+        names = ["left", "right", "bottom"]
+        dn, dd, de = lysmer_dashpots(2, bnd, names,
+                                     lambda n, name, k: eta_p if k == normal_dir(2, name) else eta_s)
+        m.dash_node, m.dash_dir, m.dash_eta = dn, dd, de
+    dt = cfl * h / vp
+    t = np.arange(nt) * dt
+    src = m.nearest_node((0.0, 0.0))
+    m.add_point_load(src, [0.0, -1.0], ricker_velocity(t, t0, f0, Ap))
+    return m, dt, nt
+
+
+def halfspace3d(nx: int = 100, ny: int = 100, nz: int = 50, h: float = 2.0, nt: int = 4001,
+                seed: int = 20260301, sigma: float = 0.1, nu: float = 0.3, rho: float = 2000.0,
+                vs_layers=(150.0, 250.0, 350.0, 450.0, 600.0, 800.0), f0: float = 3.0, t0: float = 0.4,
+                Ap: float = 1.0e6, cfl: float = 0.5, lysmer: bool = True):
+    """Returns (model, dt, nt)."""
+    Lx, Ly, Lz = nx * h, ny * h, nz * h
+    P0 = (-Lx / 2.0, -Ly / 2.0, -Lz)
+    coords, conn, bnd = make_domain_volume(nx, ny, nz, P0, (Lx / 2.0, -Ly / 2.0, -Lz),
+                                           (-Lx / 2.0, Ly / 2.0, -Lz), (-Lx / 2.0, -Ly / 2.0, 0.0))
+    ne = conn.shape[0]
+    # Element depth below the surface -> layer index (equal-thickness layers).
+    zc = coords[conn].mean(axis=1)[:, 2]
+    depth = -zc
+    nl = len(vs_layers)
+    layer = np.minimum((depth / Lz * nl).astype(np.int64), nl - 1)
+    rng = np.random.default_rng(seed)
+    z = rng.standard_normal(ne)
+    vs = np.asarray(vs_layers)[layer] * np.exp(sigma * z - 0.5 * sigma * sigma)
+    G = rho * vs * vs
+    E = 2.0 * G * (1.0 + nu)
+    vp = vs * math.sqrt(2.0 * (1.0 - nu) / (1.0 - 2.0 * nu))
+    m = Model(ndim=3, coords=coords, node_tags=tags_for(coords.shape[0]), elem_type=ELEM_HEXA8, conn=conn,
+              E=E, nu=np.full(ne, nu), rho=np.full(ne, rho), name=f"halfspace3d_{nx}x{ny}x{nz}")
+    if lysmer:
+        # Adjacent element of each node: the first element (ascending tag) touching it.
+        first = np.full(coords.shape[0], ne, dtype=np.int64)
+        flat = conn.reshape(-1)
+        np.minimum.at(first, flat, np.arange(flat.size) // conn.shape[1])
+        area = h * h
+        names = ["left", "right", "front", "back", "bottom"]
+
+        def eta(n, name, k):
+            e = first[n]
+            v = vp[e] if k == normal_dir(3, name) else vs[e]
+            return rho * v * area
+
+        dn, dd, de = lysmer_dashpots(3, bnd, names, eta)
+        m.dash_node, m.dash_dir, m.dash_eta = dn, dd, de
+    dt = cfl * float(np.min(h / vp))
+    t = np.arange(nt) * dt
+    src = m.nearest_node((0.0, 0.0, 0.0))
+    m.add_point_load(src, [0.0, 0.0, -1.0], ricker_velocity(t, t0, f0, Ap))
+    return m, dt, nt
+
+
+def dofs_of(model: Model, node: int):
+    """Free DOF indices of a node (restrained components are skipped)."""
+    f = model.free_dof()[node]
+    return [int(x) for x in f if x >= 0]
