@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--ne", type=str, default="100,100,50", help="hexa elements nx,ny,nz")
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile-nodes", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0, help="1 fused node tiles (default), 2 element-colored")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -91,7 +94,7 @@ def main():
     m, dt, nt = W.halfspace3d(nx, ny, nz, h=2.0, nt=max(4001, args.steps + args.warmup + 2))
     print(f"# model {m.name}: nodes {m.nnodes} elems {m.nelem} nfree {m.nfree} dashpots {m.dash_node.size} "
           f"dt {dt:.4e} (built in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
-    dm = DeviceModel(m)
+    dm = DeviceModel(m, tile_nodes=args.tile_nodes, block=args.block, variant=args.variant)
     info = dm.info()
     cd = CentralDifference(dm, dt)
     cd.Initialize()
@@ -129,7 +132,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"halfspace3d lin3DHexa8 {nx}x{ny}x{nz} h=2m layered+lognormal Vs, Lysmer, "
                                f"Ricker point load, CentralDifference",
-                   "nfree": m.nfree, "nelem": m.nelem, "dt": dt, "kernel_variant": info["variant"]},
+                   "nfree": m.nfree, "nelem": m.nelem, "dt": dt, "kernel_variant": info["variant"],
+                   "ntiles": info["ntiles"], "lds_bytes": info["lds_bytes"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "one CentralDifference step (all kernels of the step)",

@@ -70,6 +70,15 @@ typedef struct svlg_mesh_desc {
 
 typedef struct svlg_model svlg_model;
 
+/* Execution options (no counterpart in SVL): node-tile size of the fused explicit
+ * kernel, workgroup size, and the explicit kernel variant. Zero fields take defaults. */
+typedef struct svlg_exec_opts {
+    int32_t tile_nodes;   /* max nodes owned by one workgroup (default 448 in 3D, 1024 in 2D) */
+    int32_t block;        /* threads per workgroup of the fused kernel (default 256) */
+    int32_t variant;      /* 1 fused node-tile kernel (default), 2 element-colored kernels */
+    int32_t reserved[5];
+} svlg_exec_opts;
+
 /* Error string of the last failing call on this thread. */
 const char *svlg_last_error(void);
 /* Library version string (also forces the HIP runtime to load). */
@@ -81,14 +90,22 @@ int svlg_device_count(void);
  * builds the element coloring and node tiles, and assembles the lumped mass and the
  * dashpot damping diagonals on the device (Assembler.cpp:48-158). */
 int svlg_model_create(const svlg_mesh_desc *desc, svlg_model **out);
+/* Same with execution options (NULL = defaults). */
+int svlg_model_create_ex(const svlg_mesh_desc *desc, const svlg_exec_opts *opts, svlg_model **out);
 int svlg_model_destroy(svlg_model *m);
 /* Uses `stream` (a hipStream_t) for every later launch of this model; NULL restores
- * the model's own stream. Returns the stream in use through *current if non-NULL. */
+ * the model's own stream. svlg_model_stream returns the stream in use. */
 int svlg_model_set_stream(svlg_model *m, void *stream);
 void *svlg_model_stream(svlg_model *m);
+/* Host-only planning of the node tiles (no device needed): builds the tiles the fused
+ * kernel would use for `tile_nodes` (0 = default), checks their invariants and returns
+ * stats[0]=ntiles [1]=max owned nodes [2]=max staged nodes [3]=max elements per tile
+ * [4]=sum of tile elements (elements are recomputed by every tile they touch)
+ * [5]=sum of halo nodes [6]=LDS bytes per workgroup [7]=element colors. */
+int svlg_plan_tiles(const svlg_mesh_desc *desc, int32_t tile_nodes, int64_t *stats8);
 /* info[0]=ncolors info[1]=nelem info[2]=nnodes info[3]=nfree info[4]=explicit kernel
- * variant (0 colored, 1 tiled) info[5]=ntiles info[6]=affine (1 if all elements are
- * parallelepipeds) info[7]=reserved */
+ * variant (1 fused node tiles, 2 element-colored) info[5]=ntiles info[6]=affine (1 if
+ * every element is a parallelepiped) info[7]=dynamic LDS bytes of the fused kernel */
 int svlg_model_info(svlg_model *m, int64_t *info8);
 /* Element color of every element (ascending-tag greedy coloring; no two elements of
  * one color share a node). */

@@ -51,6 +51,11 @@ class MeshDesc(C.Structure):
     ]
 
 
+class ExecOpts(C.Structure):
+    _fields_ = [("tile_nodes", C.c_int32), ("block", C.c_int32), ("variant", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
+
+
 def _p(a, t):
     if a is None:
         return C.cast(None, t)
@@ -105,6 +110,8 @@ SIGNATURES = {
     "svlg_version": (C.c_char_p, []),
     "svlg_device_count": (C.c_int, []),
     "svlg_model_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
+    "svlg_model_create_ex": (C.c_int, [C.POINTER(MeshDesc), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "svlg_plan_tiles": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, C.POINTER(C.c_int64)]),
     "svlg_model_destroy": (C.c_int, [C.c_void_p]),
     "svlg_model_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "svlg_model_stream": (C.c_void_p, [C.c_void_p]),

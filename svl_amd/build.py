@@ -33,7 +33,7 @@ COMMON = [
 
 
 def sources():
-    return sorted(CSRC.glob("*.hip"))
+    return sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
 
 
 def build(verbose: bool = False) -> Path:

@@ -53,19 +53,20 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
-// Node tile used by the fused explicit kernel: one workgroup owns the free DOFs of
-// `nown` nodes and recomputes every element touching them from an LDS-staged copy of
-// the displacements of `nstage` nodes.
+// Node tiles of the fused explicit kernel (device copies of HostTiles): one workgroup
+// owns the contiguous device nodes [own_ptr[t], own_ptr[t+1]) and recomputes every
+// element touching them from an LDS-staged copy of U and X of owned + halo nodes.
 struct TileSet {
-    int ntiles = 0;
-    int max_stage = 0, max_elem = 0, max_own = 0, ncolor_max = 0;
-    DevBuf<int32_t> stage_ptr, stage_node;   // [ntiles+1], global node of staged slot
+    int ntiles = 0, ncolors = 0;
+    int max_stage = 0, max_elem = 0, max_own = 0;
+    int block = 256;
+    size_t lds_bytes = 0;
+    DevBuf<int32_t> own_ptr, nplain;         // [ntiles+1], [ntiles]
+    DevBuf<int32_t> halo_ptr, halo_node;     // [ntiles+1], device node of each halo slot
     DevBuf<int32_t> elem_ptr;                // [ntiles+1]
-    DevBuf<uint16_t> elem_lconn;             // [sum elems * nen] local staged slot
-    DevBuf<int32_t> elem_id;                 // [sum elems] element (tag order index)
-    DevBuf<int32_t> color_ptr;               // [ntiles * (ncolor_max+1)] offsets into tile elems
-    DevBuf<int32_t> own_ptr;                 // [ntiles+1]
-    DevBuf<uint16_t> own_slot;               // [sum own] staged slot of owned node
+    DevBuf<uint16_t> elem_lconn;             // [sum elems * nen] staged slot of each corner
+    DevBuf<double> elem_lam, elem_mu;        // [sum elems] Lame constants per tile element
+    DevBuf<int32_t> color_ptr;               // [ntiles * (ncolors+1)]
 };
 
 struct Model {
@@ -94,7 +95,7 @@ struct Model {
 
     // Diagonals (node-DOF layout).
     DevBuf<double> mdiag, cdiag;      // lumped mass, damping
-    DevBuf<double> scratch;           // [nelem*nen*ndim] per-element lumped masses
+    DevBuf<double> scratch;           // [nelem*nen] per-element lumped masses
 
     // Loads.
     int nload = 0, load_nt = 0;
@@ -102,6 +103,10 @@ struct Model {
     DevBuf<double> load_dir, load_factor, load_series;
     std::vector<int32_t> h_load_dofs; // distinct loaded node-DOFs
     DevBuf<int32_t> load_dofs;
+    DevBuf<int32_t> nl_ptr, nl_idx;   // node -> loads acting on it (ascending load index)
+
+    // Device node numbering (tile order): perm[device] = original, inv[original] = device.
+    std::vector<int32_t> perm, inv;
 
     // Central difference state.
     bool cd_ready = false;
@@ -112,12 +117,11 @@ struct Model {
     DevBuf<double> F;                 // internal-force accumulator
     DevBuf<double> Fext;              // dense external force (loaded DOFs only written)
     DevBuf<double> keff, kforce;      // diag(M/dt^2 + C/2dt), diag(M/dt^2 - C/2dt)
-    DevBuf<double> kinv_plain;        // unused placeholder
     DevBuf<double> hist;              // recorder buffer
     DevBuf<int32_t> rec;              // recorded node-DOFs
 
     TileSet tiles;
-    int variant = 0;                  // 0 colored, 1 tiled
+    int variant = 1;                  // 1 fused node tiles, 2 element-colored
 };
 
 }  // namespace svlg

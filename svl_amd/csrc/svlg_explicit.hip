@@ -77,9 +77,204 @@ __global__ void k_record(int nrec, const int32_t *__restrict__ rec, const double
     if (r < nrec) out[r] = U[rec[r]];
 }
 
-int build_tiles(Model &m) {
-    m.variant = 0;
-    m.tiles.ntiles = 0;
+// Arguments of the fused step kernel.
+struct FusedArgs {
+    const int32_t *own_ptr, *nplain, *halo_ptr, *halo_node, *elem_ptr, *color_ptr;
+    const uint16_t *lconn;
+    const double *elam, *emu;
+    const double *X, *Ucur, *Uprev;
+    double *Unext;
+    const double *keff, *kforce;
+    const int32_t *nl_ptr, *nl_idx;
+    const double *load_dir, *load_factor, *load_series;
+    int load_nt, k;
+    double th, ftol;
+    int ntiles, ncolors, max_stage;
+};
+
+__device__ __forceinline__ int color_of(const int32_t *__restrict__ cp, int i) {
+    int c = 0;
+    while (cp[c + 1] <= i) ++c;
+    return c;
+}
+
+// One CentralDifference step for one node tile (one workgroup per tile):
+//  1. stage U_{k-1} and X of owned + halo nodes in LDS;
+//  2. every element touching an owned node computes its internal force (affine modal
+//     form or the Gauss-point loop) and adds the owned corners into LDS, one element
+//     color at a time (no two elements of a color share a node: race-free and the
+//     summation order is fixed);
+//  3. owned DOFs: U_k = U_{k-1} + ((Fext - Fint) + kforce (U_{k-1} - U_{k-2})) / keff,
+//     with kforce == keff and Fext == 0 for the plain nodes at the head of the range.
+// Workgroups b and b+8 run on the same XCD; tiles are handed out so that each XCD
+// works on a contiguous range of neighbouring tiles (shared halo lines in its L2).
+template <int DIM, bool AFFINE>
+__global__ __launch_bounds__(256) void k_cd_fused(FusedArgs a) {
+    constexpr int NEN = Elem<DIM>::nen;
+    extern __shared__ double smem[];
+    const int nt = a.ntiles;
+    const int b = blockIdx.x, xcd = b & 7, j = b >> 3;
+    const int q = nt >> 3, r = nt & 7;
+    const int tile = xcd * q + min(xcd, r) + j;
+    const int o0 = a.own_ptr[tile], nown = a.own_ptr[tile + 1] - o0;
+    const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
+    double *sU = smem;
+    double *sX = smem + size_t(a.max_stage) * DIM;
+    double *sF = smem + size_t(2 * a.max_stage) * DIM;
+    const int tid = threadIdx.x, B = blockDim.x;
+    {
+        const double *Ug = a.Ucur + size_t(o0) * DIM;
+        const double *Xg = a.X + size_t(o0) * DIM;
+        for (int i = tid; i < nown * DIM; i += B) {
+            sU[i] = Ug[i];
+            sX[i] = Xg[i];
+            sF[i] = 0.0;
+        }
+        for (int i = tid; i < nh * DIM; i += B) {
+            int sl = i / DIM, c = i - sl * DIM;
+            size_t n = size_t(a.halo_node[h0 + sl]) * DIM + c;
+            sU[nown * DIM + i] = a.Ucur[n];
+            sX[nown * DIM + i] = a.X[n];
+        }
+    }
+    __syncthreads();
+    const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
+    const int32_t *cp = a.color_ptr + size_t(tile) * (a.ncolors + 1);
+    for (int base = 0; base < ne; base += B) {
+        const int i = base + tid;
+        const bool valid = i < ne;
+        double f[NEN][DIM];
+        int lc[NEN];
+        int mycol = -1;
+        if (valid) {
+            double x[NEN][DIM], u[NEN][DIM];
+            const uint16_t *l = a.lconn + size_t(e0 + i) * NEN;
+#pragma unroll
+            for (int c = 0; c < NEN; ++c) {
+                lc[c] = l[c];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    x[c][d] = sX[lc[c] * DIM + d];
+                    u[c][d] = sU[lc[c] * DIM + d];
+                }
+            }
+            const double lam = a.elam[e0 + i], mu = a.emu[e0 + i];
+            if constexpr (DIM == 3) {
+                if constexpr (AFFINE)
+                    hexa_affine_fint(x, u, lam, mu, f);
+                else
+                    hexa_fint(x, u, lam, mu, f);
+            } else {
+                if constexpr (AFFINE)
+                    quad_affine_fint(x, u, lam, mu, a.th, f);
+                else
+                    quad_fint(x, u, lam, mu, a.th, f);
+            }
+            mycol = color_of(cp, i);
+        }
+        const int clo = color_of(cp, base);
+        const int chi = color_of(cp, min(base + B, ne) - 1);
+        for (int c = clo; c <= chi; ++c) {
+            __syncthreads();
+            if (mycol == c) {
+#pragma unroll
+                for (int q2 = 0; q2 < NEN; ++q2)
+                    if (lc[q2] < nown) {
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) {
+                            double v = f[q2][d];
+                            if (fabs(v) > a.ftol) sF[lc[q2] * DIM + d] += v;
+                        }
+                    }
+            }
+        }
+    }
+    __syncthreads();
+    const int npl = a.nplain[tile];
+    for (int i = tid; i < nown * DIM; i += B) {
+        const int sl = i / DIM;
+        const size_t g = size_t(o0) * DIM + i;
+        const double u = sU[i], fi = sF[i];
+        const double kd = a.keff[g];
+        if (sl < npl) {
+            a.Unext[g] = u + ((0.0 - fi) + kd * (u - a.Uprev[g])) / kd;
+        } else if (kd == 0.0) {
+            a.Unext[g] = 0.0;
+        } else {
+            const int c = i - sl * DIM, node = o0 + sl;
+            double fe = 0.0;
+            for (int q2 = a.nl_ptr[node]; q2 < a.nl_ptr[node + 1]; ++q2) {
+                const int l = a.nl_idx[q2];
+                const double amp = (a.k >= 0 && a.k < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + a.k] : 0.0;
+                fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + c]);
+            }
+            a.Unext[g] = u + ((fe - fi) + a.kforce[g] * (u - a.Uprev[g])) / kd;
+        }
+    }
+}
+
+template <int DIM, bool AFFINE>
+static const void *fused_fn() {
+    return reinterpret_cast<const void *>(&k_cd_fused<DIM, AFFINE>);
+}
+
+static const void *fused_kernel(const Model &m) {
+    if (m.ndim == 3) return m.affine ? fused_fn<3, true>() : fused_fn<3, false>();
+    return m.affine ? fused_fn<2, true>() : fused_fn<2, false>();
+}
+
+int prepare_fused(Model &m) {
+    if (m.variant != 1) return 0;
+    SVLG_CHECK(m.tiles.block <= 256, "fused kernel block must be at most 256 threads");
+    SVLG_CHECK(m.tiles.lds_bytes <= 160 * 1024, "node tile needs more than 160 KB of LDS; use smaller tiles");
+    if (m.tiles.lds_bytes > 64 * 1024)
+        SVLG_HIP(hipFuncSetAttribute(fused_kernel(m), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     int(m.tiles.lds_bytes)));
+    return 0;
+}
+
+static int launch_fused(Model &m, int k, const double *Ucur, const double *Uprev, double *Unext) {
+    const TileSet &t = m.tiles;
+    FusedArgs a;
+    a.own_ptr = t.own_ptr.p;
+    a.nplain = t.nplain.p;
+    a.halo_ptr = t.halo_ptr.p;
+    a.halo_node = t.halo_node.p;
+    a.elem_ptr = t.elem_ptr.p;
+    a.color_ptr = t.color_ptr.p;
+    a.lconn = t.elem_lconn.p;
+    a.elam = t.elem_lam.p;
+    a.emu = t.elem_mu.p;
+    a.X = m.X.p;
+    a.Ucur = Ucur;
+    a.Uprev = Uprev;
+    a.Unext = Unext;
+    a.keff = m.keff.p;
+    a.kforce = m.kforce.p;
+    a.nl_ptr = m.nl_ptr.p;
+    a.nl_idx = m.nl_idx.p;
+    a.load_dir = m.load_dir.p;
+    a.load_factor = m.load_factor.p;
+    a.load_series = m.load_series.p;
+    a.load_nt = m.load_nt;
+    a.k = k;
+    a.th = m.th;
+    a.ftol = m.ftol;
+    a.ntiles = t.ntiles;
+    a.ncolors = t.ncolors;
+    a.max_stage = t.max_stage;
+    dim3 grid(t.ntiles), block(t.block);
+    if (m.ndim == 3) {
+        if (m.affine)
+            hipLaunchKernelGGL((k_cd_fused<3, true>), grid, block, t.lds_bytes, m.stream, a);
+        else
+            hipLaunchKernelGGL((k_cd_fused<3, false>), grid, block, t.lds_bytes, m.stream, a);
+    } else {
+        if (m.affine)
+            hipLaunchKernelGGL((k_cd_fused<2, true>), grid, block, t.lds_bytes, m.stream, a);
+        else
+            hipLaunchKernelGGL((k_cd_fused<2, false>), grid, block, t.lds_bytes, m.stream, a);
+    }
     return 0;
 }
 
@@ -87,6 +282,12 @@ static int cd_one_step(Model &m, int k) {
     double *Ucur = m.U[m.ring].p;
     double *Uprev = m.U[(m.ring + 2) % 3].p;
     double *Unext = m.U[(m.ring + 1) % 3].p;
+    if (m.variant == 1) {
+        launch_fused(m, k, Ucur, Uprev, Unext);
+        m.ring = (m.ring + 1) % 3;
+        m.steps_done++;
+        return 0;
+    }
     if (m.nload > 0)
         hipLaunchKernelGGL(k_loads, dim3(1), dim3(64), 0, m.stream, k, m.nload, m.load_nt, m.ndim,
                            (int)m.h_load_dofs.size(), m.load_dofs.p, m.load_node.p, m.load_dir.p, m.load_factor.p,

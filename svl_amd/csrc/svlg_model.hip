@@ -10,16 +10,18 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <stdexcept>
 
 #include "svlg_common.hpp"
 #include "svlg_elem.hpp"
+#include "svlg_tiles.hpp"
 
 namespace svlg {
 
 static thread_local std::string g_error;
 void set_error(const std::string &msg) { g_error = msg; }
 
-int build_tiles(Model &m);  // svlg_explicit.hip
+int prepare_fused(Model &m);  // svlg_explicit.hip
 
 // ---------------------------------------------------------------- kernels
 // Per-element row-sum lumped mass, scattered later in ascending element order.
@@ -218,7 +220,7 @@ int svlg_device_count(void) {
     return n;
 }
 
-int svlg_model_create(const svlg_mesh_desc *d, svlg_model **out) {
+static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_model **out) {
     SVLG_CHECK(d && out, "svlg_model_create: null argument");
     *out = nullptr;
     SVLG_CHECK(d->ndim == 2 || d->ndim == 3, "ndim must be 2 or 3");
@@ -226,42 +228,57 @@ int svlg_model_create(const svlg_mesh_desc *d, svlg_model **out) {
                "element type must be lin2DQuad4 in 2D or lin3DHexa8 in 3D");
     SVLG_CHECK(d->nnodes > 0 && d->nelem > 0, "empty mesh");
     SVLG_CHECK(d->coords && d->conn && d->free_dof && d->E && d->nu && d->rho, "missing mesh arrays");
-    int ndev = 0;
-    SVLG_CHECK(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, "no HIP device available");
-    auto *h = new svlg_model();
-    Model &m = h->m;
-    m.ndim = d->ndim;
-    m.nen = d->ndim == 2 ? 4 : 8;
-    m.nnodes = d->nnodes;
-    m.nelem = d->nelem;
-    m.nfree = d->nfree;
-    m.ndofn = d->nnodes * d->ndim;
-    m.th = d->thickness;
-    m.lumped = d->lumped != 0;
-    auto fail = [&](int code) {
-        delete h;
-        return code;
-    };
-    for (size_t i = 0; i < size_t(d->nelem) * m.nen; ++i)
-        if (d->conn[i] < 0 || d->conn[i] >= d->nnodes) {
-            set_error("connectivity references a node out of range");
-            return fail(11);
-        }
+    const int ndim = d->ndim, nen = ndim == 2 ? 4 : 8, nn = d->nnodes, ne = d->nelem;
+    for (size_t i = 0; i < size_t(ne) * nen; ++i)
+        SVLG_CHECK(d->conn[i] >= 0 && d->conn[i] < nn, "connectivity references a node out of range");
     if (d->mat_type)
-        for (int e = 0; e < d->nelem; ++e)
-            if (d->mat_type[e] != SVLG_MAT_ELASTIC) {
-                set_error("only Elastic2DPlaneStrain / Elastic3DLinear materials are supported by this build");
-                return fail(11);
-            }
-    m.h_free.assign(d->free_dof, d->free_dof + m.ndofn);
+        for (int e = 0; e < ne; ++e)
+            SVLG_CHECK(d->mat_type[e] == SVLG_MAT_ELASTIC,
+                       "only Elastic2DPlaneStrain / Elastic3DLinear materials are supported by this build");
     {
         int cnt = 0;
-        for (int i = 0; i < m.ndofn; ++i) {
-            if (m.h_free[i] >= d->nfree) { set_error("free_dof index >= nfree"); return fail(11); }
-            if (m.h_free[i] >= 0) cnt++;
+        for (int i = 0; i < nn * ndim; ++i) {
+            SVLG_CHECK(d->free_dof[i] >= -1 && d->free_dof[i] < d->nfree, "free_dof out of range");
+            if (d->free_dof[i] >= 0) cnt++;
         }
-        if (cnt != d->nfree) { set_error("nfree does not match free_dof"); return fail(11); }
+        SVLG_CHECK(cnt == d->nfree, "nfree does not match free_dof");
     }
+    if (d->ndash > 0) {
+        SVLG_CHECK(d->dash_node && d->dash_dir && d->dash_eta, "incomplete dashpot arrays");
+        for (int i = 0; i < d->ndash; ++i)
+            SVLG_CHECK(d->dash_node[i] >= 0 && d->dash_node[i] < nn && d->dash_dir[i] >= 0 && d->dash_dir[i] < ndim,
+                       "dashpot node or direction out of range");
+    }
+    if (d->nload > 0) {
+        SVLG_CHECK(d->load_node && d->load_dir && d->load_factor && d->load_series && d->load_nt > 0,
+                   "incomplete load arrays");
+        for (int l = 0; l < d->nload; ++l)
+            SVLG_CHECK(d->load_node[l] >= 0 && d->load_node[l] < nn, "load node out of range");
+    }
+    int ndev = 0;
+    SVLG_CHECK(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, "no HIP device available");
+
+    auto *h = new svlg_model();
+    Model &m = h->m;
+    auto fail = [&](int code) {
+        hipStream_t own = m.own_stream;
+        delete h;
+        if (own) (void)hipStreamDestroy(own);
+        return code;
+    };
+    m.ndim = ndim;
+    m.nen = nen;
+    m.nnodes = nn;
+    m.nelem = ne;
+    m.nfree = d->nfree;
+    m.ndofn = nn * ndim;
+    m.th = d->thickness;
+    m.lumped = d->lumped != 0;
+    int tile_nodes = (o && o->tile_nodes > 0) ? o->tile_nodes : (ndim == 3 ? 448 : 1024);
+    int block = (o && o->block > 0) ? o->block : 256;
+    m.variant = (o && o->variant > 0) ? o->variant : 1;
+    if (m.variant != 1 && m.variant != 2) { set_error("variant must be 1 or 2"); return fail(11); }
+    if (block % 64 != 0 || block > 1024) { set_error("block must be a multiple of 64, at most 1024"); return fail(11); }
     if (hipStreamCreateWithFlags(&m.own_stream, hipStreamNonBlocking) != hipSuccess) {
         set_error("hipStreamCreate failed");
         return fail(10);
@@ -269,110 +286,172 @@ int svlg_model_create(const svlg_mesh_desc *d, svlg_model **out) {
     m.stream = m.own_stream;
     hipStream_t s = m.stream;
 
-    if (color_elements(d->nelem, m.nen, d->nnodes, d->conn, m.h_color, m.ncolors)) {
+    // Coloring and affinity in SVL's tag order.
+    if (color_elements(ne, nen, nn, d->conn, m.h_color, m.ncolors)) {
         set_error("element coloring needs more than 64 colors");
         return fail(11);
     }
-    m.affine = all_affine(m.ndim, d->nelem, m.nen, d->conn, d->coords);
-    // Color-sorted order (stable: ascending tag inside each color).
-    m.color_off.assign(m.ncolors + 1, 0);
-    for (int e = 0; e < d->nelem; ++e) m.color_off[m.h_color[e] + 1]++;
-    for (int c = 0; c < m.ncolors; ++c) m.color_off[c + 1] += m.color_off[c];
-    std::vector<int32_t> pos(m.color_off.begin(), m.color_off.end() - 1);
-    std::vector<int32_t> conn_c(size_t(d->nelem) * m.nen), eid_c(d->nelem);
-    for (int e = 0; e < d->nelem; ++e) {
-        int p = pos[m.h_color[e]]++;
-        eid_c[p] = e;
-        std::memcpy(&conn_c[size_t(p) * m.nen], &d->conn[size_t(e) * m.nen], sizeof(int32_t) * m.nen);
-    }
-    // Lame constants of the isotropic D (c2 = lambda, c3 = mu; c1 = c2 + 2 c3).
-    std::vector<double> lam(d->nelem), mu(d->nelem), am(d->nelem, 0.0), ak(d->nelem, 0.0);
-    for (int e = 0; e < d->nelem; ++e) {
-        double E = d->E[e], nu = d->nu[e];
-        lam[e] = E * nu / (1.0 - 2.0 * nu) / (1.0 + nu);
-        mu[e] = E / (2.0 * (1.0 + nu));
+    m.affine = all_affine(ndim, ne, nen, d->conn, d->coords);
+    std::vector<double> am(ne, 0.0), ak(ne, 0.0);
+    for (int e = 0; e < ne; ++e) {
         if (d->ray_am) am[e] = d->ray_am[e];
         if (d->ray_ak) ak[e] = d->ray_ak[e];
         if (ak[e] != 0.0) m.has_ak = true;
     }
+    // Nodes that need the general update.
+    std::vector<uint8_t> special(nn, 0);
+    for (int n = 0; n < nn; ++n)
+        for (int c = 0; c < ndim; ++c)
+            if (d->free_dof[size_t(n) * ndim + c] < 0) special[n] = 1;
+    for (int i = 0; i < d->ndash; ++i) special[d->dash_node[i]] = 1;
+    for (int l = 0; l < d->nload; ++l) special[d->load_node[l]] = 1;
+    for (int e = 0; e < ne; ++e)
+        if (am[e] != 0.0)
+            for (int a = 0; a < nen; ++a) special[d->conn[size_t(e) * nen + a]] = 1;
+
+    HostTiles ht;
+    try {
+        make_tiles(ndim, nn, ne, nen, d->coords, d->conn, m.h_color.data(), m.ncolors, special.data(), tile_nodes, ht);
+    } catch (const std::exception &ex) {
+        set_error(std::string("tiling failed: ") + ex.what());
+        return fail(11);
+    }
+    m.perm = ht.perm;
+    m.inv = ht.inv;
+    // Device (tile-ordered) copies of every node-indexed array.
+    std::vector<double> Xp(size_t(nn) * ndim);
+    m.h_free.resize(size_t(nn) * ndim);
+    for (int p = 0; p < nn; ++p)
+        for (int c = 0; c < ndim; ++c) {
+            Xp[size_t(p) * ndim + c] = d->coords[size_t(m.perm[p]) * ndim + c];
+            m.h_free[size_t(p) * ndim + c] = d->free_dof[size_t(m.perm[p]) * ndim + c];
+        }
+    std::vector<int32_t> connp(size_t(ne) * nen);
+    for (size_t i = 0; i < connp.size(); ++i) connp[i] = m.inv[d->conn[i]];
+    // Color-sorted order (stable: ascending tag inside each color).
+    m.color_off.assign(m.ncolors + 1, 0);
+    for (int e = 0; e < ne; ++e) m.color_off[m.h_color[e] + 1]++;
+    for (int c = 0; c < m.ncolors; ++c) m.color_off[c + 1] += m.color_off[c];
+    std::vector<int32_t> pos(m.color_off.begin(), m.color_off.end() - 1);
+    std::vector<int32_t> conn_c(size_t(ne) * nen), eid_c(ne);
+    for (int e = 0; e < ne; ++e) {
+        int p = pos[m.h_color[e]]++;
+        eid_c[p] = e;
+        std::memcpy(&conn_c[size_t(p) * nen], &connp[size_t(e) * nen], sizeof(int32_t) * nen);
+    }
+    // Lame constants of the isotropic D (c2 = lambda, c3 = mu; c1 = c2 + 2 c3).
+    std::vector<double> lam(ne), mu(ne);
+    for (int e = 0; e < ne; ++e) {
+        double E = d->E[e], nu = d->nu[e];
+        lam[e] = E * nu / (1.0 - 2.0 * nu) / (1.0 + nu);
+        mu[e] = E / (2.0 * (1.0 + nu));
+    }
     // Node -> (element, local corner) in ascending element order.
-    std::vector<int32_t> n2e_ptr(d->nnodes + 1, 0), n2e(size_t(d->nelem) * m.nen);
-    for (size_t i = 0; i < size_t(d->nelem) * m.nen; ++i) n2e_ptr[d->conn[i] + 1]++;
-    for (int n = 0; n < d->nnodes; ++n) n2e_ptr[n + 1] += n2e_ptr[n];
+    std::vector<int32_t> n2e_ptr(nn + 1, 0), n2e(size_t(ne) * nen);
+    for (size_t i = 0; i < connp.size(); ++i) n2e_ptr[connp[i] + 1]++;
+    for (int n = 0; n < nn; ++n) n2e_ptr[n + 1] += n2e_ptr[n];
     {
         std::vector<int32_t> p(n2e_ptr.begin(), n2e_ptr.end() - 1);
-        for (int e = 0; e < d->nelem; ++e)
-            for (int a = 0; a < m.nen; ++a) n2e[p[d->conn[size_t(e) * m.nen + a]]++] = e * m.nen + a;
+        for (int e = 0; e < ne; ++e)
+            for (int a = 0; a < nen; ++a) n2e[p[connp[size_t(e) * nen + a]]++] = e * nen + a;
     }
 #define UP(buf, src, cnt)                                                   \
     if (buf.upload(src, cnt, s) != hipSuccess) {                            \
         set_error("device allocation/upload failed for " #buf);             \
         return fail(10);                                                    \
     }
-    UP(m.X, d->coords, size_t(d->nnodes) * m.ndim);
-    UP(m.conn, d->conn, size_t(d->nelem) * m.nen);
+    UP(m.X, Xp.data(), Xp.size());
+    UP(m.conn, connp.data(), connp.size());
     UP(m.conn_c, conn_c.data(), conn_c.size());
     UP(m.eid_c, eid_c.data(), eid_c.size());
     UP(m.lam, lam.data(), lam.size());
     UP(m.mu, mu.data(), mu.size());
-    UP(m.rho, d->rho, size_t(d->nelem));
+    UP(m.rho, d->rho, size_t(ne));
     UP(m.am, am.data(), am.size());
     UP(m.ak, ak.data(), ak.size());
     UP(m.fdof, m.h_free.data(), m.h_free.size());
     UP(m.n2e_ptr, n2e_ptr.data(), n2e_ptr.size());
     UP(m.n2e, n2e.data(), n2e.size());
-    // Loads.
+    // Tiles.
+    {
+        TileSet &t = m.tiles;
+        t.ntiles = ht.ntiles;
+        t.ncolors = ht.ncolors;
+        t.max_stage = ht.max_stage;
+        t.max_own = ht.max_own;
+        t.max_elem = ht.max_elem;
+        t.block = block;
+        t.lds_bytes = size_t(2 * ht.max_stage + ht.max_own) * ndim * sizeof(double);
+        std::vector<double> el(ht.elem_id.size()), em(ht.elem_id.size());
+        for (size_t i = 0; i < ht.elem_id.size(); ++i) {
+            el[i] = lam[ht.elem_id[i]];
+            em[i] = mu[ht.elem_id[i]];
+        }
+        UP(t.own_ptr, ht.own_ptr.data(), ht.own_ptr.size());
+        UP(t.nplain, ht.nplain.data(), ht.nplain.size());
+        UP(t.halo_ptr, ht.halo_ptr.data(), ht.halo_ptr.size());
+        UP(t.halo_node, ht.halo_node.data(), ht.halo_node.size());
+        UP(t.elem_ptr, ht.elem_ptr.data(), ht.elem_ptr.size());
+        UP(t.elem_lconn, ht.elem_lconn.data(), ht.elem_lconn.size());
+        UP(t.elem_lam, el.data(), el.size());
+        UP(t.elem_mu, em.data(), em.size());
+        UP(t.color_ptr, ht.color_ptr.data(), ht.color_ptr.size());
+    }
+    // Loads (device node numbering).
     m.nload = d->nload;
     m.load_nt = d->load_nt;
     if (d->nload > 0) {
-        SVLG_CHECK(d->load_node && d->load_dir && d->load_factor && d->load_series && d->load_nt > 0,
-                   "incomplete load arrays");
-        UP(m.load_node, d->load_node, size_t(d->nload));
-        UP(m.load_dir, d->load_dir, size_t(d->nload) * m.ndim);
+        std::vector<int32_t> lnode(d->nload);
+        for (int l = 0; l < d->nload; ++l) lnode[l] = m.inv[d->load_node[l]];
+        UP(m.load_node, lnode.data(), lnode.size());
+        UP(m.load_dir, d->load_dir, size_t(d->nload) * ndim);
         UP(m.load_factor, d->load_factor, size_t(d->nload));
         UP(m.load_series, d->load_series, size_t(d->nload) * d->load_nt);
         std::vector<int32_t> dofs;
-        for (int l = 0; l < d->nload; ++l) {
-            SVLG_CHECK(d->load_node[l] >= 0 && d->load_node[l] < d->nnodes, "load node out of range");
-            for (int c = 0; c < m.ndim; ++c) dofs.push_back(d->load_node[l] * m.ndim + c);
-        }
+        for (int l = 0; l < d->nload; ++l)
+            for (int c = 0; c < ndim; ++c) dofs.push_back(lnode[l] * ndim + c);
         std::sort(dofs.begin(), dofs.end());
         dofs.erase(std::unique(dofs.begin(), dofs.end()), dofs.end());
         m.h_load_dofs = dofs;
         UP(m.load_dofs, dofs.data(), dofs.size());
+        std::vector<int32_t> nlp(nn + 1, 0), nli(d->nload);
+        for (int l = 0; l < d->nload; ++l) nlp[lnode[l] + 1]++;
+        for (int n = 0; n < nn; ++n) nlp[n + 1] += nlp[n];
+        std::vector<int32_t> q(nlp.begin(), nlp.end() - 1);
+        for (int l = 0; l < d->nload; ++l) nli[q[lnode[l]]++] = l;
+        UP(m.nl_ptr, nlp.data(), nlp.size());
+        UP(m.nl_idx, nli.data(), nli.size());
+    } else {
+        std::vector<int32_t> nlp(nn + 1, 0);
+        UP(m.nl_ptr, nlp.data(), nlp.size());
     }
     // Mass and damping diagonals.
     if (m.mdiag.alloc(m.ndofn) != hipSuccess || m.cdiag.alloc(m.ndofn) != hipSuccess ||
-        m.scratch.alloc(size_t(d->nelem) * m.nen) != hipSuccess) {
+        m.scratch.alloc(size_t(ne) * nen) != hipSuccess) {
         set_error("device allocation failed (diagonals)");
         return fail(10);
     }
     {
-        int nb = (d->nelem + 127) / 128;
-        if (m.ndim == 2)
-            hipLaunchKernelGGL(k_elem_lumped_mass<2>, dim3(nb), dim3(128), 0, s, d->nelem, m.conn.p, m.X.p, m.rho.p,
-                               m.th, m.scratch.p);
+        int nb = (ne + 127) / 128;
+        if (ndim == 2)
+            hipLaunchKernelGGL(k_elem_lumped_mass<2>, dim3(nb), dim3(128), 0, s, ne, m.conn.p, m.X.p, m.rho.p, m.th,
+                               m.scratch.p);
         else
-            hipLaunchKernelGGL(k_elem_lumped_mass<3>, dim3(nb), dim3(128), 0, s, d->nelem, m.conn.p, m.X.p, m.rho.p,
+            hipLaunchKernelGGL(k_elem_lumped_mass<3>, dim3(nb), dim3(128), 0, s, ne, m.conn.p, m.X.p, m.rho.p,
                                m.th, m.scratch.p);
-        int nbn = (d->nnodes + 127) / 128;
-        hipLaunchKernelGGL(k_node_gather_mass, dim3(nbn), dim3(128), 0, s, d->nnodes, m.ndim, m.n2e_ptr.p, m.n2e.p,
-                           m.scratch.p, m.am.p, m.nen, 1e-12, m.mdiag.p, m.cdiag.p);
+        int nbn = (nn + 127) / 128;
+        hipLaunchKernelGGL(k_node_gather_mass, dim3(nbn), dim3(128), 0, s, nn, ndim, m.n2e_ptr.p, m.n2e.p,
+                           m.scratch.p, m.am.p, nen, 1e-12, m.mdiag.p, m.cdiag.p);
     }
     if (d->ndash > 0) {
-        SVLG_CHECK(d->dash_node && d->dash_dir && d->dash_eta, "incomplete dashpot arrays");
-        for (int i = 0; i < d->ndash; ++i) {
-            if (d->dash_node[i] < 0 || d->dash_node[i] >= d->nnodes || d->dash_dir[i] < 0 || d->dash_dir[i] >= m.ndim) {
-                set_error("dashpot node or direction out of range");
-                return fail(11);
-            }
-        }
+        std::vector<int32_t> dnode(d->ndash);
+        for (int i = 0; i < d->ndash; ++i) dnode[i] = m.inv[d->dash_node[i]];
         DevBuf<int32_t> dn, dd;
         DevBuf<double> de;
-        UP(dn, d->dash_node, size_t(d->ndash));
+        UP(dn, dnode.data(), dnode.size());
         UP(dd, d->dash_dir, size_t(d->ndash));
         UP(de, d->dash_eta, size_t(d->ndash));
-        hipLaunchKernelGGL(k_dashpots, dim3(1), dim3(64), 0, s, d->ndash, m.ndim, dn.p, dd.p, de.p, 1e-12, m.cdiag.p);
+        hipLaunchKernelGGL(k_dashpots, dim3(1), dim3(64), 0, s, d->ndash, ndim, dn.p, dd.p, de.p, 1e-12, m.cdiag.p);
         if (hipStreamSynchronize(s) != hipSuccess) { set_error("dashpot assembly failed"); return fail(10); }
     }
 #undef UP
@@ -380,10 +459,99 @@ int svlg_model_create(const svlg_mesh_desc *d, svlg_model **out) {
         set_error("model set-up kernels failed");
         return fail(10);
     }
-    int rc = build_tiles(m);
-    if (rc) return fail(rc);
+    if (int rc = prepare_fused(m)) return fail(rc);
     *out = h;
     return 0;
+}
+
+int svlg_model_create(const svlg_mesh_desc *d, svlg_model **out) { return create_impl(d, nullptr, out); }
+
+int svlg_plan_tiles(const svlg_mesh_desc *d, int32_t tile_nodes, int64_t *stats) {
+    SVLG_CHECK(d && stats, "null argument");
+    SVLG_CHECK(d->ndim == 2 || d->ndim == 3, "ndim must be 2 or 3");
+    const int ndim = d->ndim, nen = ndim == 2 ? 4 : 8, nn = d->nnodes, ne = d->nelem;
+    for (size_t i = 0; i < size_t(ne) * nen; ++i)
+        SVLG_CHECK(d->conn[i] >= 0 && d->conn[i] < nn, "connectivity references a node out of range");
+    std::vector<int32_t> color;
+    int ncolors = 0;
+    SVLG_CHECK(color_elements(ne, nen, nn, d->conn, color, ncolors) == 0, "coloring failed");
+    std::vector<uint8_t> special(nn, 0);
+    for (int n = 0; n < nn; ++n)
+        for (int c = 0; c < ndim; ++c)
+            if (d->free_dof[size_t(n) * ndim + c] < 0) special[n] = 1;
+    for (int i = 0; i < d->ndash; ++i) special[d->dash_node[i]] = 1;
+    for (int l = 0; l < d->nload; ++l) special[d->load_node[l]] = 1;
+    HostTiles t;
+    try {
+        make_tiles(ndim, nn, ne, nen, d->coords, d->conn, color.data(), ncolors, special.data(),
+                   tile_nodes > 0 ? tile_nodes : (ndim == 3 ? 448 : 1024), t);
+    } catch (const std::exception &ex) {
+        set_error(std::string("tiling failed: ") + ex.what());
+        return 11;
+    }
+    // Invariants: every node owned once, owned ranges contiguous, plain nodes first;
+    // each tile lists exactly the elements touching its nodes, colour-sorted; every
+    // corner maps to a staged slot holding that corner's node.
+    std::vector<int> owner(nn, -1);
+    std::vector<int32_t> seen(ne, -1);
+    int64_t tot_elem = 0, tot_halo = 0;
+    for (int ti = 0; ti < t.ntiles; ++ti) {
+        int o0 = t.own_ptr[ti], o1 = t.own_ptr[ti + 1];
+        SVLG_CHECK(o0 <= o1 && o1 - o0 <= t.max_own, "bad owned range");
+        for (int p = o0; p < o1; ++p) {
+            int n = t.perm[p];
+            SVLG_CHECK(owner[n] < 0 && t.inv[n] == p, "node owned twice or bad permutation");
+            owner[n] = ti;
+            SVLG_CHECK((p - o0 < t.nplain[ti]) == (special[n] == 0), "plain/special order broken");
+        }
+        int e0 = t.elem_ptr[ti], e1 = t.elem_ptr[ti + 1];
+        int nh = t.halo_ptr[ti + 1] - t.halo_ptr[ti];
+        int nstage = (o1 - o0) + nh;
+        tot_elem += e1 - e0;
+        tot_halo += nh;
+        const int32_t *cp = &t.color_ptr[size_t(ti) * (ncolors + 1)];
+        SVLG_CHECK(cp[0] == 0 && cp[ncolors] == e1 - e0, "bad colour offsets");
+        for (int k = e0; k < e1; ++k) {
+            int e = t.elem_id[k];
+            int c = 0;
+            while (cp[c + 1] <= k - e0) ++c;
+            SVLG_CHECK(color[e] == c, "element outside its colour segment");
+            seen[e] = ti;
+            bool touches = false;
+            for (int a = 0; a < nen; ++a) {
+                int sl = t.elem_lconn[size_t(k) * nen + a];
+                SVLG_CHECK(sl < nstage, "corner slot beyond the staged nodes");
+                int dev = sl < o1 - o0 ? o0 + sl : t.halo_node[t.halo_ptr[ti] + sl - (o1 - o0)];
+                SVLG_CHECK(t.perm[dev] == d->conn[size_t(e) * nen + a], "corner slot holds the wrong node");
+                if (sl < o1 - o0) touches = true;
+            }
+            SVLG_CHECK(touches, "tile element touches no owned node");
+        }
+        for (int p = o0; p < o1; ++p) {
+            (void)p;
+        }
+    }
+    for (int n = 0; n < nn; ++n) SVLG_CHECK(owner[n] >= 0, "node not owned by any tile");
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < nen; ++a) {
+            int ti = owner[d->conn[size_t(e) * nen + a]];
+            bool found = false;
+            for (int k = t.elem_ptr[ti]; k < t.elem_ptr[ti + 1] && !found; ++k) found = t.elem_id[k] == e;
+            SVLG_CHECK(found, "element missing from the tile owning one of its nodes");
+        }
+    stats[0] = t.ntiles;
+    stats[1] = t.max_own;
+    stats[2] = t.max_stage;
+    stats[3] = t.max_elem;
+    stats[4] = tot_elem;
+    stats[5] = tot_halo;
+    stats[6] = int64_t(size_t(2 * t.max_stage + t.max_own) * ndim * sizeof(double));
+    stats[7] = ncolors;
+    return 0;
+}
+
+int svlg_model_create_ex(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_model **out) {
+    return create_impl(d, o, out);
 }
 
 int svlg_model_destroy(svlg_model *h) {
@@ -414,7 +582,7 @@ int svlg_model_info(svlg_model *h, int64_t *info) {
     info[4] = m.variant;
     info[5] = m.tiles.ntiles;
     info[6] = m.affine ? 1 : 0;
-    info[7] = 0;
+    info[7] = int64_t(m.tiles.lds_bytes);
     return 0;
 }
 

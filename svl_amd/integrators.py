@@ -22,11 +22,13 @@ from .mesh import Model
 class DeviceModel:
     """A model resident in HBM (Mesh::Initialize + Element::SetDomain)."""
 
-    def __init__(self, model: Model):
+    def __init__(self, model: Model, tile_nodes: int = 0, block: int = 0, variant: int = 0):
         self.model = model
         self._desc, self._keep = capi.make_desc(model)
         h = C.c_void_p()
-        capi.check(capi.lib().svlg_model_create(C.byref(self._desc), C.byref(h)), "svlg_model_create")
+        opts = capi.ExecOpts(tile_nodes, block, variant)
+        capi.check(capi.lib().svlg_model_create_ex(C.byref(self._desc), C.byref(opts), C.byref(h)),
+                   "svlg_model_create")
         self.h = h
         self.nfree = model.nfree
 
@@ -44,7 +46,7 @@ class DeviceModel:
     def info(self) -> dict:
         a = (C.c_int64 * 8)()
         capi.check(capi.lib().svlg_model_info(self.h, a), "svlg_model_info")
-        keys = ["ncolors", "nelem", "nnodes", "nfree", "variant", "ntiles", "affine", "reserved"]
+        keys = ["ncolors", "nelem", "nnodes", "nfree", "variant", "ntiles", "affine", "lds_bytes"]
         return dict(zip(keys, [int(v) for v in a]))
 
     def element_colors(self) -> np.ndarray:
@@ -82,6 +84,17 @@ class DeviceModel:
 
     def sync(self):
         capi.check(capi.lib().svlg_sync(self.h), "svlg_sync")
+
+
+def plan_tiles(model: Model, tile_nodes: int = 0) -> dict:
+    """Host-only tile plan of the fused explicit kernel (no GPU needed)."""
+    desc, keep = capi.make_desc(model)
+    st = (C.c_int64 * 8)()
+    capi.check(capi.lib().svlg_plan_tiles(C.byref(desc), int(tile_nodes), st), "svlg_plan_tiles")
+    keys = ["ntiles", "max_own", "max_stage", "max_elem", "tile_elems", "halo_nodes", "lds_bytes", "ncolors"]
+    out = dict(zip(keys, [int(v) for v in st]))
+    out["redundancy"] = out["tile_elems"] / model.nelem
+    return out
 
 
 class CentralDifference:
