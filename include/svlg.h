@@ -73,10 +73,15 @@ typedef struct svlg_model svlg_model;
 /* Execution options (no counterpart in SVL): node-tile size of the fused explicit
  * kernel, workgroup size, and the explicit kernel variant. Zero fields take defaults. */
 typedef struct svlg_exec_opts {
-    int32_t tile_nodes;   /* max nodes owned by one workgroup (default 448 in 3D, 1024 in 2D) */
-    int32_t block;        /* threads per workgroup of the fused kernel (default 256) */
-    int32_t variant;      /* 1 fused node-tile kernel (default), 2 element-colored kernels */
-    int32_t reserved[5];
+    int32_t tile_nodes;   /* max nodes owned by one workgroup (default 320 in 3D, 448 in 2D) */
+    int32_t block;        /* threads per workgroup of the colour-phase fused kernel (default 512) */
+    int32_t variant;      /* 0 auto: 3 when the mesh is corner-unique and a tile fits one round,
+                           * else 1; 1 fused node tiles with colour phases, 2 element-colored
+                           * kernels, 3 fused node tiles with corner slots */
+    int32_t geo_table;    /* parallelepipeds: 0 = share (J^-1, |J|) through a table of distinct
+                           * geometries when it holds at most 65535 entries, -1 = never (stage
+                           * node coordinates in LDS instead) */
+    int32_t reserved[4];
 } svlg_exec_opts;
 
 /* Error string of the last failing call on this thread. */
@@ -104,8 +109,9 @@ void *svlg_model_stream(svlg_model *m);
  * [5]=sum of halo nodes [6]=LDS bytes per workgroup [7]=element colors. */
 int svlg_plan_tiles(const svlg_mesh_desc *desc, int32_t tile_nodes, int64_t *stats8);
 /* info[0]=ncolors info[1]=nelem info[2]=nnodes info[3]=nfree info[4]=explicit kernel
- * variant (1 fused node tiles, 2 element-colored) info[5]=ntiles info[6]=affine (1 if
- * every element is a parallelepiped) info[7]=dynamic LDS bytes of the fused kernel */
+ * variant (1 fused + colour phases, 2 element-colored, 3 fused + corner slots) info[5]=ntiles info[6]=element path
+ * of the fused kernel (0 Gauss points, 1 modal with staged coordinates, 2 modal with
+ * the geometry table) info[7]=dynamic LDS bytes of the fused kernel */
 int svlg_model_info(svlg_model *m, int64_t *info8);
 /* Element color of every element (ascending-tag greedy coloring; no two elements of
  * one color share a node). */

@@ -53,7 +53,7 @@ class MeshDesc(C.Structure):
 
 class ExecOpts(C.Structure):
     _fields_ = [("tile_nodes", C.c_int32), ("block", C.c_int32), ("variant", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("geo_table", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 def _p(a, t):

@@ -23,6 +23,7 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
 COMMON = [
     "--offload-arch=gfx950",
+    "-I", str(PKG.parent / "include"),
     "-O3",
     "-std=c++17",
     "-fPIC",

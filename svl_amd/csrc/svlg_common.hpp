@@ -7,7 +7,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/svlg.h"
+#include "svlg.h"
 
 namespace svlg {
 
@@ -66,6 +66,8 @@ struct TileSet {
     DevBuf<int32_t> elem_ptr;                // [ntiles+1]
     DevBuf<uint16_t> elem_lconn;             // [sum elems * nen] staged slot of each corner
     DevBuf<double> elem_lam, elem_mu;        // [sum elems] Lame constants per tile element
+                                             // (times |J|/8, or |J| th/4 in 2D, on the table path)
+    DevBuf<uint16_t> elem_geo;               // [sum elems] geometry-table entry (table path)
     DevBuf<int32_t> color_ptr;               // [ntiles * (ncolors+1)]
 };
 
@@ -122,6 +124,9 @@ struct Model {
 
     TileSet tiles;
     int variant = 1;                  // 1 fused node tiles, 2 element-colored
+    int elem_path = 0;                // 0 Gauss points, 1 modal + staged X, 2 modal + geometry table
+    int ngeo = 0;
+    DevBuf<double> geo;               // [ngeo * ndim * ndim] inverse Jacobians of distinct geometries
 };
 
 }  // namespace svlg

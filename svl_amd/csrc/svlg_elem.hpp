@@ -351,26 +351,12 @@ __device__ __forceinline__ double inv3(const double (&J)[3][3], double (&I)[3][3
     return fabs(det);
 }
 
-// Internal force of an affine Hexa8 (x: corner coordinates, u: corner displacements).
-__device__ __forceinline__ void hexa_affine_fint(const double (&x)[8][3], const double (&u)[8][3], double lam,
-                                                 double mu, double (&f)[8][3]) {
-    // Centre Jacobian J[k][j] = d x_j / d xi_k = m_k(x_j) / 8.
-    double J[3][3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        double v[8];
-#pragma unroll
-        for (int a = 0; a < 8; ++a) v[a] = x[a][j];
-        double mx, my, mz, mxy, myz, mzx, mxyz;
-        hexa_modes(v, mx, my, mz, mxy, myz, mzx, mxyz);
-        J[0][j] = 0.125 * mx;
-        J[1][j] = 0.125 * my;
-        J[2][j] = 0.125 * mz;
-    }
-    double Lam[3][3];
-    double adet = inv3(J, Lam);
-    // Displacement modes per component (rows of G carry m/8; the 1/8 and |J| go into
-    // the material constants).
+// Centre Jacobian of a Hexa8 (J[k][j] = d x_j / d xi_k = m_k(x_j) / 8), its inverse and |det J|.
+__device__ __host__ inline double hexa_center_geometry(const double (&x)[8][3], double (&Lam)[3][3]);
+
+// Modal internal force of an affine Hexa8: Lam = J^-1, l = lambda |J| / 8, m = mu |J| / 8.
+__device__ __forceinline__ void hexa_modal_force(const double (&Lam)[3][3], double l, double m,
+                                                 const double (&u)[8][3], double (&f)[8][3]) {
     double Mx[3], My[3], Mz[3], Mxy[3], Myz[3], Mzx[3], Mxyz[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -379,8 +365,6 @@ __device__ __forceinline__ void hexa_affine_fint(const double (&x)[8][3], const 
         for (int a = 0; a < 8; ++a) v[a] = u[a][c];
         hexa_modes(v, Mx[c], My[c], Mz[c], Mxy[c], Myz[c], Mzx[c], Mxyz[c]);
     }
-    const double s = 0.125 * adet;
-    const double l = lam * s, m = mu * s;
     const double g2 = kGP * kGP, g4 = g2 * g2;
     double G[3][3], P[3][3];
     double Fx[3], Fy[3], Fz[3], Fxy[3], Fyz[3], Fzx[3], Fxyz[3];
@@ -433,6 +417,14 @@ __device__ __forceinline__ void hexa_affine_fint(const double (&x)[8][3], const 
     }
 }
 
+// Internal force of an affine Hexa8 from corner coordinates and displacements.
+__device__ __forceinline__ void hexa_affine_fint(const double (&x)[8][3], const double (&u)[8][3], double lam,
+                                                 double mu, double (&f)[8][3]) {
+    double Lam[3][3];
+    const double s = 0.125 * hexa_center_geometry(x, Lam);
+    hexa_modal_force(Lam, lam * s, mu * s, u, f);
+}
+
 // Quad4 corner order (-,-),(+,-),(+,+),(-,+): mx = sum sx v, my, mxy.
 __device__ __forceinline__ void quad_modes(const double (&v)[4], double &mx, double &my, double &mxy) {
     double s0 = v[1] + v[0], d0 = v[1] - v[0];
@@ -451,29 +443,18 @@ __device__ __forceinline__ void quad_unmodes(double mx, double my, double mxy, d
     v[2] = s1 + d1;
 }
 
-__device__ __forceinline__ void quad_affine_fint(const double (&x)[4][2], const double (&u)[4][2], double lam,
-                                                 double mu, double th, double (&f)[4][2]) {
-    double J[2][2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        double v[4] = {x[0][j], x[1][j], x[2][j], x[3][j]};
-        double mx, my, mxy;
-        quad_modes(v, mx, my, mxy);
-        J[0][j] = 0.25 * mx;
-        J[1][j] = 0.25 * my;
-    }
-    double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    double r = 1.0 / det;
-    double Lam[2][2] = {{J[1][1] * r, -J[0][1] * r}, {-J[1][0] * r, J[0][0] * r}};
-    double adet = fabs(det);
+// Centre Jacobian of a Quad4 (J[k][j] = m_k(x_j) / 4), its inverse and |det J|.
+__device__ __host__ inline double quad_center_geometry(const double (&x)[4][2], double (&Lam)[2][2]);
+
+// Modal internal force of an affine Quad4: l = lambda |J| th / 4, m = mu |J| th / 4.
+__device__ __forceinline__ void quad_modal_force(const double (&Lam)[2][2], double l, double m,
+                                                 const double (&u)[4][2], double (&f)[4][2]) {
     double Mx[2], My[2], Mxy[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         double v[4] = {u[0][c], u[1][c], u[2][c], u[3][c]};
         quad_modes(v, Mx[c], My[c], Mxy[c]);
     }
-    const double s = 0.25 * adet * th;
-    const double l = lam * s, m = mu * s;
     const double g2 = kGP * kGP;
     double G[2][2], P[2][2], Fx[2], Fy[2], Fxy[2];
 #pragma unroll
@@ -498,6 +479,62 @@ __device__ __forceinline__ void quad_affine_fint(const double (&x)[4][2], const 
 #pragma unroll
         for (int a = 0; a < 4; ++a) f[a][c] = v[a];
     }
+}
+
+__device__ __forceinline__ void quad_affine_fint(const double (&x)[4][2], const double (&u)[4][2], double lam,
+                                                 double mu, double th, double (&f)[4][2]) {
+    double Lam[2][2];
+    const double s = 0.25 * quad_center_geometry(x, Lam) * th;
+    quad_modal_force(Lam, lam * s, mu * s, u, f);
+}
+
+// Shared host/device geometry (also used on the host to build the geometry table).
+__device__ __host__ inline double hexa_center_geometry(const double (&x)[8][3], double (&Lam)[3][3]) {
+    double J[3][3];
+    for (int j = 0; j < 3; ++j) {
+        double v[8];
+        for (int a = 0; a < 8; ++a) v[a] = x[a][j];
+        double s00 = v[1] - v[0], s10 = v[2] - v[3], s01 = v[5] - v[4], s11 = v[6] - v[7];
+        double mx = (s11 + s01) + (s10 + s00);
+        double t00 = v[1] + v[0], t10 = v[2] + v[3], t01 = v[5] + v[4], t11 = v[6] + v[7];
+        double my = (t11 - t01) + (t10 - t00);
+        double mz = (t11 + t01) - (t10 + t00);
+        J[0][j] = 0.125 * mx;
+        J[1][j] = 0.125 * my;
+        J[2][j] = 0.125 * mz;
+    }
+    double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    double r = 1.0 / det;
+    Lam[0][0] = c00 * r;
+    Lam[1][0] = c01 * r;
+    Lam[2][0] = c02 * r;
+    Lam[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r;
+    Lam[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
+    Lam[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
+    Lam[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
+    Lam[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
+    Lam[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
+    return fabs(det);
+}
+
+__device__ __host__ inline double quad_center_geometry(const double (&x)[4][2], double (&Lam)[2][2]) {
+    double J[2][2];
+    for (int j = 0; j < 2; ++j) {
+        double s0 = x[1][j] + x[0][j], d0 = x[1][j] - x[0][j];
+        double s1 = x[2][j] + x[3][j], d1 = x[2][j] - x[3][j];
+        J[0][j] = 0.25 * (d1 + d0);
+        J[1][j] = 0.25 * (s1 - s0);
+    }
+    double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    double r = 1.0 / det;
+    Lam[0][0] = J[1][1] * r;
+    Lam[0][1] = -J[0][1] * r;
+    Lam[1][0] = -J[1][0] * r;
+    Lam[1][1] = J[0][0] * r;
+    return fabs(det);
 }
 
 }  // namespace svlg

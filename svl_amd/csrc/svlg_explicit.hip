@@ -82,6 +82,8 @@ struct FusedArgs {
     const int32_t *own_ptr, *nplain, *halo_ptr, *halo_node, *elem_ptr, *color_ptr;
     const uint16_t *lconn;
     const double *elam, *emu;
+    const uint16_t *egeo;
+    const double *geo;
     const double *X, *Ucur, *Uprev;
     double *Unext;
     const double *keff, *kforce;
@@ -89,7 +91,7 @@ struct FusedArgs {
     const double *load_dir, *load_factor, *load_series;
     int load_nt, k;
     double th, ftol;
-    int ntiles, ncolors, max_stage;
+    int ntiles, ncolors, max_stage, max_own;
 };
 
 __device__ __forceinline__ int color_of(const int32_t *__restrict__ cp, int i) {
@@ -108,38 +110,75 @@ __device__ __forceinline__ int color_of(const int32_t *__restrict__ cp, int i) {
 //     with kforce == keff and Fext == 0 for the plain nodes at the head of the range.
 // Workgroups b and b+8 run on the same XCD; tiles are handed out so that each XCD
 // works on a contiguous range of neighbouring tiles (shared halo lines in its L2).
-template <int DIM, bool AFFINE>
-__global__ __launch_bounds__(256) void k_cd_fused(FusedArgs a) {
+template <int DIM, int PATH>
+__global__ __launch_bounds__(512) void k_cd_fused(FusedArgs a) {
     constexpr int NEN = Elem<DIM>::nen;
+    constexpr int NB = 4;  // loads kept in flight per thread in the staging and update loops
     extern __shared__ double smem[];
+    __shared__ int32_t cps[65];
     const int nt = a.ntiles;
     const int b = blockIdx.x, xcd = b & 7, j = b >> 3;
     const int q = nt >> 3, r = nt & 7;
     const int tile = xcd * q + min(xcd, r) + j;
     const int o0 = a.own_ptr[tile], nown = a.own_ptr[tile + 1] - o0;
     const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
+    const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
+    constexpr bool STAGE_X = PATH != 2;
     double *sU = smem;
     double *sX = smem + size_t(a.max_stage) * DIM;
-    double *sF = smem + size_t(2 * a.max_stage) * DIM;
+    double *sF = smem + size_t((STAGE_X ? 2 : 1) * a.max_stage) * DIM;
     const int tid = threadIdx.x, B = blockDim.x;
+    const int nU = nown * DIM, nH = nh * DIM;
+    for (int c = tid; c <= a.ncolors; c += B) cps[c] = a.color_ptr[size_t(tile) * (a.ncolors + 1) + c];
+    // Stage owned (contiguous) and halo (gathered) U, and X when needed, NB loads in flight.
     {
         const double *Ug = a.Ucur + size_t(o0) * DIM;
         const double *Xg = a.X + size_t(o0) * DIM;
-        for (int i = tid; i < nown * DIM; i += B) {
-            sU[i] = Ug[i];
-            sX[i] = Xg[i];
-            sF[i] = 0.0;
+        for (int base = 0; base < nU; base += NB * B) {
+            double v[NB], w[NB];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int i = base + k * B + tid;
+                v[k] = i < nU ? Ug[i] : 0.0;
+                if constexpr (STAGE_X) w[k] = i < nU ? Xg[i] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int i = base + k * B + tid;
+                if (i < nU) {
+                    sU[i] = v[k];
+                    if constexpr (STAGE_X) sX[i] = w[k];
+                    sF[i] = 0.0;
+                }
+            }
         }
-        for (int i = tid; i < nh * DIM; i += B) {
-            int sl = i / DIM, c = i - sl * DIM;
-            size_t n = size_t(a.halo_node[h0 + sl]) * DIM + c;
-            sU[nown * DIM + i] = a.Ucur[n];
-            sX[nown * DIM + i] = a.X[n];
+        for (int base = 0; base < nH; base += NB * B) {
+            size_t n[NB];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int i = base + k * B + tid;
+                const int sl = i / DIM;
+                n[k] = i < nH ? size_t(a.halo_node[h0 + sl]) * DIM + (i - sl * DIM) : 0;
+            }
+            double v[NB], w[NB];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int i = base + k * B + tid;
+                v[k] = i < nH ? a.Ucur[n[k]] : 0.0;
+                if constexpr (STAGE_X) w[k] = i < nH ? a.X[n[k]] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int i = base + k * B + tid;
+                if (i < nH) {
+                    sU[nU + i] = v[k];
+                    if constexpr (STAGE_X) sX[nU + i] = w[k];
+                }
+            }
         }
     }
     __syncthreads();
-    const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
-    const int32_t *cp = a.color_ptr + size_t(tile) * (a.ncolors + 1);
+    const int32_t *cp = cps;
     for (int base = 0; base < ne; base += B) {
         const int i = base + tid;
         const bool valid = i < ne;
@@ -147,28 +186,56 @@ __global__ __launch_bounds__(256) void k_cd_fused(FusedArgs a) {
         int lc[NEN];
         int mycol = -1;
         if (valid) {
-            double x[NEN][DIM], u[NEN][DIM];
-            const uint16_t *l = a.lconn + size_t(e0 + i) * NEN;
+            // Corner slots in one 8- or 16-byte load.
+            if constexpr (NEN == 8) {
+                const uint4 pk = *reinterpret_cast<const uint4 *>(a.lconn + size_t(e0 + i) * 8);
+                const uint32_t w4[4] = {pk.x, pk.y, pk.z, pk.w};
 #pragma unroll
-            for (int c = 0; c < NEN; ++c) {
-                lc[c] = l[c];
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) {
-                    x[c][d] = sX[lc[c] * DIM + d];
-                    u[c][d] = sU[lc[c] * DIM + d];
+                for (int c = 0; c < 4; ++c) {
+                    lc[2 * c] = int(w4[c] & 0xffffu);
+                    lc[2 * c + 1] = int(w4[c] >> 16);
                 }
+            } else {
+                const uint2 pk = *reinterpret_cast<const uint2 *>(a.lconn + size_t(e0 + i) * 4);
+                lc[0] = int(pk.x & 0xffffu);
+                lc[1] = int(pk.x >> 16);
+                lc[2] = int(pk.y & 0xffffu);
+                lc[3] = int(pk.y >> 16);
             }
             const double lam = a.elam[e0 + i], mu = a.emu[e0 + i];
-            if constexpr (DIM == 3) {
-                if constexpr (AFFINE)
-                    hexa_affine_fint(x, u, lam, mu, f);
+            double u[NEN][DIM];
+#pragma unroll
+            for (int c = 0; c < NEN; ++c)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) u[c][d] = sU[lc[c] * DIM + d];
+            if constexpr (PATH == 2) {
+                const double *g = a.geo + size_t(a.egeo[e0 + i]) * (DIM * DIM);
+                double Lam[DIM][DIM];
+#pragma unroll
+                for (int r2 = 0; r2 < DIM; ++r2)
+#pragma unroll
+                    for (int c2 = 0; c2 < DIM; ++c2) Lam[r2][c2] = g[r2 * DIM + c2];
+                if constexpr (DIM == 3)
+                    hexa_modal_force(Lam, lam, mu, u, f);
                 else
-                    hexa_fint(x, u, lam, mu, f);
+                    quad_modal_force(Lam, lam, mu, u, f);
             } else {
-                if constexpr (AFFINE)
-                    quad_affine_fint(x, u, lam, mu, a.th, f);
-                else
-                    quad_fint(x, u, lam, mu, a.th, f);
+                double x[NEN][DIM];
+#pragma unroll
+                for (int c = 0; c < NEN; ++c)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) x[c][d] = sX[lc[c] * DIM + d];
+                if constexpr (DIM == 3) {
+                    if constexpr (PATH == 1)
+                        hexa_affine_fint(x, u, lam, mu, f);
+                    else
+                        hexa_fint(x, u, lam, mu, f);
+                } else {
+                    if constexpr (PATH == 1)
+                        quad_affine_fint(x, u, lam, mu, a.th, f);
+                    else
+                        quad_fint(x, u, lam, mu, a.th, f);
+                }
             }
             mycol = color_of(cp, i);
         }
@@ -189,43 +256,241 @@ __global__ __launch_bounds__(256) void k_cd_fused(FusedArgs a) {
             }
         }
     }
-    __syncthreads();
+    // Owned DOFs: load U_{k-2} and keff with NB loads in flight, then update.
     const int npl = a.nplain[tile];
-    for (int i = tid; i < nown * DIM; i += B) {
-        const int sl = i / DIM;
-        const size_t g = size_t(o0) * DIM + i;
-        const double u = sU[i], fi = sF[i];
-        const double kd = a.keff[g];
-        if (sl < npl) {
-            a.Unext[g] = u + ((0.0 - fi) + kd * (u - a.Uprev[g])) / kd;
-        } else if (kd == 0.0) {
-            a.Unext[g] = 0.0;
-        } else {
-            const int c = i - sl * DIM, node = o0 + sl;
-            double fe = 0.0;
-            for (int q2 = a.nl_ptr[node]; q2 < a.nl_ptr[node + 1]; ++q2) {
-                const int l = a.nl_idx[q2];
-                const double amp = (a.k >= 0 && a.k < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + a.k] : 0.0;
-                fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + c]);
+    const size_t g0 = size_t(o0) * DIM;
+    for (int base = 0; base < nU; base += NB * B) {
+        double up[NB], kd[NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const int i = base + k * B + tid;
+            up[k] = i < nU ? a.Uprev[g0 + i] : 0.0;
+            kd[k] = i < nU ? a.keff[g0 + i] : 0.0;
+        }
+        if (base == 0) __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const int i = base + k * B + tid;
+            if (i >= nU) continue;
+            const int sl = i / DIM;
+            const double u = sU[i], fi = sF[i];
+            if (sl < npl) {
+                a.Unext[g0 + i] = u + ((0.0 - fi) + kd[k] * (u - up[k])) / kd[k];
+            } else if (kd[k] == 0.0) {
+                a.Unext[g0 + i] = 0.0;
+            } else {
+                const int c = i - sl * DIM, node = o0 + sl;
+                double fe = 0.0;
+                for (int q2 = a.nl_ptr[node]; q2 < a.nl_ptr[node + 1]; ++q2) {
+                    const int l = a.nl_idx[q2];
+                    const double amp =
+                        (a.k >= 0 && a.k < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + a.k] : 0.0;
+                    fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + c]);
+                }
+                a.Unext[g0 + i] = u + ((fe - fi) + a.kforce[g0 + i] * (u - up[k])) / kd[k];
             }
-            a.Unext[g] = u + ((fe - fi) + a.kforce[g] * (u - a.Uprev[g])) / kd;
         }
     }
 }
 
-template <int DIM, bool AFFINE>
-static const void *fused_fn() {
-    return reinterpret_cast<const void *>(&k_cd_fused<DIM, AFFINE>);
+// Corner-slot variant for meshes in which every (node, corner position) pair belongs
+// to at most one element (conforming structured meshes): one thread per tile element
+// and one thread per owned node, all tile elements in a single round. Each element
+// writes its corner forces to the unique LDS slot (corner a, component c, node); each
+// owned node then adds its NEN slots in corner order. Two barriers per tile, no color
+// phases, and every global load that does not depend on staged data is issued before
+// the first barrier.
+template <int DIM, int PATH>
+__global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
+    constexpr int NEN = Elem<DIM>::nen;
+    constexpr int NB = 4;
+    extern __shared__ double smem[];
+    const int nt = a.ntiles;
+    const int b = blockIdx.x, xcd = b & 7, j = b >> 3;
+    const int q = nt >> 3, r = nt & 7;
+    const int tile = xcd * q + min(xcd, r) + j;
+    const int o0 = a.own_ptr[tile], nown = a.own_ptr[tile + 1] - o0;
+    const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
+    const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
+    constexpr bool STAGE_X = PATH != 2;
+    const int MO = a.max_own;
+    double *sU = smem;
+    double *sX = smem + size_t(a.max_stage) * DIM;
+    double *sIn = smem + size_t((STAGE_X ? 2 : 1) * a.max_stage) * DIM;  // [NEN*DIM][max_own]
+    const int tid = threadIdx.x, B = blockDim.x;
+    const int nU = nown * DIM, nH = nh * DIM;
+    const size_t g0 = size_t(o0) * DIM;
+    // Prefetch: update operands of this thread's node, element data of this thread's element.
+    const bool has_node = tid < nown;
+    double up[DIM], kd[DIM];
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        up[c] = has_node ? a.Uprev[g0 + size_t(tid) * DIM + c] : 0.0;
+        kd[c] = has_node ? a.keff[g0 + size_t(tid) * DIM + c] : 0.0;
+    }
+    const bool has_elem = tid < ne;
+    int lc[NEN];
+    double lam = 0.0, mu = 0.0;
+    int gid = 0;
+    if (has_elem) {
+        if constexpr (NEN == 8) {
+            const uint4 pk = *reinterpret_cast<const uint4 *>(a.lconn + size_t(e0 + tid) * 8);
+            const uint32_t w4[4] = {pk.x, pk.y, pk.z, pk.w};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                lc[2 * c] = int(w4[c] & 0xffffu);
+                lc[2 * c + 1] = int(w4[c] >> 16);
+            }
+        } else {
+            const uint2 pk = *reinterpret_cast<const uint2 *>(a.lconn + size_t(e0 + tid) * 4);
+            lc[0] = int(pk.x & 0xffffu);
+            lc[1] = int(pk.x >> 16);
+            lc[2] = int(pk.y & 0xffffu);
+            lc[3] = int(pk.y >> 16);
+        }
+        lam = a.elam[e0 + tid];
+        mu = a.emu[e0 + tid];
+        if constexpr (PATH == 2) gid = a.egeo[e0 + tid];
+    } else {
+#pragma unroll
+        for (int c = 0; c < NEN; ++c) lc[c] = 0;
+    }
+    // Stage U (and X), zero the corner slots.
+    {
+        const double *Ug = a.Ucur + g0;
+        const double *Xg = a.X + g0;
+        for (int base = 0; base < nU; base += NB * B) {
+            double v[NB], w[NB];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int i = base + k * B + tid;
+                v[k] = i < nU ? Ug[i] : 0.0;
+                if constexpr (STAGE_X) w[k] = i < nU ? Xg[i] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int i = base + k * B + tid;
+                if (i < nU) {
+                    sU[i] = v[k];
+                    if constexpr (STAGE_X) sX[i] = w[k];
+                }
+            }
+        }
+        for (int base = 0; base < nH; base += NB * B) {
+            size_t n[NB];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int i = base + k * B + tid;
+                const int sl = i / DIM;
+                n[k] = i < nH ? size_t(a.halo_node[h0 + sl]) * DIM + (i - sl * DIM) : 0;
+            }
+            double v[NB], w[NB];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int i = base + k * B + tid;
+                v[k] = i < nH ? a.Ucur[n[k]] : 0.0;
+                if constexpr (STAGE_X) w[k] = i < nH ? a.X[n[k]] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const int i = base + k * B + tid;
+                if (i < nH) {
+                    sU[nU + i] = v[k];
+                    if constexpr (STAGE_X) sX[nU + i] = w[k];
+                }
+            }
+        }
+        for (int i = tid; i < NEN * DIM * MO; i += B) sIn[i] = 0.0;
+    }
+    __syncthreads();
+    if (has_elem) {
+        double u[NEN][DIM], f[NEN][DIM];
+#pragma unroll
+        for (int c = 0; c < NEN; ++c)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) u[c][d] = sU[lc[c] * DIM + d];
+        if constexpr (PATH == 2) {
+            const double *g = a.geo + size_t(gid) * (DIM * DIM);
+            double Lam[DIM][DIM];
+#pragma unroll
+            for (int r2 = 0; r2 < DIM; ++r2)
+#pragma unroll
+                for (int c2 = 0; c2 < DIM; ++c2) Lam[r2][c2] = g[r2 * DIM + c2];
+            if constexpr (DIM == 3)
+                hexa_modal_force(Lam, lam, mu, u, f);
+            else
+                quad_modal_force(Lam, lam, mu, u, f);
+        } else {
+            double x[NEN][DIM];
+#pragma unroll
+            for (int c = 0; c < NEN; ++c)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) x[c][d] = sX[lc[c] * DIM + d];
+            if constexpr (DIM == 3) {
+                if constexpr (PATH == 1)
+                    hexa_affine_fint(x, u, lam, mu, f);
+                else
+                    hexa_fint(x, u, lam, mu, f);
+            } else {
+                if constexpr (PATH == 1)
+                    quad_affine_fint(x, u, lam, mu, a.th, f);
+                else
+                    quad_fint(x, u, lam, mu, a.th, f);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NEN; ++c)
+            if (lc[c] < nown) {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    const double v = f[c][d];
+                    sIn[(c * DIM + d) * MO + lc[c]] = fabs(v) > a.ftol ? v : 0.0;
+                }
+            }
+    }
+    __syncthreads();
+    if (!has_node) return;
+    const int npl = a.nplain[tile];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        double fi = 0.0;
+#pragma unroll
+        for (int c = 0; c < NEN; ++c) fi += sIn[(c * DIM + d) * MO + tid];
+        const double u = sU[tid * DIM + d];
+        const size_t g = g0 + size_t(tid) * DIM + d;
+        if (tid < npl) {
+            a.Unext[g] = u + ((0.0 - fi) + kd[d] * (u - up[d])) / kd[d];
+        } else if (kd[d] == 0.0) {
+            a.Unext[g] = 0.0;
+        } else {
+            const int node = o0 + tid;
+            double fe = 0.0;
+            for (int q2 = a.nl_ptr[node]; q2 < a.nl_ptr[node + 1]; ++q2) {
+                const int l = a.nl_idx[q2];
+                const double amp = (a.k >= 0 && a.k < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + a.k] : 0.0;
+                fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + d]);
+            }
+            a.Unext[g] = u + ((fe - fi) + a.kforce[g] * (u - up[d])) / kd[d];
+        }
+    }
+}
+
+template <int DIM, int PATH>
+static const void *fused_fn(bool corner) {
+    return corner ? reinterpret_cast<const void *>(&k_cd_corner<DIM, PATH>)
+                  : reinterpret_cast<const void *>(&k_cd_fused<DIM, PATH>);
 }
 
 static const void *fused_kernel(const Model &m) {
-    if (m.ndim == 3) return m.affine ? fused_fn<3, true>() : fused_fn<3, false>();
-    return m.affine ? fused_fn<2, true>() : fused_fn<2, false>();
+    const bool cr = m.variant == 3;
+    if (m.ndim == 3)
+        return m.elem_path == 2 ? fused_fn<3, 2>(cr) : m.elem_path == 1 ? fused_fn<3, 1>(cr) : fused_fn<3, 0>(cr);
+    return m.elem_path == 2 ? fused_fn<2, 2>(cr) : m.elem_path == 1 ? fused_fn<2, 1>(cr) : fused_fn<2, 0>(cr);
 }
 
 int prepare_fused(Model &m) {
-    if (m.variant != 1) return 0;
-    SVLG_CHECK(m.tiles.block <= 256, "fused kernel block must be at most 256 threads");
+    if (m.variant != 1 && m.variant != 3) return 0;
+    SVLG_CHECK(m.tiles.block <= 512, "fused kernel block must be at most 512 threads");
     SVLG_CHECK(m.tiles.lds_bytes <= 160 * 1024, "node tile needs more than 160 KB of LDS; use smaller tiles");
     if (m.tiles.lds_bytes > 64 * 1024)
         SVLG_HIP(hipFuncSetAttribute(fused_kernel(m), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -263,18 +528,27 @@ static int launch_fused(Model &m, int k, const double *Ucur, const double *Uprev
     a.ntiles = t.ntiles;
     a.ncolors = t.ncolors;
     a.max_stage = t.max_stage;
+    a.max_own = t.max_own;
+    a.egeo = t.elem_geo.p;
+    a.geo = m.geo.p;
     dim3 grid(t.ntiles), block(t.block);
+#define SVLG_LAUNCH_FUSED(D, P)                                                                        \
+    do {                                                                                              \
+        if (m.variant == 3)                                                                           \
+            hipLaunchKernelGGL((k_cd_corner<D, P>), grid, block, t.lds_bytes, m.stream, a);          \
+        else                                                                                          \
+            hipLaunchKernelGGL((k_cd_fused<D, P>), grid, block, t.lds_bytes, m.stream, a);           \
+    } while (0)
     if (m.ndim == 3) {
-        if (m.affine)
-            hipLaunchKernelGGL((k_cd_fused<3, true>), grid, block, t.lds_bytes, m.stream, a);
-        else
-            hipLaunchKernelGGL((k_cd_fused<3, false>), grid, block, t.lds_bytes, m.stream, a);
+        if (m.elem_path == 2) SVLG_LAUNCH_FUSED(3, 2);
+        else if (m.elem_path == 1) SVLG_LAUNCH_FUSED(3, 1);
+        else SVLG_LAUNCH_FUSED(3, 0);
     } else {
-        if (m.affine)
-            hipLaunchKernelGGL((k_cd_fused<2, true>), grid, block, t.lds_bytes, m.stream, a);
-        else
-            hipLaunchKernelGGL((k_cd_fused<2, false>), grid, block, t.lds_bytes, m.stream, a);
+        if (m.elem_path == 2) SVLG_LAUNCH_FUSED(2, 2);
+        else if (m.elem_path == 1) SVLG_LAUNCH_FUSED(2, 1);
+        else SVLG_LAUNCH_FUSED(2, 0);
     }
+#undef SVLG_LAUNCH_FUSED
     return 0;
 }
 
@@ -282,7 +556,7 @@ static int cd_one_step(Model &m, int k) {
     double *Ucur = m.U[m.ring].p;
     double *Uprev = m.U[(m.ring + 2) % 3].p;
     double *Unext = m.U[(m.ring + 1) % 3].p;
-    if (m.variant == 1) {
+    if (m.variant == 1 || m.variant == 3) {
         launch_fused(m, k, Ucur, Uprev, Unext);
         m.ring = (m.ring + 1) % 3;
         m.steps_done++;

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 
@@ -274,11 +275,11 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
     m.ndofn = nn * ndim;
     m.th = d->thickness;
     m.lumped = d->lumped != 0;
-    int tile_nodes = (o && o->tile_nodes > 0) ? o->tile_nodes : (ndim == 3 ? 448 : 1024);
-    int block = (o && o->block > 0) ? o->block : 256;
-    m.variant = (o && o->variant > 0) ? o->variant : 1;
-    if (m.variant != 1 && m.variant != 2) { set_error("variant must be 1 or 2"); return fail(11); }
-    if (block % 64 != 0 || block > 1024) { set_error("block must be a multiple of 64, at most 1024"); return fail(11); }
+    int tile_nodes = (o && o->tile_nodes > 0) ? o->tile_nodes : (ndim == 3 ? 320 : 448);
+    int block = (o && o->block > 0) ? o->block : 512;
+    const int req_variant = (o && o->variant > 0) ? o->variant : 0;
+    if (req_variant < 0 || req_variant > 3) { set_error("variant must be 0 (auto), 1, 2 or 3"); return fail(11); }
+    if (block % 64 != 0 || block > 512) { set_error("block must be a multiple of 64, at most 512"); return fail(11); }
     if (hipStreamCreateWithFlags(&m.own_stream, hipStreamNonBlocking) != hipSuccess) {
         set_error("hipStreamCreate failed");
         return fail(10);
@@ -309,6 +310,17 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         if (am[e] != 0.0)
             for (int a = 0; a < nen; ++a) special[d->conn[size_t(e) * nen + a]] = 1;
 
+    // Corner-slot scatter needs every (node, corner position) pair to occur at most once.
+    bool corner_unique = true;
+    {
+        std::vector<uint8_t> used(nn, 0);
+        for (int e = 0; e < ne && corner_unique; ++e)
+            for (int a = 0; a < nen; ++a) {
+                uint8_t &u = used[d->conn[size_t(e) * nen + a]];
+                if (u & (1u << a)) { corner_unique = false; break; }
+                u |= uint8_t(1u << a);
+            }
+    }
     HostTiles ht;
     try {
         make_tiles(ndim, nn, ne, nen, d->coords, d->conn, m.h_color.data(), m.ncolors, special.data(), tile_nodes, ht);
@@ -372,6 +384,55 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
     UP(m.fdof, m.h_free.data(), m.h_free.size());
     UP(m.n2e_ptr, n2e_ptr.data(), n2e_ptr.size());
     UP(m.n2e, n2e.data(), n2e.size());
+    // Element path of the fused kernel and, for parallelepipeds, the table of distinct
+    // (J^-1, |J|) geometries (one entry for a uniform structured grid).
+    int geo_opt = o ? o->geo_table : 0;
+    m.elem_path = m.affine ? 1 : 0;
+    std::vector<double> escale(ne, 1.0), gtab;
+    std::vector<uint16_t> gid(ne, 0);
+    if (m.affine && geo_opt != -1) {
+        std::map<std::vector<uint64_t>, int> classes;
+        bool ok = true;
+        for (int e = 0; e < ne && ok; ++e) {
+            std::vector<double> lam_e(ndim * ndim);
+            double adet;
+            if (ndim == 3) {
+                double x[8][3], L[3][3];
+                for (int a = 0; a < 8; ++a)
+                    for (int c = 0; c < 3; ++c) x[a][c] = d->coords[size_t(d->conn[size_t(e) * 8 + a]) * 3 + c];
+                adet = hexa_center_geometry(x, L);
+                for (int i = 0; i < 9; ++i) lam_e[i] = L[i / 3][i % 3];
+                escale[e] = 0.125 * adet;
+            } else {
+                double x[4][2], L[2][2];
+                for (int a = 0; a < 4; ++a)
+                    for (int c = 0; c < 2; ++c) x[a][c] = d->coords[size_t(d->conn[size_t(e) * 4 + a]) * 2 + c];
+                adet = quad_center_geometry(x, L);
+                for (int i = 0; i < 4; ++i) lam_e[i] = L[i / 2][i % 2];
+                escale[e] = 0.25 * adet * m.th;
+            }
+            std::vector<uint64_t> key(lam_e.size() + 1);
+            std::memcpy(key.data(), lam_e.data(), sizeof(double) * lam_e.size());
+            std::memcpy(&key[lam_e.size()], &adet, sizeof(double));
+            auto it = classes.find(key);
+            if (it == classes.end()) {
+                if (classes.size() >= 65535) { ok = false; break; }
+                int id = int(classes.size());
+                classes.emplace(key, id);
+                gtab.insert(gtab.end(), lam_e.begin(), lam_e.end());
+                gid[e] = uint16_t(id);
+            } else {
+                gid[e] = uint16_t(it->second);
+            }
+        }
+        if (ok) {
+            m.elem_path = 2;
+            m.ngeo = int(classes.size());
+        }
+    }
+    if (m.elem_path == 2) {
+        UP(m.geo, gtab.data(), gtab.size());
+    }
     // Tiles.
     {
         TileSet &t = m.tiles;
@@ -381,12 +442,40 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         t.max_own = ht.max_own;
         t.max_elem = ht.max_elem;
         t.block = block;
-        t.lds_bytes = size_t(2 * ht.max_stage + ht.max_own) * ndim * sizeof(double);
-        std::vector<double> el(ht.elem_id.size()), em(ht.elem_id.size());
-        for (size_t i = 0; i < ht.elem_id.size(); ++i) {
-            el[i] = lam[ht.elem_id[i]];
-            em[i] = mu[ht.elem_id[i]];
+        const int staged_arrays = m.elem_path == 2 ? 1 : 2;  // U (and X unless the table is used)
+        // Fused-kernel flavour: corner slots when the mesh allows it and one round of
+        // <= 512 threads covers every tile element and owned node; colour phases otherwise.
+        const int need = std::max(ht.max_elem, ht.max_own);
+        const int cblock = std::max(64, (need + 63) / 64 * 64);
+        const size_t corner_lds =
+            (size_t(staged_arrays * ht.max_stage) * ndim + size_t(nen) * ndim * ht.max_own) * sizeof(double);
+        const bool corner_ok = corner_unique && cblock <= 512 && corner_lds <= 160 * 1024;
+        if (req_variant == 0) m.variant = corner_ok ? 3 : 1;
+        else m.variant = req_variant;
+        if (m.variant == 3 && !corner_ok) {
+            set_error("corner-slot kernel needs a corner-unique mesh and tiles of at most 512 elements");
+            return fail(11);
         }
+        if (m.variant == 3) {
+            t.block = cblock;
+            t.lds_bytes = corner_lds;
+        } else {
+            t.lds_bytes = size_t(staged_arrays * ht.max_stage + ht.max_own) * ndim * sizeof(double);
+        }
+        std::vector<double> el(ht.elem_id.size()), em(ht.elem_id.size());
+        std::vector<uint16_t> eg(ht.elem_id.size(), 0);
+        for (size_t i = 0; i < ht.elem_id.size(); ++i) {
+            int e = ht.elem_id[i];
+            if (m.elem_path == 2) {
+                el[i] = lam[e] * escale[e];
+                em[i] = mu[e] * escale[e];
+                eg[i] = gid[e];
+            } else {
+                el[i] = lam[e];
+                em[i] = mu[e];
+            }
+        }
+        UP(t.elem_geo, eg.data(), eg.size());
         UP(t.own_ptr, ht.own_ptr.data(), ht.own_ptr.size());
         UP(t.nplain, ht.nplain.data(), ht.nplain.size());
         UP(t.halo_ptr, ht.halo_ptr.data(), ht.halo_ptr.size());
@@ -484,7 +573,7 @@ int svlg_plan_tiles(const svlg_mesh_desc *d, int32_t tile_nodes, int64_t *stats)
     HostTiles t;
     try {
         make_tiles(ndim, nn, ne, nen, d->coords, d->conn, color.data(), ncolors, special.data(),
-                   tile_nodes > 0 ? tile_nodes : (ndim == 3 ? 448 : 1024), t);
+                   tile_nodes > 0 ? tile_nodes : (ndim == 3 ? 320 : 448), t);
     } catch (const std::exception &ex) {
         set_error(std::string("tiling failed: ") + ex.what());
         return 11;
@@ -581,7 +670,7 @@ int svlg_model_info(svlg_model *h, int64_t *info) {
     info[3] = m.nfree;
     info[4] = m.variant;
     info[5] = m.tiles.ntiles;
-    info[6] = m.affine ? 1 : 0;
+    info[6] = m.elem_path;
     info[7] = int64_t(m.tiles.lds_bytes);
     return 0;
 }

@@ -22,11 +22,11 @@ from .mesh import Model
 class DeviceModel:
     """A model resident in HBM (Mesh::Initialize + Element::SetDomain)."""
 
-    def __init__(self, model: Model, tile_nodes: int = 0, block: int = 0, variant: int = 0):
+    def __init__(self, model: Model, tile_nodes: int = 0, block: int = 0, variant: int = 0, geo_table: int = 0):
         self.model = model
         self._desc, self._keep = capi.make_desc(model)
         h = C.c_void_p()
-        opts = capi.ExecOpts(tile_nodes, block, variant)
+        opts = capi.ExecOpts(tile_nodes, block, variant, geo_table)
         capi.check(capi.lib().svlg_model_create_ex(C.byref(self._desc), C.byref(opts), C.byref(h)),
                    "svlg_model_create")
         self.h = h
@@ -46,8 +46,10 @@ class DeviceModel:
     def info(self) -> dict:
         a = (C.c_int64 * 8)()
         capi.check(capi.lib().svlg_model_info(self.h, a), "svlg_model_info")
-        keys = ["ncolors", "nelem", "nnodes", "nfree", "variant", "ntiles", "affine", "lds_bytes"]
-        return dict(zip(keys, [int(v) for v in a]))
+        keys = ["ncolors", "nelem", "nnodes", "nfree", "variant", "ntiles", "elem_path", "lds_bytes"]
+        d = dict(zip(keys, [int(v) for v in a]))
+        d["affine"] = int(d["elem_path"] > 0)
+        return d
 
     def element_colors(self) -> np.ndarray:
         out = np.zeros(self.model.nelem, np.int32)

@@ -72,20 +72,36 @@ def rotation():
     return Rz @ Rx @ S
 
 
-@pytest.mark.parametrize("kind", ["axis", "rotated", "distorted"])
-@pytest.mark.parametrize("tile_nodes", [24, 120, 5000])
-def test_fused_hexa_parity(kind, tile_nodes):
+def graded(m):
+    """Same block with a non-uniform spacing along every axis (still parallelepipeds,
+    one geometry per element column/row/layer combination)."""
+    c = m.coords.copy()
+    for d in range(3):
+        vals = np.unique(c[:, d])
+        new = np.cumsum(np.concatenate([[0.0], 0.7 + 0.6 * np.random.default_rng(d).random(vals.size - 1)]))
+        c[:, d] = new[np.searchsorted(vals, c[:, d])]
+    m.coords = np.ascontiguousarray(c)
+    return m
+
+
+@pytest.mark.parametrize("kind", ["axis", "rotated", "graded", "distorted"])
+@pytest.mark.parametrize("tile_nodes,geo", [(24, 0), (120, 0), (120, -1), (300, 0)])
+def test_fused_hexa_parity(kind, tile_nodes, geo):
     if kind == "axis":
         m, dt, nt, src = block_model(7, 6, 5)
     elif kind == "rotated":
         m, dt, nt, src = block_model(7, 6, 5, transform=rotation())
+    elif kind == "graded":
+        m, dt, nt, src = block_model(7, 6, 5)
+        m = graded(m)
     else:
         m, dt, nt, src = block_model(7, 6, 5, distort=0.18, seed=4)
     rec = [d for d in range(0, m.nfree, 11)]
     ref = oracle.run(m, oracle.CENTRAL_DIFFERENCE, dt, nt, rec)
-    hist, U, info = run_gpu(m, dt, nt - 1, rec, tile_nodes=tile_nodes)
-    assert info["variant"] == 1
-    assert info["affine"] == (0 if kind == "distorted" else 1)
+    hist, U, info = run_gpu(m, dt, nt - 1, rec, tile_nodes=tile_nodes, geo_table=geo)
+    assert info["variant"] == 3
+    expect = 0 if kind == "distorted" else (1 if geo == -1 else 2)
+    assert info["elem_path"] == expect
     assert rel_l2(hist, ref["U"]) < REL_TOL
     assert rel_l2(U, ref["final_U"]) < REL_TOL
 
@@ -95,9 +111,10 @@ def test_fused_quad_layer_parity(tile_nodes):
     m, dt, nt = W.layer2d(24, 12, nt=1201)
     rec = W.dofs_of(m, int(m.load_node[0])) + list(range(3, m.nfree, 17))
     ref = oracle.run(m, oracle.CENTRAL_DIFFERENCE, dt, nt, rec)
-    hist, U, info = run_gpu(m, dt, nt - 1, rec, tile_nodes=tile_nodes)
-    assert info["variant"] == 1 and info["affine"] == 1
-    assert rel_l2(hist, ref["U"]) < REL_TOL
+    for geo in (0, -1):
+        hist, U, info = run_gpu(m, dt, nt - 1, rec, tile_nodes=tile_nodes, geo_table=geo)
+        assert info["variant"] in (1, 3) and info["elem_path"] == (2 if geo == 0 else 1)
+        assert rel_l2(hist, ref["U"]) < REL_TOL
 
 
 def test_fused_quad_distorted():
@@ -118,14 +135,40 @@ def test_fused_quad_distorted():
     assert rel_l2(hist, ref["U"]) < REL_TOL
 
 
-def test_fused_matches_colored_kernels():
+@pytest.mark.parametrize("variant", [1, 3])
+def test_fused_matches_colored_kernels(variant):
+    """Both fused flavours (colour phases, corner slots) against the element-colored kernels."""
     m, dt, _ = W.halfspace3d(14, 12, 8, h=2.0, nt=301)
     rec = list(range(0, m.nfree, 13))
-    h1, U1, i1 = run_gpu(m, dt, 300, rec, tile_nodes=128)
+    h1, U1, i1 = run_gpu(m, dt, 300, rec, tile_nodes=128, variant=variant)
     h2, U2, i2 = run_gpu(m, dt, 300, rec, variant=2)
-    assert i1["variant"] == 1 and i2["variant"] == 2
+    assert i1["variant"] == variant and i2["variant"] == 2
     assert rel_l2(U1, U2) < 1e-12
     assert rel_l2(h1, h2) < 1e-12
+
+
+def test_colour_phase_kernel_on_non_corner_unique_mesh():
+    """Two blocks glued with mirrored node order break corner uniqueness: auto picks the
+    colour-phase kernel, which still matches the oracle."""
+    coords, conn, bnd = make_domain_volume(6, 5, 4, (0, 0, 0), (6, 0, 0), (0, 5, 0), (0, 0, 4))
+    conn = conn.copy()
+    # Re-orient every other element by a rotation of its corners about z (still a valid
+    # right-handed hexahedron), so a node can be corner 0 of two elements.
+    rot = np.array([1, 2, 3, 0, 5, 6, 7, 4])
+    conn[::2] = conn[::2][:, rot]
+    restrained = np.zeros(coords.shape, bool)
+    restrained[bnd["bottom"]] = True
+    m = Model(ndim=3, coords=coords, node_tags=tags_for(coords.shape[0]), elem_type=ELEM_HEXA8, conn=conn,
+              E=1e8, nu=0.3, rho=2000.0, restrained=restrained)
+    nt, dt = 300, 2e-4
+    m.add_point_load(int(bnd["top"][5]), [0.2, 0.1, -1.0], ricker_velocity(np.arange(nt) * dt, 0.02, 25.0, 1e6))
+    rec = list(range(0, m.nfree, 7))
+    ref = oracle.run(m, oracle.CENTRAL_DIFFERENCE, dt, nt, rec)
+    hist, U, info = run_gpu(m, dt, nt - 1, rec, tile_nodes=40)
+    assert info["variant"] == 1
+    assert rel_l2(hist, ref["U"]) < REL_TOL
+    with pytest.raises(Exception):
+        run_gpu(m, dt, 2, rec, tile_nodes=40, variant=3)
 
 
 def test_fused_deterministic():
