@@ -111,7 +111,7 @@ int svlg_plan_tiles(const svlg_mesh_desc *desc, int32_t tile_nodes, int64_t *sta
 /* info[0]=ncolors info[1]=nelem info[2]=nnodes info[3]=nfree info[4]=explicit kernel
  * variant (1 fused + colour phases, 2 element-colored, 3 fused + corner slots) info[5]=ntiles info[6]=element path
  * of the fused kernel (0 Gauss points, 1 modal with staged coordinates, 2 modal with
- * the geometry table) info[7]=dynamic LDS bytes of the fused kernel */
+ * the geometry table, 3 modal with a table of diagonal J^-1: axis-aligned elements) info[7]=dynamic LDS bytes of the fused kernel */
 int svlg_model_info(svlg_model *m, int64_t *info8);
 /* Element color of every element (ascending-tag greedy coloring; no two elements of
  * one color share a node). */

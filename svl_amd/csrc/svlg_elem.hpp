@@ -332,6 +332,52 @@ __device__ __forceinline__ void apply_L(const double (&G)[DIM][DIM], const doubl
         }
 }
 
+// apply_L for a diagonal Lam (rectangular / brick elements aligned with the axes):
+// H_ic = Lam_ii G_ic, P_kc = Lam_kk sigma_kc, with only the nonzero rows of G and the
+// needed rows of P formed.
+template <int DIM, bool G0, bool G1, bool G2, bool P0, bool P1, bool P2>
+__device__ __forceinline__ void apply_L_diag(const double (&G)[DIM][DIM], const double (&Lam)[DIM][DIM], double lam,
+                                             double mu, double (&P)[DIM][DIM]) {
+    constexpr bool gr[3] = {G0, G1, G2};
+    constexpr bool pr[3] = {P0, P1, P2};
+    double H[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+        if (gr[i]) {
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) H[i][c] = Lam[i][i] * G[i][c];
+        }
+    double tr = 0.0;
+    bool first = true;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+        if (gr[i]) {
+            tr = first ? H[i][i] : tr + H[i][i];
+            first = false;
+        }
+    const double lt = lam * tr, m2 = 2.0 * mu;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k)
+        if (pr[k]) {
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) {
+                double sv;
+                if (k == c) {
+                    sv = gr[k] ? fma(m2, H[k][k], lt) : lt;
+                } else if (gr[k] && gr[c]) {
+                    sv = mu * (H[k][c] + H[c][k]);
+                } else if (gr[k]) {
+                    sv = mu * H[k][c];
+                } else if (gr[c]) {
+                    sv = mu * H[c][k];
+                } else {
+                    sv = 0.0;
+                }
+                P[k][c] = Lam[k][k] * sv;
+            }
+        }
+}
+
 // 3x3 inverse (cofactors) and |det|.
 __device__ __forceinline__ double inv3(const double (&J)[3][3], double (&I)[3][3]) {
     double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
@@ -351,10 +397,23 @@ __device__ __forceinline__ double inv3(const double (&J)[3][3], double (&I)[3][3
     return fabs(det);
 }
 
+#define SVLG_APPLY_L(...) apply_L_sel<DIAG, __VA_ARGS__>
+
+template <bool DIAG, int DIM, bool G0, bool G1, bool G2, bool P0, bool P1, bool P2>
+__device__ __forceinline__ void apply_L_sel(const double (&G)[DIM][DIM], const double (&Lam)[DIM][DIM], double lam,
+                                            double mu, double (&P)[DIM][DIM]) {
+    if constexpr (DIAG)
+        apply_L_diag<DIM, G0, G1, G2, P0, P1, P2>(G, Lam, lam, mu, P);
+    else
+        apply_L<DIM, G0, G1, G2, P0, P1, P2>(G, Lam, lam, mu, P);
+}
+
 // Centre Jacobian of a Hexa8 (J[k][j] = d x_j / d xi_k = m_k(x_j) / 8), its inverse and |det J|.
 __device__ __host__ inline double hexa_center_geometry(const double (&x)[8][3], double (&Lam)[3][3]);
 
 // Modal internal force of an affine Hexa8: Lam = J^-1, l = lambda |J| / 8, m = mu |J| / 8.
+// DIAG: Lam is diagonal (axis-aligned bricks).
+template <bool DIAG = false>
 __device__ __forceinline__ void hexa_modal_force(const double (&Lam)[3][3], double l, double m,
                                                  const double (&u)[8][3], double (&f)[8][3]) {
     double Mx[3], My[3], Mz[3], Mxy[3], Myz[3], Mzx[3], Mxyz[3];
@@ -366,52 +425,61 @@ __device__ __forceinline__ void hexa_modal_force(const double (&Lam)[3][3], doub
         hexa_modes(v, Mx[c], My[c], Mz[c], Mxy[c], Myz[c], Mzx[c], Mxyz[c]);
     }
     const double g2 = kGP * kGP, g4 = g2 * g2;
+    // The Gauss weights of the hourglass terms (g^2, g^4) scale the mode amplitudes
+    // before the linear stress map.
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Mxy[c] *= g2;
+        Myz[c] *= g2;
+        Mzx[c] *= g2;
+        Mxyz[c] *= g4;
+    }
     double G[3][3], P[3][3];
     double Fx[3], Fy[3], Fz[3], Fxy[3], Fyz[3], Fzx[3], Fxyz[3];
     // Linear modes (1-point gradient): all rows.
 #pragma unroll
     for (int c = 0; c < 3; ++c) { G[0][c] = Mx[c]; G[1][c] = My[c]; G[2][c] = Mz[c]; }
-    apply_L<3, true, true, true, true, true, true>(G, Lam, l, m, P);
+    SVLG_APPLY_L(3, true, true, true, true, true, true)(G, Lam, l, m, P);
 #pragma unroll
     for (int c = 0; c < 3; ++c) { Fx[c] = P[0][c]; Fy[c] = P[1][c]; Fz[c] = P[2][c]; }
     // eta monomial: rows xi (mxy) and zeta (myz).
 #pragma unroll
     for (int c = 0; c < 3; ++c) { G[0][c] = Mxy[c]; G[2][c] = Myz[c]; }
-    apply_L<3, true, false, true, true, false, true>(G, Lam, l, m, P);
+    SVLG_APPLY_L(3, true, false, true, true, false, true)(G, Lam, l, m, P);
 #pragma unroll
     for (int c = 0; c < 3; ++c) { Fxy[c] = P[0][c]; Fyz[c] = P[2][c]; }
     // zeta monomial: rows xi (mzx) and eta (myz).
 #pragma unroll
     for (int c = 0; c < 3; ++c) { G[0][c] = Mzx[c]; G[1][c] = Myz[c]; }
-    apply_L<3, true, true, false, true, true, false>(G, Lam, l, m, P);
+    SVLG_APPLY_L(3, true, true, false, true, true, false)(G, Lam, l, m, P);
 #pragma unroll
     for (int c = 0; c < 3; ++c) { Fzx[c] = P[0][c]; Fyz[c] += P[1][c]; }
     // xi monomial: rows eta (mxy) and zeta (mzx).
 #pragma unroll
     for (int c = 0; c < 3; ++c) { G[1][c] = Mxy[c]; G[2][c] = Mzx[c]; }
-    apply_L<3, false, true, true, false, true, true>(G, Lam, l, m, P);
+    SVLG_APPLY_L(3, false, true, true, false, true, true)(G, Lam, l, m, P);
 #pragma unroll
     for (int c = 0; c < 3; ++c) { Fxy[c] += P[1][c]; Fzx[c] += P[2][c]; }
     // Trilinear mode: eta*zeta in row xi, xi*zeta in row eta, xi*eta in row zeta.
 #pragma unroll
     for (int c = 0; c < 3; ++c) G[0][c] = Mxyz[c];
-    apply_L<3, true, false, false, true, false, false>(G, Lam, l, m, P);
+    SVLG_APPLY_L(3, true, false, false, true, false, false)(G, Lam, l, m, P);
 #pragma unroll
     for (int c = 0; c < 3; ++c) Fxyz[c] = P[0][c];
 #pragma unroll
     for (int c = 0; c < 3; ++c) G[1][c] = Mxyz[c];
-    apply_L<3, false, true, false, false, true, false>(G, Lam, l, m, P);
+    SVLG_APPLY_L(3, false, true, false, false, true, false)(G, Lam, l, m, P);
 #pragma unroll
     for (int c = 0; c < 3; ++c) Fxyz[c] += P[1][c];
 #pragma unroll
     for (int c = 0; c < 3; ++c) G[2][c] = Mxyz[c];
-    apply_L<3, false, false, true, false, false, true>(G, Lam, l, m, P);
+    SVLG_APPLY_L(3, false, false, true, false, false, true)(G, Lam, l, m, P);
 #pragma unroll
     for (int c = 0; c < 3; ++c) Fxyz[c] += P[2][c];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         double v[8];
-        hexa_unmodes(Fx[c], Fy[c], Fz[c], g2 * Fxy[c], g2 * Fyz[c], g2 * Fzx[c], g4 * Fxyz[c], v);
+        hexa_unmodes(Fx[c], Fy[c], Fz[c], Fxy[c], Fyz[c], Fzx[c], Fxyz[c], v);
 #pragma unroll
         for (int a = 0; a < 8; ++a) f[a][c] = v[a];
     }
@@ -447,6 +515,7 @@ __device__ __forceinline__ void quad_unmodes(double mx, double my, double mxy, d
 __device__ __host__ inline double quad_center_geometry(const double (&x)[4][2], double (&Lam)[2][2]);
 
 // Modal internal force of an affine Quad4: l = lambda |J| th / 4, m = mu |J| th / 4.
+template <bool DIAG = false>
 __device__ __forceinline__ void quad_modal_force(const double (&Lam)[2][2], double l, double m,
                                                  const double (&u)[4][2], double (&f)[4][2]) {
     double Mx[2], My[2], Mxy[2];
@@ -456,26 +525,28 @@ __device__ __forceinline__ void quad_modal_force(const double (&Lam)[2][2], doub
         quad_modes(v, Mx[c], My[c], Mxy[c]);
     }
     const double g2 = kGP * kGP;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) Mxy[c] *= g2;
     double G[2][2], P[2][2], Fx[2], Fy[2], Fxy[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c) { G[0][c] = Mx[c]; G[1][c] = My[c]; }
-    apply_L<2, true, true, false, true, true, false>(G, Lam, l, m, P);
+    SVLG_APPLY_L(2, true, true, false, true, true, false)(G, Lam, l, m, P);
 #pragma unroll
     for (int c = 0; c < 2; ++c) { Fx[c] = P[0][c]; Fy[c] = P[1][c]; }
 #pragma unroll
     for (int c = 0; c < 2; ++c) G[0][c] = Mxy[c];
-    apply_L<2, true, false, false, true, false, false>(G, Lam, l, m, P);
+    SVLG_APPLY_L(2, true, false, false, true, false, false)(G, Lam, l, m, P);
 #pragma unroll
     for (int c = 0; c < 2; ++c) Fxy[c] = P[0][c];
 #pragma unroll
     for (int c = 0; c < 2; ++c) G[1][c] = Mxy[c];
-    apply_L<2, false, true, false, false, true, false>(G, Lam, l, m, P);
+    SVLG_APPLY_L(2, false, true, false, false, true, false)(G, Lam, l, m, P);
 #pragma unroll
     for (int c = 0; c < 2; ++c) Fxy[c] += P[1][c];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         double v[4];
-        quad_unmodes(Fx[c], Fy[c], g2 * Fxy[c], v);
+        quad_unmodes(Fx[c], Fy[c], Fxy[c], v);
 #pragma unroll
         for (int a = 0; a < 4; ++a) f[a][c] = v[a];
     }

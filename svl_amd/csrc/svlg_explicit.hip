@@ -123,7 +123,7 @@ __global__ __launch_bounds__(512) void k_cd_fused(FusedArgs a) {
     const int o0 = a.own_ptr[tile], nown = a.own_ptr[tile + 1] - o0;
     const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
     const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
-    constexpr bool STAGE_X = PATH != 2;
+    constexpr bool STAGE_X = PATH < 2;
     double *sU = smem;
     double *sX = smem + size_t(a.max_stage) * DIM;
     double *sF = smem + size_t((STAGE_X ? 2 : 1) * a.max_stage) * DIM;
@@ -152,27 +152,33 @@ __global__ __launch_bounds__(512) void k_cd_fused(FusedArgs a) {
                 }
             }
         }
-        for (int base = 0; base < nH; base += NB * B) {
+        // Halo: one index load per node, then its DIM components.
+        for (int base = 0; base < nh; base += NB * B) {
             size_t n[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) {
-                const int i = base + k * B + tid;
-                const int sl = i / DIM;
-                n[k] = i < nH ? size_t(a.halo_node[h0 + sl]) * DIM + (i - sl * DIM) : 0;
+                const int sl = base + k * B + tid;
+                n[k] = sl < nh ? size_t(a.halo_node[h0 + sl]) * DIM : 0;
             }
-            double v[NB], w[NB];
+            double v[NB][DIM], w[NB][DIM];
 #pragma unroll
             for (int k = 0; k < NB; ++k) {
-                const int i = base + k * B + tid;
-                v[k] = i < nH ? a.Ucur[n[k]] : 0.0;
-                if constexpr (STAGE_X) w[k] = i < nH ? a.X[n[k]] : 0.0;
+                const int sl = base + k * B + tid;
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) {
+                    v[k][c] = sl < nh ? a.Ucur[n[k] + c] : 0.0;
+                    if constexpr (STAGE_X) w[k][c] = sl < nh ? a.X[n[k] + c] : 0.0;
+                }
             }
 #pragma unroll
             for (int k = 0; k < NB; ++k) {
-                const int i = base + k * B + tid;
-                if (i < nH) {
-                    sU[nU + i] = v[k];
-                    if constexpr (STAGE_X) sX[nU + i] = w[k];
+                const int sl = base + k * B + tid;
+                if (sl < nh) {
+#pragma unroll
+                    for (int c = 0; c < DIM; ++c) {
+                        sU[nU + sl * DIM + c] = v[k][c];
+                        if constexpr (STAGE_X) sX[nU + sl * DIM + c] = w[k][c];
+                    }
                 }
             }
         }
@@ -208,7 +214,7 @@ __global__ __launch_bounds__(512) void k_cd_fused(FusedArgs a) {
             for (int c = 0; c < NEN; ++c)
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) u[c][d] = sU[lc[c] * DIM + d];
-            if constexpr (PATH == 2) {
+            if constexpr (PATH >= 2) {
                 const double *g = a.geo + size_t(a.egeo[e0 + i]) * (DIM * DIM);
                 double Lam[DIM][DIM];
 #pragma unroll
@@ -216,9 +222,9 @@ __global__ __launch_bounds__(512) void k_cd_fused(FusedArgs a) {
 #pragma unroll
                     for (int c2 = 0; c2 < DIM; ++c2) Lam[r2][c2] = g[r2 * DIM + c2];
                 if constexpr (DIM == 3)
-                    hexa_modal_force(Lam, lam, mu, u, f);
+                    hexa_modal_force<PATH == 3>(Lam, lam, mu, u, f);
                 else
-                    quad_modal_force(Lam, lam, mu, u, f);
+                    quad_modal_force<PATH == 3>(Lam, lam, mu, u, f);
             } else {
                 double x[NEN][DIM];
 #pragma unroll
@@ -312,7 +318,7 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
     const int o0 = a.own_ptr[tile], nown = a.own_ptr[tile + 1] - o0;
     const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
     const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
-    constexpr bool STAGE_X = PATH != 2;
+    constexpr bool STAGE_X = PATH < 2;
     const int MO = a.max_own;
     double *sU = smem;
     double *sX = smem + size_t(a.max_stage) * DIM;
@@ -350,7 +356,7 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
         }
         lam = a.elam[e0 + tid];
         mu = a.emu[e0 + tid];
-        if constexpr (PATH == 2) gid = a.egeo[e0 + tid];
+        if constexpr (PATH >= 2) gid = a.egeo[e0 + tid];
     } else {
 #pragma unroll
         for (int c = 0; c < NEN; ++c) lc[c] = 0;
@@ -376,27 +382,33 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
                 }
             }
         }
-        for (int base = 0; base < nH; base += NB * B) {
+        // Halo: one index load per node, then its DIM components.
+        for (int base = 0; base < nh; base += NB * B) {
             size_t n[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) {
-                const int i = base + k * B + tid;
-                const int sl = i / DIM;
-                n[k] = i < nH ? size_t(a.halo_node[h0 + sl]) * DIM + (i - sl * DIM) : 0;
+                const int sl = base + k * B + tid;
+                n[k] = sl < nh ? size_t(a.halo_node[h0 + sl]) * DIM : 0;
             }
-            double v[NB], w[NB];
+            double v[NB][DIM], w[NB][DIM];
 #pragma unroll
             for (int k = 0; k < NB; ++k) {
-                const int i = base + k * B + tid;
-                v[k] = i < nH ? a.Ucur[n[k]] : 0.0;
-                if constexpr (STAGE_X) w[k] = i < nH ? a.X[n[k]] : 0.0;
+                const int sl = base + k * B + tid;
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) {
+                    v[k][c] = sl < nh ? a.Ucur[n[k] + c] : 0.0;
+                    if constexpr (STAGE_X) w[k][c] = sl < nh ? a.X[n[k] + c] : 0.0;
+                }
             }
 #pragma unroll
             for (int k = 0; k < NB; ++k) {
-                const int i = base + k * B + tid;
-                if (i < nH) {
-                    sU[nU + i] = v[k];
-                    if constexpr (STAGE_X) sX[nU + i] = w[k];
+                const int sl = base + k * B + tid;
+                if (sl < nh) {
+#pragma unroll
+                    for (int c = 0; c < DIM; ++c) {
+                        sU[nU + sl * DIM + c] = v[k][c];
+                        if constexpr (STAGE_X) sX[nU + sl * DIM + c] = w[k][c];
+                    }
                 }
             }
         }
@@ -409,7 +421,7 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
         for (int c = 0; c < NEN; ++c)
 #pragma unroll
             for (int d = 0; d < DIM; ++d) u[c][d] = sU[lc[c] * DIM + d];
-        if constexpr (PATH == 2) {
+        if constexpr (PATH >= 2) {
             const double *g = a.geo + size_t(gid) * (DIM * DIM);
             double Lam[DIM][DIM];
 #pragma unroll
@@ -417,9 +429,9 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
 #pragma unroll
                 for (int c2 = 0; c2 < DIM; ++c2) Lam[r2][c2] = g[r2 * DIM + c2];
             if constexpr (DIM == 3)
-                hexa_modal_force(Lam, lam, mu, u, f);
+                hexa_modal_force<PATH == 3>(Lam, lam, mu, u, f);
             else
-                quad_modal_force(Lam, lam, mu, u, f);
+                quad_modal_force<PATH == 3>(Lam, lam, mu, u, f);
         } else {
             double x[NEN][DIM];
 #pragma unroll
@@ -484,8 +496,18 @@ static const void *fused_fn(bool corner) {
 static const void *fused_kernel(const Model &m) {
     const bool cr = m.variant == 3;
     if (m.ndim == 3)
-        return m.elem_path == 2 ? fused_fn<3, 2>(cr) : m.elem_path == 1 ? fused_fn<3, 1>(cr) : fused_fn<3, 0>(cr);
-    return m.elem_path == 2 ? fused_fn<2, 2>(cr) : m.elem_path == 1 ? fused_fn<2, 1>(cr) : fused_fn<2, 0>(cr);
+        switch (m.elem_path) {
+            case 3: return fused_fn<3, 3>(cr);
+            case 2: return fused_fn<3, 2>(cr);
+            case 1: return fused_fn<3, 1>(cr);
+            default: return fused_fn<3, 0>(cr);
+        }
+    switch (m.elem_path) {
+        case 3: return fused_fn<2, 3>(cr);
+        case 2: return fused_fn<2, 2>(cr);
+        case 1: return fused_fn<2, 1>(cr);
+        default: return fused_fn<2, 0>(cr);
+    }
 }
 
 int prepare_fused(Model &m) {
@@ -540,11 +562,13 @@ static int launch_fused(Model &m, int k, const double *Ucur, const double *Uprev
             hipLaunchKernelGGL((k_cd_fused<D, P>), grid, block, t.lds_bytes, m.stream, a);           \
     } while (0)
     if (m.ndim == 3) {
-        if (m.elem_path == 2) SVLG_LAUNCH_FUSED(3, 2);
+        if (m.elem_path == 3) SVLG_LAUNCH_FUSED(3, 3);
+        else if (m.elem_path == 2) SVLG_LAUNCH_FUSED(3, 2);
         else if (m.elem_path == 1) SVLG_LAUNCH_FUSED(3, 1);
         else SVLG_LAUNCH_FUSED(3, 0);
     } else {
-        if (m.elem_path == 2) SVLG_LAUNCH_FUSED(2, 2);
+        if (m.elem_path == 3) SVLG_LAUNCH_FUSED(2, 3);
+        else if (m.elem_path == 2) SVLG_LAUNCH_FUSED(2, 2);
         else if (m.elem_path == 1) SVLG_LAUNCH_FUSED(2, 1);
         else SVLG_LAUNCH_FUSED(2, 0);
     }

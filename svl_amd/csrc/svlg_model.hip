@@ -428,9 +428,16 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         if (ok) {
             m.elem_path = 2;
             m.ngeo = int(classes.size());
+            // Axis-aligned bricks / rectangles: every J^-1 in the table is diagonal.
+            bool diag = true;
+            for (int g = 0; g < m.ngeo && diag; ++g)
+                for (int i = 0; i < ndim; ++i)
+                    for (int k = 0; k < ndim; ++k)
+                        if (i != k && gtab[size_t(g) * ndim * ndim + i * ndim + k] != 0.0) diag = false;
+            if (diag) m.elem_path = 3;
         }
     }
-    if (m.elem_path == 2) {
+    if (m.elem_path >= 2) {
         UP(m.geo, gtab.data(), gtab.size());
     }
     // Tiles.
@@ -442,7 +449,7 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         t.max_own = ht.max_own;
         t.max_elem = ht.max_elem;
         t.block = block;
-        const int staged_arrays = m.elem_path == 2 ? 1 : 2;  // U (and X unless the table is used)
+        const int staged_arrays = m.elem_path >= 2 ? 1 : 2;  // U (and X unless the table is used)
         // Fused-kernel flavour: corner slots when the mesh allows it and one round of
         // <= 512 threads covers every tile element and owned node; colour phases otherwise.
         const int need = std::max(ht.max_elem, ht.max_own);
@@ -466,7 +473,7 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         std::vector<uint16_t> eg(ht.elem_id.size(), 0);
         for (size_t i = 0; i < ht.elem_id.size(); ++i) {
             int e = ht.elem_id[i];
-            if (m.elem_path == 2) {
+            if (m.elem_path >= 2) {
                 el[i] = lam[e] * escale[e];
                 em[i] = mu[e] * escale[e];
                 eg[i] = gid[e];

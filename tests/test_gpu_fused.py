@@ -100,7 +100,7 @@ def test_fused_hexa_parity(kind, tile_nodes, geo):
     ref = oracle.run(m, oracle.CENTRAL_DIFFERENCE, dt, nt, rec)
     hist, U, info = run_gpu(m, dt, nt - 1, rec, tile_nodes=tile_nodes, geo_table=geo)
     assert info["variant"] == 3
-    expect = 0 if kind == "distorted" else (1 if geo == -1 else 2)
+    expect = 0 if kind == "distorted" else (1 if geo == -1 else (3 if kind in ("axis", "graded") else 2))
     assert info["elem_path"] == expect
     assert rel_l2(hist, ref["U"]) < REL_TOL
     assert rel_l2(U, ref["final_U"]) < REL_TOL
@@ -113,7 +113,7 @@ def test_fused_quad_layer_parity(tile_nodes):
     ref = oracle.run(m, oracle.CENTRAL_DIFFERENCE, dt, nt, rec)
     for geo in (0, -1):
         hist, U, info = run_gpu(m, dt, nt - 1, rec, tile_nodes=tile_nodes, geo_table=geo)
-        assert info["variant"] in (1, 3) and info["elem_path"] == (2 if geo == 0 else 1)
+        assert info["variant"] in (1, 3) and info["elem_path"] == (3 if geo == 0 else 1)
         assert rel_l2(hist, ref["U"]) < REL_TOL
 
 
