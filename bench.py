@@ -56,6 +56,56 @@ class HipEvents:
         return float(ms.value)
 
 
+def copy_calibration(nbytes: int, reps: int = 20):
+    """Times svlg_stream_copy of `nbytes` (device to device) on its own stream; under
+    rocprofv3 --pmc the same launches give the counter reading for a known transfer."""
+    from svl_amd import _capi
+    L = C.CDLL("libamdhip64.so")
+    L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    L.hipFree.argtypes = [C.c_void_p]
+    L.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    L.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    L.hipStreamSynchronize.argtypes = [C.c_void_p]
+    a, b, st = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert L.hipMalloc(C.byref(a), nbytes) == 0 and L.hipMalloc(C.byref(b), nbytes) == 0
+    assert L.hipMemset(a, 0, nbytes) == 0 and L.hipStreamCreate(C.byref(st)) == 0
+    n = nbytes // 8
+    lib = _capi.lib()
+    ev = HipEvents()
+    _capi.check(lib.svlg_stream_copy(b, a, n, st), "svlg_stream_copy")
+    L.hipStreamSynchronize(st)
+    e0, e1 = ev.event(), ev.event()
+    ev.record(e0, st.value)
+    for _ in range(reps):
+        _capi.check(lib.svlg_stream_copy(b, a, n, st), "svlg_stream_copy")
+    ev.record(e1, st.value)
+    ms = ev.elapsed_ms(e0, e1)
+    L.hipStreamSynchronize(st)
+    L.hipFree(a)
+    L.hipFree(b)
+    return {"bytes_per_launch": 2 * nbytes, "launches": reps + 1, "avg_us": ms * 1e3 / reps,
+            "copy_GBs": 2 * nbytes * reps / (ms / 1e3) / 1e9}
+
+
+def traffic_from_profile(variant: int, tile_nodes: int):
+    """HBM bytes per launch of the fused step kernel from the committed rocprofv3 PMC
+    summary (profiles/halfspace3d_fused_step.md), if it covers this configuration."""
+    path = os.path.join(ROOT, "profiles", "halfspace3d_fused_step.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if line.startswith("traffic_"):
+            k, _, v = line.partition(":")
+            vals[k.strip()] = v.strip()
+    try:
+        if int(vals["traffic_variant"]) == variant and int(vals["traffic_tile_nodes"]) == tile_nodes:
+            return float(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+    return None
+
+
 def algorithmic_bytes_per_step(m) -> int:
     return 32 * m.nfree + (4 * m.nen + 16) * m.nelem + 8 * m.ndim * m.nnodes
 
@@ -72,15 +122,21 @@ def cpu_baseline(m, dt, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--ne", type=str, default="100,100,50", help="hexa elements nx,ny,nz")
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-nodes", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--variant", type=int, default=0, help="1 fused node tiles (default), 2 element-colored")
+    ap.add_argument("--variant", type=int, default=0,
+                    help="0 auto, 1 fused + colour phases, 2 element-colored, 3 fused + corner slots")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
+                    help="only run the device copy of MB megabytes (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calib:
+        print(json.dumps({"copy_calibration": copy_calibration(args.copy_calib * 1000000)}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                                    "(the multi-GPU sharding is described in the design notes, not built)"}))
@@ -135,8 +191,10 @@ def main():
                    "nfree": m.nfree, "nelem": m.nelem, "dt": dt, "kernel_variant": info["variant"],
                    "ntiles": info["ntiles"], "lds_bytes": info["lds_bytes"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "one CentralDifference step (all kernels of the step)",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile(info["variant"], args.tile_nodes or 320),
+                     "kernel": "k_cd_corner / k_cd_fused: one launch = one CentralDifference step",
+                     "avg_launch_us": step_s * 1e6,
                      "alg_bytes_per_step": bytes_step},
     }
     dm.close()

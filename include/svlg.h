@@ -148,6 +148,12 @@ int svlg_cd_get_state(svlg_model *m, double *U, double *V, double *A);
 /* Blocks until all work queued on the model's stream is done. */
 int svlg_sync(svlg_model *m);
 
+/* Device-to-device copy of n doubles with a plain coalesced kernel on `stream` (a
+ * hipStream_t, NULL = default stream). No counterpart in SVL: it gives the bytes that
+ * HBM traffic counters read for a transfer of known size and the copy bandwidth the
+ * device sustains. `dst` and `src` are device pointers. */
+int svlg_stream_copy(double *dst, const double *src, int64_t n, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,6 +125,7 @@ SIGNATURES = {
     "svlg_cd_steps_async": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "svlg_cd_get_state": (C.c_int, [C.c_void_p, f64p, f64p, f64p]),
     "svlg_sync": (C.c_int, [C.c_void_p]),
+    "svlg_stream_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
 }
 
 _LIB = None

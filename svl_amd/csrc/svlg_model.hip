@@ -71,14 +71,19 @@ __global__ void k_node_gather_mass(int nnodes, int ndim, const int32_t *__restri
     }
 }
 
-// Dashpots in element-tag order; one thread keeps the summation order of SVL.
-__global__ void k_dashpots(int ndash, int ndim, const int32_t *__restrict__ node, const int32_t *__restrict__ dir,
+// Dashpots: one thread per damped DOF adds its dashpots in element-tag order (the
+// triplet order of Assembler.cpp:148-154), after the Rayleigh terms of the continuum
+// elements (whose tags come first).
+__global__ void k_dashpots(int ndofs, const int32_t *__restrict__ dof, const int32_t *__restrict__ ptr,
                            const double *__restrict__ eta, double ktol, double *__restrict__ cdiag) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (int i = 0; i < ndash; ++i) {
-        double v = eta[i];
-        if (fabs(v) > ktol) cdiag[size_t(node[i]) * ndim + dir[i]] += v;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndofs) return;
+    double c = cdiag[dof[i]];
+    for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+        double v = eta[k];
+        if (fabs(v) > ktol) c += v;
     }
+    cdiag[dof[i]] = c;
 }
 
 // Colored Fint: every element of one color adds into F with no conflicts.
@@ -540,14 +545,29 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
                            m.scratch.p, m.am.p, nen, 1e-12, m.mdiag.p, m.cdiag.p);
     }
     if (d->ndash > 0) {
-        std::vector<int32_t> dnode(d->ndash);
-        for (int i = 0; i < d->ndash; ++i) dnode[i] = m.inv[d->dash_node[i]];
-        DevBuf<int32_t> dn, dd;
+        // Group the dashpots by device DOF, keeping their element-tag order inside a DOF.
+        std::vector<std::pair<int32_t, int32_t>> key(d->ndash);
+        for (int i = 0; i < d->ndash; ++i) key[i] = {m.inv[d->dash_node[i]] * ndim + d->dash_dir[i], i};
+        std::stable_sort(key.begin(), key.end(),
+                         [](const auto &x, const auto &y) { return x.first < y.first; });
+        std::vector<int32_t> dofs, ptr{0};
+        std::vector<double> eta(d->ndash);
+        for (int i = 0; i < d->ndash; ++i) {
+            if (dofs.empty() || dofs.back() != key[i].first) {
+                if (!dofs.empty()) ptr.push_back(i);
+                dofs.push_back(key[i].first);
+            }
+            eta[i] = d->dash_eta[key[i].second];
+        }
+        ptr.push_back(d->ndash);
+        DevBuf<int32_t> dd, dp;
         DevBuf<double> de;
-        UP(dn, dnode.data(), dnode.size());
-        UP(dd, d->dash_dir, size_t(d->ndash));
-        UP(de, d->dash_eta, size_t(d->ndash));
-        hipLaunchKernelGGL(k_dashpots, dim3(1), dim3(64), 0, s, d->ndash, ndim, dn.p, dd.p, de.p, 1e-12, m.cdiag.p);
+        UP(dd, dofs.data(), dofs.size());
+        UP(dp, ptr.data(), ptr.size());
+        UP(de, eta.data(), eta.size());
+        const int nd = int(dofs.size());
+        hipLaunchKernelGGL(k_dashpots, dim3((nd + 255) / 256), dim3(256), 0, s, nd, dd.p, dp.p, de.p, 1e-12,
+                           m.cdiag.p);
         if (hipStreamSynchronize(s) != hipSuccess) { set_error("dashpot assembly failed"); return fail(10); }
     }
 #undef UP
@@ -723,6 +743,20 @@ int svlg_internal_force(svlg_model *h, const double *Ufree, double ftol, double 
     if (int rc = gather_free(m, fn.p, uf.p)) return rc;
     SVLG_HIP(hipMemcpyAsync(Ffree, uf.p, sizeof(double) * m.nfree, hipMemcpyDeviceToHost, m.stream));
     SVLG_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+__global__ void k_stream_copy(double *__restrict__ dst, const double *__restrict__ src, int64_t n) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+int svlg_stream_copy(double *dst, const double *src, int64_t n, void *stream) {
+    SVLG_CHECK(dst && src && n >= 0, "bad copy arguments");
+    if (n == 0) return 0;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(k_stream_copy, dim3(unsigned(blocks)), dim3(256), 0, (hipStream_t)stream, dst, src, n);
+    SVLG_HIP(hipGetLastError());
     return 0;
 }
 
