@@ -148,6 +148,28 @@ int svlg_cd_get_state(svlg_model *m, double *U, double *V, double *A);
 /* Blocks until all work queued on the model's stream is done. */
 int svlg_sync(svlg_model *m);
 
+/* ---- ExtendedNewmarkBeta + Linear + LinearSystem ---------------------------------
+ * (10-Integrators/03-Newmark/ExtendedNewmarkBeta.cpp, gamma = 1/2, beta = 1/4, G = 0;
+ * 09-Algorithms/01-Linear/Linear.cpp:22-56; 11-Solvers/LinearSystem.hpp:44-66.)
+ * Needs a lumped mass, elastic materials and a corner-unique mesh (explicit variant 3).
+ * Keff = K + 4/dt^2 M + 2/dt C is applied matrix-free; the linear system is solved by
+ * preconditioned conjugate gradients to ||b - Keff x|| <= rtol ||b||. */
+/* Initialize (:36-63 + ComputeEffectiveStiffness :233-245): zero state, Keff terms and
+ * the preconditioner (SVLG_PRECOND_JACOBI). */
+int svlg_nm_init(svlg_model *m, double dt, double rtol, int32_t maxit, int32_t precond);
+/* Advances nsteps steps from step index k0 (ComputeNewStep :123-154, Feff :208-229 with
+ * the Linear algorithm). Recording as svlg_cd_steps; iters[s] and relres[s] (may be
+ * NULL) receive the CG iteration count and the final relative residual of step s. */
+int svlg_nm_steps(svlg_model *m, int32_t k0, int32_t nsteps, int32_t nrec, const int32_t *rec_dofs,
+                  double *hist, int32_t *iters, double *relres);
+/* Current U, V, A in free DOFs (ExtendedNewmarkBeta::GetDisplacements/...). */
+int svlg_nm_get_state(svlg_model *m, double *U, double *V, double *A);
+/* y = Keff x over the free DOFs (the matrix of LinearSystem::SetMatrix, applied). */
+int svlg_keff_apply(svlg_model *m, const double *x /*[nfree]*/, double *y /*[nfree]*/);
+/* Solves Keff x = b from x = 0 (LinearSystem::SetVector + SolveSystem + GetSolution). */
+int svlg_solve(svlg_model *m, const double *b /*[nfree]*/, double *x /*[nfree]*/, double rtol,
+               int32_t maxit, int32_t *iters, double *relres);
+
 /* Device-to-device copy of n doubles with a plain coalesced kernel on `stream` (a
  * hipStream_t, NULL = default stream). No counterpart in SVL: it gives the bytes that
  * HBM traffic counters read for a transfer of known size and the copy bandwidth the

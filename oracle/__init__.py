@@ -73,6 +73,8 @@ def _lib():
         L.oracle_internal_force.argtypes = [C.POINTER(MeshDesc), f64p, C.c_double, f64p]
         L.oracle_diagonals.restype = C.c_int
         L.oracle_diagonals.argtypes = [C.POINTER(MeshDesc), f64p, f64p]
+        L.oracle_newmark_keff_apply.restype = C.c_int
+        L.oracle_newmark_keff_apply.argtypes = [C.POINTER(MeshDesc), C.c_double, f64p, f64p]
         _L = L
     return _L
 
@@ -124,6 +126,16 @@ def internal_force(model, U, ftol: float = 1e-12):
     F = np.zeros(model.nfree)
     _chk(_lib().oracle_internal_force(C.byref(desc), _f(U), ftol, _f(F)), "internal_force")
     return F
+
+
+def keff_apply(model, dt: float, x):
+    """y = Keff x (ExtendedNewmarkBeta effective stiffness, free DOFs)."""
+    from svl_amd._capi import make_desc
+    desc, keep = make_desc(model)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros(model.nfree)
+    _chk(_lib().oracle_newmark_keff_apply(C.byref(desc), dt, _f(x), _f(y)), "keff_apply")
+    return y
 
 
 def diagonals(model):

@@ -900,6 +900,29 @@ int oracle_internal_force(const svlg_mesh_desc *d, const double *Ufree, double f
     }
 }
 
+// y = Keff x over the free DOFs with Keff = T^T (K + 4/dt^2 M + 2/dt C) T, the
+// effective stiffness of ExtendedNewmarkBeta::ComputeEffectiveStiffness
+// (ExtendedNewmarkBeta.cpp:233-245) with G = 0.
+int oracle_newmark_keff_apply(const svlg_mesh_desc *d, double dt, const double *xfree, double *yfree) {
+    try {
+        Model m = build(d);
+        Assembler as{&m};
+        CSC K = as.stiffness();
+        CSC M = as.mass();
+        CSC C = as.damping();
+        CSC KM = combine(1.0, K, 4.0 / dt / dt, M);
+        CSC Kt = combine(1.0, KM, 2.0 / dt, C);
+        CSC Kf = free_block(m, Kt);
+        std::vector<double> x(xfree, xfree + m.nfree), y;
+        Kf.mul(x, y);
+        std::memcpy(yfree, y.data(), sizeof(double) * y.size());
+        return 0;
+    } catch (const std::exception &ex) {
+        g_err = ex.what();
+        return 3;
+    }
+}
+
 int oracle_diagonals(const svlg_mesh_desc *d, double *Mfree, double *Cfree) {
     try {
         Model m = build(d);

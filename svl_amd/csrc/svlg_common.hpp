@@ -122,6 +122,20 @@ struct Model {
     DevBuf<double> hist;              // recorder buffer
     DevBuf<int32_t> rec;              // recorded node-DOFs
 
+    // Implicit (Extended)NewmarkBeta state and Jacobi / learned PCG work vectors.
+    bool nm_ready = false;
+    double nm_dt = 0.0, nm_rtol = 1e-8;
+    int nm_maxit = 1000, nm_precond = 0;
+    long nm_steps_done = 0;
+    DevBuf<double> nU, nV, nA, ndU;   // state and last increment (node-DOF layout)
+    DevBuf<double> dgn;               // (4/dt^2) M + (2/dt) C_diag   (lumped, per DOF)
+    DevBuf<double> jac;               // 1 / diag(Keff), 0 on restrained DOFs
+    DevBuf<double> coef_keff, coef_rhs;  // per tile element: 1 + 2 ak/dt, -ak
+    DevBuf<double> cg_r, cg_z, cg_q, cg_b, cg_p[2];
+    DevBuf<double> part;              // per-block partial sums
+    DevBuf<double> scal;              // device scalars of the solver
+    std::vector<int32_t> tile_elem_id;   // tag index of every tile element (host)
+
     TileSet tiles;
     int variant = 1;                  // 1 fused node tiles, 2 element-colored
     int elem_path = 0;                // 0 Gauss points, 1 modal + staged X, 2 modal + geometry table

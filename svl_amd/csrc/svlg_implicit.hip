@@ -1,0 +1,767 @@
+// svlg_implicit.hip -- ExtendedNewmarkBeta (gamma = 1/2, beta = 1/4) with the Linear
+// algorithm and a matrix-free, Jacobi-preconditioned conjugate-gradient solve of the
+// effective-stiffness system, all on the device in FP64.
+//
+// Replaces (RP = 02-Run_Process of the SVL source tree):
+//   ExtendedNewmarkBeta::ComputeEffectiveForce      (RP/10-Integrators/03-Newmark/ExtendedNewmarkBeta.cpp:208-229)
+//   ExtendedNewmarkBeta::ComputeEffectiveStiffness  (:233-245)
+//   ExtendedNewmarkBeta::ComputeNewStep             (:123-154)
+//   Linear::ComputeNewIncrement                     (RP/09-Algorithms/01-Linear/Linear.cpp:22-56)
+//   LinearSystem::SolveSystem / GetSolution         (RP/11-Solvers/LinearSystem.hpp:44-66): SVL
+//     factors Keff (EigenSolver LDL^T, MUMPS) or calls PETSc KSP; here CG to a relative
+//     residual tolerance ||b - Keff x|| <= rtol ||b||.
+// With lumped mass, Rayleigh damping C_e = am M_e + ak K_e and dashpots, and G = 0:
+//   Keff = sum_e (1 + 2 ak_e/dt) K_e + (4/dt^2) M + (2/dt) (sum_e am_e M_e + C_dash)
+//   Feff = Fext - sum_e K_e (U_e - ak_e V_e) + C_diag V + M (4/dt V + A)
+// The element operator reuses the node tiles of the explicit corner-slot kernel: a
+// workgroup stages the operand of its owned + halo nodes in LDS, recomputes every element
+// touching its nodes and sums the corner slots of each owned node in a fixed order, so a
+// solve is bitwise reproducible.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "svlg_common.hpp"
+#include "svlg_elem.hpp"
+
+namespace svlg {
+
+int gather_free(Model &m, const double *dnode, double *dfree);
+int scatter_free(Model &m, const double *dfree, double *dnode);
+
+// Device scalars of the solver (Model::scal).
+enum {
+    SC_RHO = 0,  // r . z
+    SC_PQ,       // p . q
+    SC_RR,       // r . r
+    SC_BB,       // b . b
+    SC_ALPHA,
+    SC_BETA,
+    SC_DONE,
+    SC_ITERS,
+    SC_TOL2,     // rtol^2
+    SC_MAXIT,
+    SC_COUNT = 16
+};
+
+enum { OP_CG = 0, OP_RHS = 1 };
+
+struct OpArgs {
+    const int32_t *own_ptr, *halo_ptr, *halo_node, *elem_ptr;
+    const uint16_t *lconn;
+    const double *elam, *emu;
+    const uint16_t *egeo;
+    const double *geo, *X;
+    const double *coef;   // OP_CG: 1 + 2 ak/dt per tile element; OP_RHS: -ak per tile element
+    int ntiles, max_stage, max_own;
+    double th;
+    const double *jac;    // 0 marks a restrained DOF
+    // OP_CG
+    const double *z, *p_old;
+    double *p_new;
+    const double *dgn;
+    double *y, *part;
+    const double *scal;
+    // OP_RHS
+    const double *U, *V, *A, *mdiag, *cdiag;
+    double dt;
+    double *b;
+    const int32_t *nl_ptr, *nl_idx;
+    const double *load_dir, *load_factor, *load_series;
+    int load_nt, k;
+};
+
+// Deterministic sum over a workgroup (blockDim a multiple of 64, at most 1024): fixed
+// shuffle tree inside each wave, then the wave sums in wave order. buf holds 17 doubles.
+__device__ __forceinline__ double block_sum(double v, double *buf) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) buf[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int i = 0; i < int(blockDim.x >> 6); ++i) r += buf[i];
+        buf[16] = r;
+    }
+    __syncthreads();
+    const double r = buf[16];
+    __syncthreads();
+    return r;
+}
+
+// Element force f = K_e u for the operand u of one tile element; lam and mu carry any
+// per-element factor (and |J|-scaling on the geometry-table paths).
+template <int DIM, int PATH>
+__device__ __forceinline__ void elem_force(const OpArgs &a, const int (&lc)[Elem<DIM>::nen], const double *sX,
+                                           double lam, double mu, int gid, const double (&u)[Elem<DIM>::nen][DIM],
+                                           double (&f)[Elem<DIM>::nen][DIM]) {
+    constexpr int NEN = Elem<DIM>::nen;
+    if constexpr (PATH >= 2) {
+        const double *g = a.geo + size_t(gid) * (DIM * DIM);
+        double Lam[DIM][DIM];
+#pragma unroll
+        for (int r = 0; r < DIM; ++r)
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) Lam[r][c] = g[r * DIM + c];
+        if constexpr (DIM == 3)
+            hexa_modal_force<PATH == 3>(Lam, lam, mu, u, f);
+        else
+            quad_modal_force<PATH == 3>(Lam, lam, mu, u, f);
+    } else {
+        double x[NEN][DIM];
+#pragma unroll
+        for (int c = 0; c < NEN; ++c)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) x[c][d] = sX[lc[c] * DIM + d];
+        if constexpr (DIM == 3) {
+            if constexpr (PATH == 1)
+                hexa_affine_fint(x, u, lam, mu, f);
+            else
+                hexa_fint(x, u, lam, mu, f);
+        } else {
+            if constexpr (PATH == 1)
+                quad_affine_fint(x, u, lam, mu, a.th, f);
+            else
+                quad_fint(x, u, lam, mu, a.th, f);
+        }
+    }
+}
+
+// One node tile of the effective-stiffness operator (one thread per tile element in the
+// element phase, one thread per owned node in the sum phase).
+//  OP_CG : stage p = z + beta p_old (owned values are written to p_new), y = Keff p on the
+//          owned DOFs (0 on restrained DOFs), per-tile partial of p . y.
+//  OP_RHS: stage U and V, y = sum_e K_e (U_e - ak_e V_e), then on the free DOFs
+//          b = (((Fext - y) + C_diag V) + M (4/dt V + A)).
+template <int DIM, int PATH, int MODE>
+__global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
+    constexpr int NEN = Elem<DIM>::nen;
+    constexpr bool STAGE_X = PATH < 2;
+    constexpr bool RHS = MODE == OP_RHS;
+    extern __shared__ double smem[];
+    __shared__ double red[17];
+    if constexpr (!RHS) {
+        if (a.scal[SC_DONE] != 0.0) return;
+    }
+    const int nt = a.ntiles;
+    // XCD-aware mapping: consecutive workgroups go to different XCDs, each XCD walks a
+    // contiguous range of tiles.
+    const int bI = blockIdx.x, xcd = bI & 7, j = bI >> 3;
+    const int q = nt >> 3, r = nt & 7;
+    const int tile = xcd * q + min(xcd, r) + j;
+    const int o0 = a.own_ptr[tile], nown = a.own_ptr[tile + 1] - o0;
+    const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
+    const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
+    const int MS = a.max_stage, MO = a.max_own;
+    double *sU = smem;
+    double *sV = smem + size_t(MS) * DIM;
+    double *sX = smem + size_t((RHS ? 2 : 1) * MS) * DIM;
+    double *sIn = smem + size_t(((RHS ? 2 : 1) + (STAGE_X ? 1 : 0)) * MS) * DIM;
+    const int tid = threadIdx.x, B = blockDim.x;
+    const size_t g0 = size_t(o0) * DIM;
+    const double beta = RHS ? 0.0 : a.scal[SC_BETA];
+    const bool has_elem = tid < ne;
+    int lc[NEN];
+    double lam = 0.0, mu = 0.0, c2 = 0.0;
+    int gid = 0;
+    if (has_elem) {
+        const uint16_t *l = a.lconn + size_t(e0 + tid) * NEN;
+#pragma unroll
+        for (int c = 0; c < NEN; ++c) lc[c] = l[c];
+        lam = a.elam[e0 + tid];
+        mu = a.emu[e0 + tid];
+        if constexpr (RHS) {
+            c2 = a.coef[e0 + tid];
+        } else {
+            const double cf = a.coef[e0 + tid];
+            lam *= cf;
+            mu *= cf;
+        }
+        if constexpr (PATH >= 2) gid = a.egeo[e0 + tid];
+    } else {
+#pragma unroll
+        for (int c = 0; c < NEN; ++c) lc[c] = 0;
+    }
+    for (int i = tid; i < nown * DIM; i += B) {
+        const size_t g = g0 + i;
+        if constexpr (RHS) {
+            sU[i] = a.U[g];
+            sV[i] = a.V[g];
+        } else {
+            const double v = a.z[g] + beta * a.p_old[g];
+            sU[i] = v;
+            a.p_new[g] = v;
+        }
+        if constexpr (STAGE_X) sX[i] = a.X[g];
+    }
+    for (int sl = tid; sl < nh; sl += B) {
+        const size_t n = size_t(a.halo_node[h0 + sl]) * DIM;
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            const int i = (nown + sl) * DIM + c;
+            if constexpr (RHS) {
+                sU[i] = a.U[n + c];
+                sV[i] = a.V[n + c];
+            } else {
+                sU[i] = a.z[n + c] + beta * a.p_old[n + c];
+            }
+            if constexpr (STAGE_X) sX[i] = a.X[n + c];
+        }
+    }
+    for (int i = tid; i < NEN * DIM * MO; i += B) sIn[i] = 0.0;
+    __syncthreads();
+    if (has_elem) {
+        double u[NEN][DIM], f[NEN][DIM];
+#pragma unroll
+        for (int c = 0; c < NEN; ++c)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                u[c][d] = sU[lc[c] * DIM + d];
+                if constexpr (RHS) u[c][d] += c2 * sV[lc[c] * DIM + d];
+            }
+        elem_force<DIM, PATH>(a, lc, sX, lam, mu, gid, u, f);
+#pragma unroll
+        for (int c = 0; c < NEN; ++c)
+            if (lc[c] < nown) {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) sIn[(c * DIM + d) * MO + lc[c]] = f[c][d];
+            }
+    }
+    __syncthreads();
+    double acc = 0.0;
+    if (tid < nown) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            double fi = 0.0;
+#pragma unroll
+            for (int c = 0; c < NEN; ++c) fi += sIn[(c * DIM + d) * MO + tid];
+            const size_t g = g0 + size_t(tid) * DIM + d;
+            const bool freedof = a.jac[g] != 0.0;
+            if constexpr (RHS) {
+                double fe = 0.0;
+                const int node = o0 + tid;
+                for (int q2 = a.nl_ptr[node]; q2 < a.nl_ptr[node + 1]; ++q2) {
+                    const int l = a.nl_idx[q2];
+                    const double amp =
+                        (a.k >= 0 && a.k < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + a.k] : 0.0;
+                    fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + d]);
+                }
+                const double v = sV[tid * DIM + d];
+                a.b[g] = freedof ? ((fe - fi) + a.cdiag[g] * v) + a.mdiag[g] * (4.0 / a.dt * v + a.A[g]) : 0.0;
+            } else {
+                const double x = sU[tid * DIM + d];
+                const double yv = freedof ? fi + a.dgn[g] * x : 0.0;
+                a.y[g] = yv;
+                acc += x * yv;
+            }
+        }
+    }
+    if constexpr (!RHS) {
+        const double s = block_sum(acc, red);
+        if (tid == 0) a.part[tile] = s;
+    }
+}
+
+__global__ void k_solver_reset(double *scal, double tol2, double maxit) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        for (int i = 0; i < SC_COUNT; ++i) scal[i] = 0.0;
+        scal[SC_TOL2] = tol2;
+        scal[SC_MAXIT] = maxit;
+    }
+}
+
+// r = b - q, z = jac r; per-block partials of r.z, r.r, b.b (fixed grid => fixed order).
+__global__ __launch_bounds__(256) void k_cg_init(int n, const double *__restrict__ b, const double *__restrict__ q,
+                                                 const double *__restrict__ jac, double *__restrict__ r,
+                                                 double *__restrict__ z, double *__restrict__ part) {
+    __shared__ double red[17];
+    double rz = 0.0, rr = 0.0, bb = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double ri = jac[i] != 0.0 ? b[i] - q[i] : 0.0;
+        const double zi = jac[i] * ri;
+        r[i] = ri;
+        z[i] = zi;
+        rz += ri * zi;
+        rr += ri * ri;
+        bb += b[i] * b[i];
+    }
+    rz = block_sum(rz, red);
+    rr = block_sum(rr, red);
+    bb = block_sum(bb, red);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x + 0] = rz;
+        part[3 * blockIdx.x + 1] = rr;
+        part[3 * blockIdx.x + 2] = bb;
+    }
+}
+
+// x += alpha p; r -= alpha q; z = jac r; per-block partials of r.z and r.r.
+__global__ __launch_bounds__(256) void k_cg_update(int n, const double *__restrict__ scal, double *__restrict__ x,
+                                                   const double *__restrict__ p, const double *__restrict__ q,
+                                                   double *__restrict__ r, double *__restrict__ z,
+                                                   const double *__restrict__ jac, double *__restrict__ part) {
+    __shared__ double red[17];
+    if (scal[SC_DONE] != 0.0) return;
+    const double alpha = scal[SC_ALPHA];
+    double rz = 0.0, rr = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * q[i];
+        const double zi = jac[i] * ri;
+        r[i] = ri;
+        z[i] = zi;
+        rz += ri * zi;
+        rr += ri * ri;
+    }
+    rz = block_sum(rz, red);
+    rr = block_sum(rr, red);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x + 0] = rz;
+        part[3 * blockIdx.x + 1] = rr;
+    }
+}
+
+enum { RED_INIT = 0, RED_PQ = 1, RED_RZ = 2 };
+
+// One workgroup: sums the partials in a fixed order, then the scalar recurrences of
+// preconditioned CG (alpha = rho / p.q, beta = rho_new / rho) and the stopping test.
+__global__ __launch_bounds__(1024) void k_reduce(int mode, int count, int stride, const double *__restrict__ part,
+                                                 double *scal) {
+    __shared__ double red[17];
+    if (mode != RED_INIT && scal[SC_DONE] != 0.0) return;
+    const int ncomp = mode == RED_PQ ? 1 : (mode == RED_INIT ? 3 : 2);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int i = threadIdx.x; i < count; i += blockDim.x) {
+        const double *pp = part + size_t(i) * stride;
+        s0 += pp[0];
+        if (ncomp > 1) s1 += pp[1];
+        if (ncomp > 2) s2 += pp[2];
+    }
+    s0 = block_sum(s0, red);
+    if (ncomp > 1) s1 = block_sum(s1, red);
+    if (ncomp > 2) s2 = block_sum(s2, red);
+    if (threadIdx.x != 0) return;
+    if (mode == RED_PQ) {
+        scal[SC_PQ] = s0;
+        scal[SC_ALPHA] = s0 != 0.0 ? scal[SC_RHO] / s0 : 0.0;
+    } else if (mode == RED_INIT) {
+        scal[SC_RHO] = s0;
+        scal[SC_RR] = s1;
+        scal[SC_BB] = s2;
+        scal[SC_BETA] = 0.0;
+        scal[SC_ITERS] = 0.0;
+        scal[SC_DONE] = (s1 <= scal[SC_TOL2] * s2) ? 1.0 : 0.0;
+    } else {
+        scal[SC_BETA] = scal[SC_RHO] != 0.0 ? s0 / scal[SC_RHO] : 0.0;
+        scal[SC_RHO] = s0;
+        scal[SC_RR] = s1;
+        scal[SC_ITERS] += 1.0;
+        if (s1 <= scal[SC_TOL2] * scal[SC_BB] || scal[SC_ITERS] >= scal[SC_MAXIT]) scal[SC_DONE] = 1.0;
+    }
+}
+
+// A zero right-hand side has the zero solution whatever the initial guess.
+__global__ void k_cg_zero_rhs(int n, const double *__restrict__ scal, double *__restrict__ x) {
+    if (scal[SC_BB] != 0.0) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = 0.0;
+}
+
+// U += dU; A = 4/dt^2 dU - 4/dt V - A; V = 2/dt dU - V   (ExtendedNewmarkBeta.cpp:141-150).
+__global__ void k_nm_update(int n, double dt, const double *__restrict__ dU, const double *__restrict__ jac,
+                            double *__restrict__ U, double *__restrict__ V, double *__restrict__ A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || jac[i] == 0.0) return;
+    const double du = dU[i];
+    U[i] += du;
+    const double v = V[i];
+    A[i] = 4.0 / dt / dt * du - 4.0 / dt * v - A[i];
+    V[i] = 2.0 / dt * du - v;
+}
+
+// Diagonal of sum_e (1 + 2 ak_e/dt) K_e by unit probes through the Gauss-point routine
+// (valid for every element shape), one colour class per launch so no two threads of a
+// launch touch the same node.
+template <int DIM>
+__global__ void k_kdiag_colored(int e0, int e1, const int32_t *__restrict__ conn_c, const int32_t *__restrict__ eid_c,
+                                const double *__restrict__ X, const double *__restrict__ lam,
+                                const double *__restrict__ mu, const double *__restrict__ ak, double inv_dt, double th,
+                                double *__restrict__ kd) {
+    const int e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= e1) return;
+    constexpr int nen = Elem<DIM>::nen;
+    const int tag = eid_c[e];
+    int nodes[nen];
+    double x[nen][DIM];
+#pragma unroll
+    for (int a = 0; a < nen; ++a) {
+        nodes[a] = conn_c[size_t(e) * nen + a];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) x[a][d] = X[size_t(nodes[a]) * DIM + d];
+    }
+    const double cf = 1.0 + 2.0 * ak[tag] * inv_dt;
+#pragma unroll 1
+    for (int a = 0; a < nen; ++a)
+#pragma unroll 1
+        for (int d = 0; d < DIM; ++d) {
+            double u[nen][DIM], f[nen][DIM];
+#pragma unroll
+            for (int b = 0; b < nen; ++b)
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) u[b][c] = (b == a && c == d) ? 1.0 : 0.0;
+            if constexpr (DIM == 2)
+                quad_fint(x, u, lam[tag], mu[tag], th, f);
+            else
+                hexa_fint(x, u, lam[tag], mu[tag], f);
+            kd[size_t(nodes[a]) * DIM + d] += cf * f[a][d];
+        }
+}
+
+__global__ void k_jacobi(int n, const int32_t *__restrict__ fdof, const double *__restrict__ kd,
+                         const double *__restrict__ dgn, double *__restrict__ jac, int *__restrict__ bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (fdof[i] < 0) {
+        jac[i] = 0.0;
+        return;
+    }
+    const double d = kd[i] + dgn[i];
+    if (!(d > 0.0)) atomicOr(bad, 1);
+    jac[i] = d > 0.0 ? 1.0 / d : 0.0;
+}
+
+__global__ void k_dgn(int n, double dt, const double *__restrict__ M, const double *__restrict__ C,
+                      const int32_t *__restrict__ fdof, double *__restrict__ dgn) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dgn[i] = fdof[i] < 0 ? 0.0 : 4.0 / dt / dt * M[i] + 2.0 / dt * C[i];
+}
+
+__global__ void k_record_nm(int nrec, const int32_t *__restrict__ rec, const double *__restrict__ U,
+                            double *__restrict__ out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrec) out[r] = U[rec[r]];
+}
+
+// ------------------------------------------------------------------ host side
+static constexpr int kVecGrid = 512, kVecBlock = 256;
+
+static OpArgs op_args(const Model &m) {
+    const TileSet &t = m.tiles;
+    OpArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.own_ptr = t.own_ptr.p;
+    a.halo_ptr = t.halo_ptr.p;
+    a.halo_node = t.halo_node.p;
+    a.elem_ptr = t.elem_ptr.p;
+    a.lconn = t.elem_lconn.p;
+    a.elam = t.elem_lam.p;
+    a.emu = t.elem_mu.p;
+    a.egeo = t.elem_geo.p;
+    a.geo = m.geo.p;
+    a.X = m.X.p;
+    a.ntiles = t.ntiles;
+    a.max_stage = t.max_stage;
+    a.max_own = t.max_own;
+    a.th = m.th;
+    a.jac = m.jac.p;
+    a.dgn = m.dgn.p;
+    a.scal = m.scal.p;
+    return a;
+}
+
+static size_t op_lds(const Model &m, int mode) {
+    const int staged = (mode == OP_RHS ? 2 : 1) + (m.elem_path < 2 ? 1 : 0);
+    return (size_t(staged) * m.tiles.max_stage * m.ndim + size_t(m.nen) * m.ndim * m.tiles.max_own) * sizeof(double);
+}
+
+template <int DIM, int MODE>
+static const void *op_fn_t(int path) {
+    switch (path) {
+        case 3: return reinterpret_cast<const void *>(&k_nm_op<DIM, 3, MODE>);
+        case 2: return reinterpret_cast<const void *>(&k_nm_op<DIM, 2, MODE>);
+        case 1: return reinterpret_cast<const void *>(&k_nm_op<DIM, 1, MODE>);
+        default: return reinterpret_cast<const void *>(&k_nm_op<DIM, 0, MODE>);
+    }
+}
+
+static const void *op_fn(const Model &m, int mode) {
+    if (m.ndim == 3) return mode == OP_CG ? op_fn_t<3, OP_CG>(m.elem_path) : op_fn_t<3, OP_RHS>(m.elem_path);
+    return mode == OP_CG ? op_fn_t<2, OP_CG>(m.elem_path) : op_fn_t<2, OP_RHS>(m.elem_path);
+}
+
+static int launch_op(Model &m, int mode, const OpArgs &a) {
+    void *args[] = {const_cast<OpArgs *>(&a)};
+    SVLG_HIP(hipLaunchKernel(op_fn(m, mode), dim3(m.tiles.ntiles), dim3(m.tiles.block), args, op_lds(m, mode),
+                             m.stream));
+    return 0;
+}
+
+// y = Keff x (node-DOF layout) through the CG operator with beta = 0.
+static int keff_apply_node(Model &m, const double *x, double *y) {
+    hipLaunchKernelGGL(k_solver_reset, dim3(1), dim3(64), 0, m.stream, m.scal.p, 0.0, 0.0);
+    OpArgs a = op_args(m);
+    a.coef = m.coef_keff.p;
+    a.z = x;
+    a.p_old = x;
+    a.p_new = m.cg_p[1].p;
+    a.y = y;
+    a.part = m.part.p;
+    return launch_op(m, OP_CG, a);
+}
+
+// Solves Keff x = b (node-DOF layout; x holds the initial guess on entry).
+static int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int *iters, double *relres) {
+    hipStream_t s = m.stream;
+    const int n = m.ndofn;
+    hipLaunchKernelGGL(k_solver_reset, dim3(1), dim3(64), 0, s, m.scal.p, rtol * rtol, double(maxit));
+    OpArgs a = op_args(m);
+    a.coef = m.coef_keff.p;
+    a.part = m.part.p;
+    a.y = m.cg_q.p;
+    // q = Keff x0 (x0 staged as z with beta = 0; cg_p[1] receives a copy of x0).
+    a.z = x;
+    a.p_old = x;
+    a.p_new = m.cg_p[1].p;
+    if (int rc = launch_op(m, OP_CG, a)) return rc;
+    hipLaunchKernelGGL(k_cg_init, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, b, m.cg_q.p, m.jac.p, m.cg_r.p, m.cg_z.p,
+                       m.part.p);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, int(RED_INIT), kVecGrid, 3, m.part.p, m.scal.p);
+    double *vpart = m.part.p + m.tiles.ntiles;
+    int cur = 1;  // p_old of the first iteration (beta = 0 there, so p = z)
+    double hs[SC_COUNT];
+    a.z = m.cg_z.p;
+    for (int it = 0; it < maxit; ++it) {
+        a.p_old = m.cg_p[cur].p;
+        a.p_new = m.cg_p[1 - cur].p;
+        if (int rc = launch_op(m, OP_CG, a)) return rc;
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, int(RED_PQ), m.tiles.ntiles, 1, m.part.p, m.scal.p);
+        hipLaunchKernelGGL(k_cg_update, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, m.scal.p, x, m.cg_p[1 - cur].p,
+                           m.cg_q.p, m.cg_r.p, m.cg_z.p, m.jac.p, vpart);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, int(RED_RZ), kVecGrid, 3, vpart, m.scal.p);
+        cur = 1 - cur;
+        // Iterations queued after convergence are no-ops on the device (every kernel checks
+        // the done flag), so the host polls only every 16 iterations.
+        if ((it & 15) == 15 || it == maxit - 1) {
+            SVLG_HIP(hipMemcpyAsync(hs, m.scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
+            SVLG_HIP(hipStreamSynchronize(s));
+            if (hs[SC_DONE] != 0.0) break;
+        }
+    }
+    hipLaunchKernelGGL(k_cg_zero_rhs, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, m.scal.p, x);
+    SVLG_HIP(hipGetLastError());
+    SVLG_HIP(hipMemcpyAsync(hs, m.scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
+    SVLG_HIP(hipStreamSynchronize(s));
+    if (iters) *iters = int(hs[SC_ITERS]);
+    const double rel = hs[SC_BB] > 0.0 ? std::sqrt(hs[SC_RR] / hs[SC_BB]) : 0.0;
+    if (relres) *relres = rel;
+    if (!(rel <= rtol)) {
+        set_error("conjugate gradient did not reach the tolerance within the iteration limit");
+        return 12;
+    }
+    return 0;
+}
+
+}  // namespace svlg
+
+using namespace svlg;
+
+extern "C" {
+
+int svlg_nm_init(svlg_model *h, double dt, double rtol, int32_t maxit, int32_t precond) {
+    SVLG_CHECK(h, "null model");
+    Model &m = h->m;
+    SVLG_CHECK(dt > 0.0 && rtol > 0.0 && maxit > 0, "dt, rtol and maxit must be positive");
+    SVLG_CHECK(m.lumped, "the device Newmark path needs MassFormulation = LUMPED");
+    SVLG_CHECK(m.variant == 3, "the device Newmark path needs corner-slot node tiles (a corner-unique mesh)");
+    SVLG_CHECK(precond == SVLG_PRECOND_JACOBI, "only the Jacobi preconditioner is available");
+    for (int mode : {OP_CG, OP_RHS})
+        SVLG_CHECK(op_lds(m, mode) <= 160 * 1024, "Newmark operator tile needs more than 160 KB of LDS; use smaller tiles");
+    hipStream_t s = m.stream;
+    const int n = m.ndofn;
+    m.nm_ready = false;
+    m.nm_dt = dt;
+    m.nm_rtol = rtol;
+    m.nm_maxit = maxit;
+    m.nm_precond = precond;
+    for (DevBuf<double> *b : {&m.nU, &m.nV, &m.nA, &m.ndU, &m.dgn, &m.jac, &m.cg_r, &m.cg_z, &m.cg_q, &m.cg_b,
+                              &m.cg_p[0], &m.cg_p[1]}) {
+        SVLG_HIP(b->alloc(n));
+        SVLG_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * n, s));
+    }
+    SVLG_HIP(m.part.alloc(size_t(m.tiles.ntiles) + 3 * kVecGrid));
+    SVLG_HIP(m.scal.alloc(SC_COUNT));
+    SVLG_HIP(hipMemsetAsync(m.scal.p, 0, sizeof(double) * SC_COUNT, s));
+    // Per tile-element coefficients: 1 + 2 ak/dt (operator) and -ak (right-hand side).
+    std::vector<double> ak_tag(m.nelem);
+    SVLG_HIP(hipMemcpyAsync(ak_tag.data(), m.ak.p, sizeof(double) * m.nelem, hipMemcpyDeviceToHost, s));
+    SVLG_HIP(hipStreamSynchronize(s));
+    std::vector<double> ck(m.tile_elem_id.size()), cr(m.tile_elem_id.size());
+    for (size_t i = 0; i < ck.size(); ++i) {
+        const double ak = ak_tag[m.tile_elem_id[i]];
+        ck[i] = 1.0 + 2.0 * ak / dt;
+        cr[i] = -ak;
+    }
+    SVLG_HIP(m.coef_keff.upload(ck.data(), ck.size(), s));
+    SVLG_HIP(m.coef_rhs.upload(cr.data(), cr.size(), s));
+    // dgn = (4/dt^2) M + (2/dt) C_diag; Jacobi = 1 / diag(Keff).
+    const int nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_dgn, dim3(nb), dim3(256), 0, s, n, dt, m.mdiag.p, m.cdiag.p, m.fdof.p, m.dgn.p);
+    DevBuf<double> kd;
+    SVLG_HIP(kd.alloc(n));
+    SVLG_HIP(hipMemsetAsync(kd.p, 0, sizeof(double) * n, s));
+    for (int c = 0; c < m.ncolors; ++c) {
+        const int e0 = m.color_off[c], e1 = m.color_off[c + 1];
+        if (e1 <= e0) continue;
+        if (m.ndim == 3)
+            hipLaunchKernelGGL(k_kdiag_colored<3>, dim3((e1 - e0 + 127) / 128), dim3(128), 0, s, e0, e1, m.conn_c.p,
+                               m.eid_c.p, m.X.p, m.lam.p, m.mu.p, m.ak.p, 1.0 / dt, m.th, kd.p);
+        else
+            hipLaunchKernelGGL(k_kdiag_colored<2>, dim3((e1 - e0 + 127) / 128), dim3(128), 0, s, e0, e1, m.conn_c.p,
+                               m.eid_c.p, m.X.p, m.lam.p, m.mu.p, m.ak.p, 1.0 / dt, m.th, kd.p);
+    }
+    DevBuf<int> bad;
+    SVLG_HIP(bad.alloc(1));
+    SVLG_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(256), 0, s, n, m.fdof.p, kd.p, m.dgn.p, m.jac.p, bad.p);
+    SVLG_HIP(hipGetLastError());
+    int hb = 0;
+    SVLG_HIP(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    SVLG_HIP(hipStreamSynchronize(s));
+    SVLG_CHECK(hb == 0, "effective stiffness has a non-positive diagonal entry");
+    for (int mode : {OP_CG, OP_RHS}) {
+        const size_t lds = op_lds(m, mode);
+        if (lds > 64 * 1024)
+            SVLG_HIP(hipFuncSetAttribute(op_fn(m, mode), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    }
+    m.nm_steps_done = 0;
+    m.nm_ready = true;
+    return 0;
+}
+
+int svlg_nm_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const int32_t *rec_dofs, double *hist,
+                  int32_t *iters, double *relres) {
+    SVLG_CHECK(h, "null model");
+    Model &m = h->m;
+    SVLG_CHECK(m.nm_ready, "svlg_nm_init has not been called");
+    SVLG_CHECK(nsteps >= 0 && nrec >= 0, "negative count");
+    hipStream_t s = m.stream;
+    DevBuf<int32_t> rec;
+    DevBuf<double> hbuf;
+    if (nrec > 0 && nsteps > 0) {
+        SVLG_CHECK(rec_dofs && hist, "recorder arrays missing");
+        std::vector<int32_t> f2n(m.nfree, -1);
+        for (int i = 0; i < m.ndofn; ++i)
+            if (m.h_free[i] >= 0) f2n[m.h_free[i]] = i;
+        std::vector<int32_t> rn(nrec);
+        for (int r = 0; r < nrec; ++r) {
+            SVLG_CHECK(rec_dofs[r] >= 0 && rec_dofs[r] < m.nfree, "recorded DOF is not a free DOF");
+            rn[r] = f2n[rec_dofs[r]];
+        }
+        SVLG_HIP(rec.upload(rn.data(), rn.size(), s));
+        SVLG_HIP(hbuf.alloc(size_t(nrec) * nsteps));
+    }
+    OpArgs a = op_args(m);
+    a.coef = m.coef_rhs.p;
+    a.U = m.nU.p;
+    a.V = m.nV.p;
+    a.A = m.nA.p;
+    a.mdiag = m.mdiag.p;
+    a.cdiag = m.cdiag.p;
+    a.dt = m.nm_dt;
+    a.b = m.cg_b.p;
+    a.nl_ptr = m.nl_ptr.p;
+    a.nl_idx = m.nl_idx.p;
+    a.load_dir = m.load_dir.p;
+    a.load_factor = m.load_factor.p;
+    a.load_series = m.load_series.p;
+    a.load_nt = m.load_nt;
+    const int nb = (m.ndofn + 255) / 256;
+    for (int st = 0; st < nsteps; ++st) {
+        // Feff (ExtendedNewmarkBeta::ComputeEffectiveForce with the Linear algorithm's dU = 0).
+        a.k = k0 + st;
+        if (int rc = launch_op(m, OP_RHS, a)) return rc;
+        // Keff dU = Feff; the previous increment is the initial guess.
+        int it = 0;
+        double rel = 0.0;
+        const int rc = cg_solve(m, m.cg_b.p, m.ndU.p, m.nm_rtol, m.nm_maxit, &it, &rel);
+        if (iters) iters[st] = it;
+        if (relres) relres[st] = rel;
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_nm_update, dim3(nb), dim3(256), 0, s, m.ndofn, m.nm_dt, m.ndU.p, m.jac.p, m.nU.p, m.nV.p,
+                           m.nA.p);
+        if (nrec > 0)
+            hipLaunchKernelGGL(k_record_nm, dim3((nrec + 63) / 64), dim3(64), 0, s, nrec, rec.p, m.nU.p,
+                               hbuf.p + size_t(st) * nrec);
+        m.nm_steps_done++;
+    }
+    SVLG_HIP(hipGetLastError());
+    if (nrec > 0 && nsteps > 0)
+        SVLG_HIP(hipMemcpyAsync(hist, hbuf.p, sizeof(double) * size_t(nrec) * nsteps, hipMemcpyDeviceToHost, s));
+    SVLG_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int svlg_nm_get_state(svlg_model *h, double *U, double *V, double *A) {
+    SVLG_CHECK(h, "null model");
+    Model &m = h->m;
+    SVLG_CHECK(m.nm_ready, "svlg_nm_init has not been called");
+    DevBuf<double> tmp;
+    SVLG_HIP(tmp.alloc(std::max(m.nfree, 1)));
+    double *outs[3] = {U, V, A};
+    const double *src[3] = {m.nU.p, m.nV.p, m.nA.p};
+    for (int i = 0; i < 3; ++i) {
+        if (!outs[i]) continue;
+        if (int rc = gather_free(m, src[i], tmp.p)) return rc;
+        SVLG_HIP(hipMemcpyAsync(outs[i], tmp.p, sizeof(double) * m.nfree, hipMemcpyDeviceToHost, m.stream));
+        SVLG_HIP(hipStreamSynchronize(m.stream));
+    }
+    return 0;
+}
+
+int svlg_keff_apply(svlg_model *h, const double *xfree, double *yfree) {
+    SVLG_CHECK(h, "null model");
+    Model &m = h->m;
+    SVLG_CHECK(m.nm_ready, "svlg_nm_init has not been called");
+    SVLG_CHECK(xfree && yfree, "null vector");
+    DevBuf<double> xf, xn, yn;
+    SVLG_HIP(xf.upload(xfree, m.nfree, m.stream));
+    SVLG_HIP(xn.alloc(m.ndofn));
+    SVLG_HIP(yn.alloc(m.ndofn));
+    SVLG_HIP(hipMemsetAsync(xn.p, 0, sizeof(double) * m.ndofn, m.stream));
+    if (int rc = scatter_free(m, xf.p, xn.p)) return rc;
+    if (int rc = keff_apply_node(m, xn.p, yn.p)) return rc;
+    if (int rc = gather_free(m, yn.p, xf.p)) return rc;
+    SVLG_HIP(hipMemcpyAsync(yfree, xf.p, sizeof(double) * m.nfree, hipMemcpyDeviceToHost, m.stream));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+int svlg_solve(svlg_model *h, const double *bfree, double *xfree, double rtol, int32_t maxit, int32_t *iters,
+               double *relres) {
+    SVLG_CHECK(h, "null model");
+    Model &m = h->m;
+    SVLG_CHECK(m.nm_ready, "svlg_nm_init has not been called");
+    SVLG_CHECK(bfree && xfree, "null vector");
+    SVLG_CHECK(rtol > 0.0 && maxit > 0, "rtol and maxit must be positive");
+    DevBuf<double> f, bn, xn;
+    SVLG_HIP(f.upload(bfree, m.nfree, m.stream));
+    SVLG_HIP(bn.alloc(m.ndofn));
+    SVLG_HIP(xn.alloc(m.ndofn));
+    SVLG_HIP(hipMemsetAsync(bn.p, 0, sizeof(double) * m.ndofn, m.stream));
+    SVLG_HIP(hipMemsetAsync(xn.p, 0, sizeof(double) * m.ndofn, m.stream));
+    if (int rc = scatter_free(m, f.p, bn.p)) return rc;
+    int it = 0;
+    double rel = 0.0;
+    const int rc = cg_solve(m, bn.p, xn.p, rtol, maxit, &it, &rel);
+    if (iters) *iters = it;
+    if (relres) *relres = rel;
+    if (rc) return rc;
+    if (int rc2 = gather_free(m, xn.p, f.p)) return rc2;
+    SVLG_HIP(hipMemcpyAsync(xfree, f.p, sizeof(double) * m.nfree, hipMemcpyDeviceToHost, m.stream));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -488,6 +488,7 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
             }
         }
         UP(t.elem_geo, eg.data(), eg.size());
+        m.tile_elem_id = ht.elem_id;
         UP(t.own_ptr, ht.own_ptr.data(), ht.own_ptr.size());
         UP(t.nplain, ht.nplain.data(), ht.nplain.size());
         UP(t.halo_ptr, ht.halo_ptr.data(), ht.halo_ptr.size());

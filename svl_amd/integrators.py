@@ -5,6 +5,8 @@ semantics follow SVL (RP = 02-Run_Process):
   * Assembler            RP/07-Assembler/Assembler.hpp:50-181
   * Integrator           RP/10-Integrators/Integrator.hpp:53-144
   * CentralDifference    RP/10-Integrators/02-CentralDifference/CentralDifference.hpp:63
+  * ExtendedNewmarkBeta  RP/10-Integrators/03-Newmark/ExtendedNewmarkBeta.cpp (G = 0)
+  * LinearSystem         RP/11-Solvers/LinearSystem.hpp:44-66
   * DynamicAnalysis      RP/08-Analysis/02-Dynamic/DynamicAnalysis.cpp:25-64
 Every numerical call goes through libsvlg (HIP on gfx950); a failing call raises
 :class:`SvlgError` or, where SVL returns ``bool stop``, returns ``True``.
@@ -145,6 +147,110 @@ class CentralDifference:
 
     def GetAccelerations(self) -> np.ndarray:
         return self._state()[2]
+
+
+PRECOND_JACOBI = 0
+PRECOND_LEARNED = 1
+
+
+class ExtendedNewmarkBeta:
+    """ExtendedNewmarkBeta(mesh, TimeStep, mtol, ktol, ftol) with the Linear algorithm on the
+    device: gamma = 1/2, beta = 1/4, Keff = K + 4/dt^2 M + 2/dt C applied matrix-free and
+    solved by preconditioned CG to ``||b - Keff x|| <= rtol ||b||`` (SVL factors Keff
+    directly; the tolerance stands in for the direct solve's round-off)."""
+
+    def __init__(self, mesh: DeviceModel, TimeStep: float, mtol: float = 1e-12, ktol: float = 1e-12,
+                 ftol: float = 1e-12, rtol: float = 1e-10, maxit: int = 5000, precond: int = PRECOND_JACOBI):
+        self.mesh = mesh
+        self.dt = float(TimeStep)
+        self.rtol = float(rtol)
+        self.maxit = int(maxit)
+        self.precond = int(precond)
+        self.iters = np.zeros(0, np.int32)
+        self.relres = np.zeros(0)
+
+    def Initialize(self):
+        capi.check(capi.lib().svlg_nm_init(self.mesh.h, self.dt, self.rtol, self.maxit, self.precond),
+                   "svlg_nm_init")
+
+    def ComputeNewStep(self, k: int) -> bool:
+        """One step with load sample k; returns SVL's stop flag."""
+        it = np.zeros(1, np.int32)
+        rr = np.zeros(1)
+        rc = capi.lib().svlg_nm_steps(self.mesh.h, int(k), 1, 0, None, None, capi.iptr(it), capi.fptr(rr))
+        self.iters, self.relres = it, rr
+        return rc != 0
+
+    def advance(self, k0: int, nsteps: int, rec_dofs=None) -> np.ndarray | None:
+        """nsteps steps from step index k0; returns U history [nsteps, nrec] if recording.
+        CG iteration counts and final relative residuals land in ``iters`` / ``relres``."""
+        it = np.zeros(max(nsteps, 1), np.int32)
+        rr = np.zeros(max(nsteps, 1))
+        if rec_dofs is None or len(rec_dofs) == 0:
+            rc = capi.lib().svlg_nm_steps(self.mesh.h, int(k0), int(nsteps), 0, None, None, capi.iptr(it),
+                                          capi.fptr(rr))
+            hist = None
+        else:
+            rec = np.ascontiguousarray(rec_dofs, dtype=np.int32)
+            hist = np.zeros((nsteps, rec.size))
+            rc = capi.lib().svlg_nm_steps(self.mesh.h, int(k0), int(nsteps), int(rec.size), capi.iptr(rec),
+                                          capi.fptr(hist), capi.iptr(it), capi.fptr(rr))
+        self.iters, self.relres = it[:nsteps], rr[:nsteps]
+        capi.check(rc, "svlg_nm_steps")
+        return hist
+
+    def _state(self):
+        n = self.mesh.nfree
+        U, V, A = np.zeros(n), np.zeros(n), np.zeros(n)
+        capi.check(capi.lib().svlg_nm_get_state(self.mesh.h, capi.fptr(U), capi.fptr(V), capi.fptr(A)),
+                   "svlg_nm_get_state")
+        return U, V, A
+
+    def GetDisplacements(self) -> np.ndarray:
+        return self._state()[0]
+
+    def GetVelocities(self) -> np.ndarray:
+        return self._state()[1]
+
+    def GetAccelerations(self) -> np.ndarray:
+        return self._state()[2]
+
+
+class LinearSystem:
+    """The effective-stiffness system of an initialised :class:`ExtendedNewmarkBeta`
+    (LinearSystem::SetMatrix / SetVector / SolveSystem / GetSolution)."""
+
+    def __init__(self, integrator: ExtendedNewmarkBeta):
+        self.mesh = integrator.mesh
+        self.rtol = integrator.rtol
+        self.maxit = integrator.maxit
+        self._b = None
+        self._x = None
+        self.iters = 0
+        self.relres = 0.0
+
+    def Apply(self, x: np.ndarray) -> np.ndarray:
+        """y = Keff x (free DOFs)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.mesh.nfree)
+        capi.check(capi.lib().svlg_keff_apply(self.mesh.h, capi.fptr(x), capi.fptr(y)), "svlg_keff_apply")
+        return y
+
+    def SetVector(self, b: np.ndarray):
+        self._b = np.ascontiguousarray(b, dtype=np.float64)
+
+    def SolveSystem(self) -> bool:
+        x = np.zeros(self.mesh.nfree)
+        it = np.zeros(1, np.int32)
+        rr = np.zeros(1)
+        rc = capi.lib().svlg_solve(self.mesh.h, capi.fptr(self._b), capi.fptr(x), self.rtol, self.maxit,
+                                   capi.iptr(it), capi.fptr(rr))
+        self.iters, self.relres = int(it[0]), float(rr[0])
+        self._x = x
+        return rc != 0
+
+    def GetSolution(self) -> np.ndarray:
+        return self._x
 
 
 class DynamicAnalysis:

@@ -13,6 +13,13 @@
   adjacent element's rho * V * h^2; vertical Ricker f0 = 3 Hz, t0 = 0.4 s at the
   surface centre; dt = 0.5 min_e(h / Vp_e). At 100 x 100 x 50 elements of 2 m this is
   the 1,560,753-DOF half-space.
+* ``block3d_implicit`` -- 3D elastic block of lin3DHexa8 elements of h m, base fixed,
+  per-element Vs ~ U[200, 1000] m/s drawn with ``numpy.random.default_rng(seed)`` in
+  element-tag order, nu = 0.3, rho = 2000, Rayleigh damping of 5 % at 1 and 10 Hz
+  (am = 2 zeta w1 w2 / (w1 + w2) = 0.5712, ak = 2 zeta / (w1 + w2) = 1.447e-3), lumped
+  mass, vertical Ricker f0 = 5 Hz (t0 = 0.25 s, Ap = 1e6 N) at the top centre node,
+  ExtendedNewmarkBeta with dt = 2e-3 s. At 64 x 64 x 64 this is the 811,200-DOF block
+  of 500 implicit steps.
 
 Lysmer convention (Builder.py:1086-1131, per-node generalisation): a boundary node's
 dashpot on the boundary-normal direction takes rho Vp A, the tangential ones rho Vs A,
@@ -94,6 +101,36 @@ def halfspace3d(nx: int = 100, ny: int = 100, nz: int = 50, h: float = 2.0, nt: 
         dn, dd, de = lysmer_dashpots(3, bnd, names, eta)
         m.dash_node, m.dash_dir, m.dash_eta = dn, dd, de
     dt = cfl * float(np.min(h / vp))
+    t = np.arange(nt) * dt
+    src = m.nearest_node((0.0, 0.0, 0.0))
+    m.add_point_load(src, [0.0, 0.0, -1.0], ricker_velocity(t, t0, f0, Ap))
+    return m, dt, nt
+
+
+def rayleigh(zeta: float, f1: float, f2: float):
+    """(am, ak) giving damping ratio zeta at the frequencies f1 and f2 (Hz)."""
+    w1, w2 = 2.0 * math.pi * f1, 2.0 * math.pi * f2
+    return 2.0 * zeta * w1 * w2 / (w1 + w2), 2.0 * zeta / (w1 + w2)
+
+
+def block3d_implicit(nx: int = 64, ny: int = 64, nz: int = 64, h: float = 1.0, nt: int = 501,
+                     seed: int = 20260302, vs_range=(200.0, 1000.0), nu: float = 0.3, rho: float = 2000.0,
+                     zeta: float = 0.05, f0: float = 5.0, t0: float = 0.25, Ap: float = 1.0e6, dt: float = 2e-3):
+    """Returns (model, dt, nt)."""
+    Lx, Ly, Lz = nx * h, ny * h, nz * h
+    P0 = (-Lx / 2.0, -Ly / 2.0, -Lz)
+    coords, conn, bnd = make_domain_volume(nx, ny, nz, P0, (Lx / 2.0, -Ly / 2.0, -Lz),
+                                           (-Lx / 2.0, Ly / 2.0, -Lz), (-Lx / 2.0, -Ly / 2.0, 0.0))
+    ne = conn.shape[0]
+    rng = np.random.default_rng(seed)
+    vs = rng.uniform(vs_range[0], vs_range[1], ne)
+    E = 2.0 * rho * vs * vs * (1.0 + nu)
+    restrained = np.zeros(coords.shape, bool)
+    restrained[bnd["bottom"]] = True
+    am, ak = rayleigh(zeta, 1.0, 10.0)
+    m = Model(ndim=3, coords=coords, node_tags=tags_for(coords.shape[0]), elem_type=ELEM_HEXA8, conn=conn,
+              E=E, nu=np.full(ne, nu), rho=np.full(ne, rho), restrained=restrained, ray_am=np.full(ne, am),
+              ray_ak=np.full(ne, ak), name=f"block3d_{nx}x{ny}x{nz}")
     t = np.arange(nt) * dt
     src = m.nearest_node((0.0, 0.0, 0.0))
     m.add_point_load(src, [0.0, 0.0, -1.0], ricker_velocity(t, t0, f0, Ap))
