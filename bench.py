@@ -119,6 +119,52 @@ def cpu_baseline(m, dt, steps):
                       f"same {m.name} model, time loop only ({secs:.2f} s)"}
 
 
+def implicit_bench(args):
+    """Implicit ExtendedNewmarkBeta step time on the 64^3 block: Jacobi-PCG against flexible CG
+    with the trained graph-convolution preconditioner, both to the same relative residual."""
+    import numpy as np
+    from svl_amd import workloads as W
+    from svl_amd.integrators import DeviceModel, ExtendedNewmarkBeta, LearnedPreconditioner
+    n = args.implicit_ne
+    m, dt, nt = W.block3d_implicit(n, n, n, nt=max(501, 2 * args.implicit_steps + 10))
+    dm = DeviceModel(m)
+    nm = ExtendedNewmarkBeta(dm, dt, rtol=args.implicit_rtol, maxit=5000)
+    out = {"workload": f"block3d lin3DHexa8 {n}x{n}x{n} h=1m, Vs~U[200,1000] (seed 20260302), Rayleigh 5% at "
+                       f"1 and 10 Hz, fixed base, Ricker point load, ExtendedNewmarkBeta dt={dt}",
+           "nfree": m.nfree, "rtol": args.implicit_rtol, "steps": args.implicit_steps}
+
+    def timed(label):
+        dm.sync()
+        t = time.perf_counter()
+        nm.advance(1, args.implicit_steps)
+        el = (time.perf_counter() - t) / args.implicit_steps
+        its = nm.iters.astype(float)
+        print(f"# implicit {label}: {el * 1e3:.3f} ms/step, CG iterations mean {its.mean():.2f}",
+              file=sys.stderr, flush=True)
+        return {"s_per_step": el, "iters_mean": float(its.mean()), "iters_max": int(its.max()),
+                "relres_max": float(nm.relres.max())}
+
+    nm.Initialize()
+    nm.advance(1, 2)
+    nm.Initialize()
+    out["jacobi_pcg"] = timed("Jacobi-PCG")
+    nm.Initialize()
+    t0 = time.perf_counter()
+    pc = LearnedPreconditioner(nm, channels=args.gnn_channels, layers=args.gnn_layers, mfma=-1)
+    loss = pc.train(nsteps=24, every=3, sweeps=2, epochs=args.gnn_epochs, lr=2e-3)
+    train_s = time.perf_counter() - t0
+    pc.use(True)
+    info = pc.info()
+    lf = timed("learned FCG")
+    lf.update({"channels": info["channels"], "layers": info["layers"], "mfma_bf16": bool(info["mfma"]),
+               "train_s": train_s, "loss_first": float(loss[0]) if len(loss) else None,
+               "loss_last": float(loss[-1]) if len(loss) else None})
+    out["learned_fcg"] = lf
+    out["speedup_vs_jacobi"] = out["jacobi_pcg"]["s_per_step"] / lf["s_per_step"]
+    dm.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +177,13 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0,
                     help="0 auto, 1 fused + colour phases, 2 element-colored, 3 fused + corner slots")
+    ap.add_argument("--implicit-steps", type=int, default=30)
+    ap.add_argument("--implicit-ne", type=int, default=64, help="elements per side of the implicit block")
+    ap.add_argument("--implicit-rtol", type=float, default=1e-8)
+    ap.add_argument("--gnn-channels", type=int, default=1)
+    ap.add_argument("--gnn-layers", type=int, default=6)
+    ap.add_argument("--gnn-epochs", type=int, default=40)
+    ap.add_argument("--no-implicit", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="only run the device copy of MB megabytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -198,6 +251,8 @@ def main():
                      "alg_bytes_per_step": bytes_step},
     }
     dm.close()
+    if not args.no_implicit:
+        out["implicit"] = implicit_bench(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, dt, args.cpu_steps)
         out["cpu_baseline"]["cores"] = 1

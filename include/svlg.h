@@ -81,7 +81,10 @@ typedef struct svlg_exec_opts {
     int32_t geo_table;    /* parallelepipeds: 0 = share (J^-1, |J|) through a table of distinct
                            * geometries when it holds at most 65535 entries, -1 = never (stage
                            * node coordinates in LDS instead) */
-    int32_t reserved[4];
+    int32_t elem_precision; /* explicit element step: 0 = FP64 (the parity path), 1 = BF16 MFMA
+                           * with FP32 accumulation (Hexa8 meshes with one element geometry and
+                           * corner-slot tiles; displacements, forces and updates stay FP64) */
+    int32_t reserved[3];
 } svlg_exec_opts;
 
 /* Error string of the last failing call on this thread. */
@@ -111,8 +114,9 @@ int svlg_plan_tiles(const svlg_mesh_desc *desc, int32_t tile_nodes, int64_t *sta
 /* info[0]=ncolors info[1]=nelem info[2]=nnodes info[3]=nfree info[4]=explicit kernel
  * variant (1 fused + colour phases, 2 element-colored, 3 fused + corner slots) info[5]=ntiles info[6]=element path
  * of the fused kernel (0 Gauss points, 1 modal with staged coordinates, 2 modal with
- * the geometry table, 3 modal with a table of diagonal J^-1: axis-aligned elements) info[7]=dynamic LDS bytes of the fused kernel */
-int svlg_model_info(svlg_model *m, int64_t *info8);
+ * the geometry table, 3 modal with a table of diagonal J^-1: axis-aligned elements) info[7]=dynamic LDS bytes of the fused kernel
+ * info[8]=explicit element precision (0 FP64, 1 BF16 MFMA) info[9]=distinct element geometries */
+int svlg_model_info(svlg_model *m, int64_t *info10);
 /* Element color of every element (ascending-tag greedy coloring; no two elements of
  * one color share a node). */
 int svlg_element_colors(svlg_model *m, int32_t *colors /*[nelem]*/);
@@ -164,11 +168,43 @@ int svlg_nm_steps(svlg_model *m, int32_t k0, int32_t nsteps, int32_t nrec, const
                   double *hist, int32_t *iters, double *relres);
 /* Current U, V, A in free DOFs (ExtendedNewmarkBeta::GetDisplacements/...). */
 int svlg_nm_get_state(svlg_model *m, double *U, double *V, double *A);
+/* Switches the preconditioner of later solves (SVLG_PRECOND_LEARNED needs svlg_gnn_init;
+ * it runs flexible CG). No SVL counterpart: SVL selects a LinearSystem at start-up
+ * (Driver.hpp:1768-1783). */
+int svlg_nm_set_precond(svlg_model *m, int32_t precond);
 /* y = Keff x over the free DOFs (the matrix of LinearSystem::SetMatrix, applied). */
 int svlg_keff_apply(svlg_model *m, const double *x /*[nfree]*/, double *y /*[nfree]*/);
 /* Solves Keff x = b from x = 0 (LinearSystem::SetVector + SolveSystem + GetSolution). */
 int svlg_solve(svlg_model *m, const double *b /*[nfree]*/, double *x /*[nfree]*/, double rtol,
                int32_t maxit, int32_t *iters, double *relres);
+
+/* ---- Learned preconditioner (no SVL counterpart; it replaces the factorisation inside
+ * LinearSystem::SolveSystem with a trained preconditioner for flexible CG) ------------
+ * A linear graph convolution network over Ehat = D^-1/2 Keff D^-1/2 (D = diag Keff):
+ *   x0 = s r, H^1 = x0 c_1^T, H^l = H^(l-1) A_l + (Ehat H^(l-1)) B_l + x0 c_l^T, z = s (H^L w)
+ * with s = D^-1/2, C channels and L layers. theta = for l = 1..L [A_l (CxC) | B_l (CxC) |
+ * c_l (C)], then w (C); size L (2C^2 + C) + C. Needs svlg_nm_init (its dt fixes Keff). */
+/* channels in [1,4], layers in [2,16]; mfma: 1 = BF16 MFMA element step (Hexa8, one element
+ * geometry), 0 = FP64 element routine, -1 = MFMA when possible; theta NULL = Chebyshev
+ * initialisation on power-iteration bounds of the spectrum of Ehat. */
+int svlg_gnn_init(svlg_model *m, int32_t channels, int32_t layers, int32_t mfma, const double *theta);
+/* info[0]=channels [1]=layers [2]=MFMA in use [3]=parameter count; bounds = (lmin, lmax). */
+int svlg_gnn_info(svlg_model *m, int32_t *info4, double *bounds2);
+int svlg_gnn_get_params(svlg_model *m, double *theta);
+int svlg_gnn_set_params(svlg_model *m, const double *theta);
+/* z = P r over the free DOFs (mfma selects the element step of this call). */
+int svlg_gnn_apply(svlg_model *m, int32_t mfma, const double *r /*[nfree]*/, double *z /*[nfree]*/);
+/* Mean energy-norm loss ||P r_p - e_p||^2_Keff / ||e_p||^2_Keff over npairs pairs (r_p, e_p),
+ * [npairs][nfree] each with Keff e_p = r_p, and its gradient w.r.t. theta (FP64 element
+ * routine, hand-written backward kernels). grad may be NULL. */
+int svlg_gnn_loss_grad(svlg_model *m, int32_t npairs, const double *r, const double *e, double *loss,
+                       double *grad);
+/* Trains theta with Adam (learning rate lr) for `epochs` full-batch epochs on pairs from
+ * the model's own Newmark solves: steps 1..nsteps from rest, every `every`-th step gives
+ * (Feff, dU) plus `sweeps` Jacobi-smoothed pairs. The state is back at rest afterwards.
+ * loss_hist[epochs] may be NULL. */
+int svlg_gnn_train(svlg_model *m, int32_t nsteps, int32_t every, int32_t sweeps, int32_t epochs, double lr,
+                   double *loss_hist);
 
 /* Device-to-device copy of n doubles with a plain coalesced kernel on `stream` (a
  * hipStream_t, NULL = default stream). No counterpart in SVL: it gives the bytes that

@@ -73,6 +73,8 @@ def _lib():
         L.oracle_internal_force.argtypes = [C.POINTER(MeshDesc), f64p, C.c_double, f64p]
         L.oracle_diagonals.restype = C.c_int
         L.oracle_diagonals.argtypes = [C.POINTER(MeshDesc), f64p, f64p]
+        L.oracle_newmark_keff_dense.restype = C.c_int
+        L.oracle_newmark_keff_dense.argtypes = [C.POINTER(MeshDesc), C.c_double, f64p]
         L.oracle_newmark_keff_apply.restype = C.c_int
         L.oracle_newmark_keff_apply.argtypes = [C.POINTER(MeshDesc), C.c_double, f64p, f64p]
         _L = L
@@ -136,6 +138,18 @@ def keff_apply(model, dt: float, x):
     y = np.zeros(model.nfree)
     _chk(_lib().oracle_newmark_keff_apply(C.byref(desc), dt, _f(x), _f(y)), "keff_apply")
     return y
+
+
+def keff_dense(model, dt: float):
+    """Dense Keff = T^T (K + 4/dt^2 M + 2/dt C) T over the free DOFs (small models only)."""
+    from svl_amd._capi import make_desc
+    desc, keep = make_desc(model)
+    n = model.nfree
+    if n > 6000:
+        raise ValueError("keff_dense is meant for models of at most 6000 free DOFs")
+    K = np.zeros((n, n))
+    _chk(_lib().oracle_newmark_keff_dense(C.byref(desc), dt, _f(K)), "keff_dense")
+    return K
 
 
 def diagonals(model):

@@ -923,6 +923,28 @@ int oracle_newmark_keff_apply(const svlg_mesh_desc *d, double dt, const double *
     }
 }
 
+// Dense T^T (K + 4/dt^2 M + 2/dt C) T ([nfree][nfree], row-major) for small models.
+int oracle_newmark_keff_dense(const svlg_mesh_desc *d, double dt, double *Kd) {
+    try {
+        Model m = build(d);
+        Assembler as{&m};
+        CSC K = as.stiffness();
+        CSC M = as.mass();
+        CSC C = as.damping();
+        CSC KM = combine(1.0, K, 4.0 / dt / dt, M);
+        CSC Kt = combine(1.0, KM, 2.0 / dt, C);
+        CSC Kf = free_block(m, Kt);
+        const size_t n = size_t(m.nfree);
+        std::fill(Kd, Kd + n * n, 0.0);
+        for (int j = 0; j < Kf.n; ++j)
+            for (int k = Kf.colptr[j]; k < Kf.colptr[j + 1]; ++k) Kd[size_t(Kf.row[k]) * n + j] += Kf.val[k];
+        return 0;
+    } catch (const std::exception &ex) {
+        g_err = ex.what();
+        return 3;
+    }
+}
+
 int oracle_diagonals(const svlg_mesh_desc *d, double *Mfree, double *Cfree) {
     try {
         Model m = build(d);

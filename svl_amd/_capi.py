@@ -53,7 +53,7 @@ class MeshDesc(C.Structure):
 
 class ExecOpts(C.Structure):
     _fields_ = [("tile_nodes", C.c_int32), ("block", C.c_int32), ("variant", C.c_int32),
-                ("geo_table", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("geo_table", C.c_int32), ("elem_precision", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 def _p(a, t):
@@ -130,6 +130,14 @@ SIGNATURES = {
     "svlg_nm_get_state": (C.c_int, [C.c_void_p, f64p, f64p, f64p]),
     "svlg_keff_apply": (C.c_int, [C.c_void_p, f64p, f64p]),
     "svlg_solve": (C.c_int, [C.c_void_p, f64p, f64p, C.c_double, C.c_int32, i32p, f64p]),
+    "svlg_nm_set_precond": (C.c_int, [C.c_void_p, C.c_int32]),
+    "svlg_gnn_init": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, f64p]),
+    "svlg_gnn_info": (C.c_int, [C.c_void_p, i32p, f64p]),
+    "svlg_gnn_get_params": (C.c_int, [C.c_void_p, f64p]),
+    "svlg_gnn_set_params": (C.c_int, [C.c_void_p, f64p]),
+    "svlg_gnn_apply": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p]),
+    "svlg_gnn_loss_grad": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p, f64p, f64p]),
+    "svlg_gnn_train": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, f64p]),
     "svlg_stream_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
 }
 

@@ -134,13 +134,34 @@ struct Model {
     DevBuf<double> cg_r, cg_z, cg_q, cg_b, cg_p[2];
     DevBuf<double> part;              // per-block partial sums
     DevBuf<double> scal;              // device scalars of the solver
+    DevBuf<unsigned> counters;        // arrival counters of the last-workgroup reductions
     std::vector<int32_t> tile_elem_id;   // tag index of every tile element (host)
+
+    // Learned graph-convolution preconditioner (svlg_gcn.hip). Hidden states are FP32
+    // planes [channel][ndofn]; parameters theta live on the host (FP64) and the device (FP32).
+    bool gcn_ready = false;
+    int gcn_C = 1, gcn_L = 1, gcn_mfma = 0;
+    double gcn_lmin = 0.0, gcn_lmax = 0.0;
+    std::vector<double> gcn_theta;
+    DevBuf<float> gcn_w;              // theta in FP32
+    DevBuf<double> gcn_s;             // D^-1/2 (0 on restrained DOFs)
+    DevBuf<float> gcn_sf;             // the same in FP32
+    DevBuf<double> gcn_elam, gcn_emu; // per tile element (lambda, mu) * (1 + 2 ak/dt)
+    DevBuf<float> gcn_kmat;           // [2][24][24] K_lambda, K_mu of the single geometry class (MFMA path)
+    DevBuf<float> gcn_x0;             // s * r
+    DevBuf<float> gcn_h[2];           // inference ping-pong hidden states
+    DevBuf<float> gcn_H, gcn_Y;       // training: H^l and Ehat H^(l-1) for every layer
+    DevBuf<float> gcn_G[2];           // training: backward ping-pong
+    DevBuf<double> gcn_part;          // training: per-tile gradient partials
+    DevBuf<double> gcn_tmp[3];        // training: FP64 work vectors (node-DOF layout)
 
     TileSet tiles;
     int variant = 1;                  // 1 fused node tiles, 2 element-colored
     int elem_path = 0;                // 0 Gauss points, 1 modal + staged X, 2 modal + geometry table
     int ngeo = 0;
     DevBuf<double> geo;               // [ngeo * ndim * ndim] inverse Jacobians of distinct geometries
+    int elem_bf16 = 0;                // explicit element step on BF16 MFMA (single geometry, Hexa8)
+    DevBuf<float> cd_kmat;            // its K_lambda, K_mu
 };
 
 }  // namespace svlg

@@ -608,4 +608,45 @@ __device__ __host__ inline double quad_center_geometry(const double (&x)[4][2], 
     return fabs(det);
 }
 
+// Element force f = K_e u for the operand u of one tile element; lam and mu carry any
+// per-element factor (and |J|-scaling on the geometry-table paths). PATH: 0 Gauss points on
+// staged coordinates sX, 1 modal form on staged coordinates, 2 modal form with the inverse
+// Jacobian from the geometry table, 3 the same with a diagonal inverse Jacobian.
+template <int DIM, int PATH>
+__device__ __forceinline__ void tile_elem_force(const double *geo_table, double th, const int (&lc)[Elem<DIM>::nen],
+                                                const double *sX,
+                                           double lam, double mu, int gid, const double (&u)[Elem<DIM>::nen][DIM],
+                                           double (&f)[Elem<DIM>::nen][DIM]) {
+    constexpr int NEN = Elem<DIM>::nen;
+    if constexpr (PATH >= 2) {
+        const double *g = geo_table + size_t(gid) * (DIM * DIM);
+        double Lam[DIM][DIM];
+#pragma unroll
+        for (int r = 0; r < DIM; ++r)
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) Lam[r][c] = g[r * DIM + c];
+        if constexpr (DIM == 3)
+            hexa_modal_force<PATH == 3>(Lam, lam, mu, u, f);
+        else
+            quad_modal_force<PATH == 3>(Lam, lam, mu, u, f);
+    } else {
+        double x[NEN][DIM];
+#pragma unroll
+        for (int c = 0; c < NEN; ++c)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) x[c][d] = sX[lc[c] * DIM + d];
+        if constexpr (DIM == 3) {
+            if constexpr (PATH == 1)
+                hexa_affine_fint(x, u, lam, mu, f);
+            else
+                hexa_fint(x, u, lam, mu, f);
+        } else {
+            if constexpr (PATH == 1)
+                quad_affine_fint(x, u, lam, mu, th, f);
+            else
+                quad_fint(x, u, lam, mu, th, f);
+        }
+    }
+}
+
 }  // namespace svlg

@@ -17,6 +17,7 @@
 
 #include "svlg_common.hpp"
 #include "svlg_elem.hpp"
+#include "svlg_mfma.hpp"
 
 namespace svlg {
 
@@ -92,6 +93,7 @@ struct FusedArgs {
     int load_nt, k;
     double th, ftol;
     int ntiles, ncolors, max_stage, max_own;
+    const float *kmat;  // PATH 4: K_lambda, K_mu of the single element geometry
 };
 
 __device__ __forceinline__ int color_of(const int32_t *__restrict__ cp, int i) {
@@ -334,7 +336,7 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
         up[c] = has_node ? a.Uprev[g0 + size_t(tid) * DIM + c] : 0.0;
         kd[c] = has_node ? a.keff[g0 + size_t(tid) * DIM + c] : 0.0;
     }
-    const bool has_elem = tid < ne;
+    const bool has_elem = PATH != 4 && tid < ne;
     int lc[NEN];
     double lam = 0.0, mu = 0.0;
     int gid = 0;
@@ -415,7 +417,12 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
         for (int i = tid; i < NEN * DIM * MO; i += B) sIn[i] = 0.0;
     }
     __syncthreads();
-    if (has_elem) {
+    if constexpr (PATH == 4) {
+        // BF16 MFMA element step (single element geometry), FP64 slots and update.
+        bf16x8 Ak[3];
+        mfma_load_kblocks(a.kmat, Ak);
+        mfma_hexa_elements<double>(a.lconn, a.elam, a.emu, e0, ne, nown, sU, sIn, MO, Ak, a.ftol);
+    } else if (has_elem) {
         double u[NEN][DIM], f[NEN][DIM];
 #pragma unroll
         for (int c = 0; c < NEN; ++c)
@@ -495,6 +502,7 @@ static const void *fused_fn(bool corner) {
 
 static const void *fused_kernel(const Model &m) {
     const bool cr = m.variant == 3;
+    if (m.elem_bf16) return reinterpret_cast<const void *>(&k_cd_corner<3, 4>);
     if (m.ndim == 3)
         switch (m.elem_path) {
             case 3: return fused_fn<3, 3>(cr);
@@ -561,7 +569,10 @@ static int launch_fused(Model &m, int k, const double *Ucur, const double *Uprev
         else                                                                                          \
             hipLaunchKernelGGL((k_cd_fused<D, P>), grid, block, t.lds_bytes, m.stream, a);           \
     } while (0)
-    if (m.ndim == 3) {
+    a.kmat = m.cd_kmat.p;
+    if (m.elem_bf16) {
+        hipLaunchKernelGGL((k_cd_corner<3, 4>), grid, block, t.lds_bytes, m.stream, a);
+    } else if (m.ndim == 3) {
         if (m.elem_path == 3) SVLG_LAUNCH_FUSED(3, 3);
         else if (m.elem_path == 2) SVLG_LAUNCH_FUSED(3, 2);
         else if (m.elem_path == 1) SVLG_LAUNCH_FUSED(3, 1);

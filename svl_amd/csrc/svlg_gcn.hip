@@ -1,0 +1,942 @@
+// svlg_gcn.hip -- learned preconditioner of the effective-stiffness solve: a linear graph
+// convolution network on the mesh's DOF graph whose propagation operator is the
+// Jacobi-normalised effective stiffness Ehat = D^-1/2 Keff D^-1/2 (D = diag Keff):
+//
+//   x0  = s * r / sigma                    s = D^-1/2 (0 on restrained DOFs), sigma = rms(r)
+//   H^1 = x0 c_1^T
+//   H^l = H^(l-1) A_l + (Ehat H^(l-1)) B_l + x0 c_l^T          l = 2..L
+//   z   = sigma s * (H^L w)
+//
+// H^l holds C channels per DOF. With C = 1, A_l = 1, B_l = -omega_l, c_l = omega_l, w = 1 the
+// network is L steps of Richardson iteration on Ehat y = x0 with step sizes omega_l; with the
+// omega_l the reciprocal Chebyshev nodes of [lmin, lmax] it is the Chebyshev polynomial
+// preconditioner, which is the initial state. Training (the forward and backward kernels
+// below) minimises the energy-norm error ||z - e||^2_Keff / ||e||^2_Keff over
+// residual/solution pairs (r, e), Keff e = r, that the model's own Newmark solves produce.
+// z is linear in r, but the map is not symmetric for C > 1, so the solver runs flexible CG.
+//
+// Ehat is applied on the node tiles of the Newmark operator: one workgroup stages s * H of
+// its owned + halo nodes, recomputes every element touching its nodes and sums corner slots
+// in a fixed order. The element step is either the FP64 element routine (any mesh) or, for
+// Hexa8 meshes with a single element geometry, a BF16 MFMA product with the element's
+// K_lambda and K_mu (FP32 accumulation): f_e = lambda_e K_lambda u_e + mu_e K_mu u_e.
+// Hidden states are FP32; the diagonal part, z and every CG scalar stay FP64.
+//
+// Parameter vector theta (FP64 on the host, FP32 on the device): for l = 1..L the block
+// [A_l (C x C, row = input channel) | B_l (C x C) | c_l (C)], then w (C). A_1 and B_1 are
+// unused (H^0 = 0).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "svlg_common.hpp"
+#include "svlg_elem.hpp"
+#include "svlg_mfma.hpp"
+#include "svlg_solver.hpp"
+
+namespace svlg {
+
+int gather_free(Model &m, const double *dnode, double *dfree);
+int scatter_free(Model &m, const double *dfree, double *dnode);
+int keff_apply_node(Model &m, const double *x, double *y);
+
+static constexpr int kMaxC = 4;
+static constexpr int kMaxL = 16;
+
+static inline int theta_layer(int C) { return 2 * C * C + C; }
+static inline int theta_size(int C, int L) { return L * theta_layer(C) + C; }
+
+enum { G_FWD = 0, G_BWD = 1, G_OUT = 2 };
+
+struct GcnArgs {
+    const int32_t *own_ptr, *halo_ptr, *halo_node, *elem_ptr;
+    const uint16_t *lconn, *egeo;
+    const double *elam, *emu;  // (lambda, mu) * (1 + 2 ak/dt) per tile element
+    const double *geo, *X;
+    double th;
+    int ntiles, max_stage, max_own;
+    const float *kmat;         // [2][24][24] (MFMA path)
+    const double *dgn, *s;
+    const float *sf;
+    int C, ndofn;
+    const float *in;           // [C][ndofn]
+    float *out;                // [C][ndofn]
+    float *ystore;             // G_FWD: Ehat in, if not null
+    const float *x0;
+    const float *M1, *M2, *v;  // out = in M1 + (Ehat in) M2 + x0 v^T (v may be null)
+    int trans;                 // use M1^T and M2^T
+    // G_BWD: weight-gradient partials per tile [2 C^2 + C]
+    const float *hprev, *yl;
+    double *gpart;
+    // G_OUT
+    const float *w;
+    const double *r, *q;
+    double *z, *part, *scal;
+    unsigned *counter;
+    int first, normalize;
+};
+
+__device__ __forceinline__ float mixw(const float *M, int C, int c, int k, int trans) {
+    return trans ? M[k * C + c] : M[c * C + k];
+}
+
+// sigma of the input scaling: rms of the residual (from the solver's r.r) or 1.
+__device__ __forceinline__ double gcn_sigma(const double *scal, int normalize, int n) {
+    if (!normalize) return 1.0;
+    const double rr = scal[SC_RR];
+    return rr > 0.0 ? sqrt(rr / double(n)) : 1.0;
+}
+
+// ---------------------------------------------------------------- element phase
+// FP64 element routine on the staged FP32 operand of one channel.
+template <int DIM, int PATH>
+__device__ __forceinline__ void gcn_elem_valu(const GcnArgs &a, int e0, int ne, int nown, const float *sXc,
+                                              const double *sCoord, double *sIn, int MO) {
+    constexpr int NEN = Elem<DIM>::nen;
+    const int tid = threadIdx.x;
+    if (tid >= ne) return;
+    int lc[NEN];
+    const uint16_t *l = a.lconn + size_t(e0 + tid) * NEN;
+#pragma unroll
+    for (int c = 0; c < NEN; ++c) lc[c] = l[c];
+    const double lam = a.elam[e0 + tid], mu = a.emu[e0 + tid];
+    const int gid = PATH >= 2 ? int(a.egeo[e0 + tid]) : 0;
+    double u[NEN][DIM], f[NEN][DIM];
+#pragma unroll
+    for (int c = 0; c < NEN; ++c)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) u[c][d] = double(sXc[lc[c] * DIM + d]);
+    tile_elem_force<DIM, PATH>(a.geo, a.th, lc, sCoord, lam, mu, gid, u, f);
+#pragma unroll
+    for (int c = 0; c < NEN; ++c)
+        if (lc[c] < nown) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) sIn[(c * DIM + d) * MO + lc[c]] = f[c][d];
+        }
+}
+
+// ---------------------------------------------------------------- layer kernel
+// One node tile of a network layer: stages s * in (every channel) for owned + halo nodes,
+// applies Ehat per channel, then mixes channels on the owned DOFs.
+//  G_FWD: out = in M1 + (Ehat in) M2 + x0 v^T; optionally stores Ehat in.
+//  G_BWD: out = in M1^T + (Ehat in) M2^T (trans = 1) and the per-tile gradient partials
+//         dA[c][k] = sum hprev_c in_k, dB[c][k] = sum yl_c in_k, dc[k] = sum x0 in_k.
+//  G_OUT: as G_FWD, then z = sigma s * (out w) (FP64), per-tile z.r and z.q; the last tile
+//         sets the flexible-CG scalars RHO = z.r and BETA = first ? 0 : -z.q / PQ.
+template <int DIM, int PATH, bool MFMA, int MODE>
+__global__ __launch_bounds__(512) void k_gcn_layer(GcnArgs a) {
+    constexpr int NEN = Elem<DIM>::nen;
+    constexpr bool STAGE_X = PATH < 2;
+    extern __shared__ double smem[];
+    __shared__ double red[17];
+    __shared__ int last;
+    if constexpr (MODE != G_BWD) {
+        if (a.scal && a.scal[SC_DONE] != 0.0) return;
+    }
+    const int nt = a.ntiles;
+    const int bI = blockIdx.x, xcd = bI & 7, j = bI >> 3;
+    const int q = nt >> 3, r = nt & 7;
+    const int tile = xcd * q + min(xcd, r) + j;
+    const int o0 = a.own_ptr[tile], nown = a.own_ptr[tile + 1] - o0;
+    const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
+    const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
+    const int MS = a.max_stage, MO = a.max_own, C = a.C, N = a.ndofn;
+    double *sIn = smem;                                              // [NEN*DIM][MO]
+    double *sCoord = sIn + size_t(NEN * DIM) * MO;                   // [MS*DIM] if STAGE_X
+    float *sX = reinterpret_cast<float *>(sCoord + (STAGE_X ? size_t(MS) * DIM : 0));  // [C][MS*DIM]
+    const int tid = threadIdx.x, B = blockDim.x;
+    const size_t g0 = size_t(o0) * DIM;
+    // Stage s * in for every channel (and the coordinates).
+    for (int i = tid; i < nown * DIM; i += B) {
+        const float sv = a.sf[g0 + i];
+        for (int c = 0; c < C; ++c) sX[c * MS * DIM + i] = sv * a.in[size_t(c) * N + g0 + i];
+        if constexpr (STAGE_X) sCoord[i] = a.X[g0 + i];
+    }
+    for (int sl = tid; sl < nh; sl += B) {
+        const size_t nd = size_t(a.halo_node[h0 + sl]) * DIM;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const int i = (nown + sl) * DIM + d;
+            const float sv = a.sf[nd + d];
+            for (int c = 0; c < C; ++c) sX[c * MS * DIM + i] = sv * a.in[size_t(c) * N + nd + d];
+            if constexpr (STAGE_X) sCoord[i] = a.X[nd + d];
+        }
+    }
+    bf16x8 Ak[3];
+    if constexpr (MFMA) mfma_load_kblocks(a.kmat, Ak);
+    double yv[kMaxC][DIM];
+    for (int c = 0; c < C; ++c) {
+        for (int i = tid; i < NEN * DIM * MO; i += B) sIn[i] = 0.0;
+        __syncthreads();
+        if constexpr (MFMA)
+            mfma_hexa_elements<float>(a.lconn, a.elam, a.emu, e0, ne, nown, sX + c * MS * DIM, sIn, MO, Ak, -1.0);
+        else
+            gcn_elem_valu<DIM, PATH>(a, e0, ne, nown, sX + c * MS * DIM, sCoord, sIn, MO);
+        __syncthreads();
+        if (tid < nown) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                double fi = 0.0;
+#pragma unroll
+                for (int cc = 0; cc < NEN; ++cc) fi += sIn[(cc * DIM + d) * MO + tid];
+                const size_t g = g0 + size_t(tid) * DIM + d;
+                const double sg = a.s[g];
+                const double xh = sg * double(a.in[size_t(c) * N + g]);
+                yv[c][d] = sg * (fi + a.dgn[g] * xh);
+            }
+        }
+        __syncthreads();
+    }
+    // Channel mixing on the owned DOFs.
+    double acc0 = 0.0, acc1 = 0.0;
+    constexpr int NP = 2 * kMaxC * kMaxC + kMaxC;
+    double gp[NP];
+    if constexpr (MODE == G_BWD) {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) gp[i] = 0.0;
+    }
+    if (tid < nown) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const size_t g = g0 + size_t(tid) * DIM + d;
+            float inv[kMaxC];
+            for (int c = 0; c < C; ++c) inv[c] = a.in[size_t(c) * N + g];
+            const float x0 = a.x0[g];
+            double zsum = 0.0;
+            for (int k = 0; k < C; ++k) {
+                double o = 0.0;
+                for (int c = 0; c < C; ++c)
+                    o += double(inv[c]) * mixw(a.M1, C, c, k, a.trans) + yv[c][d] * mixw(a.M2, C, c, k, a.trans);
+                if (MODE != G_BWD && a.v) o += double(x0) * a.v[k];
+                const float of = float(o);
+                a.out[size_t(k) * N + g] = of;
+                if constexpr (MODE == G_OUT) zsum += double(of) * a.w[k];
+            }
+            if constexpr (MODE == G_FWD) {
+                if (a.ystore)
+                    for (int c = 0; c < C; ++c) a.ystore[size_t(c) * N + g] = float(yv[c][d]);
+            }
+            if constexpr (MODE == G_BWD) {
+                // in = G^l here; hprev = H^(l-1), yl = Ehat H^(l-1).
+                for (int k = 0; k < C; ++k) {
+                    const double gk = inv[k];
+                    for (int c = 0; c < C; ++c) {
+                        gp[c * C + k] += double(a.hprev[size_t(c) * N + g]) * gk;
+                        gp[C * C + c * C + k] += double(a.yl[size_t(c) * N + g]) * gk;
+                    }
+                    gp[2 * C * C + k] += double(x0) * gk;
+                }
+            }
+            if constexpr (MODE == G_OUT) {
+                const double sig = gcn_sigma(a.scal, a.normalize, N);
+                const double zv = sig * a.s[g] * zsum;
+                a.z[g] = zv;
+                acc0 += zv * a.r[g];
+                acc1 += zv * a.q[g];
+            }
+        }
+    }
+    if constexpr (MODE == G_BWD) {
+        const int np = 2 * C * C + C;
+        for (int i = 0; i < np; ++i) {
+            const double sv = block_sum(gp[i], red);
+            if (tid == 0) a.gpart[size_t(tile) * np + i] = sv;
+        }
+    }
+    if constexpr (MODE == G_OUT) {
+        acc0 = block_sum(acc0, red);
+        acc1 = block_sum(acc1, red);
+        if (tid == 0) {
+            a.part[2 * tile] = acc0;
+            a.part[2 * tile + 1] = acc1;
+        }
+        if (arrive_last(a.counter, gridDim.x, &last)) {
+            const double zr = sum_partials(a.part, nt, 2, red);
+            const double zq = sum_partials(a.part + 1, nt, 2, red);
+            if (tid == 0) {
+                const double pq = a.scal[SC_PQ];
+                a.scal[SC_BETA] = (a.first || pq == 0.0) ? 0.0 : -zq / pq;
+                a.scal[SC_RHO] = zr;
+            }
+            release_counter(a.counter);
+        }
+    }
+}
+
+// x0 = s r / sigma, H^1 = x0 c_1^T.
+__global__ void k_gcn_first(int n, int C, int normalize, const double *__restrict__ r, const double *__restrict__ s,
+                            const float *__restrict__ c1, const double *scal, float *__restrict__ x0,
+                            float *__restrict__ h1) {
+    if (normalize && scal[SC_DONE] != 0.0) return;
+    const double sig = gcn_sigma(scal, normalize, n);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float xv = float(s[i] * r[i] / sig);
+        x0[i] = xv;
+        for (int k = 0; k < C; ++k) h1[size_t(k) * n + i] = xv * c1[k];
+    }
+}
+
+// Training: z = s * (H^L w).
+__global__ void k_gcn_out(int n, int C, const float *__restrict__ hL, const float *__restrict__ w,
+                          const double *__restrict__ s, double *__restrict__ z) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int k = 0; k < C; ++k) acc += double(hL[size_t(k) * n + i]) * w[k];
+        z[i] = s[i] * acc;
+    }
+}
+
+// Per-block partials of (z - e).(Keff z - r) and e.r (loss of one pair).
+__global__ __launch_bounds__(256) void k_gcn_lossdots(int n, const double *__restrict__ z, const double *__restrict__ e,
+                                                      const double *__restrict__ kz, const double *__restrict__ r,
+                                                      double *__restrict__ part) {
+    __shared__ double red[17];
+    double a0 = 0.0, a1 = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        a0 += (z[i] - e[i]) * (kz[i] - r[i]);
+        a1 += e[i] * r[i];
+    }
+    a0 = block_sum(a0, red);
+    a1 = block_sum(a1, red);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = a0;
+        part[2 * blockIdx.x + 1] = a1;
+    }
+}
+
+// Output gradient: gz = scale (Keff z - r); G^L_k = w_k s gz; partials dw_k = sum H^L_k s gz.
+__global__ __launch_bounds__(256) void k_gcn_outgrad(int n, int C, double scale, const double *__restrict__ kz,
+                                                     const double *__restrict__ r, const double *__restrict__ s,
+                                                     const float *__restrict__ w, const float *__restrict__ hL,
+                                                     float *__restrict__ G, double *__restrict__ part) {
+    __shared__ double red[17];
+    double dw[kMaxC] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double sg = s[i] * scale * (kz[i] - r[i]);
+        for (int k = 0; k < C; ++k) {
+            G[size_t(k) * n + i] = float(w[k] * sg);
+            dw[k] += double(hL[size_t(k) * n + i]) * sg;
+        }
+    }
+    for (int k = 0; k < C; ++k) {
+        const double v = block_sum(dw[k], red);
+        if (threadIdx.x == 0) part[size_t(blockIdx.x) * C + k] = v;
+    }
+}
+
+// Layer-1 gradient partials dc_1[k] = sum x0 G^1_k.
+__global__ __launch_bounds__(256) void k_gcn_first_grad(int n, int C, const float *__restrict__ x0,
+                                                        const float *__restrict__ G, double *__restrict__ part) {
+    __shared__ double red[17];
+    double dc[kMaxC] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int k = 0; k < C; ++k) dc[k] += double(x0[i]) * double(G[size_t(k) * n + i]);
+    for (int k = 0; k < C; ++k) {
+        const double v = block_sum(dc[k], red);
+        if (threadIdx.x == 0) part[size_t(blockIdx.x) * C + k] = v;
+    }
+}
+
+// grad[off + i] += sum over rows of part[row][i] (fixed row order), one workgroup per i.
+__global__ __launch_bounds__(256) void k_gcn_accum(int rows, int width, const double *__restrict__ part, double *grad,
+                                                   int off) {
+    __shared__ double red[17];
+    const int i = blockIdx.x;
+    double acc = 0.0;
+    for (int row = threadIdx.x; row < rows; row += blockDim.x) acc += part[size_t(row) * width + i];
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) grad[off + i] += acc;
+}
+
+// Columns of K_lambda and K_mu of geometry class 0 (unit Lame constants) by unit probes.
+__global__ void k_gcn_kmat(const double *__restrict__ geo, int diag, float *__restrict__ kmat) {
+    const int col = threadIdx.x;  // 0..47
+    if (col >= 48) return;
+    const int which = col / 24, k = col % 24;
+    double Lam[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int jj = 0; jj < 3; ++jj) Lam[i][jj] = geo[i * 3 + jj];
+    double u[8][3], f[8][3];
+    for (int c = 0; c < 8; ++c)
+        for (int d = 0; d < 3; ++d) u[c][d] = (c * 3 + d == k) ? 1.0 : 0.0;
+    const double lam = which == 0 ? 1.0 : 0.0, mu = which == 0 ? 0.0 : 1.0;
+    if (diag)
+        hexa_modal_force<true>(Lam, lam, mu, u, f);
+    else
+        hexa_modal_force<false>(Lam, lam, mu, u, f);
+    for (int c = 0; c < 8; ++c)
+        for (int d = 0; d < 3; ++d) kmat[which * 576 + (c * 3 + d) * 24 + k] = float(f[c][d]);
+}
+
+__global__ void k_gcn_scale(int n, const double *__restrict__ jac, double *__restrict__ s, float *__restrict__ sf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = jac[i] > 0.0 ? sqrt(jac[i]) : 0.0;
+    s[i] = v;
+    sf[i] = float(v);
+}
+
+__global__ void k_gcn_scale_mul(int n, const double *__restrict__ jac, double a, double *__restrict__ d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = a * jac[i] * d[i];
+}
+
+__global__ void k_axpby(int n, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+// ---------------------------------------------------------------- host side
+static constexpr int kVecGrid = 512, kVecBlock = 256;
+
+// K_lambda, K_mu of the single element geometry of a Hexa8 model (for the MFMA element step).
+int build_hexa_kmat(Model &m, DevBuf<float> &kmat) {
+    SVLG_CHECK(m.ndim == 3 && m.elem_path >= 2 && m.ngeo == 1,
+               "the MFMA element step needs a Hexa8 mesh with a single element geometry");
+    SVLG_HIP(kmat.alloc(2 * 576));
+    hipLaunchKernelGGL(k_gcn_kmat, dim3(1), dim3(64), 0, m.stream, m.geo.p, m.elem_path == 3 ? 1 : 0, kmat.p);
+    SVLG_HIP(hipGetLastError());
+    return 0;
+}
+
+static size_t gcn_lds(const Model &m) {
+    const int nen = m.nen, dim = m.ndim;
+    size_t b = size_t(nen * dim) * m.tiles.max_own * sizeof(double);
+    if (m.elem_path < 2) b += size_t(m.tiles.max_stage) * dim * sizeof(double);
+    b += size_t(m.gcn_C) * m.tiles.max_stage * dim * sizeof(float);
+    return b;
+}
+
+template <int DIM, int PATH, bool MFMA>
+static const void *gcn_fn_t(int mode) {
+    switch (mode) {
+        case G_BWD: return reinterpret_cast<const void *>(&k_gcn_layer<DIM, PATH, MFMA, G_BWD>);
+        case G_OUT: return reinterpret_cast<const void *>(&k_gcn_layer<DIM, PATH, MFMA, G_OUT>);
+        default: return reinterpret_cast<const void *>(&k_gcn_layer<DIM, PATH, MFMA, G_FWD>);
+    }
+}
+
+static const void *gcn_fn(const Model &m, int mode, bool mfma) {
+    if (m.ndim == 3) {
+        if (mfma) return m.elem_path == 3 ? gcn_fn_t<3, 3, true>(mode) : gcn_fn_t<3, 2, true>(mode);
+        switch (m.elem_path) {
+            case 3: return gcn_fn_t<3, 3, false>(mode);
+            case 2: return gcn_fn_t<3, 2, false>(mode);
+            case 1: return gcn_fn_t<3, 1, false>(mode);
+            default: return gcn_fn_t<3, 0, false>(mode);
+        }
+    }
+    switch (m.elem_path) {
+        case 3: return gcn_fn_t<2, 3, false>(mode);
+        case 2: return gcn_fn_t<2, 2, false>(mode);
+        case 1: return gcn_fn_t<2, 1, false>(mode);
+        default: return gcn_fn_t<2, 0, false>(mode);
+    }
+}
+
+static GcnArgs gcn_args(const Model &m) {
+    const TileSet &t = m.tiles;
+    GcnArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.own_ptr = t.own_ptr.p;
+    a.halo_ptr = t.halo_ptr.p;
+    a.halo_node = t.halo_node.p;
+    a.elem_ptr = t.elem_ptr.p;
+    a.lconn = t.elem_lconn.p;
+    a.egeo = t.elem_geo.p;
+    a.elam = m.gcn_elam.p;
+    a.emu = m.gcn_emu.p;
+    a.geo = m.geo.p;
+    a.X = m.X.p;
+    a.th = m.th;
+    a.ntiles = t.ntiles;
+    a.max_stage = t.max_stage;
+    a.max_own = t.max_own;
+    a.kmat = m.gcn_kmat.p;
+    a.dgn = m.dgn.p;
+    a.s = m.gcn_s.p;
+    a.sf = m.gcn_sf.p;
+    a.C = m.gcn_C;
+    a.ndofn = m.ndofn;
+    a.x0 = m.gcn_x0.p;
+    return a;
+}
+
+static int launch_gcn(Model &m, int mode, bool mfma, const GcnArgs &a) {
+    void *args[] = {const_cast<GcnArgs *>(&a)};
+    SVLG_HIP(hipLaunchKernel(gcn_fn(m, mode, mfma), dim3(m.tiles.ntiles), dim3(m.tiles.block), args, gcn_lds(m),
+                             m.stream));
+    return 0;
+}
+
+static const float *theta_A(const Model &m, int l) { return m.gcn_w.p + size_t(l - 1) * theta_layer(m.gcn_C); }
+static const float *theta_B(const Model &m, int l) { return theta_A(m, l) + m.gcn_C * m.gcn_C; }
+static const float *theta_c(const Model &m, int l) { return theta_A(m, l) + 2 * m.gcn_C * m.gcn_C; }
+static const float *theta_w(const Model &m) { return m.gcn_w.p + size_t(m.gcn_L) * theta_layer(m.gcn_C); }
+
+static int upload_theta(Model &m) {
+    std::vector<float> f(m.gcn_theta.begin(), m.gcn_theta.end());
+    SVLG_HIP(hipMemcpyAsync(m.gcn_w.p, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice, m.stream));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+// Inference: z = P(r) (node layout). In the solver, normalize = 1 (sigma from RR), and the
+// last layer sets RHO / BETA of flexible CG.
+static int gcn_run(Model &m, const double *r, const double *q, double *z, bool first, int normalize) {
+    const int n = m.ndofn, C = m.gcn_C, L = m.gcn_L;
+    hipLaunchKernelGGL(k_gcn_first, dim3(kVecGrid), dim3(kVecBlock), 0, m.stream, n, C, normalize, r, m.gcn_s.p,
+                       theta_c(m, 1), m.scal.p, m.gcn_x0.p, m.gcn_h[0].p);
+    GcnArgs a = gcn_args(m);
+    int cur = 0;
+    for (int l = 2; l <= L; ++l) {
+        a.in = m.gcn_h[cur].p;
+        a.out = m.gcn_h[1 - cur].p;
+        a.M1 = theta_A(m, l);
+        a.M2 = theta_B(m, l);
+        a.v = theta_c(m, l);
+        a.trans = 0;
+        a.scal = m.scal.p;
+        const int mode = l == L ? G_OUT : G_FWD;
+        if (mode == G_OUT) {
+            a.w = theta_w(m);
+            a.r = r;
+            a.q = q;
+            a.z = z;
+            a.part = m.part.p;
+            a.scal = m.scal.p;
+            a.counter = m.counters.p + CNT_PRE;
+            a.first = first ? 1 : 0;
+            a.normalize = normalize;
+        }
+        if (int rc = launch_gcn(m, mode, m.gcn_mfma != 0, a)) return rc;
+        cur = 1 - cur;
+    }
+    return 0;
+}
+
+int gcn_precond(Model &m, const double *r, const double *q, double *z, bool first) {
+    return gcn_run(m, r, q, z, first, 1);
+}
+
+// Loss and gradient of a batch of pairs (node layout, device); grad is FP64 on the device,
+// accumulated (+=) over the pairs. loss_sum receives sum over pairs of the pair losses.
+static int gcn_loss_grad(Model &m, int np, const double *const *R, const double *const *E, double weight,
+                         double *grad_dev, double *loss_sum) {
+    const int n = m.ndofn, C = m.gcn_C, L = m.gcn_L, TL = theta_layer(C);
+    const size_t plane = size_t(C) * n;
+    hipStream_t s = m.stream;
+    double *z = m.gcn_tmp[0].p, *kz = m.gcn_tmp[1].p;
+    std::vector<double> hp(2 * kVecGrid);
+    double total = 0.0;
+    for (int p = 0; p < np; ++p) {
+        // Forward with stored states: H^l at gcn_H + l * plane (l = 1..L), Ehat H^(l-1) at gcn_Y + l * plane.
+        hipLaunchKernelGGL(k_gcn_first, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, C, 0, R[p], m.gcn_s.p,
+                           theta_c(m, 1), m.scal.p, m.gcn_x0.p, m.gcn_H.p + plane);
+        GcnArgs a = gcn_args(m);
+        for (int l = 2; l <= L; ++l) {
+            a.in = m.gcn_H.p + size_t(l - 1) * plane;
+            a.out = m.gcn_H.p + size_t(l) * plane;
+            a.ystore = m.gcn_Y.p + size_t(l) * plane;
+            a.M1 = theta_A(m, l);
+            a.M2 = theta_B(m, l);
+            a.v = theta_c(m, l);
+            a.trans = 0;
+            if (int rc = launch_gcn(m, G_FWD, false, a)) return rc;
+        }
+        hipLaunchKernelGGL(k_gcn_out, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, C, m.gcn_H.p + size_t(L) * plane,
+                           theta_w(m), m.gcn_s.p, z);
+        if (int rc = keff_apply_node(m, z, kz)) return rc;
+        hipLaunchKernelGGL(k_gcn_lossdots, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, z, E[p], kz, R[p], m.gcn_part.p);
+        SVLG_HIP(hipMemcpyAsync(hp.data(), m.gcn_part.p, sizeof(double) * 2 * kVecGrid, hipMemcpyDeviceToHost, s));
+        SVLG_HIP(hipStreamSynchronize(s));
+        double num = 0.0, den = 0.0;
+        for (int i = 0; i < kVecGrid; ++i) {
+            num += hp[2 * i];
+            den += hp[2 * i + 1];
+        }
+        if (!(den > 0.0)) {
+            set_error("training pair with e . r <= 0");
+            return 13;
+        }
+        total += weight * num / den;
+        // Backward.
+        const double scale = weight * 2.0 / den;
+        int gc = L & 1;
+        hipLaunchKernelGGL(k_gcn_outgrad, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, C, scale, kz, R[p], m.gcn_s.p,
+                           theta_w(m), m.gcn_H.p + size_t(L) * plane, m.gcn_G[gc].p, m.gcn_part.p);
+        hipLaunchKernelGGL(k_gcn_accum, dim3(C), dim3(256), 0, s, kVecGrid, C, m.gcn_part.p, grad_dev, L * TL);
+        for (int l = L; l >= 2; --l) {
+            a = gcn_args(m);
+            a.in = m.gcn_G[gc].p;
+            a.out = m.gcn_G[1 - gc].p;
+            a.M1 = theta_A(m, l);
+            a.M2 = theta_B(m, l);
+            a.v = nullptr;
+            a.trans = 1;
+            a.hprev = m.gcn_H.p + size_t(l - 1) * plane;
+            a.yl = m.gcn_Y.p + size_t(l) * plane;
+            a.gpart = m.gcn_part.p;
+            if (int rc = launch_gcn(m, G_BWD, false, a)) return rc;
+            hipLaunchKernelGGL(k_gcn_accum, dim3(TL), dim3(256), 0, s, m.tiles.ntiles, TL, m.gcn_part.p, grad_dev,
+                               (l - 1) * TL);
+            gc = 1 - gc;
+        }
+        hipLaunchKernelGGL(k_gcn_first_grad, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, C, m.gcn_x0.p, m.gcn_G[gc].p,
+                           m.gcn_part.p);
+        hipLaunchKernelGGL(k_gcn_accum, dim3(C), dim3(256), 0, s, kVecGrid, C, m.gcn_part.p, grad_dev, 2 * C * C);
+    }
+    SVLG_HIP(hipGetLastError());
+    *loss_sum = total;
+    return 0;
+}
+
+// Chebyshev (Richardson) initial parameters on [lmin, lmax]; extra channels start small.
+static void theta_chebyshev(int C, int L, double lmin, double lmax, std::vector<double> &th) {
+    th.assign(theta_size(C, L), 0.0);
+    const int TL = theta_layer(C);
+    std::vector<double> om(L);
+    for (int l = 0; l < L; ++l) {
+        const double node = 0.5 * (lmax + lmin) + 0.5 * (lmax - lmin) * std::cos(M_PI * (2.0 * l + 1.0) / (2.0 * L));
+        om[l] = 1.0 / node;
+    }
+    // Alternate large and small steps (a simple stable ordering of the Richardson factors).
+    std::vector<double> ord;
+    for (int i = 0, jx = L - 1; i <= jx; ++i, --jx) {
+        ord.push_back(om[i]);
+        if (i != jx) ord.push_back(om[jx]);
+    }
+    std::mt19937_64 rng(20260302);
+    std::normal_distribution<double> nd(0.0, 0.02);
+    for (int l = 1; l <= L; ++l) {
+        double *A = th.data() + size_t(l - 1) * TL, *Bm = A + C * C, *c = A + 2 * C * C;
+        for (int i = 0; i < C; ++i)
+            for (int k = 0; k < C; ++k) {
+                A[i * C + k] = (i == k) ? 1.0 : nd(rng);
+                Bm[i * C + k] = (i == 0 && k == 0) ? -ord[l - 1] : nd(rng);
+            }
+        c[0] = ord[l - 1];
+        for (int k = 1; k < C; ++k) c[k] = nd(rng);
+    }
+    double *w = th.data() + size_t(L) * TL;
+    w[0] = 1.0;
+    for (int k = 1; k < C; ++k) w[k] = nd(rng);
+}
+
+// Extreme eigenvalues of Ehat by power iteration (FP64 Keff applies): lmax, then the
+// largest eigenvalue of lmax I - Ehat.
+static int ehat_bounds(Model &m, int iters, double *lmin, double *lmax) {
+    const int n = m.ndofn;
+    std::vector<double> hv(n), hs(n);
+    SVLG_HIP(hipMemcpyAsync(hs.data(), m.gcn_s.p, sizeof(double) * n, hipMemcpyDeviceToHost, m.stream));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    std::mt19937_64 rng(7);
+    std::uniform_real_distribution<double> ud(-1.0, 1.0);
+    for (int i = 0; i < n; ++i) hv[i] = hs[i] > 0.0 ? ud(rng) : 0.0;
+    double *x = m.gcn_tmp[0].p, *y = m.gcn_tmp[1].p, *t = m.gcn_tmp[2].p;
+    auto ehat = [&](const std::vector<double> &in, std::vector<double> &out) -> int {
+        std::vector<double> xs(n);
+        for (int i = 0; i < n; ++i) xs[i] = hs[i] * in[i];
+        SVLG_HIP(hipMemcpyAsync(x, xs.data(), sizeof(double) * n, hipMemcpyHostToDevice, m.stream));
+        if (int rc = keff_apply_node(m, x, y)) return rc;
+        SVLG_HIP(hipMemcpyAsync(out.data(), y, sizeof(double) * n, hipMemcpyDeviceToHost, m.stream));
+        SVLG_HIP(hipStreamSynchronize(m.stream));
+        for (int i = 0; i < n; ++i) out[i] *= hs[i];
+        return 0;
+    };
+    (void)t;
+    auto norm = [&](std::vector<double> &v) {
+        double s2 = 0.0;
+        for (double e : v) s2 += e * e;
+        const double nv = std::sqrt(s2);
+        for (double &e : v) e /= nv;
+        return nv;
+    };
+    norm(hv);
+    std::vector<double> w(n);
+    double lam = 0.0;
+    for (int it = 0; it < iters; ++it) {
+        if (int rc = ehat(hv, w)) return rc;
+        lam = norm(w);
+        hv.swap(w);
+    }
+    *lmax = lam;
+    for (int i = 0; i < n; ++i) hv[i] = hs[i] > 0.0 ? ud(rng) : 0.0;
+    norm(hv);
+    double mu = 0.0;
+    for (int it = 0; it < iters; ++it) {
+        if (int rc = ehat(hv, w)) return rc;
+        for (int i = 0; i < n; ++i) w[i] = lam * hv[i] - w[i];
+        mu = norm(w);
+        hv.swap(w);
+    }
+    *lmin = std::max(lam - mu, 1e-3 * lam);
+    return 0;
+}
+
+}  // namespace svlg
+
+using namespace svlg;
+
+extern "C" {
+
+int svlg_gnn_init(svlg_model *h, int32_t channels, int32_t layers, int32_t mfma, const double *theta) {
+    SVLG_CHECK(h, "null model");
+    Model &m = h->m;
+    SVLG_CHECK(m.nm_ready, "svlg_nm_init has not been called");
+    SVLG_CHECK(channels >= 1 && channels <= kMaxC, "channels must be in [1, 4]");
+    SVLG_CHECK(layers >= 2 && layers <= kMaxL, "layers must be in [2, 16]");
+    const bool mfma_ok = m.ndim == 3 && m.elem_path >= 2 && m.ngeo == 1;
+    SVLG_CHECK(!(mfma == 1 && !mfma_ok), "the MFMA element step needs a Hexa8 mesh with a single element geometry");
+    m.gcn_ready = false;
+    m.gcn_C = channels;
+    m.gcn_L = layers;
+    m.gcn_mfma = (mfma == 1 || (mfma < 0 && mfma_ok)) ? 1 : 0;
+    SVLG_CHECK(gcn_lds(m) <= 160 * 1024, "network layer tile needs more than 160 KB of LDS; use smaller tiles");
+    hipStream_t s = m.stream;
+    const int n = m.ndofn;
+    SVLG_HIP(m.gcn_s.alloc(n));
+    SVLG_HIP(m.gcn_sf.alloc(n));
+    hipLaunchKernelGGL(k_gcn_scale, dim3((n + 255) / 256), dim3(256), 0, s, n, m.jac.p, m.gcn_s.p, m.gcn_sf.p);
+    // (lambda, mu) * (1 + 2 ak/dt) per tile element.
+    {
+        const size_t ne = m.tile_elem_id.size();
+        std::vector<double> el(ne), em(ne), ck(ne);
+        SVLG_HIP(hipMemcpyAsync(el.data(), m.tiles.elem_lam.p, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+        SVLG_HIP(hipMemcpyAsync(em.data(), m.tiles.elem_mu.p, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+        SVLG_HIP(hipMemcpyAsync(ck.data(), m.coef_keff.p, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+        SVLG_HIP(hipStreamSynchronize(s));
+        for (size_t i = 0; i < ne; ++i) {
+            el[i] *= ck[i];
+            em[i] *= ck[i];
+        }
+        SVLG_HIP(m.gcn_elam.upload(el.data(), ne, s));
+        SVLG_HIP(m.gcn_emu.upload(em.data(), ne, s));
+    }
+    if (m.gcn_mfma)
+        if (int rc = build_hexa_kmat(m, m.gcn_kmat)) return rc;
+    SVLG_HIP(m.gcn_x0.alloc(n));
+    for (int i = 0; i < 2; ++i) SVLG_HIP(m.gcn_h[i].alloc(size_t(channels) * n));
+    for (int i = 0; i < 3; ++i) SVLG_HIP(m.gcn_tmp[i].alloc(n));
+    const size_t np = std::max<size_t>(size_t(m.tiles.ntiles) * theta_layer(channels), 2 * kVecGrid * kMaxC);
+    SVLG_HIP(m.gcn_part.alloc(np));
+    SVLG_HIP(m.gcn_w.alloc(theta_size(channels, layers)));
+    SVLG_HIP(hipGetLastError());
+    if (theta) {
+        m.gcn_theta.assign(theta, theta + theta_size(channels, layers));
+    } else {
+        double lmin = 0.0, lmax = 0.0;
+        if (int rc = ehat_bounds(m, 40, &lmin, &lmax)) return rc;
+        m.gcn_lmin = lmin;
+        m.gcn_lmax = lmax;
+        // Slightly widened interval: power iteration underestimates lmax and overestimates lmin.
+        theta_chebyshev(channels, layers, 0.9 * lmin, 1.02 * lmax, m.gcn_theta);
+    }
+    if (int rc = upload_theta(m)) return rc;
+    const size_t lds = gcn_lds(m);
+    if (lds > 64 * 1024)
+        for (int mode : {G_FWD, G_BWD, G_OUT})
+            for (bool mf : {false, true}) {
+                if (mf && !m.gcn_mfma) continue;
+                SVLG_HIP(hipFuncSetAttribute(gcn_fn(m, mode, mf), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+            }
+    m.gcn_ready = true;
+    return 0;
+}
+
+int svlg_gnn_info(svlg_model *h, int32_t *info4, double *bounds2) {
+    SVLG_CHECK(h, "null model");
+    Model &m = h->m;
+    SVLG_CHECK(m.gcn_ready, "svlg_gnn_init has not been called");
+    if (info4) {
+        info4[0] = m.gcn_C;
+        info4[1] = m.gcn_L;
+        info4[2] = m.gcn_mfma;
+        info4[3] = theta_size(m.gcn_C, m.gcn_L);
+    }
+    if (bounds2) {
+        bounds2[0] = m.gcn_lmin;
+        bounds2[1] = m.gcn_lmax;
+    }
+    return 0;
+}
+
+int svlg_gnn_get_params(svlg_model *h, double *theta) {
+    SVLG_CHECK(h && theta, "null argument");
+    Model &m = h->m;
+    SVLG_CHECK(m.gcn_ready, "svlg_gnn_init has not been called");
+    std::copy(m.gcn_theta.begin(), m.gcn_theta.end(), theta);
+    return 0;
+}
+
+int svlg_gnn_set_params(svlg_model *h, const double *theta) {
+    SVLG_CHECK(h && theta, "null argument");
+    Model &m = h->m;
+    SVLG_CHECK(m.gcn_ready, "svlg_gnn_init has not been called");
+    m.gcn_theta.assign(theta, theta + theta_size(m.gcn_C, m.gcn_L));
+    return upload_theta(m);
+}
+
+int svlg_gnn_apply(svlg_model *h, int32_t mfma, const double *rfree, double *zfree) {
+    SVLG_CHECK(h && rfree && zfree, "null argument");
+    Model &m = h->m;
+    SVLG_CHECK(m.gcn_ready, "svlg_gnn_init has not been called");
+    SVLG_CHECK(!(mfma && m.gcn_kmat.p == nullptr), "the MFMA element step is not set up for this model");
+    DevBuf<double> f, rn, zn;
+    SVLG_HIP(f.upload(rfree, m.nfree, m.stream));
+    SVLG_HIP(rn.alloc(m.ndofn));
+    SVLG_HIP(zn.alloc(m.ndofn));
+    SVLG_HIP(hipMemsetAsync(rn.p, 0, sizeof(double) * m.ndofn, m.stream));
+    if (int rc = scatter_free(m, f.p, rn.p)) return rc;
+    const int saved = m.gcn_mfma;
+    m.gcn_mfma = mfma ? 1 : 0;
+    // The output layer also writes flexible-CG scalars: give it a scratch scalar block.
+    SVLG_HIP(hipMemsetAsync(m.scal.p, 0, sizeof(double) * SC_COUNT, m.stream));
+    const int rc = gcn_run(m, rn.p, rn.p, zn.p, true, 0);
+    m.gcn_mfma = saved;
+    if (rc) return rc;
+    if (int rc2 = gather_free(m, zn.p, f.p)) return rc2;
+    SVLG_HIP(hipMemcpyAsync(zfree, f.p, sizeof(double) * m.nfree, hipMemcpyDeviceToHost, m.stream));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+// Ensures the training buffers exist.
+static int gcn_train_buffers(Model &m) {
+    const size_t plane = size_t(m.gcn_C) * m.ndofn;
+    if (m.gcn_H.n != plane * (m.gcn_L + 1)) {
+        SVLG_HIP(m.gcn_H.alloc(plane * (m.gcn_L + 1)));
+        SVLG_HIP(m.gcn_Y.alloc(plane * (m.gcn_L + 1)));
+        for (int i = 0; i < 2; ++i) SVLG_HIP(m.gcn_G[i].alloc(plane));
+    }
+    return 0;
+}
+
+int svlg_gnn_loss_grad(svlg_model *h, int32_t npairs, const double *r, const double *e, double *loss,
+                       double *grad) {
+    SVLG_CHECK(h && r && e && loss, "null argument");
+    Model &m = h->m;
+    SVLG_CHECK(m.gcn_ready, "svlg_gnn_init has not been called");
+    SVLG_CHECK(npairs > 0, "npairs must be positive");
+    if (int rc = gcn_train_buffers(m)) return rc;
+    const int nt = theta_size(m.gcn_C, m.gcn_L);
+    DevBuf<double> pr, pe, f, gd;
+    SVLG_HIP(pr.alloc(size_t(npairs) * m.ndofn));
+    SVLG_HIP(pe.alloc(size_t(npairs) * m.ndofn));
+    SVLG_HIP(f.alloc(m.nfree));
+    SVLG_HIP(gd.alloc(nt));
+    SVLG_HIP(hipMemsetAsync(pr.p, 0, sizeof(double) * pr.n, m.stream));
+    SVLG_HIP(hipMemsetAsync(pe.p, 0, sizeof(double) * pe.n, m.stream));
+    SVLG_HIP(hipMemsetAsync(gd.p, 0, sizeof(double) * nt, m.stream));
+    std::vector<const double *> R(npairs), E(npairs);
+    for (int p = 0; p < npairs; ++p) {
+        SVLG_HIP(hipMemcpyAsync(f.p, r + size_t(p) * m.nfree, sizeof(double) * m.nfree, hipMemcpyHostToDevice,
+                                m.stream));
+        if (int rc = scatter_free(m, f.p, pr.p + size_t(p) * m.ndofn)) return rc;
+        SVLG_HIP(hipMemcpyAsync(f.p, e + size_t(p) * m.nfree, sizeof(double) * m.nfree, hipMemcpyHostToDevice,
+                                m.stream));
+        if (int rc = scatter_free(m, f.p, pe.p + size_t(p) * m.ndofn)) return rc;
+        SVLG_HIP(hipStreamSynchronize(m.stream));
+        R[p] = pr.p + size_t(p) * m.ndofn;
+        E[p] = pe.p + size_t(p) * m.ndofn;
+    }
+    double ls = 0.0;
+    if (int rc = gcn_loss_grad(m, npairs, R.data(), E.data(), 1.0 / npairs, gd.p, &ls)) return rc;
+    *loss = ls;
+    if (grad) {
+        SVLG_HIP(hipMemcpyAsync(grad, gd.p, sizeof(double) * nt, hipMemcpyDeviceToHost, m.stream));
+        SVLG_HIP(hipStreamSynchronize(m.stream));
+    }
+    return 0;
+}
+
+int svlg_nm_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const int32_t *rec_dofs, double *hist,
+                  int32_t *iters, double *relres);
+
+// Generates residual/solution pairs from the model's own Newmark solves and trains the
+// network with Adam. From rest, steps k = 1..nsteps are solved with the current
+// preconditioner to rtol_pairs; every `every`-th step contributes the pair (Feff, dU) and
+// `sweeps` Jacobi-smoothed pairs (Feff - Keff x_j, dU - x_j), x_j = x_(j-1) + D^-1 (Feff -
+// Keff x_(j-1)) / 2. The state is reset to rest afterwards. loss_hist[epoch] (may be NULL)
+// receives the mean loss of each epoch.
+int svlg_gnn_train(svlg_model *h, int32_t nsteps, int32_t every, int32_t sweeps, int32_t epochs, double lr,
+                   double *loss_hist) {
+    SVLG_CHECK(h, "null model");
+    Model &m = h->m;
+    SVLG_CHECK(m.gcn_ready, "svlg_gnn_init has not been called");
+    SVLG_CHECK(nsteps > 0 && every > 0 && sweeps >= 0 && epochs >= 0 && lr > 0.0, "invalid training settings");
+    if (int rc = gcn_train_buffers(m)) return rc;
+    hipStream_t s = m.stream;
+    const int n = m.ndofn;
+    const int npairs = (nsteps / every) * (1 + sweeps);
+    SVLG_CHECK(npairs > 0, "no training pairs (nsteps < every)");
+    DevBuf<double> pr, pe;
+    SVLG_HIP(pr.alloc(size_t(npairs) * n));
+    SVLG_HIP(pe.alloc(size_t(npairs) * n));
+    // Pairs from the Newmark solves (the preconditioner in use is unchanged).
+    int p = 0;
+    const int saved_pc = m.nm_precond;
+    m.nm_precond = SVLG_PRECOND_JACOBI;
+    for (int k = 1; k <= nsteps; ++k) {
+        if (int rc = svlg_nm_steps(h, k, 1, 0, nullptr, nullptr, nullptr, nullptr)) {
+            m.nm_precond = saved_pc;
+            return rc;
+        }
+        if (k % every) continue;
+        double *rb = pr.p + size_t(p) * n, *eb = pe.p + size_t(p) * n;
+        SVLG_HIP(hipMemcpyAsync(rb, m.cg_b.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        SVLG_HIP(hipMemcpyAsync(eb, m.ndU.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        ++p;
+        // Jacobi-smoothed pairs: r_j = r_(j-1) - Keff d, e_j = e_(j-1) - d, d = jac r_(j-1) / 2.
+        for (int sw = 0; sw < sweeps; ++sw) {
+            double *rp = pr.p + size_t(p - 1) * n, *ep = pe.p + size_t(p - 1) * n;
+            double *rn = pr.p + size_t(p) * n, *en = pe.p + size_t(p) * n;
+            double *d = m.gcn_tmp[0].p, *kd = m.gcn_tmp[1].p;
+            // d = jac * rp / 2 via two axpby passes on copies.
+            SVLG_HIP(hipMemcpyAsync(d, rp, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_gcn_scale_mul, dim3((n + 255) / 256), dim3(256), 0, s, n, m.jac.p, 0.5, d);
+            if (int rc = keff_apply_node(m, d, kd)) return rc;
+            SVLG_HIP(hipMemcpyAsync(rn, rp, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            SVLG_HIP(hipMemcpyAsync(en, ep, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_axpby, dim3((n + 255) / 256), dim3(256), 0, s, n, -1.0, kd, 1.0, rn);
+            hipLaunchKernelGGL(k_axpby, dim3((n + 255) / 256), dim3(256), 0, s, n, -1.0, d, 1.0, en);
+            ++p;
+        }
+    }
+    m.nm_precond = saved_pc;
+    // Back to rest.
+    for (DevBuf<double> *b : {&m.nU, &m.nV, &m.nA, &m.ndU})
+        SVLG_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * n, s));
+    m.nm_steps_done = 0;
+    // Adam over full-batch gradients.
+    const int nt = theta_size(m.gcn_C, m.gcn_L);
+    DevBuf<double> gd;
+    SVLG_HIP(gd.alloc(nt));
+    std::vector<const double *> R(npairs), E(npairs);
+    for (int i = 0; i < npairs; ++i) {
+        R[i] = pr.p + size_t(i) * n;
+        E[i] = pe.p + size_t(i) * n;
+    }
+    std::vector<double> g(nt), m1(nt, 0.0), m2(nt, 0.0);
+    const double b1 = 0.9, b2 = 0.999, eps = 1e-12;
+    for (int ep = 0; ep < epochs; ++ep) {
+        SVLG_HIP(hipMemsetAsync(gd.p, 0, sizeof(double) * nt, s));
+        double ls = 0.0;
+        if (int rc = gcn_loss_grad(m, npairs, R.data(), E.data(), 1.0 / npairs, gd.p, &ls)) return rc;
+        SVLG_HIP(hipMemcpyAsync(g.data(), gd.p, sizeof(double) * nt, hipMemcpyDeviceToHost, s));
+        SVLG_HIP(hipStreamSynchronize(s));
+        if (loss_hist) loss_hist[ep] = ls;
+        const double c1 = 1.0 - std::pow(b1, ep + 1), c2 = 1.0 - std::pow(b2, ep + 1);
+        for (int i = 0; i < nt; ++i) {
+            m1[i] = b1 * m1[i] + (1.0 - b1) * g[i];
+            m2[i] = b2 * m2[i] + (1.0 - b2) * g[i] * g[i];
+            m.gcn_theta[i] -= lr * (m1[i] / c1) / (std::sqrt(m2[i] / c2) + eps);
+        }
+        if (int rc = upload_theta(m)) return rc;
+    }
+    return 0;
+}
+
+}  // extern "C"

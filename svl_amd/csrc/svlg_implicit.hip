@@ -24,26 +24,12 @@
 
 #include "svlg_common.hpp"
 #include "svlg_elem.hpp"
+#include "svlg_solver.hpp"
 
 namespace svlg {
 
 int gather_free(Model &m, const double *dnode, double *dfree);
 int scatter_free(Model &m, const double *dfree, double *dnode);
-
-// Device scalars of the solver (Model::scal).
-enum {
-    SC_RHO = 0,  // r . z
-    SC_PQ,       // p . q
-    SC_RR,       // r . r
-    SC_BB,       // b . b
-    SC_ALPHA,
-    SC_BETA,
-    SC_DONE,
-    SC_ITERS,
-    SC_TOL2,     // rtol^2
-    SC_MAXIT,
-    SC_COUNT = 16
-};
 
 enum { OP_CG = 0, OP_RHS = 1 };
 
@@ -62,7 +48,9 @@ struct OpArgs {
     double *p_new;
     const double *dgn;
     double *y, *part;
-    const double *scal;
+    double *scal;
+    unsigned *counter;
+    int reduce;           // OP_CG: 1 = per-tile p.y partials, the last tile sets PQ and ALPHA
     // OP_RHS
     const double *U, *V, *A, *mdiag, *cdiag;
     double dt;
@@ -71,63 +59,6 @@ struct OpArgs {
     const double *load_dir, *load_factor, *load_series;
     int load_nt, k;
 };
-
-// Deterministic sum over a workgroup (blockDim a multiple of 64, at most 1024): fixed
-// shuffle tree inside each wave, then the wave sums in wave order. buf holds 17 doubles.
-__device__ __forceinline__ double block_sum(double v, double *buf) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) buf[w] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = 0.0;
-        for (int i = 0; i < int(blockDim.x >> 6); ++i) r += buf[i];
-        buf[16] = r;
-    }
-    __syncthreads();
-    const double r = buf[16];
-    __syncthreads();
-    return r;
-}
-
-// Element force f = K_e u for the operand u of one tile element; lam and mu carry any
-// per-element factor (and |J|-scaling on the geometry-table paths).
-template <int DIM, int PATH>
-__device__ __forceinline__ void elem_force(const OpArgs &a, const int (&lc)[Elem<DIM>::nen], const double *sX,
-                                           double lam, double mu, int gid, const double (&u)[Elem<DIM>::nen][DIM],
-                                           double (&f)[Elem<DIM>::nen][DIM]) {
-    constexpr int NEN = Elem<DIM>::nen;
-    if constexpr (PATH >= 2) {
-        const double *g = a.geo + size_t(gid) * (DIM * DIM);
-        double Lam[DIM][DIM];
-#pragma unroll
-        for (int r = 0; r < DIM; ++r)
-#pragma unroll
-            for (int c = 0; c < DIM; ++c) Lam[r][c] = g[r * DIM + c];
-        if constexpr (DIM == 3)
-            hexa_modal_force<PATH == 3>(Lam, lam, mu, u, f);
-        else
-            quad_modal_force<PATH == 3>(Lam, lam, mu, u, f);
-    } else {
-        double x[NEN][DIM];
-#pragma unroll
-        for (int c = 0; c < NEN; ++c)
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) x[c][d] = sX[lc[c] * DIM + d];
-        if constexpr (DIM == 3) {
-            if constexpr (PATH == 1)
-                hexa_affine_fint(x, u, lam, mu, f);
-            else
-                hexa_fint(x, u, lam, mu, f);
-        } else {
-            if constexpr (PATH == 1)
-                quad_affine_fint(x, u, lam, mu, a.th, f);
-            else
-                quad_fint(x, u, lam, mu, a.th, f);
-        }
-    }
-}
 
 // One node tile of the effective-stiffness operator (one thread per tile element in the
 // element phase, one thread per owned node in the sum phase).
@@ -221,7 +152,7 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
                 u[c][d] = sU[lc[c] * DIM + d];
                 if constexpr (RHS) u[c][d] += c2 * sV[lc[c] * DIM + d];
             }
-        elem_force<DIM, PATH>(a, lc, sX, lam, mu, gid, u, f);
+        tile_elem_force<DIM, PATH>(a.geo, a.th, lc, sX, lam, mu, gid, u, f);
 #pragma unroll
         for (int c = 0; c < NEN; ++c)
             if (lc[c] < nown) {
@@ -259,8 +190,18 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
         }
     }
     if constexpr (!RHS) {
+        if (!a.reduce) return;
+        __shared__ int last;
         const double s = block_sum(acc, red);
         if (tid == 0) a.part[tile] = s;
+        if (arrive_last(a.counter, gridDim.x, &last)) {
+            const double pq = sum_partials(a.part, nt, 1, red);
+            if (tid == 0) {
+                a.scal[SC_PQ] = pq;
+                a.scal[SC_ALPHA] = pq != 0.0 ? a.scal[SC_RHO] / pq : 0.0;
+            }
+            release_counter(a.counter);
+        }
     }
 }
 
@@ -272,18 +213,24 @@ __global__ void k_solver_reset(double *scal, double tol2, double maxit) {
     }
 }
 
-// r = b - q, z = jac r; per-block partials of r.z, r.r, b.b (fixed grid => fixed order).
-__global__ __launch_bounds__(256) void k_cg_init(int n, const double *__restrict__ b, const double *__restrict__ q,
-                                                 const double *__restrict__ jac, double *__restrict__ r,
-                                                 double *__restrict__ z, double *__restrict__ part) {
+// r = b - q and, for the Jacobi preconditioner (flexible = 0), z = jac r. The last
+// workgroup sums the per-workgroup partials in workgroup order and starts the recurrence:
+// RHO = r.z (0 when flexible: the preconditioner sets it), RR = r.r, BB = b.b, BETA = 0.
+__global__ __launch_bounds__(256) void k_cg_init(int n, int flexible, const double *__restrict__ b,
+                                                 const double *__restrict__ q, const double *__restrict__ jac,
+                                                 double *__restrict__ r, double *__restrict__ z,
+                                                 double *__restrict__ part, double *scal, unsigned *counter) {
     __shared__ double red[17];
+    __shared__ int last;
     double rz = 0.0, rr = 0.0, bb = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double ri = jac[i] != 0.0 ? b[i] - q[i] : 0.0;
-        const double zi = jac[i] * ri;
         r[i] = ri;
-        z[i] = zi;
-        rz += ri * zi;
+        if (!flexible) {
+            const double zi = jac[i] * ri;
+            z[i] = zi;
+            rz += ri * zi;
+        }
         rr += ri * ri;
         bb += b[i] * b[i];
     }
@@ -295,70 +242,65 @@ __global__ __launch_bounds__(256) void k_cg_init(int n, const double *__restrict
         part[3 * blockIdx.x + 1] = rr;
         part[3 * blockIdx.x + 2] = bb;
     }
+    if (arrive_last(counter, gridDim.x, &last)) {
+        const double s0 = sum_partials(part + 0, gridDim.x, 3, red);
+        const double s1 = sum_partials(part + 1, gridDim.x, 3, red);
+        const double s2 = sum_partials(part + 2, gridDim.x, 3, red);
+        if (threadIdx.x == 0) {
+            scal[SC_RHO] = s0;
+            scal[SC_RR] = s1;
+            scal[SC_BB] = s2;
+            scal[SC_BETA] = 0.0;
+            scal[SC_ITERS] = 0.0;
+            scal[SC_DONE] = (s1 <= scal[SC_TOL2] * s2) ? 1.0 : 0.0;
+        }
+        release_counter(counter);
+    }
 }
 
-// x += alpha p; r -= alpha q; z = jac r; per-block partials of r.z and r.r.
-__global__ __launch_bounds__(256) void k_cg_update(int n, const double *__restrict__ scal, double *__restrict__ x,
+// Iteration `it`: x += alpha p; r -= alpha q; for Jacobi also z = jac r. The last
+// workgroup sums the partials in workgroup order, applies the stopping test
+// ||r||^2 <= rtol^2 ||b||^2 (or it + 1 = maxit) and, for Jacobi, BETA = r.z / RHO.
+__global__ __launch_bounds__(256) void k_cg_update(int n, int it, int flexible, double *scal, double *__restrict__ x,
                                                    const double *__restrict__ p, const double *__restrict__ q,
                                                    double *__restrict__ r, double *__restrict__ z,
-                                                   const double *__restrict__ jac, double *__restrict__ part) {
+                                                   const double *__restrict__ jac, double *__restrict__ part,
+                                                   unsigned *counter) {
     __shared__ double red[17];
+    __shared__ int last;
     if (scal[SC_DONE] != 0.0) return;
     const double alpha = scal[SC_ALPHA];
     double rz = 0.0, rr = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         x[i] += alpha * p[i];
         const double ri = r[i] - alpha * q[i];
-        const double zi = jac[i] * ri;
         r[i] = ri;
-        z[i] = zi;
-        rz += ri * zi;
+        if (!flexible) {
+            const double zi = jac[i] * ri;
+            z[i] = zi;
+            rz += ri * zi;
+        }
         rr += ri * ri;
     }
     rz = block_sum(rz, red);
     rr = block_sum(rr, red);
     if (threadIdx.x == 0) {
-        part[3 * blockIdx.x + 0] = rz;
-        part[3 * blockIdx.x + 1] = rr;
+        part[2 * blockIdx.x + 0] = rz;
+        part[2 * blockIdx.x + 1] = rr;
     }
-}
-
-enum { RED_INIT = 0, RED_PQ = 1, RED_RZ = 2 };
-
-// One workgroup: sums the partials in a fixed order, then the scalar recurrences of
-// preconditioned CG (alpha = rho / p.q, beta = rho_new / rho) and the stopping test.
-__global__ __launch_bounds__(1024) void k_reduce(int mode, int count, int stride, const double *__restrict__ part,
-                                                 double *scal) {
-    __shared__ double red[17];
-    if (mode != RED_INIT && scal[SC_DONE] != 0.0) return;
-    const int ncomp = mode == RED_PQ ? 1 : (mode == RED_INIT ? 3 : 2);
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int i = threadIdx.x; i < count; i += blockDim.x) {
-        const double *pp = part + size_t(i) * stride;
-        s0 += pp[0];
-        if (ncomp > 1) s1 += pp[1];
-        if (ncomp > 2) s2 += pp[2];
-    }
-    s0 = block_sum(s0, red);
-    if (ncomp > 1) s1 = block_sum(s1, red);
-    if (ncomp > 2) s2 = block_sum(s2, red);
-    if (threadIdx.x != 0) return;
-    if (mode == RED_PQ) {
-        scal[SC_PQ] = s0;
-        scal[SC_ALPHA] = s0 != 0.0 ? scal[SC_RHO] / s0 : 0.0;
-    } else if (mode == RED_INIT) {
-        scal[SC_RHO] = s0;
-        scal[SC_RR] = s1;
-        scal[SC_BB] = s2;
-        scal[SC_BETA] = 0.0;
-        scal[SC_ITERS] = 0.0;
-        scal[SC_DONE] = (s1 <= scal[SC_TOL2] * s2) ? 1.0 : 0.0;
-    } else {
-        scal[SC_BETA] = scal[SC_RHO] != 0.0 ? s0 / scal[SC_RHO] : 0.0;
-        scal[SC_RHO] = s0;
-        scal[SC_RR] = s1;
-        scal[SC_ITERS] += 1.0;
-        if (s1 <= scal[SC_TOL2] * scal[SC_BB] || scal[SC_ITERS] >= scal[SC_MAXIT]) scal[SC_DONE] = 1.0;
+    if (arrive_last(counter, gridDim.x, &last)) {
+        const double s0 = sum_partials(part + 0, gridDim.x, 2, red);
+        const double s1 = sum_partials(part + 1, gridDim.x, 2, red);
+        if (threadIdx.x == 0) {
+            if (!flexible) {
+                scal[SC_BETA] = scal[SC_RHO] != 0.0 ? s0 / scal[SC_RHO] : 0.0;
+                scal[SC_RHO] = s0;
+            }
+            scal[SC_RR] = s1;
+            scal[SC_ITERS] = double(it + 1);
+            if (s1 <= scal[SC_TOL2] * scal[SC_BB] || double(it + 1) >= scal[SC_MAXIT]) scal[SC_DONE] = 1.0;
+        }
+        release_counter(counter);
     }
 }
 
@@ -499,7 +441,7 @@ static int launch_op(Model &m, int mode, const OpArgs &a) {
 }
 
 // y = Keff x (node-DOF layout) through the CG operator with beta = 0.
-static int keff_apply_node(Model &m, const double *x, double *y) {
+int keff_apply_node(Model &m, const double *x, double *y) {
     hipLaunchKernelGGL(k_solver_reset, dim3(1), dim3(64), 0, m.stream, m.scal.p, 0.0, 0.0);
     OpArgs a = op_args(m);
     a.coef = m.coef_keff.p;
@@ -508,41 +450,50 @@ static int keff_apply_node(Model &m, const double *x, double *y) {
     a.p_new = m.cg_p[1].p;
     a.y = y;
     a.part = m.part.p;
+    a.reduce = 0;
     return launch_op(m, OP_CG, a);
 }
 
-// Solves Keff x = b (node-DOF layout; x holds the initial guess on entry).
-static int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int *iters, double *relres) {
+// Solves Keff x = b (node-DOF layout; x holds the initial guess on entry) by CG with the
+// Jacobi preconditioner (precond = SVLG_PRECOND_JACOBI) or by flexible CG with the learned
+// preconditioner (SVLG_PRECOND_LEARNED; Polak-Ribiere-type beta = -z.q / p.q). Per iteration
+// the operator kernel and the vector kernel each end in a last-workgroup reduction, so no
+// separate reduction launches are needed; the host polls the done flag every 16 iterations.
+static int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int precond, int *iters,
+                    double *relres) {
     hipStream_t s = m.stream;
     const int n = m.ndofn;
+    const int flexible = precond == SVLG_PRECOND_LEARNED ? 1 : 0;
     hipLaunchKernelGGL(k_solver_reset, dim3(1), dim3(64), 0, s, m.scal.p, rtol * rtol, double(maxit));
     OpArgs a = op_args(m);
     a.coef = m.coef_keff.p;
     a.part = m.part.p;
+    a.counter = m.counters.p + CNT_OP;
     a.y = m.cg_q.p;
     // q = Keff x0 (x0 staged as z with beta = 0; cg_p[1] receives a copy of x0).
     a.z = x;
     a.p_old = x;
     a.p_new = m.cg_p[1].p;
+    a.reduce = 0;
     if (int rc = launch_op(m, OP_CG, a)) return rc;
-    hipLaunchKernelGGL(k_cg_init, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, b, m.cg_q.p, m.jac.p, m.cg_r.p, m.cg_z.p,
-                       m.part.p);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, int(RED_INIT), kVecGrid, 3, m.part.p, m.scal.p);
     double *vpart = m.part.p + m.tiles.ntiles;
+    hipLaunchKernelGGL(k_cg_init, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, flexible, b, m.cg_q.p, m.jac.p,
+                       m.cg_r.p, m.cg_z.p, vpart, m.scal.p, m.counters.p + CNT_VEC);
+    if (flexible)
+        if (int rc = gcn_precond(m, m.cg_r.p, m.cg_q.p, m.cg_z.p, true)) return rc;
     int cur = 1;  // p_old of the first iteration (beta = 0 there, so p = z)
     double hs[SC_COUNT];
     a.z = m.cg_z.p;
+    a.reduce = 1;
     for (int it = 0; it < maxit; ++it) {
         a.p_old = m.cg_p[cur].p;
         a.p_new = m.cg_p[1 - cur].p;
         if (int rc = launch_op(m, OP_CG, a)) return rc;
-        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, int(RED_PQ), m.tiles.ntiles, 1, m.part.p, m.scal.p);
-        hipLaunchKernelGGL(k_cg_update, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, m.scal.p, x, m.cg_p[1 - cur].p,
-                           m.cg_q.p, m.cg_r.p, m.cg_z.p, m.jac.p, vpart);
-        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, int(RED_RZ), kVecGrid, 3, vpart, m.scal.p);
+        hipLaunchKernelGGL(k_cg_update, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, it, flexible, m.scal.p, x,
+                           m.cg_p[1 - cur].p, m.cg_q.p, m.cg_r.p, m.cg_z.p, m.jac.p, vpart, m.counters.p + CNT_VEC);
+        if (flexible)
+            if (int rc = gcn_precond(m, m.cg_r.p, m.cg_q.p, m.cg_z.p, false)) return rc;
         cur = 1 - cur;
-        // Iterations queued after convergence are no-ops on the device (every kernel checks
-        // the done flag), so the host polls only every 16 iterations.
         if ((it & 15) == 15 || it == maxit - 1) {
             SVLG_HIP(hipMemcpyAsync(hs, m.scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
             SVLG_HIP(hipStreamSynchronize(s));
@@ -575,12 +526,14 @@ int svlg_nm_init(svlg_model *h, double dt, double rtol, int32_t maxit, int32_t p
     SVLG_CHECK(dt > 0.0 && rtol > 0.0 && maxit > 0, "dt, rtol and maxit must be positive");
     SVLG_CHECK(m.lumped, "the device Newmark path needs MassFormulation = LUMPED");
     SVLG_CHECK(m.variant == 3, "the device Newmark path needs corner-slot node tiles (a corner-unique mesh)");
-    SVLG_CHECK(precond == SVLG_PRECOND_JACOBI, "only the Jacobi preconditioner is available");
+    SVLG_CHECK(precond == SVLG_PRECOND_JACOBI,
+               "svlg_nm_init takes SVLG_PRECOND_JACOBI; switch with svlg_nm_set_precond after svlg_gnn_init");
     for (int mode : {OP_CG, OP_RHS})
         SVLG_CHECK(op_lds(m, mode) <= 160 * 1024, "Newmark operator tile needs more than 160 KB of LDS; use smaller tiles");
     hipStream_t s = m.stream;
     const int n = m.ndofn;
     m.nm_ready = false;
+    m.gcn_ready = false;
     m.nm_dt = dt;
     m.nm_rtol = rtol;
     m.nm_maxit = maxit;
@@ -590,9 +543,13 @@ int svlg_nm_init(svlg_model *h, double dt, double rtol, int32_t maxit, int32_t p
         SVLG_HIP(b->alloc(n));
         SVLG_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * n, s));
     }
-    SVLG_HIP(m.part.alloc(size_t(m.tiles.ntiles) + 3 * kVecGrid));
+    // Per-tile partials (one per tile for the operator, two for the preconditioner's output
+    // layer) and per-workgroup partials of the vector kernels, which start at ntiles.
+    SVLG_HIP(m.part.alloc(2 * size_t(m.tiles.ntiles) + 3 * kVecGrid));
     SVLG_HIP(m.scal.alloc(SC_COUNT));
     SVLG_HIP(hipMemsetAsync(m.scal.p, 0, sizeof(double) * SC_COUNT, s));
+    SVLG_HIP(m.counters.alloc(CNT_COUNT));
+    SVLG_HIP(hipMemsetAsync(m.counters.p, 0, sizeof(unsigned) * CNT_COUNT, s));
     // Per tile-element coefficients: 1 + 2 ak/dt (operator) and -ak (right-hand side).
     std::vector<double> ak_tag(m.nelem);
     SVLG_HIP(hipMemcpyAsync(ak_tag.data(), m.ak.p, sizeof(double) * m.nelem, hipMemcpyDeviceToHost, s));
@@ -685,7 +642,7 @@ int svlg_nm_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
         // Keff dU = Feff; the previous increment is the initial guess.
         int it = 0;
         double rel = 0.0;
-        const int rc = cg_solve(m, m.cg_b.p, m.ndU.p, m.nm_rtol, m.nm_maxit, &it, &rel);
+        const int rc = cg_solve(m, m.cg_b.p, m.ndU.p, m.nm_rtol, m.nm_maxit, m.nm_precond, &it, &rel);
         if (iters) iters[st] = it;
         if (relres) relres[st] = rel;
         if (rc) return rc;
@@ -700,6 +657,16 @@ int svlg_nm_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
     if (nrec > 0 && nsteps > 0)
         SVLG_HIP(hipMemcpyAsync(hist, hbuf.p, sizeof(double) * size_t(nrec) * nsteps, hipMemcpyDeviceToHost, s));
     SVLG_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int svlg_nm_set_precond(svlg_model *h, int32_t precond) {
+    SVLG_CHECK(h, "null model");
+    Model &m = h->m;
+    SVLG_CHECK(m.nm_ready, "svlg_nm_init has not been called");
+    SVLG_CHECK(precond == SVLG_PRECOND_JACOBI || precond == SVLG_PRECOND_LEARNED, "unknown preconditioner");
+    SVLG_CHECK(precond != SVLG_PRECOND_LEARNED || m.gcn_ready, "svlg_gnn_init has not been called");
+    m.nm_precond = precond;
     return 0;
 }
 
@@ -754,7 +721,7 @@ int svlg_solve(svlg_model *h, const double *bfree, double *xfree, double rtol, i
     if (int rc = scatter_free(m, f.p, bn.p)) return rc;
     int it = 0;
     double rel = 0.0;
-    const int rc = cg_solve(m, bn.p, xn.p, rtol, maxit, &it, &rel);
+    const int rc = cg_solve(m, bn.p, xn.p, rtol, maxit, m.nm_precond, &it, &rel);
     if (iters) *iters = it;
     if (relres) *relres = rel;
     if (rc) return rc;

@@ -22,7 +22,8 @@ namespace svlg {
 static thread_local std::string g_error;
 void set_error(const std::string &msg) { g_error = msg; }
 
-int prepare_fused(Model &m);  // svlg_explicit.hip
+int prepare_fused(Model &m);                           // svlg_explicit.hip
+int build_hexa_kmat(Model &m, DevBuf<float> &kmat);    // svlg_gcn.hip
 
 // ---------------------------------------------------------------- kernels
 // Per-element row-sum lumped mass, scattered later in ascending element order.
@@ -576,6 +577,21 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         set_error("model set-up kernels failed");
         return fail(10);
     }
+    {
+        const int prec = o ? o->elem_precision : 0;
+        if (prec == 1) {
+            if (!(ndim == 3 && m.variant == 3 && m.elem_path >= 2 && m.ngeo == 1)) {
+                set_error("elem_precision 1 (BF16 MFMA) needs a Hexa8 mesh with one element geometry and "
+                          "corner-slot tiles");
+                return fail(11);
+            }
+            if (int rc = build_hexa_kmat(m, m.cd_kmat)) return fail(rc);
+            m.elem_bf16 = 1;
+        } else if (prec != 0) {
+            set_error("elem_precision must be 0 (FP64) or 1 (BF16 MFMA)");
+            return fail(11);
+        }
+    }
     if (int rc = prepare_fused(m)) return fail(rc);
     *out = h;
     return 0;
@@ -700,6 +716,8 @@ int svlg_model_info(svlg_model *h, int64_t *info) {
     info[5] = m.tiles.ntiles;
     info[6] = m.elem_path;
     info[7] = int64_t(m.tiles.lds_bytes);
+    info[8] = m.elem_bf16;
+    info[9] = m.ngeo;
     return 0;
 }
 

@@ -24,11 +24,12 @@ from .mesh import Model
 class DeviceModel:
     """A model resident in HBM (Mesh::Initialize + Element::SetDomain)."""
 
-    def __init__(self, model: Model, tile_nodes: int = 0, block: int = 0, variant: int = 0, geo_table: int = 0):
+    def __init__(self, model: Model, tile_nodes: int = 0, block: int = 0, variant: int = 0, geo_table: int = 0,
+                 elem_precision: int = 0):
         self.model = model
         self._desc, self._keep = capi.make_desc(model)
         h = C.c_void_p()
-        opts = capi.ExecOpts(tile_nodes, block, variant, geo_table)
+        opts = capi.ExecOpts(tile_nodes, block, variant, geo_table, elem_precision)
         capi.check(capi.lib().svlg_model_create_ex(C.byref(self._desc), C.byref(opts), C.byref(h)),
                    "svlg_model_create")
         self.h = h
@@ -46,9 +47,10 @@ class DeviceModel:
             pass
 
     def info(self) -> dict:
-        a = (C.c_int64 * 8)()
+        a = (C.c_int64 * 10)()
         capi.check(capi.lib().svlg_model_info(self.h, a), "svlg_model_info")
-        keys = ["ncolors", "nelem", "nnodes", "nfree", "variant", "ntiles", "elem_path", "lds_bytes"]
+        keys = ["ncolors", "nelem", "nnodes", "nfree", "variant", "ntiles", "elem_path", "lds_bytes",
+                "elem_precision", "ngeo"]
         d = dict(zip(keys, [int(v) for v in a]))
         d["affine"] = int(d["elem_path"] > 0)
         return d
@@ -160,17 +162,16 @@ class ExtendedNewmarkBeta:
     directly; the tolerance stands in for the direct solve's round-off)."""
 
     def __init__(self, mesh: DeviceModel, TimeStep: float, mtol: float = 1e-12, ktol: float = 1e-12,
-                 ftol: float = 1e-12, rtol: float = 1e-10, maxit: int = 5000, precond: int = PRECOND_JACOBI):
+                 ftol: float = 1e-12, rtol: float = 1e-10, maxit: int = 5000):
         self.mesh = mesh
         self.dt = float(TimeStep)
         self.rtol = float(rtol)
         self.maxit = int(maxit)
-        self.precond = int(precond)
         self.iters = np.zeros(0, np.int32)
         self.relres = np.zeros(0)
 
     def Initialize(self):
-        capi.check(capi.lib().svlg_nm_init(self.mesh.h, self.dt, self.rtol, self.maxit, self.precond),
+        capi.check(capi.lib().svlg_nm_init(self.mesh.h, self.dt, self.rtol, self.maxit, PRECOND_JACOBI),
                    "svlg_nm_init")
 
     def ComputeNewStep(self, k: int) -> bool:
@@ -214,6 +215,70 @@ class ExtendedNewmarkBeta:
 
     def GetAccelerations(self) -> np.ndarray:
         return self._state()[2]
+
+
+class LearnedPreconditioner:
+    """Graph-convolution preconditioner of an initialised :class:`ExtendedNewmarkBeta`.
+
+    ``channels`` C in [1, 4], ``layers`` L in [2, 16]; ``mfma`` 1 / 0 / -1 (auto) selects the
+    BF16 MFMA element step. ``theta`` None starts from the Chebyshev preconditioner."""
+
+    def __init__(self, integrator: ExtendedNewmarkBeta, channels: int = 1, layers: int = 6, mfma: int = -1,
+                 theta=None):
+        self.integrator = integrator
+        self.mesh = integrator.mesh
+        t = None
+        if theta is not None:
+            t = np.ascontiguousarray(theta, dtype=np.float64)
+        capi.check(capi.lib().svlg_gnn_init(self.mesh.h, int(channels), int(layers), int(mfma),
+                                            capi.fptr(t) if t is not None else None), "svlg_gnn_init")
+        self.loss_hist = np.zeros(0)
+
+    def info(self) -> dict:
+        a = np.zeros(4, np.int32)
+        b = np.zeros(2)
+        capi.check(capi.lib().svlg_gnn_info(self.mesh.h, capi.iptr(a), capi.fptr(b)), "svlg_gnn_info")
+        return {"channels": int(a[0]), "layers": int(a[1]), "mfma": int(a[2]), "nparams": int(a[3]),
+                "lmin": float(b[0]), "lmax": float(b[1])}
+
+    @property
+    def theta(self) -> np.ndarray:
+        out = np.zeros(self.info()["nparams"])
+        capi.check(capi.lib().svlg_gnn_get_params(self.mesh.h, capi.fptr(out)), "svlg_gnn_get_params")
+        return out
+
+    @theta.setter
+    def theta(self, value):
+        v = np.ascontiguousarray(value, dtype=np.float64)
+        capi.check(capi.lib().svlg_gnn_set_params(self.mesh.h, capi.fptr(v)), "svlg_gnn_set_params")
+
+    def apply(self, r: np.ndarray, mfma: bool = False) -> np.ndarray:
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros(self.mesh.nfree)
+        capi.check(capi.lib().svlg_gnn_apply(self.mesh.h, int(bool(mfma)), capi.fptr(r), capi.fptr(z)),
+                   "svlg_gnn_apply")
+        return z
+
+    def loss_grad(self, R: np.ndarray, E: np.ndarray):
+        R = np.ascontiguousarray(R, dtype=np.float64)
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        loss = np.zeros(1)
+        g = np.zeros(self.info()["nparams"])
+        capi.check(capi.lib().svlg_gnn_loss_grad(self.mesh.h, int(R.shape[0]), capi.fptr(R), capi.fptr(E),
+                                                 capi.fptr(loss), capi.fptr(g)), "svlg_gnn_loss_grad")
+        return float(loss[0]), g
+
+    def train(self, nsteps: int = 40, every: int = 4, sweeps: int = 2, epochs: int = 60, lr: float = 2e-3):
+        hist = np.zeros(max(epochs, 1))
+        capi.check(capi.lib().svlg_gnn_train(self.mesh.h, int(nsteps), int(every), int(sweeps), int(epochs),
+                                             float(lr), capi.fptr(hist)), "svlg_gnn_train")
+        self.loss_hist = hist[:epochs]
+        return self.loss_hist
+
+    def use(self, on: bool = True):
+        """Selects this preconditioner (flexible CG) or Jacobi-PCG for later steps."""
+        capi.check(capi.lib().svlg_nm_set_precond(self.mesh.h, PRECOND_LEARNED if on else PRECOND_JACOBI),
+                   "svlg_nm_set_precond")
 
 
 class LinearSystem:
