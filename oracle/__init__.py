@@ -45,12 +45,24 @@ class OracleOpts(C.Structure):
         ("mtol", C.c_double),
         ("ktol", C.c_double),
         ("ftol", C.c_double),
+        ("algorithm", C.c_int32),
+        ("nmax", C.c_int32),
+        ("cnvgtest", C.c_int32),
+        ("pad", C.c_int32),
+        ("cnvgtol", C.c_double),
     ]
 
 
 CENTRAL_DIFFERENCE = 0
 EXTENDED_NEWMARK = 1
 NEWMARK = 2
+
+LINEAR = 0
+NEWTON = 1
+# Algorithm::ComputeConvergence flags (SVL's 'cnvgtest' names).
+CNVG = {"UnbalanceForce": 1, "IncrementalDisplacement": 2, "InterpolatedEnergy": 3,
+        "RelativeUnbalanceForce": 4, "RelativeIncrementalDisplacement": 5, "RelativeInterpolatedEnergy": 6,
+        "TotalRelativeIncrementalDisplacement": 7, "MaxIterations": 8}
 
 _L = None
 
@@ -66,13 +78,15 @@ def _lib():
         L.oracle_last_error.restype = C.c_char_p
         L.oracle_run.restype = C.c_int
         L.oracle_run.argtypes = [C.POINTER(MeshDesc), C.POINTER(OracleOpts), C.c_int32, i32p, f64p, f64p, f64p,
-                                 f64p, f64p]
+                                 f64p, f64p, i32p]
         L.oracle_element_matrices.restype = C.c_int
         L.oracle_element_matrices.argtypes = [C.POINTER(MeshDesc), C.c_int32, f64p, f64p, f64p]
         L.oracle_internal_force.restype = C.c_int
         L.oracle_internal_force.argtypes = [C.POINTER(MeshDesc), f64p, C.c_double, f64p]
         L.oracle_diagonals.restype = C.c_int
         L.oracle_diagonals.argtypes = [C.POINTER(MeshDesc), f64p, f64p]
+        L.oracle_j2_path.restype = C.c_int
+        L.oracle_j2_path.argtypes = [f64p, C.c_int32, C.c_int32, f64p, f64p, f64p, f64p]
         L.oracle_newmark_keff_dense.restype = C.c_int
         L.oracle_newmark_keff_dense.argtypes = [C.POINTER(MeshDesc), C.c_double, f64p]
         L.oracle_newmark_keff_apply.restype = C.c_int
@@ -90,9 +104,11 @@ def _f(a):
     return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
 
 
-def run(model, integrator: int, dt: float, nt: int, rec_dofs=(), ftol: float = 1e-12, want_va: bool = False):
+def run(model, integrator: int, dt: float, nt: int, rec_dofs=(), ftol: float = 1e-12, want_va: bool = False,
+        algorithm: int = LINEAR, nmax: int = 20, cnvgtest: int = 5, cnvgtol: float = 1e-6):
     """Runs k = 1..nt-1; returns dict with U (and V, A) histories [nt-1, nrec],
-    the final free-DOF displacement and the time-loop wall time in seconds."""
+    the final free-DOF displacement, the time-loop wall time in seconds and, for Newton,
+    the iterations of every step."""
     from svl_amd._capi import make_desc
     desc, keep = make_desc(model)
     rec = np.ascontiguousarray(rec_dofs, dtype=np.int32)
@@ -102,10 +118,11 @@ def run(model, integrator: int, dt: float, nt: int, rec_dofs=(), ftol: float = 1
     ha = np.zeros((ns, rec.size)) if want_va else None
     secs = C.c_double(0.0)
     uf = np.zeros(model.nfree)
-    o = OracleOpts(integrator, nt, dt, 1e-12, 1e-12, ftol)
+    o = OracleOpts(integrator, nt, dt, 1e-12, 1e-12, ftol, algorithm, nmax, cnvgtest, 0, cnvgtol)
+    its = np.zeros(max(ns, 1), np.int32)
     _chk(_lib().oracle_run(C.byref(desc), C.byref(o), rec.size, rec.ctypes.data_as(C.POINTER(C.c_int32)), _f(hu),
-                           _f(hv), _f(ha), C.byref(secs), _f(uf)), "run")
-    out = {"U": hu, "final_U": uf, "seconds": secs.value}
+                           _f(hv), _f(ha), C.byref(secs), _f(uf), its.ctypes.data_as(C.POINTER(C.c_int32))), "run")
+    out = {"U": hu, "final_U": uf, "seconds": secs.value, "iters": its[:ns]}
     if want_va:
         out["V"] = hv
         out["A"] = ha
@@ -150,6 +167,21 @@ def keff_dense(model, dt: float):
     K = np.zeros((n, n))
     _chk(_lib().oracle_newmark_keff_dense(C.byref(desc), dt, _f(K)), "keff_dense")
     return K
+
+
+def j2_path(params, strains, plane: bool = False):
+    """One J2 Gauss point driven through engineering strains [n, 6] (or [n, 3] plane
+    strain), one UpdateState per row: returns (stress [n, nv], tangent [n, nv, nv], alpha [n])."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    e = np.ascontiguousarray(strains, dtype=np.float64)
+    n, nv = e.shape
+    if nv != (3 if plane else 6) or p.size != 5:
+        raise ValueError("j2_path: strains must have 3 columns in plane strain, 6 in 3D; 5 parameters")
+    sig = np.zeros((n, nv))
+    tan = np.zeros((n, nv, nv))
+    al = np.zeros(n)
+    _chk(_lib().oracle_j2_path(_f(p), int(plane), n, _f(e), _f(sig), _f(tan), _f(al)), "j2_path")
+    return sig, tan, al
 
 
 def diagonals(model):

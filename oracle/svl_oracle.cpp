@@ -256,10 +256,102 @@ struct GaussCache {
     std::vector<double> w, xi;
 };
 
+// ---------------------------------------------------------------- J2 plasticity
+// Plastic3DJ2 / PlasticPlaneStrainJ2 state of one Gauss point (Plastic3DJ2.cpp:5-52). The
+// 6-vectors are tensor components [11, 22, 33, 12, 23, 31] (shears not doubled).
+struct J2State {
+    double eps[6] = {0, 0, 0, 0, 0, 0};
+    double sig[6] = {0, 0, 0, 0, 0, 0};
+    double ep[6] = {0, 0, 0, 0, 0, 0};   // PlasticStrain
+    double q[6] = {0, 0, 0, 0, 0, 0};    // BackStress
+    double alpha = 0.0;
+    double C[36];                         // TangentStiffness, row-major
+};
+
+struct J2Params {
+    double K, G, sy, H, beta;
+};
+
+// K D + 2G (I - D/3): D has ones in the normal 3x3 block, I = diag(1,1,1,1/2,1/2,1/2)
+// (ComputeIdentityTensor / ComputeIdentityOperator, Plastic3DJ2.cpp:282-312).
+void j2_elastic_tangent(const J2Params &p, double *C) {
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+            const double Dij = (i < 3 && j < 3) ? 1.0 : 0.0;
+            const double Iij = (i == j) ? (i < 3 ? 1.0 : 0.5) : 0.0;
+            C[i * 6 + j] = p.K * Dij + 2.0 * p.G * (Iij - Dij / 3.0);
+        }
+}
+
+// ComputeTensorNorm (Plastic3DJ2.cpp:321-329): shears counted twice.
+double j2_norm(const double *t) {
+    return std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2] + 2.0 * (t[3] * t[3] + t[4] * t[4] + t[5] * t[5]));
+}
+
+// Plastic3DJ2::UpdateState with cond = 1 (Plastic3DJ2.cpp:206-258); PlasticPlaneStrainJ2 is
+// the same on its embedded strain (PlasticPlaneStrainJ2.cpp:227-279). `et` is the tensor
+// strain. alpha, BackStress and PlasticStrain are advanced from their current values, not
+// from the committed ones, so repeated calls within one step accumulate (as in SVL).
+void j2_update(J2State &s, const J2Params &p, const double *et) {
+    for (int i = 0; i < 6; ++i) s.eps[i] = et[i];
+    const double tr = et[0] + et[1] + et[2];
+    const double one[6] = {1, 1, 1, 0, 0, 0};
+    double strial[6], xi[6];
+    for (int i = 0; i < 6; ++i) {
+        const double dev = et[i] - 1.0 / 3.0 * tr * one[i];
+        strial[i] = 2.0 * p.G * (dev - s.ep[i]);
+        xi[i] = strial[i] - s.q[i];
+    }
+    const double nx = j2_norm(xi);
+    const double F = nx - std::sqrt(2.0 / 3.0) * (p.sy + s.alpha * p.beta * p.H);
+    double Id[36];
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+            const double Dij = (i < 3 && j < 3) ? 1.0 : 0.0;
+            const double Iij = (i == j) ? (i < 3 ? 1.0 : 0.5) : 0.0;
+            Id[i * 6 + j] = Iij - 1.0 / 3.0 * Dij;
+        }
+    if (F <= 0) {
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) {
+                const double Dij = (i < 3 && j < 3) ? 1.0 : 0.0;
+                s.C[i * 6 + j] = p.K * Dij + 2.0 * p.G * Id[i * 6 + j];
+            }
+        for (int i = 0; i < 6; ++i) s.sig[i] = p.K * tr * one[i] + strial[i];
+        return;
+    }
+    double n[6];
+    for (int i = 0; i < 6; ++i) n[i] = xi[i] / nx;
+    const double dg = F / (2.0 * p.G + 2.0 / 3.0 * p.H);
+    s.alpha += std::sqrt(2.0 / 3.0) * dg;
+    for (int i = 0; i < 6; ++i) {
+        s.q[i] += 2.0 / 3.0 * (1.0 - p.beta) * p.H * dg * n[i];
+        s.ep[i] += dg * n[i];
+        s.sig[i] = p.K * tr * one[i] + strial[i] - 2.0 * p.G * dg * n[i];
+    }
+    const double c1 = 1.0 / (1.0 + p.H / p.G / 3.0);
+    const double c2 = 4.0 * p.G * p.G * dg / nx;
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+            const double Dij = (i < 3 && j < 3) ? 1.0 : 0.0;
+            const double nn = n[i] * n[j];
+            s.C[i * 6 + j] = p.K * Dij + 2.0 * p.G * (Id[i * 6 + j] - c1 * nn) - c2 * (Id[i * 6 + j] - nn);
+        }
+}
+
+// Engineering strain of the element (B u) -> tensor strain of the material:
+// 3D [e_xx, e_yy, e_zz, g_xy/2, g_yz/2, g_zx/2] (Plastic3DJ2.cpp:217); plane strain
+// embeds [0, e_xx, e_yy, 0, g_xy/2, 0] (PlasticPlaneStrainJ2.cpp:236) and reads back the
+// components (1, 2, 4) (:115-158).
+static const int kPsIdx[3] = {1, 2, 4};
+
 // One continuum element of the restated mesh.
 struct Continuum {
     int nodes[8];
     double E, nu, rho, am, ak;
+    int mat = 0;                  // SVLG_MAT_ELASTIC or SVLG_MAT_J2
+    J2Params j2p{0, 0, 0, 0, 0};
+    std::vector<J2State> j2;      // [ngp]
     std::vector<double> strain;  // [ngp * nvoigt] material Strain (trial)
     std::vector<double> strain_n; // committed newStrain
 };
@@ -332,14 +424,37 @@ Mat element_mass(const Model &m, const Continuum &e) {
     return M;
 }
 
-// lin*::ComputeStiffnessMatrix / ComputeInitialStiffnessMatrix (elastic tangent).
-Mat element_stiffness(const Model &m, const Continuum &e) {
+// Material matrix of Gauss point g in the element's Voigt order: the elastic D, or for J2
+// the initial tangent K D + 2G (I - D/3) (GetInitialTangentStiffness, Plastic3DJ2.cpp:146-159)
+// or the current consistent tangent (GetTangentStiffness), restricted to (1, 2, 4) in 2D.
+Mat material_D(const Model &m, const Continuum &e, int g, bool tangent) {
+    if (e.mat != SVLG_MAT_J2) return elastic_D(m.dim, e.E, e.nu);
+    double C6[36];
+    if (tangent)
+        std::memcpy(C6, e.j2[g].C, sizeof(C6));
+    else
+        j2_elastic_tangent(e.j2p, C6);
+    if (m.dim == 3) {
+        Mat D(6, 6);
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) D(i, j) = C6[i * 6 + j];
+        return D;
+    }
+    Mat D(3, 3);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) D(i, j) = C6[kPsIdx[i] * 6 + kPsIdx[j]];
+    return D;
+}
+
+// lin*::ComputeInitialStiffnessMatrix (tangent = false) / ComputeStiffnessMatrix (tangent =
+// true; lin3DHexa8.cpp:289-318): sum w t |J| B^T C B.
+Mat element_stiffness(const Model &m, const Continuum &e, bool tangent = false) {
     int nd = m.nen * m.dim, nv = m.nvoigt;
     Mat K(nd, nd);
     double X[24];
     m.element_X(e, X);
-    Mat C = elastic_D(m.dim, e.E, e.nu);
     for (int g = 0; g < m.ngp; ++g) {
+        const Mat C = material_D(m, e, g, tangent);
         Mat J = m.jac(X, g);
         Mat Ji; double det;
         lu_inverse(J, Ji, det);
@@ -387,12 +502,17 @@ void element_fint(const Model &m, const Continuum &e, double *f) {
         Mat Ji; double det;
         lu_inverse(J, Ji, det);
         Mat B = m.bmat(g, J);
-        const double *eps = &e.strain[size_t(g) * nv];
         double sig[6];
-        for (int i = 0; i < nv; ++i) {
-            double t = 0.0;
-            for (int k = 0; k < nv; ++k) t += C(i, k) * eps[k];
-            sig[i] = t;
+        if (e.mat == SVLG_MAT_J2) {
+            // Plastic*J2::GetStress: the stress of the last UpdateState.
+            for (int i = 0; i < nv; ++i) sig[i] = m.dim == 3 ? e.j2[g].sig[i] : e.j2[g].sig[kPsIdx[i]];
+        } else {
+            const double *eps = &e.strain[size_t(g) * nv];
+            for (int i = 0; i < nv; ++i) {
+                double t = 0.0;
+                for (int k = 0; k < nv; ++k) t += C(i, k) * eps[k];
+                sig[i] = t;
+            }
         }
         double s = m.gc.w[g] * m.thick() * std::fabs(det);
         for (int i = 0; i < nd; ++i) {
@@ -417,6 +537,17 @@ void element_update(const Model &m, Continuum &e, const std::vector<double> &Ut)
             double t = 0.0;
             for (int k = 0; k < nd; ++k) t += B(i, k) * u[k];
             e.strain[size_t(g) * nv + i] = t;
+        }
+        if (e.mat == SVLG_MAT_J2) {
+            const double *es = &e.strain[size_t(g) * nv];
+            double et[6] = {0, 0, 0, 0, 0, 0};
+            if (m.dim == 3) {
+                et[0] = es[0]; et[1] = es[1]; et[2] = es[2];
+                et[3] = 0.5 * es[3]; et[4] = 0.5 * es[4]; et[5] = 0.5 * es[5];
+            } else {
+                et[1] = es[0]; et[2] = es[1]; et[4] = 0.5 * es[2];
+            }
+            j2_update(e.j2[g], e.j2p, et);
         }
     }
 }
@@ -599,8 +730,17 @@ Model build(const svlg_mesh_desc *d) {
     for (int e = 0; e < d->nelem; ++e) {
         Continuum &c = m.elems[e];
         for (int a = 0; a < m.nen; ++a) c.nodes[a] = d->conn[size_t(e) * m.nen + a];
-        if (d->mat_type && d->mat_type[e] != SVLG_MAT_ELASTIC)
-            throw std::runtime_error("oracle: only elastic materials are restated here");
+        c.mat = d->mat_type ? d->mat_type[e] : SVLG_MAT_ELASTIC;
+        if (c.mat == SVLG_MAT_J2) {
+            if (!d->j2) throw std::runtime_error("oracle: J2 element without J2 parameters");
+            const double *q = d->j2 + size_t(e) * 5;
+            c.j2p = {q[0], q[1], q[2], q[3], q[4]};
+            J2State s0;
+            j2_elastic_tangent(c.j2p, s0.C);
+            c.j2.assign(m.ngp, s0);
+        } else if (c.mat != SVLG_MAT_ELASTIC) {
+            throw std::runtime_error("oracle: unknown material type");
+        }
         c.E = d->E[e]; c.nu = d->nu[e]; c.rho = d->rho[e];
         c.am = d->ray_am ? d->ray_am[e] : 0.0;
         c.ak = d->ray_ak ? d->ray_ak[e] : 0.0;
@@ -669,14 +809,16 @@ struct Assembler {
         }
         return CSC::from(s);
     }
-    // Assembler::ComputeStiffnessMatrix (Assembler.cpp:71-113); dashpots add k = 0.
-    CSC stiffness() const {
+    // Assembler::ComputeStiffnessMatrix (Assembler.cpp:71-113); dashpots add k = 0. With
+    // tangent = true the elements use the current material tangent (Newton), else the
+    // initial one.
+    CSC stiffness(bool tangent = false) const {
         Sparse s; s.n = m->ntotal;
         int dofs[24];
         int nd = m->nen * m->dim;
         for (auto &e : m->elems) {
             elem_dofs(e, dofs);
-            Mat Ke = element_stiffness(*m, e);
+            Mat Ke = element_stiffness(*m, e, tangent);
             for (int j = 0; j < nd; ++j)
                 for (int i = 0; i < nd; ++i)
                     if (std::fabs(Ke(i, j)) > ktol) s.add(dofs[i], dofs[j], Ke(i, j));
@@ -755,14 +897,56 @@ typedef struct oracle_opts {
     int32_t nt;          // Analysis 'nt': steps k = 1..nt-1
     double dt;
     double mtol, ktol, ftol;
+    int32_t algorithm;   // 0 Linear, 1 NewtonRaphson (Newmark integrators only)
+    int32_t nmax;        // Newton 'nstep'
+    int32_t cnvgtest;    // Algorithm::ComputeConvergence flag 1..8
+    int32_t pad;
+    double cnvgtol;      // Newton 'cnvgtol'
 } oracle_opts;
+
+// Algorithm::ComputeConvergence (Algorithm.cpp:122-188) on one process: Force = Feff,
+// Delta = accumulated increment, delta = this iteration's solution, k = iteration index.
+double convergence(int flag, const std::vector<double> &F, const std::vector<double> &Dl,
+                   const std::vector<double> &dl, int k, double &NormFactor) {
+    auto nrm = [](const std::vector<double> &v) {
+        double s2 = 0.0;
+        for (double x : v) s2 += x * x;
+        return std::sqrt(s2);
+    };
+    auto energy = [&]() {
+        double s2 = 0.0;
+        for (size_t i = 0; i < F.size(); ++i) s2 += (dl[i] * F[i]) * (dl[i] * F[i]);
+        return std::sqrt(s2);
+    };
+    switch (flag) {
+        case 1: return nrm(F);
+        case 2: return nrm(dl);
+        case 3: return energy();
+        case 4:
+            if (k != 0) return nrm(F) / NormFactor;
+            NormFactor = nrm(F);
+            return NormFactor;
+        case 5:
+            if (k != 0) return nrm(dl) / NormFactor;
+            NormFactor = nrm(Dl);
+            return nrm(dl);
+        case 6:
+            if (k != 0) return energy() / NormFactor;
+            NormFactor = energy();
+            return NormFactor;
+        case 7: return nrm(dl) / nrm(Dl);
+        case 8: return -1.0;
+        default: return 0.0;
+    }
+}
 
 const char *oracle_last_error(void) { return g_err.c_str(); }
 
 // Runs a dynamic analysis; hist_* are [(nt-1) * nrec] (any may be NULL). rec_dofs are
 // free DOF indices. `seconds` (optional) receives the wall time of the time loop only.
 int oracle_run(const svlg_mesh_desc *d, const oracle_opts *o, int32_t nrec, const int32_t *rec_dofs,
-               double *hist_U, double *hist_V, double *hist_A, double *seconds, double *final_U) {
+               double *hist_U, double *hist_V, double *hist_A, double *seconds, double *final_U,
+               int32_t *newton_iters) {
     try {
         Model m = build(d);
         Assembler as{&m, o->mtol, o->ktol, o->ftol};
@@ -784,11 +968,15 @@ int oracle_run(const svlg_mesh_desc *d, const oracle_opts *o, int32_t nrec, cons
             CSC KM = combine(1.0, K, 4.0 / dt / dt, M);
             Kt = combine(1.0, KM, 2.0 / dt, C);
         }
-        CSC Kf = free_block(m, Kt);
-        if (!solver.factor(Kf)) {
-            g_err = "EigenSolver::SolveSystem() : Solution for the linear system failed.";
-            return 1;
+        const bool newton = o->integrator != 0 && o->algorithm == 1;
+        if (!newton) {
+            CSC Kf = free_block(m, Kt);
+            if (!solver.factor(Kf)) {
+                g_err = "EigenSolver::SolveSystem() : Solution for the linear system failed.";
+                return 1;
+            }
         }
+        double NormFactor = 0.0;  // Algorithm member, kept across steps
         std::vector<double> Fint, Fext, Ft(m.ntotal), Feff, x, dUt, tmp, tmp2;
         std::vector<int> rec_tot(nrec, -1);
         for (int r = 0; r < nrec; ++r) {
@@ -798,6 +986,56 @@ int oracle_run(const svlg_mesh_desc *d, const oracle_opts *o, int32_t nrec, cons
         }
         auto t0 = std::chrono::steady_clock::now();
         for (int k = 1; k < o->nt; ++k) {
+            if (newton) {
+                // NewtonRaphson::ComputeNewIncrement (NewtonRaphson.cpp:24-75): Feff and the
+                // tangent Keff from the current iterate, solve, dU += du, convergence test,
+                // then UpdateStatesIncrements(U + dU).
+                std::vector<double> dUf(m.nfree, 0.0), du, w(m.ntotal), v2(m.ntotal), dUtot;
+                double Residual = 0.0;
+                int it = 0;
+                do {
+                    to_total(m, dUf, dUtot);
+                    as.fint(Fint);
+                    as.fext(k, Fext);
+                    if (o->integrator == 2) {
+                        // NewmarkBeta::ComputeEffectiveForce (NewmarkBeta.cpp:212-228):
+                        // Fext + Fbar - Fint + M (4/dt V + A - 4/dt^2 dU) + C (V - 2/dt dU).
+                        for (int i = 0; i < m.ntotal; ++i) {
+                            w[i] = 4.0 / dt * V[i] + A[i] - 4.0 / dt / dt * dUtot[i];
+                            v2[i] = V[i] - 2.0 / dt * dUtot[i];
+                        }
+                        M.mul(w, tmp2);
+                        C.mul(v2, tmp);
+                        for (int i = 0; i < m.ntotal; ++i) Ft[i] = (((Fext[i] + 0.0) - Fint[i]) + tmp2[i]) + tmp[i];
+                    } else {
+                        // ExtendedNewmarkBeta::ComputeEffectiveForce (:219-225): no dU terms.
+                        for (int i = 0; i < m.ntotal; ++i) w[i] = 4.0 / dt * V[i] + A[i];
+                        C.mul(V, tmp);
+                        M.mul(w, tmp2);
+                        for (int i = 0; i < m.ntotal; ++i) Ft[i] = (((Fext[i] + 0.0) - Fint[i]) + tmp[i]) + tmp2[i];
+                    }
+                    to_free(m, Ft, Feff);
+                    // ComputeEffectiveStiffness: tangent K + 4/dt^2 M + 2/dt C, refactorised.
+                    CSC Ktan = as.stiffness(true);
+                    CSC KM = combine(1.0, Ktan, 4.0 / dt / dt, M);
+                    CSC Kn = combine(1.0, KM, 2.0 / dt, C);
+                    Skyline sk;
+                    if (!sk.factor(free_block(m, Kn))) {
+                        g_err = "EigenSolver::SolveSystem() : Solution for the linear system failed.";
+                        return 1;
+                    }
+                    sk.solve(Feff, du);
+                    for (int i = 0; i < m.nfree; ++i) dUf[i] += du[i];
+                    Residual = convergence(o->cnvgtest, Feff, dUf, du, it, NormFactor);
+                    to_total(m, dUf, dUtot);
+                    std::vector<double> Un(m.ntotal);
+                    for (int i = 0; i < m.ntotal; ++i) Un[i] = U[i] + dUtot[i];
+                    update_states(m, Un);
+                    ++it;
+                } while ((Residual > o->cnvgtol) && (it < o->nmax));
+                if (newton_iters) newton_iters[k - 1] = it;
+                dUt = dUtot;
+            } else {
             as.fint(Fint);
             as.fext(k, Fext);
             if (o->integrator == 0) {
@@ -823,6 +1061,7 @@ int oracle_run(const svlg_mesh_desc *d, const oracle_opts *o, int32_t nrec, cons
             std::vector<double> Un(m.ntotal);
             for (int i = 0; i < m.ntotal; ++i) Un[i] = U[i] + dUt[i];
             update_states(m, Un);
+            }
             if (o->integrator == 0) {
                 // CentralDifference::ComputeNewStep (:141-148).
                 for (int i = 0; i < m.ntotal; ++i) {
@@ -943,6 +1182,39 @@ int oracle_newmark_keff_dense(const svlg_mesh_desc *d, double dt, double *Kd) {
         g_err = ex.what();
         return 3;
     }
+}
+
+// Drives one Plastic3DJ2 (plane = 0) or PlasticPlaneStrainJ2 (plane = 1) Gauss point
+// through n engineering strains (6 or 3 per row), one UpdateState(strain, 1) per row, and
+// returns per row the stress and the consistent tangent in the element's Voigt order and
+// alpha. params5 = K, G, sigma_y, H, beta.
+int oracle_j2_path(const double *params5, int32_t plane, int32_t n, const double *strain, double *stress,
+                   double *tangent, double *alpha) {
+    const J2Params p{params5[0], params5[1], params5[2], params5[3], params5[4]};
+    J2State s;
+    j2_elastic_tangent(p, s.C);
+    const int nv = plane ? 3 : 6;
+    for (int r = 0; r < n; ++r) {
+        const double *es = strain + size_t(r) * nv;
+        double et[6] = {0, 0, 0, 0, 0, 0};
+        if (!plane) {
+            et[0] = es[0]; et[1] = es[1]; et[2] = es[2];
+            et[3] = 0.5 * es[3]; et[4] = 0.5 * es[4]; et[5] = 0.5 * es[5];
+        } else {
+            et[1] = es[0]; et[2] = es[1]; et[4] = 0.5 * es[2];
+        }
+        j2_update(s, p, et);
+        for (int i = 0; i < nv; ++i) {
+            const int ii = plane ? kPsIdx[i] : i;
+            if (stress) stress[size_t(r) * nv + i] = s.sig[ii];
+            for (int j = 0; j < nv; ++j) {
+                const int jj = plane ? kPsIdx[j] : j;
+                if (tangent) tangent[(size_t(r) * nv + i) * nv + j] = s.C[ii * 6 + jj];
+            }
+        }
+        if (alpha) alpha[r] = s.alpha;
+    }
+    return 0;
 }
 
 int oracle_diagonals(const svlg_mesh_desc *d, double *Mfree, double *Cfree) {

@@ -71,9 +71,9 @@ class Model:
         self.conn = np.ascontiguousarray(self.conn, dtype=np.int32)
         n = self.coords.shape[0]
         ne = self.conn.shape[0]
-        self.E = np.ascontiguousarray(np.broadcast_to(self.E, (ne,)), dtype=np.float64)
-        self.nu = np.ascontiguousarray(np.broadcast_to(self.nu, (ne,)), dtype=np.float64)
-        self.rho = np.ascontiguousarray(np.broadcast_to(self.rho, (ne,)), dtype=np.float64)
+        self.E = np.array(np.broadcast_to(self.E, (ne,)), dtype=np.float64)
+        self.nu = np.array(np.broadcast_to(self.nu, (ne,)), dtype=np.float64)
+        self.rho = np.array(np.broadcast_to(self.rho, (ne,)), dtype=np.float64)
         if self.restrained is None:
             self.restrained = np.zeros((n, self.ndim), dtype=bool)
         self.restrained = np.ascontiguousarray(self.restrained, dtype=bool)
@@ -87,8 +87,8 @@ class Model:
             self.ray_am = np.zeros(ne)
         if self.ray_ak is None:
             self.ray_ak = np.zeros(ne)
-        self.ray_am = np.ascontiguousarray(np.broadcast_to(self.ray_am, (ne,)), dtype=np.float64)
-        self.ray_ak = np.ascontiguousarray(np.broadcast_to(self.ray_ak, (ne,)), dtype=np.float64)
+        self.ray_am = np.array(np.broadcast_to(self.ray_am, (ne,)), dtype=np.float64)
+        self.ray_ak = np.array(np.broadcast_to(self.ray_ak, (ne,)), dtype=np.float64)
         self.dash_node = np.ascontiguousarray(self.dash_node, dtype=np.int32)
         self.dash_dir = np.ascontiguousarray(self.dash_dir, dtype=np.int32)
         self.dash_eta = np.ascontiguousarray(self.dash_eta, dtype=np.float64)
@@ -102,6 +102,18 @@ class Model:
             self.load_series = np.zeros((0, 1))
         self.load_dir = np.ascontiguousarray(self.load_dir)
         self.load_series = np.ascontiguousarray(self.load_series)
+
+    def set_j2(self, K, G, sigma_y, H, beta, elems=None):
+        """Makes elements J2 (Plastic3DJ2 / PlasticPlaneStrainJ2) with bulk modulus K, shear
+        modulus G, yield stress sigma_y, hardening modulus H and mixing beta (1 isotropic,
+        0 kinematic); E and nu are set to the equivalent elastic constants."""
+        idx = np.arange(self.nelem) if elems is None else np.asarray(elems)
+        self.mat_type[idx] = MAT_J2
+        self.j2[idx] = np.stack(np.broadcast_arrays(K, G, sigma_y, H, beta), axis=-1)
+        K = np.broadcast_to(K, idx.shape)
+        G = np.broadcast_to(G, idx.shape)
+        self.E[idx] = 9.0 * K * G / (3.0 * K + G)
+        self.nu[idx] = (3.0 * K - 2.0 * G) / (2.0 * (3.0 * K + G))
 
     # ------------------------------------------------------------------ sizes
     @property

@@ -15,6 +15,9 @@ Cases (SVL case name, what it pins):
   J02  1 lin3DHexa8, Elastic3DLinear, lumped mass, Newmark dt=0.025, 200 steps
   F06  100 lin2DQuad4 soil column, Rayleigh, 2 Viscous1DLinear base dashpots, Newmark
   J05  100 lin3DHexa8 soil column, Rayleigh, 8 Viscous1DLinear base dashpots, Newmark
+  F03  1 lin2DQuad4, PlasticPlaneStrainJ2, massless, Newton + Newmark dt=0.001, 999 steps
+  F07  100 lin2DQuad4 J2 soil column, Rayleigh, 2 base dashpots, Newton + Newmark
+       (first 4000 of 9999 steps kept)
 """
 from __future__ import annotations
 
@@ -32,7 +35,10 @@ CASES = {
     "J02": ("J02-DY_Lin_3DPointLoad_Elastic_Hexa8", "StressLoad.txt"),
     "F06": ("F06-DY_Lin_2DSoilColumn_ElasticPStrain_Quad4", "ricker.in"),
     "J05": ("J05-DY_Lin_3DSoilColumn_Elastic_Hexa8", "ricker.in"),
+    "F03": ("F03-DY_Lin_2DPointLoad_J2PStrain_Quad4", "ricker.in"),
+    "F07": ("F07-DY_Lin_2DSoilColumn_J2PStrain_Quad4", "ricker-SoilColumnPlasticPlaneStrainJ2.in"),
 }
+KEEP_ROWS = {"F07": 4000}
 
 
 def read_member(z: zipfile.ZipFile, suffix: str) -> str:
@@ -62,6 +68,9 @@ def main(svl_root: str):
                         break
                     except FileNotFoundError:
                         continue
+        if key in KEEP_ROWS:
+            nk = KEEP_ROWS[key]
+            out = {k: (v[:nk + 1] if k == "load" else v[:nk]) for k, v in out.items()}
         np.savez_compressed(HERE / f"{key}.npz", **out)
         print(key, {k: v.shape for k, v in out.items()})
 
