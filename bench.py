@@ -158,7 +158,10 @@ def implicit_bench(args):
     lf = timed("learned FCG")
     lf.update({"channels": info["channels"], "layers": info["layers"], "mfma_bf16": bool(info["mfma"]),
                "train_s": train_s, "loss_first": float(loss[0]) if len(loss) else None,
-               "loss_last": float(loss[-1]) if len(loss) else None})
+               "loss_last": float(loss[-1]) if len(loss) else None,
+               "trained_params_accepted": bool(pc.accepted),
+               "validation_iters_init_vs_trained": [info["val_iters_before"], info["val_iters_after"]],
+               "spectrum_ehat_lanczos": [info["lmin"], info["lmax"]]})
     out["learned_fcg"] = lf
     out["speedup_vs_jacobi"] = out["jacobi_pcg"]["s_per_step"] / lf["s_per_step"]
     dm.close()
@@ -181,8 +184,8 @@ def main():
     ap.add_argument("--implicit-ne", type=int, default=64, help="elements per side of the implicit block")
     ap.add_argument("--implicit-rtol", type=float, default=1e-8)
     ap.add_argument("--gnn-channels", type=int, default=1)
-    ap.add_argument("--gnn-layers", type=int, default=6)
-    ap.add_argument("--gnn-epochs", type=int, default=40)
+    ap.add_argument("--gnn-layers", type=int, default=8)
+    ap.add_argument("--gnn-epochs", type=int, default=30)
     ap.add_argument("--no-implicit", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="only run the device copy of MB megabytes (counter calibration) and exit")

@@ -188,8 +188,10 @@ int svlg_solve(svlg_model *m, const double *b /*[nfree]*/, double *x /*[nfree]*/
  * geometry), 0 = FP64 element routine, -1 = MFMA when possible; theta NULL = Chebyshev
  * initialisation on power-iteration bounds of the spectrum of Ehat. */
 int svlg_gnn_init(svlg_model *m, int32_t channels, int32_t layers, int32_t mfma, const double *theta);
-/* info[0]=channels [1]=layers [2]=MFMA in use [3]=parameter count; bounds = (lmin, lmax). */
-int svlg_gnn_info(svlg_model *m, int32_t *info4, double *bounds2);
+/* info[0]=channels [1]=layers [2]=MFMA in use [3]=parameter count [4], [5]=validation
+ * flexible-CG iterations before / after the last training; bounds = Lanczos (lmin, lmax)
+ * of Ehat. */
+int svlg_gnn_info(svlg_model *m, int32_t *info6, double *bounds2);
 int svlg_gnn_get_params(svlg_model *m, double *theta);
 int svlg_gnn_set_params(svlg_model *m, const double *theta);
 /* z = P r over the free DOFs (mfma selects the element step of this call). */
@@ -201,10 +203,12 @@ int svlg_gnn_loss_grad(svlg_model *m, int32_t npairs, const double *r, const dou
                        double *grad);
 /* Trains theta with Adam (learning rate lr) for `epochs` full-batch epochs on pairs from
  * the model's own Newmark solves: steps 1..nsteps from rest, every `every`-th step gives
- * (Feff, dU) plus `sweeps` Jacobi-smoothed pairs. The state is back at rest afterwards.
- * loss_hist[epochs] may be NULL. */
+ * (Feff, dU), `sweeps` Jacobi-smoothed pairs and one white-noise pair (Keff e, e). The
+ * trained theta is kept only if it lowers the flexible-CG iterations of a validation solve
+ * (*accepted = 1), else the previous theta is restored. The state is back at rest afterwards.
+ * loss_hist[epochs] and accepted may be NULL. */
 int svlg_gnn_train(svlg_model *m, int32_t nsteps, int32_t every, int32_t sweeps, int32_t epochs, double lr,
-                   double *loss_hist);
+                   double *loss_hist, int32_t *accepted);
 
 /* Device-to-device copy of n doubles with a plain coalesced kernel on `stream` (a
  * hipStream_t, NULL = default stream). No counterpart in SVL: it gives the bytes that

@@ -137,7 +137,7 @@ SIGNATURES = {
     "svlg_gnn_set_params": (C.c_int, [C.c_void_p, f64p]),
     "svlg_gnn_apply": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p]),
     "svlg_gnn_loss_grad": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p, f64p, f64p]),
-    "svlg_gnn_train": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, f64p]),
+    "svlg_gnn_train": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, f64p, i32p]),
     "svlg_stream_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
 }
 

@@ -132,9 +132,9 @@ struct Model {
     DevBuf<double> jac;               // 1 / diag(Keff), 0 on restrained DOFs
     DevBuf<double> coef_keff, coef_rhs;  // per tile element: 1 + 2 ak/dt, -ak
     DevBuf<double> cg_r, cg_z, cg_q, cg_b, cg_p[2];
-    DevBuf<double> part;              // per-block partial sums
+    DevBuf<double> part;              // partial sums of a solve (CgParts)
     DevBuf<double> scal;              // device scalars of the solver
-    DevBuf<unsigned> counters;        // arrival counters of the last-workgroup reductions
+    int cg_last_iters[2] = {0, 0};    // iterations of the last solve (Jacobi, flexible)
     std::vector<int32_t> tile_elem_id;   // tag index of every tile element (host)
 
     // Learned graph-convolution preconditioner (svlg_gcn.hip). Hidden states are FP32
@@ -142,12 +142,14 @@ struct Model {
     bool gcn_ready = false;
     int gcn_C = 1, gcn_L = 1, gcn_mfma = 0;
     double gcn_lmin = 0.0, gcn_lmax = 0.0;
+    int gcn_val_iters[2] = {0, 0};    // validation iterations before / after the last training
     std::vector<double> gcn_theta;
     DevBuf<float> gcn_w;              // theta in FP32
     DevBuf<double> gcn_s;             // D^-1/2 (0 on restrained DOFs)
     DevBuf<float> gcn_sf;             // the same in FP32
     DevBuf<double> gcn_elam, gcn_emu; // per tile element (lambda, mu) * (1 + 2 ak/dt)
     DevBuf<float> gcn_kmat;           // [2][24][24] K_lambda, K_mu of the single geometry class (MFMA path)
+    DevBuf<uint4> gcn_kimg;           // [3][64] per-lane BF16 A operands of the MFMA element step
     DevBuf<float> gcn_x0;             // s * r
     DevBuf<float> gcn_h[2];           // inference ping-pong hidden states
     DevBuf<float> gcn_H, gcn_Y;       // training: H^l and Ehat H^(l-1) for every layer

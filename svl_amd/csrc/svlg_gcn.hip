@@ -41,6 +41,7 @@ namespace svlg {
 int gather_free(Model &m, const double *dnode, double *dfree);
 int scatter_free(Model &m, const double *dfree, double *dnode);
 int keff_apply_node(Model &m, const double *x, double *y);
+int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int precond, int *iters, double *relres);
 
 static constexpr int kMaxC = 4;
 static constexpr int kMaxL = 16;
@@ -57,7 +58,7 @@ struct GcnArgs {
     const double *geo, *X;
     double th;
     int ntiles, max_stage, max_own;
-    const float *kmat;         // [2][24][24] (MFMA path)
+    const float *kmat;         // [2][24][24] (MFMA path); the fast kernel takes the A-operand image
     const double *dgn, *s;
     const float *sf;
     int C, ndofn;
@@ -73,20 +74,21 @@ struct GcnArgs {
     // G_OUT
     const float *w;
     const double *r, *q;
-    double *z, *part, *scal;
-    unsigned *counter;
-    int first, normalize;
+    double *z, *scal;
+    CgParts cg;
+    int iter, normalize;
 };
 
 __device__ __forceinline__ float mixw(const float *M, int C, int c, int k, int trans) {
     return trans ? M[k * C + c] : M[c * C + k];
 }
 
-// sigma of the input scaling: rms of the residual (from the solver's r.r) or 1.
-__device__ __forceinline__ double gcn_sigma(const double *scal, int normalize, int n) {
+// sigma of the input scaling: rms of r_j (from the solver's partials) or 1.
+__device__ __forceinline__ double gcn_sigma(const CgParts &c, int j, int normalize, int n, double *buf) {
     if (!normalize) return 1.0;
-    const double rr = scal[SC_RR];
-    return rr > 0.0 ? sqrt(rr / double(n)) : 1.0;
+    double v[3];
+    cg_vec(c, j, buf, v);
+    return v[1] > 0.0 ? sqrt(v[1] / double(n)) : 1.0;
 }
 
 // ---------------------------------------------------------------- element phase
@@ -130,11 +132,12 @@ __global__ __launch_bounds__(512) void k_gcn_layer(GcnArgs a) {
     constexpr int NEN = Elem<DIM>::nen;
     constexpr bool STAGE_X = PATH < 2;
     extern __shared__ double smem[];
-    __shared__ double red[17];
-    __shared__ int last;
+    __shared__ double red[3 * 17];
     if constexpr (MODE != G_BWD) {
         if (a.scal && a.scal[SC_DONE] != 0.0) return;
     }
+    double sig = 1.0;
+    if constexpr (MODE == G_OUT) sig = gcn_sigma(a.cg, a.iter, a.normalize, a.ndofn, red);
     const int nt = a.ntiles;
     const int bI = blockIdx.x, xcd = bI & 7, j = bI >> 3;
     const int q = nt >> 3, r = nt & 7;
@@ -230,7 +233,6 @@ __global__ __launch_bounds__(512) void k_gcn_layer(GcnArgs a) {
                 }
             }
             if constexpr (MODE == G_OUT) {
-                const double sig = gcn_sigma(a.scal, a.normalize, N);
                 const double zv = sig * a.s[g] * zsum;
                 a.z[g] = zv;
                 acc0 += zv * a.r[g];
@@ -249,28 +251,178 @@ __global__ __launch_bounds__(512) void k_gcn_layer(GcnArgs a) {
         acc0 = block_sum(acc0, red);
         acc1 = block_sum(acc1, red);
         if (tid == 0) {
-            a.part[2 * tile] = acc0;
-            a.part[2 * tile + 1] = acc1;
-        }
-        if (arrive_last(a.counter, gridDim.x, &last)) {
-            const double zr = sum_partials(a.part, nt, 2, red);
-            const double zq = sum_partials(a.part + 1, nt, 2, red);
-            if (tid == 0) {
-                const double pq = a.scal[SC_PQ];
-                a.scal[SC_BETA] = (a.first || pq == 0.0) ? 0.0 : -zq / pq;
-                a.scal[SC_RHO] = zr;
-            }
-            release_counter(a.counter);
+            double *t = a.cg.tp + size_t(a.iter & 1) * 3 * nt;
+            t[3 * tile] = acc0;
+            t[3 * tile + 1] = acc1;
+            t[3 * tile + 2] = 0.0;
         }
     }
 }
 
+// Inference layer for one channel with the BF16 MFMA element step (Hexa8, single element
+// geometry): out = a_l in + b_l (Ehat in) + c_l x0, and on the output layer z = sigma s (w out)
+// with the tile partials of z.r and z.q. The element data of the workgroup's element blocks
+// are prefetched before staging, the corner slots and the node-phase arithmetic are FP32
+// (preconditioner precision); z and the partials are FP64.
+// 16-bit staged slot of corner 2w + h from the packed connectivity (four 32-bit words).
+__device__ __forceinline__ int packed_half(uint32_t w, int h) { return int((w >> (16 * h)) & 0xffffu); }
+
+template <bool OUT>
+__global__ __launch_bounds__(512) void k_gcn_fast(GcnArgs a) {
+    constexpr int MAXR = 4;  // element rounds per wave: tiles hold at most 512 elements
+    extern __shared__ double smem[];
+    __shared__ double red[3 * 17];
+    if (a.scal && a.scal[SC_DONE] != 0.0) return;
+    double sig = 1.0;
+    if constexpr (OUT) sig = gcn_sigma(a.cg, a.iter, a.normalize, a.ndofn, red);
+    const int nt = a.ntiles;
+    const int bI = blockIdx.x, xcd = bI & 7, jb = bI >> 3;
+    const int qq = nt >> 3, rr = nt & 7;
+    const int tile = xcd * qq + min(xcd, rr) + jb;
+    const int o0 = a.own_ptr[tile], nown = a.own_ptr[tile + 1] - o0;
+    const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
+    const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
+    const int MS = a.max_stage, MO = a.max_own;
+    // LDS: corner slots [24][MO] + one trash row, staged operand [MS * 3] + one zero cell.
+    float *sIn = reinterpret_cast<float *>(smem);
+    const int trash = 24 * MO;
+    float *sX = sIn + size_t(25) * MO;
+    const int zcell = 3 * MS;
+    const int tid = threadIdx.x, B = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nw = B >> 6, n16 = lane & 15, g = lane >> 4;
+    // Prefetch the A-operand image (four 16 x 32 BF16 blocks) and this lane's element of every
+    // round. k-slot order: lane group g carries corners 2g and 2g+1 (k = 8g + jj -> element DOF
+    // 6g + jj, jj < 6; jj = 6, 7 are zero). Row order: blocks 0, 1 (K_lambda) and 2, 3 (K_mu)
+    // give lane group g the rows of corner 4(b%2) + g (row 4g + i -> DOF 3 corner + i, i < 3).
+    const uint4 *kimg = reinterpret_cast<const uint4 *>(a.kmat);
+    uint4 ak[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) ak[b] = kimg[b * 64 + lane];
+    uint4 pk[MAXR];
+    float lamr[MAXR], mur[MAXR];
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) {
+        const int el = min(wave * 16 + r * nw * 16 + n16, max(ne - 1, 0));
+        pk[r] = *reinterpret_cast<const uint4 *>(a.lconn + size_t(e0 + el) * 8);
+        lamr[r] = float(a.elam[e0 + el]);
+        mur[r] = float(a.emu[e0 + el]);
+    }
+    const size_t g0 = size_t(o0) * 3;
+    for (int i = tid; i < nown * 3; i += B) sX[i] = a.sf[g0 + i] * a.in[g0 + i];
+    for (int sl = tid; sl < nh; sl += B) {
+        const size_t nd = size_t(a.halo_node[h0 + sl]) * 3;
+        const float s0 = a.sf[nd], s1 = a.sf[nd + 1], s2 = a.sf[nd + 2];
+        const float v0 = a.in[nd], v1 = a.in[nd + 1], v2 = a.in[nd + 2];
+        float *d = sX + (nown + sl) * 3;
+        d[0] = s0 * v0;
+        d[1] = s1 * v1;
+        d[2] = s2 * v2;
+    }
+    for (int i = tid; i < 24 * MO; i += B) sIn[i] = 0.f;
+    if (tid == 0) sX[zcell] = 0.f;
+    __syncthreads();
+    bf16x8 Ak[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) Ak[b] = *reinterpret_cast<const bf16x8 *>(&ak[b]);
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) {
+        const int eb = wave * 16 + r * nw * 16;
+        if (eb >= ne) break;  // wave-uniform
+        const bool valid = eb + n16 < ne;
+        // Gather: word g of the packed connectivity holds corners 2g, 2g+1.
+        const uint32_t wg = g == 0 ? pk[r].x : g == 1 ? pk[r].y : g == 2 ? pk[r].z : pk[r].w;
+        const int sa = valid ? packed_half(wg, 0) * 3 : zcell;
+        const int sb = valid ? packed_half(wg, 1) * 3 : zcell;
+        const int za = valid ? 1 : 0;
+        bf16x8 Bv;
+#pragma unroll
+        for (int jj = 0; jj < 3; ++jj) {
+            Bv[jj] = __bf16(sX[sa + jj * za]);
+            Bv[3 + jj] = __bf16(sX[sb + jj * za]);
+        }
+        Bv[6] = __bf16(0.f);
+        Bv[7] = __bf16(0.f);
+        const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+        const f32x4 l0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ak[0], Bv, zero, 0, 0, 0);
+        const f32x4 l1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ak[1], Bv, zero, 0, 0, 0);
+        const f32x4 m0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ak[2], Bv, zero, 0, 0, 0);
+        const f32x4 m1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ak[3], Bv, zero, 0, 0, 0);
+        // Scatter: corner g (word g/2, half g&1) from blocks 0/2, corner 4 + g from blocks 1/3.
+        const uint32_t wlo = (g >> 1) ? pk[r].y : pk[r].x, whi = (g >> 1) ? pk[r].w : pk[r].z;
+        const int c0 = g, c1 = 4 + g;
+        const int s0 = packed_half(wlo, g & 1), s1 = packed_half(whi, g & 1);
+        const bool w0 = valid && s0 < nown, w1 = valid && s1 < nown;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            sIn[w0 ? (c0 * 3 + i) * MO + s0 : trash] = lamr[r] * l0[i] + mur[r] * m0[i];
+            sIn[w1 ? (c1 * 3 + i) * MO + s1 : trash] = lamr[r] * l1[i] + mur[r] * m1[i];
+        }
+    }
+    __syncthreads();
+    const float am = a.M1[0], bm = a.M2[0], cm = a.v[0];
+    double acc0 = 0.0, acc1 = 0.0;
+    if (tid < nown) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            float fi = 0.f;
+#pragma unroll
+            for (int cc = 0; cc < 8; ++cc) fi += sIn[(cc * 3 + d) * MO + tid];
+            const size_t gd = g0 + size_t(tid) * 3 + d;
+            const float sg = a.sf[gd];
+            const float xh = sX[tid * 3 + d];
+            const float y = sg * (fi + float(a.dgn[gd]) * xh);
+            const float o = am * a.in[gd] + bm * y + cm * a.x0[gd];
+            if constexpr (OUT) {
+                const double zv = sig * a.s[gd] * double(o * a.w[0]);
+                a.z[gd] = zv;
+                acc0 += zv * a.r[gd];
+                acc1 += zv * a.q[gd];
+            } else {
+                a.out[gd] = o;
+            }
+        }
+    }
+    if constexpr (OUT) {
+        acc0 = block_sum(acc0, red);
+        acc1 = block_sum(acc1, red);
+        if (tid == 0) {
+            double *t = a.cg.tp + size_t(a.iter & 1) * 3 * nt;
+            t[3 * tile] = acc0;
+            t[3 * tile + 1] = acc1;
+            t[3 * tile + 2] = 0.0;
+        }
+    }
+}
+
+// A-operand image of the fast MFMA element step ([4][64] x 16 B): for block b and lane l
+// (row m = l % 16, k-slots 8 (l / 16) .. +7) the BF16 entries K[row DOF][column DOF] in the
+// k-slot and row orders described in k_gcn_fast (zero on padding slots).
+__global__ void k_gcn_kimg(const float *__restrict__ kmat, uint4 *__restrict__ img) {
+    const int lane = threadIdx.x;
+    if (lane >= 64) return;
+    const int m = lane & 15, g = lane >> 4;
+    const int gr = m >> 2, ir = m & 3;
+    for (int b = 0; b < 4; ++b) {
+        const float *K = kmat + (b < 2 ? 0 : 576);
+        const int rdof = ir < 3 ? (4 * (b & 1) + gr) * 3 + ir : -1;
+        bf16x8 v;
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+            const int cdof = jj < 6 ? 6 * g + jj : -1;
+            v[jj] = __bf16((rdof >= 0 && cdof >= 0) ? K[rdof * 24 + cdof] : 0.f);
+        }
+        img[b * 64 + lane] = *reinterpret_cast<const uint4 *>(&v);
+    }
+}
+
 // x0 = s r / sigma, H^1 = x0 c_1^T.
-__global__ void k_gcn_first(int n, int C, int normalize, const double *__restrict__ r, const double *__restrict__ s,
-                            const float *__restrict__ c1, const double *scal, float *__restrict__ x0,
-                            float *__restrict__ h1) {
+__global__ __launch_bounds__(512) void k_gcn_first(int n, int C, int normalize, CgParts cg, int j,
+                                                   const double *__restrict__ r, const double *__restrict__ s,
+                                                   const float *__restrict__ c1, const double *scal,
+                                                   float *__restrict__ x0, float *__restrict__ h1) {
+    __shared__ double red[3 * 17];
     if (normalize && scal[SC_DONE] != 0.0) return;
-    const double sig = gcn_sigma(scal, normalize, n);
+    const double sig = gcn_sigma(cg, j, normalize, n, red);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float xv = float(s[i] * r[i] / sig);
         x0[i] = xv;
@@ -289,7 +441,7 @@ __global__ void k_gcn_out(int n, int C, const float *__restrict__ hL, const floa
 }
 
 // Per-block partials of (z - e).(Keff z - r) and e.r (loss of one pair).
-__global__ __launch_bounds__(256) void k_gcn_lossdots(int n, const double *__restrict__ z, const double *__restrict__ e,
+__global__ __launch_bounds__(512) void k_gcn_lossdots(int n, const double *__restrict__ z, const double *__restrict__ e,
                                                       const double *__restrict__ kz, const double *__restrict__ r,
                                                       double *__restrict__ part) {
     __shared__ double red[17];
@@ -307,7 +459,7 @@ __global__ __launch_bounds__(256) void k_gcn_lossdots(int n, const double *__res
 }
 
 // Output gradient: gz = scale (Keff z - r); G^L_k = w_k s gz; partials dw_k = sum H^L_k s gz.
-__global__ __launch_bounds__(256) void k_gcn_outgrad(int n, int C, double scale, const double *__restrict__ kz,
+__global__ __launch_bounds__(512) void k_gcn_outgrad(int n, int C, double scale, const double *__restrict__ kz,
                                                      const double *__restrict__ r, const double *__restrict__ s,
                                                      const float *__restrict__ w, const float *__restrict__ hL,
                                                      float *__restrict__ G, double *__restrict__ part) {
@@ -327,7 +479,7 @@ __global__ __launch_bounds__(256) void k_gcn_outgrad(int n, int C, double scale,
 }
 
 // Layer-1 gradient partials dc_1[k] = sum x0 G^1_k.
-__global__ __launch_bounds__(256) void k_gcn_first_grad(int n, int C, const float *__restrict__ x0,
+__global__ __launch_bounds__(512) void k_gcn_first_grad(int n, int C, const float *__restrict__ x0,
                                                         const float *__restrict__ G, double *__restrict__ part) {
     __shared__ double red[17];
     double dc[kMaxC] = {0.0, 0.0, 0.0, 0.0};
@@ -378,6 +530,20 @@ __global__ void k_gcn_scale(int n, const double *__restrict__ jac, double *__res
     sf[i] = float(v);
 }
 
+// e = s * w with w uniform in [-1, 1] from a counter-based hash of (seed, i).
+__global__ void k_gcn_white(int n, unsigned long long seed, const double *__restrict__ s, double *__restrict__ e) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long z = seed * 0x9E3779B97F4A7C15ull + (unsigned long long)(i) * 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const double u = double(z >> 11) * (1.0 / 9007199254740992.0);  // [0, 1)
+    e[i] = s[i] * (2.0 * u - 1.0);
+}
+
 __global__ void k_gcn_scale_mul(int n, const double *__restrict__ jac, double a, double *__restrict__ d) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) d[i] = a * jac[i] * d[i];
@@ -389,7 +555,7 @@ __global__ void k_axpby(int n, double a, const double *__restrict__ x, double b,
 }
 
 // ---------------------------------------------------------------- host side
-static constexpr int kVecGrid = 512, kVecBlock = 256;
+static constexpr int kVecGrid = 256, kVecBlock = 512;
 
 // K_lambda, K_mu of the single element geometry of a Hexa8 model (for the MFMA element step).
 int build_hexa_kmat(Model &m, DevBuf<float> &kmat) {
@@ -407,6 +573,10 @@ static size_t gcn_lds(const Model &m) {
     if (m.elem_path < 2) b += size_t(m.tiles.max_stage) * dim * sizeof(double);
     b += size_t(m.gcn_C) * m.tiles.max_stage * dim * sizeof(float);
     return b;
+}
+
+static size_t gcn_fast_lds(const Model &m) {
+    return size_t(25) * m.tiles.max_own * sizeof(float) + (size_t(m.tiles.max_stage) * 3 + 1) * sizeof(float);
 }
 
 template <int DIM, int PATH, bool MFMA>
@@ -483,12 +653,12 @@ static int upload_theta(Model &m) {
     return 0;
 }
 
-// Inference: z = P(r) (node layout). In the solver, normalize = 1 (sigma from RR), and the
-// last layer sets RHO / BETA of flexible CG.
-static int gcn_run(Model &m, const double *r, const double *q, double *z, bool first, int normalize) {
+// Inference: z = P(r) (node layout). In the solver (normalize = 1) r is r_j of iteration j
+// with its partials in `cg`; the output layer writes the tp[j&1] partials.
+static int gcn_run(Model &m, const CgParts &cg, int j, const double *r, const double *q, double *z, int normalize) {
     const int n = m.ndofn, C = m.gcn_C, L = m.gcn_L;
-    hipLaunchKernelGGL(k_gcn_first, dim3(kVecGrid), dim3(kVecBlock), 0, m.stream, n, C, normalize, r, m.gcn_s.p,
-                       theta_c(m, 1), m.scal.p, m.gcn_x0.p, m.gcn_h[0].p);
+    hipLaunchKernelGGL(k_gcn_first, dim3(kVecGrid), dim3(kVecBlock), 0, m.stream, n, C, normalize, cg, j, r,
+                       m.gcn_s.p, theta_c(m, 1), m.scal.p, m.gcn_x0.p, m.gcn_h[0].p);
     GcnArgs a = gcn_args(m);
     int cur = 0;
     for (int l = 2; l <= L; ++l) {
@@ -505,20 +675,26 @@ static int gcn_run(Model &m, const double *r, const double *q, double *z, bool f
             a.r = r;
             a.q = q;
             a.z = z;
-            a.part = m.part.p;
-            a.scal = m.scal.p;
-            a.counter = m.counters.p + CNT_PRE;
-            a.first = first ? 1 : 0;
+            a.cg = cg;
+            a.iter = j;
             a.normalize = normalize;
         }
-        if (int rc = launch_gcn(m, mode, m.gcn_mfma != 0, a)) return rc;
+        if (m.gcn_mfma && C == 1) {
+            a.kmat = reinterpret_cast<const float *>(m.gcn_kimg.p);
+            void *args[] = {&a};
+            const void *fn = mode == G_OUT ? reinterpret_cast<const void *>(&k_gcn_fast<true>)
+                                           : reinterpret_cast<const void *>(&k_gcn_fast<false>);
+            SVLG_HIP(hipLaunchKernel(fn, dim3(m.tiles.ntiles), dim3(m.tiles.block), args, gcn_fast_lds(m), m.stream));
+        } else if (int rc = launch_gcn(m, mode, m.gcn_mfma != 0, a)) {
+            return rc;
+        }
         cur = 1 - cur;
     }
     return 0;
 }
 
-int gcn_precond(Model &m, const double *r, const double *q, double *z, bool first) {
-    return gcn_run(m, r, q, z, first, 1);
+int gcn_precond(Model &m, const CgParts &c, int j, const double *r, const double *q, double *z) {
+    return gcn_run(m, c, j, r, q, z, 1);
 }
 
 // Loss and gradient of a batch of pairs (node layout, device); grad is FP64 on the device,
@@ -533,8 +709,8 @@ static int gcn_loss_grad(Model &m, int np, const double *const *R, const double 
     double total = 0.0;
     for (int p = 0; p < np; ++p) {
         // Forward with stored states: H^l at gcn_H + l * plane (l = 1..L), Ehat H^(l-1) at gcn_Y + l * plane.
-        hipLaunchKernelGGL(k_gcn_first, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, C, 0, R[p], m.gcn_s.p,
-                           theta_c(m, 1), m.scal.p, m.gcn_x0.p, m.gcn_H.p + plane);
+        hipLaunchKernelGGL(k_gcn_first, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, C, 0, CgParts{}, 0, R[p],
+                           m.gcn_s.p, theta_c(m, 1), m.scal.p, m.gcn_x0.p, m.gcn_H.p + plane);
         GcnArgs a = gcn_args(m);
         for (int l = 2; l <= L; ++l) {
             a.in = m.gcn_H.p + size_t(l - 1) * plane;
@@ -625,54 +801,96 @@ static void theta_chebyshev(int C, int L, double lmin, double lmax, std::vector<
     for (int k = 1; k < C; ++k) w[k] = nd(rng);
 }
 
-// Extreme eigenvalues of Ehat by power iteration (FP64 Keff applies): lmax, then the
-// largest eigenvalue of lmax I - Ehat.
+// Eigenvalues of a symmetric tridiagonal matrix (diagonal a, off-diagonal b) by cyclic
+// Jacobi rotations on the dense k x k copy (k is a few tens).
+static void tridiag_eigs(const std::vector<double> &a, const std::vector<double> &b, std::vector<double> &ev) {
+    const int k = int(a.size());
+    std::vector<double> T(size_t(k) * k, 0.0);
+    for (int i = 0; i < k; ++i) {
+        T[size_t(i) * k + i] = a[i];
+        if (i + 1 < k) T[size_t(i) * k + i + 1] = T[size_t(i + 1) * k + i] = b[i];
+    }
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0;
+        for (int p = 0; p < k; ++p)
+            for (int q = p + 1; q < k; ++q) off += T[size_t(p) * k + q] * T[size_t(p) * k + q];
+        if (off < 1e-30) break;
+        for (int p = 0; p < k; ++p)
+            for (int q = p + 1; q < k; ++q) {
+                const double apq = T[size_t(p) * k + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                const double app = T[size_t(p) * k + p], aqq = T[size_t(q) * k + q];
+                const double th = 0.5 * (aqq - app) / apq;
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int r = 0; r < k; ++r) {
+                    const double arp = T[size_t(r) * k + p], arq = T[size_t(r) * k + q];
+                    T[size_t(r) * k + p] = c * arp - sn * arq;
+                    T[size_t(r) * k + q] = sn * arp + c * arq;
+                }
+                for (int r = 0; r < k; ++r) {
+                    const double apr = T[size_t(p) * k + r], aqr = T[size_t(q) * k + r];
+                    T[size_t(p) * k + r] = c * apr - sn * aqr;
+                    T[size_t(q) * k + r] = sn * apr + c * aqr;
+                }
+            }
+    }
+    ev.resize(k);
+    for (int i = 0; i < k; ++i) ev[i] = T[size_t(i) * k + i];
+    std::sort(ev.begin(), ev.end());
+}
+
+// Extreme eigenvalues of Ehat from `iters` Lanczos steps (FP64 Keff applies, no
+// reorthogonalisation: the extreme Ritz values converge first and are what is needed).
+// The Ritz values lie inside the spectrum: lmax from below, lmin from above.
 static int ehat_bounds(Model &m, int iters, double *lmin, double *lmax) {
     const int n = m.ndofn;
-    std::vector<double> hv(n), hs(n);
+    std::vector<double> hs(n), v(n, 0.0), vprev(n, 0.0), w(n), xs(n);
     SVLG_HIP(hipMemcpyAsync(hs.data(), m.gcn_s.p, sizeof(double) * n, hipMemcpyDeviceToHost, m.stream));
     SVLG_HIP(hipStreamSynchronize(m.stream));
     std::mt19937_64 rng(7);
     std::uniform_real_distribution<double> ud(-1.0, 1.0);
-    for (int i = 0; i < n; ++i) hv[i] = hs[i] > 0.0 ? ud(rng) : 0.0;
-    double *x = m.gcn_tmp[0].p, *y = m.gcn_tmp[1].p, *t = m.gcn_tmp[2].p;
-    auto ehat = [&](const std::vector<double> &in, std::vector<double> &out) -> int {
-        std::vector<double> xs(n);
-        for (int i = 0; i < n; ++i) xs[i] = hs[i] * in[i];
+    double nv = 0.0;
+    for (int i = 0; i < n; ++i) {
+        v[i] = hs[i] > 0.0 ? ud(rng) : 0.0;
+        nv += v[i] * v[i];
+    }
+    nv = std::sqrt(nv);
+    for (double &e : v) e /= nv;
+    double *x = m.gcn_tmp[0].p, *y = m.gcn_tmp[1].p;
+    std::vector<double> al, be;
+    double bprev = 0.0;
+    for (int it = 0; it < iters; ++it) {
+        for (int i = 0; i < n; ++i) xs[i] = hs[i] * v[i];
         SVLG_HIP(hipMemcpyAsync(x, xs.data(), sizeof(double) * n, hipMemcpyHostToDevice, m.stream));
         if (int rc = keff_apply_node(m, x, y)) return rc;
-        SVLG_HIP(hipMemcpyAsync(out.data(), y, sizeof(double) * n, hipMemcpyDeviceToHost, m.stream));
+        SVLG_HIP(hipMemcpyAsync(w.data(), y, sizeof(double) * n, hipMemcpyDeviceToHost, m.stream));
         SVLG_HIP(hipStreamSynchronize(m.stream));
-        for (int i = 0; i < n; ++i) out[i] *= hs[i];
-        return 0;
-    };
-    (void)t;
-    auto norm = [&](std::vector<double> &v) {
-        double s2 = 0.0;
-        for (double e : v) s2 += e * e;
-        const double nv = std::sqrt(s2);
-        for (double &e : v) e /= nv;
-        return nv;
-    };
-    norm(hv);
-    std::vector<double> w(n);
-    double lam = 0.0;
-    for (int it = 0; it < iters; ++it) {
-        if (int rc = ehat(hv, w)) return rc;
-        lam = norm(w);
-        hv.swap(w);
+        double a = 0.0;
+        for (int i = 0; i < n; ++i) {
+            w[i] *= hs[i];
+            a += w[i] * v[i];
+        }
+        double b = 0.0;
+        for (int i = 0; i < n; ++i) {
+            w[i] -= a * v[i] + bprev * vprev[i];
+            b += w[i] * w[i];
+        }
+        b = std::sqrt(b);
+        al.push_back(a);
+        if (b < 1e-14 * std::fabs(a) || it == iters - 1) break;
+        be.push_back(b);
+        for (int i = 0; i < n; ++i) {
+            vprev[i] = v[i];
+            v[i] = w[i] / b;
+        }
+        bprev = b;
     }
-    *lmax = lam;
-    for (int i = 0; i < n; ++i) hv[i] = hs[i] > 0.0 ? ud(rng) : 0.0;
-    norm(hv);
-    double mu = 0.0;
-    for (int it = 0; it < iters; ++it) {
-        if (int rc = ehat(hv, w)) return rc;
-        for (int i = 0; i < n; ++i) w[i] = lam * hv[i] - w[i];
-        mu = norm(w);
-        hv.swap(w);
-    }
-    *lmin = std::max(lam - mu, 1e-3 * lam);
+    be.resize(al.size() > 0 ? al.size() - 1 : 0);
+    std::vector<double> ev;
+    tridiag_eigs(al, be, ev);
+    *lmin = std::max(ev.front(), 1e-6 * ev.back());
+    *lmax = ev.back();
     return 0;
 }
 
@@ -715,8 +933,11 @@ int svlg_gnn_init(svlg_model *h, int32_t channels, int32_t layers, int32_t mfma,
         SVLG_HIP(m.gcn_elam.upload(el.data(), ne, s));
         SVLG_HIP(m.gcn_emu.upload(em.data(), ne, s));
     }
-    if (m.gcn_mfma)
+    if (m.gcn_mfma) {
         if (int rc = build_hexa_kmat(m, m.gcn_kmat)) return rc;
+        SVLG_HIP(m.gcn_kimg.alloc(4 * 64));
+        hipLaunchKernelGGL(k_gcn_kimg, dim3(1), dim3(64), 0, s, m.gcn_kmat.p, m.gcn_kimg.p);
+    }
     SVLG_HIP(m.gcn_x0.alloc(n));
     for (int i = 0; i < 2; ++i) SVLG_HIP(m.gcn_h[i].alloc(size_t(channels) * n));
     for (int i = 0; i < 3; ++i) SVLG_HIP(m.gcn_tmp[i].alloc(n));
@@ -731,10 +952,14 @@ int svlg_gnn_init(svlg_model *h, int32_t channels, int32_t layers, int32_t mfma,
         if (int rc = ehat_bounds(m, 40, &lmin, &lmax)) return rc;
         m.gcn_lmin = lmin;
         m.gcn_lmax = lmax;
-        // Slightly widened interval: power iteration underestimates lmax and overestimates lmin.
-        theta_chebyshev(channels, layers, 0.9 * lmin, 1.02 * lmax, m.gcn_theta);
+        // Widened interval: the Ritz values lie inside the spectrum.
+        theta_chebyshev(channels, layers, 0.9 * lmin, 1.05 * lmax, m.gcn_theta);
     }
     if (int rc = upload_theta(m)) return rc;
+    if (gcn_fast_lds(m) > 64 * 1024)
+        for (const void *fn : {reinterpret_cast<const void *>(&k_gcn_fast<true>),
+                               reinterpret_cast<const void *>(&k_gcn_fast<false>)})
+            SVLG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(gcn_fast_lds(m))));
     const size_t lds = gcn_lds(m);
     if (lds > 64 * 1024)
         for (int mode : {G_FWD, G_BWD, G_OUT})
@@ -746,7 +971,8 @@ int svlg_gnn_init(svlg_model *h, int32_t channels, int32_t layers, int32_t mfma,
     return 0;
 }
 
-int svlg_gnn_info(svlg_model *h, int32_t *info4, double *bounds2) {
+int svlg_gnn_info(svlg_model *h, int32_t *info6, double *bounds2) {
+    int32_t *info4 = info6;
     SVLG_CHECK(h, "null model");
     Model &m = h->m;
     SVLG_CHECK(m.gcn_ready, "svlg_gnn_init has not been called");
@@ -755,6 +981,8 @@ int svlg_gnn_info(svlg_model *h, int32_t *info4, double *bounds2) {
         info4[1] = m.gcn_L;
         info4[2] = m.gcn_mfma;
         info4[3] = theta_size(m.gcn_C, m.gcn_L);
+        info4[4] = m.gcn_val_iters[0];
+        info4[5] = m.gcn_val_iters[1];
     }
     if (bounds2) {
         bounds2[0] = m.gcn_lmin;
@@ -792,9 +1020,14 @@ int svlg_gnn_apply(svlg_model *h, int32_t mfma, const double *rfree, double *zfr
     if (int rc = scatter_free(m, f.p, rn.p)) return rc;
     const int saved = m.gcn_mfma;
     m.gcn_mfma = mfma ? 1 : 0;
-    // The output layer also writes flexible-CG scalars: give it a scratch scalar block.
+    // The output layer also writes flexible-CG partials: point them at a scratch block.
     SVLG_HIP(hipMemsetAsync(m.scal.p, 0, sizeof(double) * SC_COUNT, m.stream));
-    const int rc = gcn_run(m, rn.p, rn.p, zn.p, true, 0);
+    DevBuf<double> tp;
+    SVLG_HIP(tp.alloc(6 * size_t(m.tiles.ntiles)));
+    CgParts c{};
+    c.ntiles = m.tiles.ntiles;
+    c.tp = tp.p;
+    const int rc = gcn_run(m, c, 0, rn.p, rn.p, zn.p, 0);
     m.gcn_mfma = saved;
     if (rc) return rc;
     if (int rc2 = gather_free(m, zn.p, f.p)) return rc2;
@@ -855,14 +1088,18 @@ int svlg_gnn_loss_grad(svlg_model *h, int32_t npairs, const double *r, const dou
 int svlg_nm_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const int32_t *rec_dofs, double *hist,
                   int32_t *iters, double *relres);
 
-// Generates residual/solution pairs from the model's own Newmark solves and trains the
-// network with Adam. From rest, steps k = 1..nsteps are solved with the current
-// preconditioner to rtol_pairs; every `every`-th step contributes the pair (Feff, dU) and
-// `sweeps` Jacobi-smoothed pairs (Feff - Keff x_j, dU - x_j), x_j = x_(j-1) + D^-1 (Feff -
-// Keff x_(j-1)) / 2. The state is reset to rest afterwards. loss_hist[epoch] (may be NULL)
-// receives the mean loss of each epoch.
+// Generates residual/solution pairs and trains the network with Adam. From rest, steps
+// k = 1..nsteps are solved with Jacobi-PCG; every `every`-th step contributes the pair
+// (Feff, dU) of the model's own Newmark solve, `sweeps` Jacobi-smoothed pairs
+// (Feff - Keff x_j, dU - x_j), x_j = x_(j-1) + D^-1 (Feff - Keff x_(j-1)) / 2, and one
+// white pair (Keff e, e), e = D^-1/2 w with w uniform: the Newmark pairs carry the smooth
+// part of the spectrum that the solves meet, the white pairs keep every mode in the loss.
+// The trained parameters are kept only if a validation solve (the first Newmark pair's Feff,
+// flexible CG at the model's tolerance) needs fewer iterations than with the parameters the
+// training started from. The state is back at rest afterwards. loss_hist[epochs] (may be
+// NULL) receives the mean loss of each epoch; accepted (may be NULL) 1 if kept.
 int svlg_gnn_train(svlg_model *h, int32_t nsteps, int32_t every, int32_t sweeps, int32_t epochs, double lr,
-                   double *loss_hist) {
+                   double *loss_hist, int32_t *accepted) {
     SVLG_CHECK(h, "null model");
     Model &m = h->m;
     SVLG_CHECK(m.gcn_ready, "svlg_gnn_init has not been called");
@@ -870,12 +1107,13 @@ int svlg_gnn_train(svlg_model *h, int32_t nsteps, int32_t every, int32_t sweeps,
     if (int rc = gcn_train_buffers(m)) return rc;
     hipStream_t s = m.stream;
     const int n = m.ndofn;
-    const int npairs = (nsteps / every) * (1 + sweeps);
-    SVLG_CHECK(npairs > 0, "no training pairs (nsteps < every)");
+    const int nbase = nsteps / every;
+    const int npairs = nbase * (2 + sweeps);
+    SVLG_CHECK(nbase > 0, "no training pairs (nsteps < every)");
     DevBuf<double> pr, pe;
     SVLG_HIP(pr.alloc(size_t(npairs) * n));
     SVLG_HIP(pe.alloc(size_t(npairs) * n));
-    // Pairs from the Newmark solves (the preconditioner in use is unchanged).
+    const int nb = (n + 255) / 256;
     int p = 0;
     const int saved_pc = m.nm_precond;
     m.nm_precond = SVLG_PRECOND_JACOBI;
@@ -889,27 +1127,42 @@ int svlg_gnn_train(svlg_model *h, int32_t nsteps, int32_t every, int32_t sweeps,
         SVLG_HIP(hipMemcpyAsync(rb, m.cg_b.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         SVLG_HIP(hipMemcpyAsync(eb, m.ndU.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         ++p;
-        // Jacobi-smoothed pairs: r_j = r_(j-1) - Keff d, e_j = e_(j-1) - d, d = jac r_(j-1) / 2.
         for (int sw = 0; sw < sweeps; ++sw) {
             double *rp = pr.p + size_t(p - 1) * n, *ep = pe.p + size_t(p - 1) * n;
             double *rn = pr.p + size_t(p) * n, *en = pe.p + size_t(p) * n;
             double *d = m.gcn_tmp[0].p, *kd = m.gcn_tmp[1].p;
-            // d = jac * rp / 2 via two axpby passes on copies.
             SVLG_HIP(hipMemcpyAsync(d, rp, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            hipLaunchKernelGGL(k_gcn_scale_mul, dim3((n + 255) / 256), dim3(256), 0, s, n, m.jac.p, 0.5, d);
+            hipLaunchKernelGGL(k_gcn_scale_mul, dim3(nb), dim3(256), 0, s, n, m.jac.p, 0.5, d);
             if (int rc = keff_apply_node(m, d, kd)) return rc;
             SVLG_HIP(hipMemcpyAsync(rn, rp, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
             SVLG_HIP(hipMemcpyAsync(en, ep, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-            hipLaunchKernelGGL(k_axpby, dim3((n + 255) / 256), dim3(256), 0, s, n, -1.0, kd, 1.0, rn);
-            hipLaunchKernelGGL(k_axpby, dim3((n + 255) / 256), dim3(256), 0, s, n, -1.0, d, 1.0, en);
+            hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, s, n, -1.0, kd, 1.0, rn);
+            hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, s, n, -1.0, d, 1.0, en);
             ++p;
         }
+        double *rw = pr.p + size_t(p) * n, *ew = pe.p + size_t(p) * n;
+        hipLaunchKernelGGL(k_gcn_white, dim3(nb), dim3(256), 0, s, n, (unsigned long long)(1000 + k), m.gcn_s.p, ew);
+        if (int rc = keff_apply_node(m, ew, rw)) return rc;
+        ++p;
     }
     m.nm_precond = saved_pc;
-    // Back to rest.
     for (DevBuf<double> *b : {&m.nU, &m.nV, &m.nA, &m.ndU})
         SVLG_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * n, s));
     m.nm_steps_done = 0;
+    // Validation: flexible-CG iterations on the first Newmark right-hand side.
+    auto validate = [&](int *its) -> int {
+        SVLG_HIP(hipMemsetAsync(m.gcn_tmp[2].p, 0, sizeof(double) * n, s));
+        double rel = 0.0;
+        const int rc = cg_solve(m, pr.p, m.gcn_tmp[2].p, m.nm_rtol, m.nm_maxit, SVLG_PRECOND_LEARNED, its, &rel);
+        if (rc == 12) {  // not converged within maxit: count as maxit
+            *its = m.nm_maxit;
+            return 0;
+        }
+        return rc;
+    };
+    int it0 = 0;
+    if (int rc = validate(&it0)) return rc;
+    const std::vector<double> theta0 = m.gcn_theta;
     // Adam over full-batch gradients.
     const int nt = theta_size(m.gcn_C, m.gcn_L);
     DevBuf<double> gd;
@@ -936,6 +1189,16 @@ int svlg_gnn_train(svlg_model *h, int32_t nsteps, int32_t every, int32_t sweeps,
         }
         if (int rc = upload_theta(m)) return rc;
     }
+    int it1 = 0;
+    if (int rc = validate(&it1)) return rc;
+    const bool keep = it1 < it0;
+    if (!keep) {
+        m.gcn_theta = theta0;
+        if (int rc = upload_theta(m)) return rc;
+    }
+    m.gcn_val_iters[0] = it0;
+    m.gcn_val_iters[1] = it1;
+    if (accepted) *accepted = keep ? 1 : 0;
     return 0;
 }
 

@@ -47,10 +47,11 @@ struct OpArgs {
     const double *z, *p_old;
     double *p_new;
     const double *dgn;
-    double *y, *part;
+    double *y;
     double *scal;
-    unsigned *counter;
-    int reduce;           // OP_CG: 1 = per-tile p.y partials, the last tile sets PQ and ALPHA
+    CgParts cg;
+    int reduce;           // OP_CG: 1 = iteration `iter` of a solve (beta, stopping test, p.q partials)
+    int iter;
     // OP_RHS
     const double *U, *V, *A, *mdiag, *cdiag;
     double dt;
@@ -72,9 +73,38 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
     constexpr bool STAGE_X = PATH < 2;
     constexpr bool RHS = MODE == OP_RHS;
     extern __shared__ double smem[];
-    __shared__ double red[17];
+    __shared__ double red[3 * 17];
+    double beta = 0.0;
     if constexpr (!RHS) {
-        if (a.scal[SC_DONE] != 0.0) return;
+        if (a.reduce) {
+            if (a.scal[SC_DONE] != 0.0) return;
+            // Stopping test and beta of iteration j from the partials of the previous kernels;
+            // scal holds r.z and p.q of iteration j-1 (left by the vector kernel).
+            const int j = a.iter;
+            double v[3];
+            cg_vec(a.cg, j, red, v);
+            const double rr = v[1], bb = v[2];
+            const bool done = rr <= a.scal[SC_TOL2] * bb || bb == 0.0;
+            if (j > 0) {
+                if (a.cg.flexible) {
+                    double zz[2];
+                    cg_pre(a.cg, j, red, zz);
+                    const double pq = a.scal[SC_PQ];
+                    beta = pq != 0.0 ? -zz[1] / pq : 0.0;
+                } else {
+                    const double rz0 = a.scal[SC_RHO];
+                    beta = rz0 != 0.0 ? v[0] / rz0 : 0.0;
+                }
+            }
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                a.scal[SC_RR] = rr;
+                a.scal[SC_BB] = bb;
+                a.scal[SC_ITERS] = double(j);
+                a.scal[SC_BETA] = beta;
+                if (done) a.scal[SC_DONE] = 1.0;
+            }
+            if (done) return;
+        }
     }
     const int nt = a.ntiles;
     // XCD-aware mapping: consecutive workgroups go to different XCDs, each XCD walks a
@@ -92,7 +122,6 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
     double *sIn = smem + size_t(((RHS ? 2 : 1) + (STAGE_X ? 1 : 0)) * MS) * DIM;
     const int tid = threadIdx.x, B = blockDim.x;
     const size_t g0 = size_t(o0) * DIM;
-    const double beta = RHS ? 0.0 : a.scal[SC_BETA];
     const bool has_elem = tid < ne;
     int lc[NEN];
     double lam = 0.0, mu = 0.0, c2 = 0.0;
@@ -191,17 +220,8 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
     }
     if constexpr (!RHS) {
         if (!a.reduce) return;
-        __shared__ int last;
-        const double s = block_sum(acc, red);
-        if (tid == 0) a.part[tile] = s;
-        if (arrive_last(a.counter, gridDim.x, &last)) {
-            const double pq = sum_partials(a.part, nt, 1, red);
-            if (tid == 0) {
-                a.scal[SC_PQ] = pq;
-                a.scal[SC_ALPHA] = pq != 0.0 ? a.scal[SC_RHO] / pq : 0.0;
-            }
-            release_counter(a.counter);
-        }
+        const double sp = block_sum(acc, red);
+        if (tid == 0) a.cg.pq[size_t(a.iter & 1) * nt + tile] = sp;
     }
 }
 
@@ -213,20 +233,17 @@ __global__ void k_solver_reset(double *scal, double tol2, double maxit) {
     }
 }
 
-// r = b - q and, for the Jacobi preconditioner (flexible = 0), z = jac r. The last
-// workgroup sums the per-workgroup partials in workgroup order and starts the recurrence:
-// RHO = r.z (0 when flexible: the preconditioner sets it), RR = r.r, BB = b.b, BETA = 0.
-__global__ __launch_bounds__(256) void k_cg_init(int n, int flexible, const double *__restrict__ b,
+// r_0 = b - q and, for the Jacobi preconditioner, z_0 = jac r_0; partials (r.z, r.r, b.b)
+// into vp[0] and b.b into ip.
+__global__ __launch_bounds__(512) void k_cg_init(int n, CgParts c, const double *__restrict__ b,
                                                  const double *__restrict__ q, const double *__restrict__ jac,
-                                                 double *__restrict__ r, double *__restrict__ z,
-                                                 double *__restrict__ part, double *scal, unsigned *counter) {
+                                                 double *__restrict__ r, double *__restrict__ z) {
     __shared__ double red[17];
-    __shared__ int last;
     double rz = 0.0, rr = 0.0, bb = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double ri = jac[i] != 0.0 ? b[i] - q[i] : 0.0;
         r[i] = ri;
-        if (!flexible) {
+        if (!c.flexible) {
             const double zi = jac[i] * ri;
             z[i] = zi;
             rz += ri * zi;
@@ -238,44 +255,45 @@ __global__ __launch_bounds__(256) void k_cg_init(int n, int flexible, const doub
     rr = block_sum(rr, red);
     bb = block_sum(bb, red);
     if (threadIdx.x == 0) {
-        part[3 * blockIdx.x + 0] = rz;
-        part[3 * blockIdx.x + 1] = rr;
-        part[3 * blockIdx.x + 2] = bb;
-    }
-    if (arrive_last(counter, gridDim.x, &last)) {
-        const double s0 = sum_partials(part + 0, gridDim.x, 3, red);
-        const double s1 = sum_partials(part + 1, gridDim.x, 3, red);
-        const double s2 = sum_partials(part + 2, gridDim.x, 3, red);
-        if (threadIdx.x == 0) {
-            scal[SC_RHO] = s0;
-            scal[SC_RR] = s1;
-            scal[SC_BB] = s2;
-            scal[SC_BETA] = 0.0;
-            scal[SC_ITERS] = 0.0;
-            scal[SC_DONE] = (s1 <= scal[SC_TOL2] * s2) ? 1.0 : 0.0;
-        }
-        release_counter(counter);
+        c.vp[3 * blockIdx.x + 0] = rz;
+        c.vp[3 * blockIdx.x + 1] = rr;
+        c.vp[3 * blockIdx.x + 2] = bb;
+        c.ip[blockIdx.x] = bb;
     }
 }
 
-// Iteration `it`: x += alpha p; r -= alpha q; for Jacobi also z = jac r. The last
-// workgroup sums the partials in workgroup order, applies the stopping test
-// ||r||^2 <= rtol^2 ||b||^2 (or it + 1 = maxit) and, for Jacobi, BETA = r.z / RHO.
-__global__ __launch_bounds__(256) void k_cg_update(int n, int it, int flexible, double *scal, double *__restrict__ x,
+// Iteration j: alpha_j = r_j.z_j / p_j.q_j; x += alpha p; r -= alpha q; for Jacobi also
+// z = jac r. Partials (r.z (Jacobi), r.r, b.b) of iteration j+1 into vp[(j+1)&1]; the first
+// workgroup leaves r_j.z_j and p_j.q_j in scal for the next operator kernel.
+__global__ __launch_bounds__(512) void k_cg_update(int n, int j, CgParts c, double *scal, double *__restrict__ x,
                                                    const double *__restrict__ p, const double *__restrict__ q,
                                                    double *__restrict__ r, double *__restrict__ z,
-                                                   const double *__restrict__ jac, double *__restrict__ part,
-                                                   unsigned *counter) {
-    __shared__ double red[17];
-    __shared__ int last;
+                                                   const double *__restrict__ jac) {
+    __shared__ double red[3 * 17];
     if (scal[SC_DONE] != 0.0) return;
-    const double alpha = scal[SC_ALPHA];
+    const double pq = cg_pq(c, j, red);
+    double rzj;
+    if (c.flexible) {
+        double zz[2];
+        cg_pre(c, j, red, zz);
+        rzj = zz[0];
+    } else {
+        double v[3];
+        cg_vec(c, j, red, v);
+        rzj = v[0];
+    }
+    const double alpha = pq != 0.0 ? rzj / pq : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[SC_RHO] = rzj;
+        scal[SC_PQ] = pq;
+        scal[SC_ALPHA] = alpha;
+    }
     double rz = 0.0, rr = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         x[i] += alpha * p[i];
         const double ri = r[i] - alpha * q[i];
         r[i] = ri;
-        if (!flexible) {
+        if (!c.flexible) {
             const double zi = jac[i] * ri;
             z[i] = zi;
             rz += ri * zi;
@@ -285,22 +303,24 @@ __global__ __launch_bounds__(256) void k_cg_update(int n, int it, int flexible, 
     rz = block_sum(rz, red);
     rr = block_sum(rr, red);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x + 0] = rz;
-        part[2 * blockIdx.x + 1] = rr;
+        double *v = c.vp + size_t((j + 1) & 1) * 3 * c.nvb;
+        v[3 * blockIdx.x + 0] = rz;
+        v[3 * blockIdx.x + 1] = rr;
+        v[3 * blockIdx.x + 2] = c.ip[blockIdx.x];
     }
-    if (arrive_last(counter, gridDim.x, &last)) {
-        const double s0 = sum_partials(part + 0, gridDim.x, 2, red);
-        const double s1 = sum_partials(part + 1, gridDim.x, 2, red);
-        if (threadIdx.x == 0) {
-            if (!flexible) {
-                scal[SC_BETA] = scal[SC_RHO] != 0.0 ? s0 / scal[SC_RHO] : 0.0;
-                scal[SC_RHO] = s0;
-            }
-            scal[SC_RR] = s1;
-            scal[SC_ITERS] = double(it + 1);
-            if (s1 <= scal[SC_TOL2] * scal[SC_BB] || double(it + 1) >= scal[SC_MAXIT]) scal[SC_DONE] = 1.0;
-        }
-        release_counter(counter);
+}
+
+// After `niter` iterations without convergence: the stopping test on r_niter.
+__global__ __launch_bounds__(256) void k_cg_final(int niter, CgParts c, double *scal) {
+    __shared__ double red[3 * 17];
+    if (scal[SC_DONE] != 0.0) return;
+    double v[3];
+    cg_vec(c, niter, red, v);
+    if (threadIdx.x == 0) {
+        scal[SC_RR] = v[1];
+        scal[SC_BB] = v[2];
+        scal[SC_ITERS] = double(niter);
+        scal[SC_DONE] = (v[1] <= scal[SC_TOL2] * v[2] || v[2] == 0.0) ? 1.0 : 0.0;
     }
 }
 
@@ -387,7 +407,7 @@ __global__ void k_record_nm(int nrec, const int32_t *__restrict__ rec, const dou
 }
 
 // ------------------------------------------------------------------ host side
-static constexpr int kVecGrid = 512, kVecBlock = 256;
+static constexpr int kVecGrid = 256, kVecBlock = 512;
 
 static OpArgs op_args(const Model &m) {
     const TileSet &t = m.tiles;
@@ -449,26 +469,40 @@ int keff_apply_node(Model &m, const double *x, double *y) {
     a.p_old = x;
     a.p_new = m.cg_p[1].p;
     a.y = y;
-    a.part = m.part.p;
     a.reduce = 0;
     return launch_op(m, OP_CG, a);
 }
 
+// Partial-sum buffers of a solve inside Model::part.
+static CgParts cg_parts(Model &m, int flexible) {
+    CgParts c;
+    c.ntiles = m.tiles.ntiles;
+    c.nvb = kVecGrid;
+    c.flexible = flexible;
+    c.pq = m.part.p;
+    c.vp = c.pq + 2 * size_t(c.ntiles);
+    c.tp = c.vp + 6 * size_t(c.nvb);
+    c.ip = c.tp + 6 * size_t(c.ntiles);
+    return c;
+}
+
+static size_t cg_parts_size(const Model &m) { return 8 * size_t(m.tiles.ntiles) + 7 * size_t(kVecGrid); }
+
 // Solves Keff x = b (node-DOF layout; x holds the initial guess on entry) by CG with the
 // Jacobi preconditioner (precond = SVLG_PRECOND_JACOBI) or by flexible CG with the learned
-// preconditioner (SVLG_PRECOND_LEARNED; Polak-Ribiere-type beta = -z.q / p.q). Per iteration
-// the operator kernel and the vector kernel each end in a last-workgroup reduction, so no
-// separate reduction launches are needed; the host polls the done flag every 16 iterations.
-static int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int precond, int *iters,
-                    double *relres) {
+// preconditioner (SVLG_PRECOND_LEARNED; beta = -z.q / p.q, Notay's FCG(1)). Per iteration:
+// one operator launch, one vector launch (plus the preconditioner's); the host polls the
+// done flag every 16 iterations.
+int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int precond, int *iters,
+             double *relres) {
     hipStream_t s = m.stream;
     const int n = m.ndofn;
     const int flexible = precond == SVLG_PRECOND_LEARNED ? 1 : 0;
+    const CgParts c = cg_parts(m, flexible);
     hipLaunchKernelGGL(k_solver_reset, dim3(1), dim3(64), 0, s, m.scal.p, rtol * rtol, double(maxit));
     OpArgs a = op_args(m);
     a.coef = m.coef_keff.p;
-    a.part = m.part.p;
-    a.counter = m.counters.p + CNT_OP;
+    a.cg = c;
     a.y = m.cg_q.p;
     // q = Keff x0 (x0 staged as z with beta = 0; cg_p[1] receives a copy of x0).
     a.z = x;
@@ -476,38 +510,45 @@ static int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit
     a.p_new = m.cg_p[1].p;
     a.reduce = 0;
     if (int rc = launch_op(m, OP_CG, a)) return rc;
-    double *vpart = m.part.p + m.tiles.ntiles;
-    hipLaunchKernelGGL(k_cg_init, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, flexible, b, m.cg_q.p, m.jac.p,
-                       m.cg_r.p, m.cg_z.p, vpart, m.scal.p, m.counters.p + CNT_VEC);
+    hipLaunchKernelGGL(k_cg_init, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, c, b, m.cg_q.p, m.jac.p, m.cg_r.p,
+                       m.cg_z.p);
     if (flexible)
-        if (int rc = gcn_precond(m, m.cg_r.p, m.cg_q.p, m.cg_z.p, true)) return rc;
+        if (int rc = gcn_precond(m, c, 0, m.cg_r.p, m.cg_q.p, m.cg_z.p)) return rc;
     int cur = 1;  // p_old of the first iteration (beta = 0 there, so p = z)
     double hs[SC_COUNT];
     a.z = m.cg_z.p;
     a.reduce = 1;
-    for (int it = 0; it < maxit; ++it) {
+    int it = 0;
+    for (; it < maxit; ++it) {
+        a.iter = it;
         a.p_old = m.cg_p[cur].p;
         a.p_new = m.cg_p[1 - cur].p;
         if (int rc = launch_op(m, OP_CG, a)) return rc;
-        hipLaunchKernelGGL(k_cg_update, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, it, flexible, m.scal.p, x,
-                           m.cg_p[1 - cur].p, m.cg_q.p, m.cg_r.p, m.cg_z.p, m.jac.p, vpart, m.counters.p + CNT_VEC);
+        hipLaunchKernelGGL(k_cg_update, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, it, c, m.scal.p, x,
+                           m.cg_p[1 - cur].p, m.cg_q.p, m.cg_r.p, m.cg_z.p, m.jac.p);
         if (flexible)
-            if (int rc = gcn_precond(m, m.cg_r.p, m.cg_q.p, m.cg_z.p, false)) return rc;
+            if (int rc = gcn_precond(m, c, it + 1, m.cg_r.p, m.cg_q.p, m.cg_z.p)) return rc;
         cur = 1 - cur;
-        if ((it & 15) == 15 || it == maxit - 1) {
+        // The done flag is read back every iteration from one before the previous solve's
+        // count (successive time steps need nearly the same number), every 16 before that;
+        // iterations queued past convergence exit at once on the device.
+        const int expect = m.cg_last_iters[flexible];
+        if ((expect > 0 && it + 2 >= expect) || (it & 15) == 15) {
             SVLG_HIP(hipMemcpyAsync(hs, m.scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
             SVLG_HIP(hipStreamSynchronize(s));
             if (hs[SC_DONE] != 0.0) break;
         }
     }
+    hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(256), 0, s, it < maxit ? it + 1 : maxit, c, m.scal.p);
     hipLaunchKernelGGL(k_cg_zero_rhs, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, m.scal.p, x);
     SVLG_HIP(hipGetLastError());
     SVLG_HIP(hipMemcpyAsync(hs, m.scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
     SVLG_HIP(hipStreamSynchronize(s));
     if (iters) *iters = int(hs[SC_ITERS]);
+    m.cg_last_iters[flexible] = int(hs[SC_ITERS]);
     const double rel = hs[SC_BB] > 0.0 ? std::sqrt(hs[SC_RR] / hs[SC_BB]) : 0.0;
     if (relres) *relres = rel;
-    if (!(rel <= rtol)) {
+    if (hs[SC_DONE] == 0.0 || !(rel <= rtol)) {
         set_error("conjugate gradient did not reach the tolerance within the iteration limit");
         return 12;
     }
@@ -543,13 +584,9 @@ int svlg_nm_init(svlg_model *h, double dt, double rtol, int32_t maxit, int32_t p
         SVLG_HIP(b->alloc(n));
         SVLG_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * n, s));
     }
-    // Per-tile partials (one per tile for the operator, two for the preconditioner's output
-    // layer) and per-workgroup partials of the vector kernels, which start at ntiles.
-    SVLG_HIP(m.part.alloc(2 * size_t(m.tiles.ntiles) + 3 * kVecGrid));
+    SVLG_HIP(m.part.alloc(cg_parts_size(m)));
     SVLG_HIP(m.scal.alloc(SC_COUNT));
     SVLG_HIP(hipMemsetAsync(m.scal.p, 0, sizeof(double) * SC_COUNT, s));
-    SVLG_HIP(m.counters.alloc(CNT_COUNT));
-    SVLG_HIP(hipMemsetAsync(m.counters.p, 0, sizeof(unsigned) * CNT_COUNT, s));
     // Per tile-element coefficients: 1 + 2 ak/dt (operator) and -ak (right-hand side).
     std::vector<double> ak_tag(m.nelem);
     SVLG_HIP(hipMemcpyAsync(ak_tag.data(), m.ak.p, sizeof(double) * m.nelem, hipMemcpyDeviceToHost, s));

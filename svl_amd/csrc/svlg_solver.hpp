@@ -1,13 +1,14 @@
 // svlg_solver.hpp -- device helpers shared by the conjugate-gradient solver and the
-// learned preconditioner: the solver's device scalars, deterministic workgroup sums and
-// the last-workgroup reduction.
+// learned preconditioner: the solver's device scalars, deterministic workgroup sums and the
+// partial-sum buffers through which the kernels of one iteration hand over their dot products.
 #pragma once
 
 #include "svlg_common.hpp"
 
 namespace svlg {
 
-// Device scalars of one solve (Model::scal).
+// Device scalars of one solve (Model::scal); written by the first workgroup of the kernels
+// for the host (progress, stopping) and read by later kernels (TOL2, DONE).
 enum {
     SC_RHO = 0,  // r . z of the current residual
     SC_PQ,       // p . Keff p of the current direction
@@ -15,15 +16,12 @@ enum {
     SC_BB,       // b . b
     SC_ALPHA,
     SC_BETA,
-    SC_DONE,     // 1 once converged or out of iterations
+    SC_DONE,     // 1 once converged
     SC_ITERS,
     SC_TOL2,     // rtol^2
     SC_MAXIT,
     SC_COUNT = 16
 };
-
-// Counters of the last-workgroup reductions (Model::counters), one per kernel kind.
-enum { CNT_OP = 0, CNT_VEC, CNT_PRE, CNT_COUNT = 8 };
 
 // Deterministic sum over a workgroup (blockDim a multiple of 64, at most 1024): a fixed
 // shuffle tree inside each wave, then the wave sums in wave order. buf holds 17 doubles.
@@ -44,42 +42,75 @@ __device__ __forceinline__ double block_sum(double v, double *buf) {
     return r;
 }
 
-// Called by every thread of every workgroup of a grid after thread 0 has stored the
-// workgroup's partial sums: returns true in exactly one workgroup, the last to arrive,
-// after which the partials of all workgroups are visible to it. The winner resets the
-// counter (release_counter) once it is done with the partials.
-__device__ __forceinline__ bool arrive_last(unsigned *counter, unsigned nblocks, int *flag) {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned t = atomicAdd(counter, 1u);
-        *flag = (t == nblocks - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    const bool last = *flag != 0;
-    if (last) __threadfence();
-    return last;
-}
-
-__device__ __forceinline__ void release_counter(unsigned *counter) {
-    if (threadIdx.x == 0) atomicExch(counter, 0u);
-}
-
-// Coherent read of a partial written by another workgroup.
-__device__ __forceinline__ double load_partial(const double *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Fixed-order sum of count partials (stride apart) by one workgroup.
 __device__ __forceinline__ double sum_partials(const double *part, int count, int stride, double *buf) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < count; i += blockDim.x) s += load_partial(part + size_t(i) * stride);
+    for (int i = threadIdx.x; i < count; i += blockDim.x) s += part[size_t(i) * stride];
     return block_sum(s, buf);
 }
 
+// Fixed-order sums of the first K components of count records (stride doubles apart), one
+// pass and one workgroup round. buf holds 17 * K doubles.
+template <int K>
+__device__ __forceinline__ void sum_records(const double *part, int count, int stride, double *buf, double (&out)[K]) {
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = 0.0;
+    for (int i = threadIdx.x; i < count; i += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] += part[size_t(i) * stride + k];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) buf[k * 17 + w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < K; ++k) {
+            double r = 0.0;
+            for (int i = 0; i < int(blockDim.x >> 6); ++i) r += buf[k * 17 + i];
+            buf[k * 17 + 16] = r;
+        }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = buf[k * 17 + 16];
+    __syncthreads();
+}
+
+// Partial sums of one solve. Iteration j works on r_j, z_j, p_j, q_j = Keff p_j. Every kernel
+// writes its per-workgroup partials into the half of a ping-pong pair picked by the parity of
+// the iteration it produces, and the kernels that need a total re-sum the partials in
+// workgroup order at their start: the kernel boundary makes them visible, and every workgroup
+// derives bitwise the same scalars, with no fence, atomic or reduction launch.
+//   vp[j&1]  [nvb][3]       (r_j . z_j (Jacobi) or 0, r_j . r_j, b . b)  (vector kernel of j-1 / init)
+//   tp[j&1]  [ntiles][3]    (z_j . r_j, z_j . q_(j-1), 0) per tile        (preconditioner after j-1)
+//   pq[j&1]  [ntiles]       p_j . q_j per tile                            (operator kernel of j)
+//   ip       [nvb]          b . b per vector workgroup                    (init)
+// The totals a kernel needs from the previous iteration (r_(j-1).z_(j-1), p_(j-1).q_(j-1)) are
+// left in Model::scal by the first workgroup of the vector kernel that used them.
+struct CgParts {
+    double *pq, *vp, *tp, *ip;
+    int ntiles, nvb, flexible;
+};
+
+// (r.z, r.r, b.b) of iteration j from the vector partials.
+__device__ __forceinline__ void cg_vec(const CgParts &c, int j, double *buf, double (&o)[3]) {
+    sum_records<3>(c.vp + size_t(j & 1) * 3 * c.nvb, c.nvb, 3, buf, o);
+}
+__device__ __forceinline__ double cg_pq(const CgParts &c, int j, double *buf) {
+    return sum_partials(c.pq + size_t(j & 1) * c.ntiles, c.ntiles, 1, buf);
+}
+// (z.r, z.q) of iteration j from the preconditioner partials.
+__device__ __forceinline__ void cg_pre(const CgParts &c, int j, double *buf, double (&o)[2]) {
+    sum_records<2>(c.tp + size_t(j & 1) * 3 * c.ntiles, c.ntiles, 3, buf, o);
+}
+
 // Preconditioner hook of the flexible CG loop (svlg_gcn.hip): z = P(r) on the node-DOF
-// layout; the last kernel it launches also forms z.r and z.q and sets the CG scalars
-// RHO and BETA (BETA = 0 when first).
-int gcn_precond(Model &m, const double *r, const double *q, double *z, bool first);
+// layout for iteration j (z = z_j; r = r_j; q = q_(j-1)); writes the tp[j&1] partials
+// (z.r, z.q per tile).
+int gcn_precond(Model &m, const CgParts &c, int j, const double *r, const double *q, double *z);
 
 }  // namespace svlg

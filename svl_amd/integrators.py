@@ -235,11 +235,12 @@ class LearnedPreconditioner:
         self.loss_hist = np.zeros(0)
 
     def info(self) -> dict:
-        a = np.zeros(4, np.int32)
+        a = np.zeros(6, np.int32)
         b = np.zeros(2)
         capi.check(capi.lib().svlg_gnn_info(self.mesh.h, capi.iptr(a), capi.fptr(b)), "svlg_gnn_info")
         return {"channels": int(a[0]), "layers": int(a[1]), "mfma": int(a[2]), "nparams": int(a[3]),
-                "lmin": float(b[0]), "lmax": float(b[1])}
+                "val_iters_before": int(a[4]), "val_iters_after": int(a[5]), "lmin": float(b[0]),
+                "lmax": float(b[1])}
 
     @property
     def theta(self) -> np.ndarray:
@@ -269,10 +270,14 @@ class LearnedPreconditioner:
         return float(loss[0]), g
 
     def train(self, nsteps: int = 40, every: int = 4, sweeps: int = 2, epochs: int = 60, lr: float = 2e-3):
+        """Trains on pairs from the model's own solves; returns the loss history. ``accepted``
+        tells whether the trained parameters beat the previous ones on the validation solve."""
         hist = np.zeros(max(epochs, 1))
+        acc = np.zeros(1, np.int32)
         capi.check(capi.lib().svlg_gnn_train(self.mesh.h, int(nsteps), int(every), int(sweeps), int(epochs),
-                                             float(lr), capi.fptr(hist)), "svlg_gnn_train")
+                                             float(lr), capi.fptr(hist), capi.iptr(acc)), "svlg_gnn_train")
         self.loss_hist = hist[:epochs]
+        self.accepted = bool(acc[0])
         return self.loss_hist
 
     def use(self, on: bool = True):
