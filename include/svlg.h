@@ -178,6 +178,29 @@ int svlg_keff_apply(svlg_model *m, const double *x /*[nfree]*/, double *y /*[nfr
 int svlg_solve(svlg_model *m, const double *b /*[nfree]*/, double *x /*[nfree]*/, double rtol,
                int32_t maxit, int32_t *iters, double *relres);
 
+/* ---- NewmarkBeta + NewtonRaphson (models with J2 elements) --------------------------
+ * (10-Integrators/03-Newmark/NewmarkBeta.cpp:115-144, 198-235, gamma = 1/2, beta = 1/4;
+ * 09-Algorithms/02-Newton/NewtonRaphson.cpp:24-75; Algorithm.cpp:18-56, 122-188;
+ * 02-Materials/02-NonLinear/Plastic3DJ2.cpp, PlasticPlaneStrainJ2.cpp.)
+ * Each Newton iteration forms Feff = Fext - Fint + M (4/dt V + A - 4/dt^2 dU) + C (V - 2/dt dU)
+ * from the Gauss-point stresses, solves (K_tangent + 4/dt^2 M + 2/dt C) du = Feff by
+ * Jacobi-preconditioned CG to cg_rtol, sets dU += du, evaluates ComputeConvergence(cnvgtest)
+ * and updates the Gauss-point states at U + dU, while residual > cnvgtol and iterations < nmax.
+ * C = am M + ak K_initial + dashpots. Elastic elements take part unchanged. */
+/* Initialize: zero state, J2 Gauss-point states at rest, Keff terms. cnvgtest = 1..8 as
+ * SVL's flags (1 UnbalanceForce ... 5 RelativeIncrementalDisplacement ... 8 none). */
+int svlg_nl_init(svlg_model *m, double dt, double cg_rtol, int32_t cg_maxit, double cnvgtol, int32_t nmax,
+                 int32_t cnvgtest);
+/* Advances nsteps steps from step index k0; recording as svlg_cd_steps; newton_iters[s]
+ * (may be NULL) receives the Newton iteration count of step s and newton_res[s * nmax + i]
+ * (may be NULL; entries past the last iteration are left as they are) the convergence-test
+ * value of iteration i. State: svlg_nm_get_state. */
+int svlg_nl_steps(svlg_model *m, int32_t k0, int32_t nsteps, int32_t nrec, const int32_t *rec_dofs,
+                  double *hist, int32_t *newton_iters, double *newton_res);
+/* Stresses of the Gauss points of J2 element elem (lin*::GetStress): [ngp][3] (xx, yy, xy)
+ * for Quad4 plane strain, [ngp][6] (xx, yy, zz, xy, yz, zx) for Hexa8. */
+int svlg_gp_stress(svlg_model *m, int32_t elem, double *stress);
+
 /* ---- Learned preconditioner (no SVL counterpart; it replaces the factorisation inside
  * LinearSystem::SolveSystem with a trained preconditioner for flexible CG) ------------
  * A linear graph convolution network over Ehat = D^-1/2 Keff D^-1/2 (D = diag Keff):

@@ -946,7 +946,7 @@ const char *oracle_last_error(void) { return g_err.c_str(); }
 // free DOF indices. `seconds` (optional) receives the wall time of the time loop only.
 int oracle_run(const svlg_mesh_desc *d, const oracle_opts *o, int32_t nrec, const int32_t *rec_dofs,
                double *hist_U, double *hist_V, double *hist_A, double *seconds, double *final_U,
-               int32_t *newton_iters) {
+               int32_t *newton_iters, double *newton_res) {
     try {
         Model m = build(d);
         Assembler as{&m, o->mtol, o->ktol, o->ftol};
@@ -1027,6 +1027,7 @@ int oracle_run(const svlg_mesh_desc *d, const oracle_opts *o, int32_t nrec, cons
                     sk.solve(Feff, du);
                     for (int i = 0; i < m.nfree; ++i) dUf[i] += du[i];
                     Residual = convergence(o->cnvgtest, Feff, dUf, du, it, NormFactor);
+                    if (newton_res) newton_res[size_t(k - 1) * o->nmax + it] = Residual;
                     to_total(m, dUf, dUtot);
                     std::vector<double> Un(m.ntotal);
                     for (int i = 0; i < m.ntotal; ++i) Un[i] = U[i] + dUtot[i];

@@ -136,6 +136,19 @@ struct Model {
     DevBuf<double> scal;              // device scalars of the solver
     int cg_last_iters[2] = {0, 0};    // iterations of the last solve (Jacobi, flexible)
     std::vector<int32_t> tile_elem_id;   // tag index of every tile element (host)
+    DevBuf<int32_t> tile_eid;         // the same on the device
+
+    // J2 materials (Plastic3DJ2 / PlasticPlaneStrainJ2) and the Newton-Raphson path.
+    bool has_j2 = false;
+    DevBuf<int32_t> mat;              // [nelem] material type
+    DevBuf<double> j2p;               // [nelem * 5] K, G, sigma_y, H, beta
+    DevBuf<double> gp_state;          // [nelem * ngp * kJ2State] Gauss-point states
+    bool nl_ready = false;
+    double nl_tol = 1e-6;
+    int nl_nmax = 20, nl_test = 5;
+    double nl_normfactor = 0.0;       // Algorithm::NormFactor, kept across steps
+    DevBuf<double> nl_dUa, nl_un, nl_w;  // Newton increment, U + dU, V - 2/dt dU
+    DevBuf<double> nl_part;           // reduction partials
 
     // Learned graph-convolution preconditioner (svlg_gcn.hip). Hidden states are FP32
     // planes [channel][ndofn]; parameters theta live on the host (FP64) and the device (FP32).

@@ -622,6 +622,7 @@ int svlg_cd_init(svlg_model *h, double dt, double ftol) {
     SVLG_CHECK(dt > 0.0, "dt must be positive");
     SVLG_CHECK(m.lumped, "CentralDifference on the device needs MassFormulation = LUMPED");
     SVLG_CHECK(!m.has_ak, "CentralDifference on the device needs a diagonal C (Rayleigh ak must be 0)");
+    SVLG_CHECK(!m.has_j2, "the explicit kernels restate elastic materials only; J2 models use svlg_nl_steps");
     m.dt = dt;
     m.ftol = ftol;
     for (auto &b : m.U) {

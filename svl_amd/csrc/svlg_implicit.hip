@@ -24,6 +24,7 @@
 
 #include "svlg_common.hpp"
 #include "svlg_elem.hpp"
+#include "svlg_j2.hpp"
 #include "svlg_solver.hpp"
 
 namespace svlg {
@@ -59,6 +60,14 @@ struct OpArgs {
     const int32_t *nl_ptr, *nl_idx;
     const double *load_dir, *load_factor, *load_series;
     int load_nt, k;
+    // Models with J2 elements (NL kernels): element -> material, parameters, Gauss-point
+    // states; for the Newton residual the staged operands are U + dU and V - 2/dt dU, and the
+    // mass term uses the step's V and dU.
+    const int32_t *teid;   // tile element -> element tag index
+    const int32_t *mat;    // [nelem]
+    const double *j2p;     // [nelem * 5]
+    const double *state;   // [nelem * ngp * kJ2State]
+    const double *Vm, *dUn;
 };
 
 // One node tile of the effective-stiffness operator (one thread per tile element in the
@@ -67,8 +76,12 @@ struct OpArgs {
 //          owned DOFs (0 on restrained DOFs), per-tile partial of p . y.
 //  OP_RHS: stage U and V, y = sum_e K_e (U_e - ak_e V_e), then on the free DOFs
 //          b = (((Fext - y) + C_diag V) + M (4/dt V + A)).
-template <int DIM, int PATH, int MODE>
+//  NL: elements of J2 material use their Gauss-point states instead: the internal force from
+//      the stored stresses (OP_RHS) or the consistent tangent (OP_CG), each plus the Rayleigh
+//      term of the initial stiffness (needs staged coordinates, PATH 0 or 1).
+template <int DIM, int PATH, int MODE, bool NL>
 __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
+    static_assert(!NL || PATH < 2, "J2 elements need staged coordinates");
     constexpr int NEN = Elem<DIM>::nen;
     constexpr bool STAGE_X = PATH < 2;
     constexpr bool RHS = MODE == OP_RHS;
@@ -174,14 +187,35 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
     __syncthreads();
     if (has_elem) {
         double u[NEN][DIM], f[NEN][DIM];
+        bool j2 = false;
+        if constexpr (NL) {
+            const int tag = a.teid[e0 + tid];
+            if (a.mat[tag] == SVLG_MAT_J2) {
+                j2 = true;
+                const double *q5 = a.j2p + size_t(tag) * 5;
+                const J2Par jp{q5[0], q5[1], q5[2], q5[3], q5[4]};
+                double x[NEN][DIM];
 #pragma unroll
-        for (int c = 0; c < NEN; ++c)
+                for (int c = 0; c < NEN; ++c)
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                u[c][d] = sU[lc[c] * DIM + d];
-                if constexpr (RHS) u[c][d] += c2 * sV[lc[c] * DIM + d];
+                    for (int d = 0; d < DIM; ++d) {
+                        x[c][d] = sX[lc[c] * DIM + d];
+                        u[c][d] = RHS ? sV[lc[c] * DIM + d] : sU[lc[c] * DIM + d];
+                    }
+                const double cR = RHS ? c2 : a.coef[e0 + tid] - 1.0;
+                j2_elem_force<DIM>(x, u, jp, a.state + size_t(tag) * Elem<DIM>::ngp * kJ2State, cR, !RHS, a.th, f);
             }
-        tile_elem_force<DIM, PATH>(a.geo, a.th, lc, sX, lam, mu, gid, u, f);
+        }
+        if (!j2) {
+#pragma unroll
+            for (int c = 0; c < NEN; ++c)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    u[c][d] = sU[lc[c] * DIM + d];
+                    if constexpr (RHS) u[c][d] += c2 * sV[lc[c] * DIM + d];
+                }
+            tile_elem_force<DIM, PATH>(a.geo, a.th, lc, sX, lam, mu, gid, u, f);
+        }
 #pragma unroll
         for (int c = 0; c < NEN; ++c)
             if (lc[c] < nown) {
@@ -209,7 +243,11 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
                     fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + d]);
                 }
                 const double v = sV[tid * DIM + d];
-                a.b[g] = freedof ? ((fe - fi) + a.cdiag[g] * v) + a.mdiag[g] * (4.0 / a.dt * v + a.A[g]) : 0.0;
+                // Newton residual (Vm set): the staged V is V - 2/dt dU, the mass term uses
+                // the step's V and dU.
+                const double mt = a.Vm ? 4.0 / a.dt * a.Vm[g] + a.A[g] - 4.0 / a.dt / a.dt * a.dUn[g]
+                                       : 4.0 / a.dt * v + a.A[g];
+                a.b[g] = freedof ? ((fe - fi) + a.cdiag[g] * v) + a.mdiag[g] * mt : 0.0;
             } else {
                 const double x = sU[tid * DIM + d];
                 const double yv = freedof ? fi + a.dgn[g] * x : 0.0;
@@ -430,6 +468,10 @@ static OpArgs op_args(const Model &m) {
     a.jac = m.jac.p;
     a.dgn = m.dgn.p;
     a.scal = m.scal.p;
+    a.teid = m.tile_eid.p;
+    a.mat = m.mat.p;
+    a.j2p = m.j2p.p;
+    a.state = m.gp_state.p;
     return a;
 }
 
@@ -439,18 +481,22 @@ static size_t op_lds(const Model &m, int mode) {
 }
 
 template <int DIM, int MODE>
-static const void *op_fn_t(int path) {
+static const void *op_fn_t(int path, bool nl) {
+    if (nl) return path == 1 ? reinterpret_cast<const void *>(&k_nm_op<DIM, 1, MODE, true>)
+                             : reinterpret_cast<const void *>(&k_nm_op<DIM, 0, MODE, true>);
     switch (path) {
-        case 3: return reinterpret_cast<const void *>(&k_nm_op<DIM, 3, MODE>);
-        case 2: return reinterpret_cast<const void *>(&k_nm_op<DIM, 2, MODE>);
-        case 1: return reinterpret_cast<const void *>(&k_nm_op<DIM, 1, MODE>);
-        default: return reinterpret_cast<const void *>(&k_nm_op<DIM, 0, MODE>);
+        case 3: return reinterpret_cast<const void *>(&k_nm_op<DIM, 3, MODE, false>);
+        case 2: return reinterpret_cast<const void *>(&k_nm_op<DIM, 2, MODE, false>);
+        case 1: return reinterpret_cast<const void *>(&k_nm_op<DIM, 1, MODE, false>);
+        default: return reinterpret_cast<const void *>(&k_nm_op<DIM, 0, MODE, false>);
     }
 }
 
 static const void *op_fn(const Model &m, int mode) {
-    if (m.ndim == 3) return mode == OP_CG ? op_fn_t<3, OP_CG>(m.elem_path) : op_fn_t<3, OP_RHS>(m.elem_path);
-    return mode == OP_CG ? op_fn_t<2, OP_CG>(m.elem_path) : op_fn_t<2, OP_RHS>(m.elem_path);
+    const bool nl = m.has_j2;
+    if (m.ndim == 3)
+        return mode == OP_CG ? op_fn_t<3, OP_CG>(m.elem_path, nl) : op_fn_t<3, OP_RHS>(m.elem_path, nl);
+    return mode == OP_CG ? op_fn_t<2, OP_CG>(m.elem_path, nl) : op_fn_t<2, OP_RHS>(m.elem_path, nl);
 }
 
 static int launch_op(Model &m, int mode, const OpArgs &a) {
@@ -555,6 +601,30 @@ int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int p
     return 0;
 }
 
+// NewmarkBeta residual of a Newton iterate (NewmarkBeta.cpp:212-228): b = Fext(k) - Fint(U + dU) + M (4/dt V + A - 4/dt^2 dU) + C (V - 2/dt dU), with
+// un = U + dU and w = V - 2/dt dU already formed.
+int launch_nl_rhs(Model &m, int k, const double *un, const double *w, const double *dU, double *b) {
+    OpArgs a = op_args(m);
+    a.coef = m.coef_rhs.p;
+    a.U = un;
+    a.V = w;
+    a.Vm = m.nV.p;
+    a.A = m.nA.p;
+    a.dUn = dU;
+    a.mdiag = m.mdiag.p;
+    a.cdiag = m.cdiag.p;
+    a.dt = m.nm_dt;
+    a.b = b;
+    a.nl_ptr = m.nl_ptr.p;
+    a.nl_idx = m.nl_idx.p;
+    a.load_dir = m.load_dir.p;
+    a.load_factor = m.load_factor.p;
+    a.load_series = m.load_series.p;
+    a.load_nt = m.load_nt;
+    a.k = k;
+    return launch_op(m, OP_RHS, a);
+}
+
 }  // namespace svlg
 
 using namespace svlg;
@@ -639,6 +709,7 @@ int svlg_nm_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
     SVLG_CHECK(h, "null model");
     Model &m = h->m;
     SVLG_CHECK(m.nm_ready, "svlg_nm_init has not been called");
+    SVLG_CHECK(!m.has_j2, "models with J2 materials step with svlg_nl_steps (Newton-Raphson)");
     SVLG_CHECK(nsteps >= 0 && nrec >= 0, "negative count");
     hipStream_t s = m.stream;
     DevBuf<int32_t> rec;

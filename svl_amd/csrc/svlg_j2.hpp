@@ -1,0 +1,233 @@
+// svlg_j2.hpp -- J2 plasticity at one Gauss point on the device: Plastic3DJ2 and
+// PlasticPlaneStrainJ2 (RP/02-Materials/02-NonLinear/Plastic3DJ2.cpp:206-258,
+// PlasticPlaneStrainJ2.cpp:227-279).
+//
+// State of a Gauss point, kJ2State doubles (tensor components [11, 22, 33, 12, 23, 31]):
+//   [0,6) plastic strain, [6,12) back stress, [12] alpha, [13,19) stress,
+//   [19,25) unit normal n of the last plastic update, [25] a1, [26] a2,
+// where the consistent tangent of the last update is
+//   C eps = K tr(eps) (1,1,1,0,0,0) + a1 Id eps - a2 (n . eps) n,
+//   Id eps = (eps_i - tr/3 for i < 3, eps_i / 2 for shears), eps engineering,
+// i.e. K D + (2G - c2) Id - (2G c1 - c2) n n^T with c1 = 1 / (1 + H / 3G),
+// c2 = 4 G^2 dgamma / ||xi|| (SVL's TangentStiffness); a1 = 2G, a2 = 0 when elastic.
+// As in SVL, the plastic strain, back stress and alpha advance from their current values on
+// every call, so the Newton iterates of one step accumulate.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "svlg_elem.hpp"
+
+namespace svlg {
+
+constexpr int kJ2State = 27;
+
+struct J2Par {
+    double K, G, sy, H, beta;
+};
+
+__device__ __host__ inline void j2_init_state(const J2Par &p, double *st) {
+    for (int i = 0; i < kJ2State; ++i) st[i] = 0.0;
+    st[25] = 2.0 * p.G;
+}
+
+// One UpdateState(strain, 1) with the tensor strain et.
+__device__ inline void j2_update_gp(const J2Par &p, const double (&et)[6], double *st) {
+    double *ep = st, *q = st + 6, *sig = st + 13, *nn = st + 19;
+    const double tr = et[0] + et[1] + et[2];
+    double strial[6], xi[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const double one = i < 3 ? 1.0 : 0.0;
+        const double dev = et[i] - 1.0 / 3.0 * tr * one;
+        strial[i] = 2.0 * p.G * (dev - ep[i]);
+        xi[i] = strial[i] - q[i];
+    }
+    const double nx = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2] +
+                           2.0 * (xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]));
+    const double F = nx - sqrt(2.0 / 3.0) * (p.sy + st[12] * p.beta * p.H);
+    if (F <= 0) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const double one = i < 3 ? 1.0 : 0.0;
+            sig[i] = p.K * tr * one + strial[i];
+            nn[i] = 0.0;
+        }
+        st[25] = 2.0 * p.G;
+        st[26] = 0.0;
+        return;
+    }
+    const double dg = F / (2.0 * p.G + 2.0 / 3.0 * p.H);
+    st[12] += sqrt(2.0 / 3.0) * dg;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const double one = i < 3 ? 1.0 : 0.0;
+        const double n = xi[i] / nx;
+        q[i] += 2.0 / 3.0 * (1.0 - p.beta) * p.H * dg * n;
+        ep[i] += dg * n;
+        sig[i] = p.K * tr * one + strial[i] - 2.0 * p.G * dg * n;
+        nn[i] = n;
+    }
+    const double c1 = 1.0 / (1.0 + p.H / p.G / 3.0);
+    const double c2 = 4.0 * p.G * p.G * dg / nx;
+    st[25] = 2.0 * p.G - c2;
+    st[26] = 2.0 * p.G * c1 - c2;
+}
+
+// sigma' = C_t eps for an engineering strain eps (6 components) with the tangent of st.
+__device__ inline void j2_tangent_apply(const J2Par &p, const double *st, const double (&e)[6], double (&s)[6]) {
+    const double tr = e[0] + e[1] + e[2];
+    const double *n = st + 19;
+    double ne = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) ne += n[j] * e[j];
+    const double a1 = st[25], a2 = st[26];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const double id = i < 3 ? e[i] - tr / 3.0 : 0.5 * e[i];
+        s[i] = (i < 3 ? p.K * tr : 0.0) + a1 * id - a2 * ne * n[i];
+    }
+}
+
+// Initial tangent K D + 2G (I - D/3) applied to an engineering strain (GetInitialTangentStiffness).
+__device__ inline void j2_initial_apply(const J2Par &p, const double (&e)[6], double (&s)[6]) {
+    const double tr = e[0] + e[1] + e[2];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const double id = i < 3 ? e[i] - tr / 3.0 : 0.5 * e[i];
+        s[i] = (i < 3 ? p.K * tr : 0.0) + 2.0 * p.G * id;
+    }
+}
+
+// Engineering strain of the element's Voigt order -> tensor strain of the material and the
+// 6-component engineering strain the tangent acts on (plane strain: slots 1, 2, 4).
+template <int DIM>
+__device__ inline void j2_embed(const double *ev, double (&et)[6], double (&e6)[6]) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) et[i] = e6[i] = 0.0;
+    if constexpr (DIM == 3) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            e6[i] = ev[i];
+            et[i] = i < 3 ? ev[i] : 0.5 * ev[i];
+        }
+    } else {
+        e6[1] = ev[0];
+        e6[2] = ev[1];
+        e6[4] = ev[2];
+        et[1] = ev[0];
+        et[2] = ev[1];
+        et[4] = 0.5 * ev[2];
+    }
+}
+
+// Stress components of the element's Voigt order from a 6-vector.
+template <int DIM>
+__device__ inline void j2_extract(const double *s6, double *sv) {
+    if constexpr (DIM == 3) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) sv[i] = s6[i];
+    } else {
+        sv[0] = s6[1];
+        sv[1] = s6[2];
+        sv[2] = s6[4];
+    }
+}
+
+
+// Physical shape-function derivatives at Gauss point g of a quad / hexa with corner
+// coordinates x; returns w |J| (times the thickness th for the plane-strain quad).
+template <int DIM>
+__device__ __forceinline__ double gp_dn(const double (&x)[Elem<DIM>::nen][DIM], int g, double th,
+                                        double (&dn)[Elem<DIM>::nen][DIM]) {
+    double r, s, t;
+    gauss_point(g, r, s, t);
+    if constexpr (DIM == 2)
+        return th * quad_dn(x, r, s, dn);
+    else
+        return hexa_dn(x, r, s, t, dn);
+}
+
+// Engineering strain B u in the element's Voigt order ([xx,yy,xy] / [xx,yy,zz,xy,yz,zx]).
+template <int DIM>
+__device__ __forceinline__ void gp_strain(const double (&dn)[Elem<DIM>::nen][DIM],
+                                          const double (&u)[Elem<DIM>::nen][DIM], double *ev) {
+    constexpr int NEN = Elem<DIM>::nen;
+    if constexpr (DIM == 2) {
+        ev[0] = ev[1] = ev[2] = 0.0;
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) {
+            ev[0] += dn[a][0] * u[a][0];
+            ev[1] += dn[a][1] * u[a][1];
+            ev[2] += dn[a][1] * u[a][0] + dn[a][0] * u[a][1];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) ev[i] = 0.0;
+#pragma unroll
+        for (int a = 0; a < NEN; ++a) {
+            ev[0] += dn[a][0] * u[a][0];
+            ev[1] += dn[a][1] * u[a][1];
+            ev[2] += dn[a][2] * u[a][2];
+            ev[3] += dn[a][1] * u[a][0] + dn[a][0] * u[a][1];
+            ev[4] += dn[a][2] * u[a][1] + dn[a][1] * u[a][2];
+            ev[5] += dn[a][2] * u[a][0] + dn[a][0] * u[a][2];
+        }
+    }
+}
+
+// f += wj B^T sigma (sigma in the element's Voigt order).
+template <int DIM>
+__device__ __forceinline__ void gp_btsig(const double (&dn)[Elem<DIM>::nen][DIM], const double *sv, double wj,
+                                         double (&f)[Elem<DIM>::nen][DIM]) {
+    constexpr int NEN = Elem<DIM>::nen;
+#pragma unroll
+    for (int a = 0; a < NEN; ++a) {
+        if constexpr (DIM == 2) {
+            f[a][0] += wj * (dn[a][0] * sv[0] + dn[a][1] * sv[2]);
+            f[a][1] += wj * (dn[a][1] * sv[1] + dn[a][0] * sv[2]);
+        } else {
+            f[a][0] += wj * (dn[a][0] * sv[0] + dn[a][1] * sv[3] + dn[a][2] * sv[5]);
+            f[a][1] += wj * (dn[a][1] * sv[1] + dn[a][0] * sv[3] + dn[a][2] * sv[4]);
+            f[a][2] += wj * (dn[a][2] * sv[2] + dn[a][1] * sv[4] + dn[a][0] * sv[5]);
+        }
+    }
+}
+
+// Element force of a J2 element from its Gauss-point states:
+//  RHS (tangent = false): f = sum_g wj B^T (sigma_g + c D0 B w)          (c = -ak)
+//  TAN (tangent = true):  f = sum_g wj B^T (C_t,g B u + c D0 B u)       (c = 2 ak / dt)
+// D0 is the initial tangent; u is the staged operand (p for TAN, ignored for RHS), w the
+// Rayleigh operand (V - 2/dt dU for RHS).
+template <int DIM>
+__device__ inline void j2_elem_force(const double (&x)[Elem<DIM>::nen][DIM], const double (&u)[Elem<DIM>::nen][DIM],
+                                     const J2Par &p, const double *st, double c, bool tangent, double th,
+                                     double (&f)[Elem<DIM>::nen][DIM]) {
+    constexpr int NEN = Elem<DIM>::nen, NV = DIM == 2 ? 3 : 6, NGP = Elem<DIM>::ngp;
+#pragma unroll
+    for (int a = 0; a < NEN; ++a)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) f[a][d] = 0.0;
+#pragma unroll 1
+    for (int g = 0; g < NGP; ++g) {
+        double dn[NEN][DIM];
+        const double wj = gp_dn<DIM>(x, g, th, dn);
+        double ev[NV], et[6], e6[6], s6[6], i6[6], sv[NV];
+        gp_strain<DIM>(dn, u, ev);
+        j2_embed<DIM>(ev, et, e6);
+        j2_initial_apply(p, e6, i6);
+        const double *sg = st + size_t(g) * kJ2State;
+        if (tangent) {
+            j2_tangent_apply(p, sg, e6, s6);
+#pragma unroll
+            for (int i = 0; i < 6; ++i) s6[i] += c * i6[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) s6[i] = sg[13 + i] + c * i6[i];
+        }
+        j2_extract<DIM>(s6, sv);
+        gp_btsig<DIM>(dn, sv, wj, f);
+    }
+}
+
+}  // namespace svlg

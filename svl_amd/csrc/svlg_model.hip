@@ -238,10 +238,18 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
     const int ndim = d->ndim, nen = ndim == 2 ? 4 : 8, nn = d->nnodes, ne = d->nelem;
     for (size_t i = 0; i < size_t(ne) * nen; ++i)
         SVLG_CHECK(d->conn[i] >= 0 && d->conn[i] < nn, "connectivity references a node out of range");
+    bool any_j2 = false;
     if (d->mat_type)
-        for (int e = 0; e < ne; ++e)
-            SVLG_CHECK(d->mat_type[e] == SVLG_MAT_ELASTIC,
-                       "only Elastic2DPlaneStrain / Elastic3DLinear materials are supported by this build");
+        for (int e = 0; e < ne; ++e) {
+            SVLG_CHECK(d->mat_type[e] == SVLG_MAT_ELASTIC || d->mat_type[e] == SVLG_MAT_J2, "unknown material type");
+            if (d->mat_type[e] == SVLG_MAT_J2) {
+                SVLG_CHECK(d->j2 != nullptr, "J2 elements need the j2 parameter array");
+                const double *q = d->j2 + size_t(e) * 5;
+                SVLG_CHECK(q[0] > 0.0 && q[1] > 0.0 && q[2] > 0.0 && q[3] >= 0.0 && q[4] >= 0.0 && q[4] <= 1.0,
+                           "J2 parameters must satisfy K > 0, G > 0, sigma_y > 0, H >= 0, 0 <= beta <= 1");
+                any_j2 = true;
+            }
+        }
     {
         int cnt = 0;
         for (int i = 0; i < nn * ndim; ++i) {
@@ -358,12 +366,24 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         std::memcpy(&conn_c[size_t(p) * nen], &connp[size_t(e) * nen], sizeof(int32_t) * nen);
     }
     // Lame constants of the isotropic D (c2 = lambda, c3 = mu; c1 = c2 + 2 c3).
+    // J2 elements: the initial tangent K D + 2G (I - D/3) has lambda = K - 2G/3, mu = G.
     std::vector<double> lam(ne), mu(ne);
+    std::vector<int32_t> mat(ne, SVLG_MAT_ELASTIC);
+    std::vector<double> j2p(size_t(ne) * 5, 0.0);
     for (int e = 0; e < ne; ++e) {
+        if (any_j2 && d->mat_type[e] == SVLG_MAT_J2) {
+            mat[e] = SVLG_MAT_J2;
+            const double *q = d->j2 + size_t(e) * 5;
+            for (int i = 0; i < 5; ++i) j2p[size_t(e) * 5 + i] = q[i];
+            lam[e] = q[0] - 2.0 / 3.0 * q[1];
+            mu[e] = q[1];
+            continue;
+        }
         double E = d->E[e], nu = d->nu[e];
         lam[e] = E * nu / (1.0 - 2.0 * nu) / (1.0 + nu);
         mu[e] = E / (2.0 * (1.0 + nu));
     }
+    m.has_j2 = any_j2;
     // Node -> (element, local corner) in ascending element order.
     std::vector<int32_t> n2e_ptr(nn + 1, 0), n2e(size_t(ne) * nen);
     for (size_t i = 0; i < connp.size(); ++i) n2e_ptr[connp[i] + 1]++;
@@ -390,9 +410,12 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
     UP(m.fdof, m.h_free.data(), m.h_free.size());
     UP(m.n2e_ptr, n2e_ptr.data(), n2e_ptr.size());
     UP(m.n2e, n2e.data(), n2e.size());
+    UP(m.mat, mat.data(), mat.size());
+    if (any_j2) UP(m.j2p, j2p.data(), j2p.size());
     // Element path of the fused kernel and, for parallelepipeds, the table of distinct
     // (J^-1, |J|) geometries (one entry for a uniform structured grid).
     int geo_opt = o ? o->geo_table : 0;
+    if (any_j2) geo_opt = -1;  // J2 Gauss-point loops need the staged coordinates
     m.elem_path = m.affine ? 1 : 0;
     std::vector<double> escale(ne, 1.0), gtab;
     std::vector<uint16_t> gid(ne, 0);
@@ -490,6 +513,7 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         }
         UP(t.elem_geo, eg.data(), eg.size());
         m.tile_elem_id = ht.elem_id;
+        UP(m.tile_eid, ht.elem_id.data(), ht.elem_id.size());
         UP(t.own_ptr, ht.own_ptr.data(), ht.own_ptr.size());
         UP(t.nplain, ht.nplain.data(), ht.nplain.size());
         UP(t.halo_ptr, ht.halo_ptr.data(), ht.halo_ptr.size());
@@ -752,6 +776,7 @@ int svlg_damping_diagonal(svlg_model *h, double *cfree) {
 int svlg_internal_force(svlg_model *h, const double *Ufree, double ftol, double *Ffree) {
     SVLG_CHECK(h && Ufree && Ffree, "null argument");
     Model &m = h->m;
+    SVLG_CHECK(!m.has_j2, "svlg_internal_force restates elastic elements; J2 forces come from their states");
     DevBuf<double> uf, un, fn;
     SVLG_HIP(uf.upload(Ufree, m.nfree, m.stream));
     SVLG_HIP(un.alloc(m.ndofn));

@@ -34,6 +34,7 @@ class DeviceModel:
                    "svlg_model_create")
         self.h = h
         self.nfree = model.nfree
+        self.ndim = model.ndim
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -215,6 +216,57 @@ class ExtendedNewmarkBeta:
 
     def GetAccelerations(self) -> np.ndarray:
         return self._state()[2]
+
+
+class NewmarkBeta(ExtendedNewmarkBeta):
+    """NewmarkBeta(mesh, TimeStep) with the NewtonRaphson algorithm (nstep = nmax,
+    cnvgtol, cnvgtest as SVL's convergence flags) on the device, for models with J2
+    elements: each Newton solve is Jacobi-preconditioned CG on the tangent effective
+    stiffness to ``rtol`` (SVL factors it directly)."""
+
+    def __init__(self, mesh: DeviceModel, TimeStep: float, cnvgtol: float = 1e-6, nmax: int = 50,
+                 cnvgtest: int = 5, rtol: float = 1e-12, maxit: int = 5000):
+        super().__init__(mesh, TimeStep, rtol=rtol, maxit=maxit)
+        self.cnvgtol = float(cnvgtol)
+        self.nmax = int(nmax)
+        self.cnvgtest = int(cnvgtest)
+
+    def Initialize(self):
+        capi.check(capi.lib().svlg_nl_init(self.mesh.h, self.dt, self.rtol, self.maxit, self.cnvgtol, self.nmax,
+                                           self.cnvgtest), "svlg_nl_init")
+
+    def ComputeNewStep(self, k: int) -> bool:
+        it = np.zeros(1, np.int32)
+        res = np.full((1, self.nmax), np.nan)
+        rc = capi.lib().svlg_nl_steps(self.mesh.h, int(k), 1, 0, None, None, capi.iptr(it), capi.fptr(res))
+        self.iters, self.residuals = it, res
+        return rc != 0
+
+    def advance(self, k0: int, nsteps: int, rec_dofs=None) -> np.ndarray | None:
+        """nsteps steps from step index k0; Newton iteration counts land in ``iters`` and the
+        convergence-test value of every iteration in ``residuals`` [nsteps, nmax] (NaN past
+        the last iteration)."""
+        it = np.zeros(max(nsteps, 1), np.int32)
+        res = np.full((max(nsteps, 1), self.nmax), np.nan)
+        if rec_dofs is None or len(rec_dofs) == 0:
+            rc = capi.lib().svlg_nl_steps(self.mesh.h, int(k0), int(nsteps), 0, None, None, capi.iptr(it),
+                                          capi.fptr(res))
+            hist = None
+        else:
+            rec = np.ascontiguousarray(rec_dofs, dtype=np.int32)
+            hist = np.zeros((nsteps, rec.size))
+            rc = capi.lib().svlg_nl_steps(self.mesh.h, int(k0), int(nsteps), int(rec.size), capi.iptr(rec),
+                                          capi.fptr(hist), capi.iptr(it), capi.fptr(res))
+        self.iters, self.residuals = it[:nsteps], res[:nsteps]
+        capi.check(rc, "svlg_nl_steps")
+        return hist
+
+    def GetStress(self, elem: int) -> np.ndarray:
+        """Gauss-point stresses of J2 element ``elem``: [ngp, 3] (2D) or [ngp, 6] (3D)."""
+        nd = self.mesh.ndim
+        s = np.zeros((4, 3) if nd == 2 else (8, 6))
+        capi.check(capi.lib().svlg_gp_stress(self.mesh.h, int(elem), capi.fptr(s)), "svlg_gp_stress")
+        return s
 
 
 class LearnedPreconditioner:
