@@ -168,6 +168,52 @@ def implicit_bench(args):
     return out
 
 
+def nonlinear_bench(args):
+    """Newton-Raphson + NewmarkBeta step time on the 80x80x40 J2 soil-structure block: every
+    Newton solve by Jacobi-PCG against flexible CG with the graph-convolution preconditioner
+    (Chebyshev-initialised, BF16 MFMA layers) of the initial effective stiffness, both to the
+    same relative residual. Both runs start from rest and time the same window of steps."""
+    import numpy as np
+    from svl_amd import workloads as W
+    from svl_amd.integrators import DeviceModel, LearnedPreconditioner, NewmarkBeta
+    skip, steps = args.nonlinear_skip, args.nonlinear_steps
+    m, dt, nt = W.soil_structure_j2(nt=max(1001, skip + steps + 2))
+    out = {"workload": "soil_structure_j2 lin3DHexa8 80x80x40 h=2m, 4 Plastic3DJ2 layers (c~U[50,150] kPa, seed "
+                       "20260305, H=0.1G, beta=1) + elastic 10x10x8 block (30 GPa), Rayleigh 5% at 1 and 10 Hz, "
+                       f"fixed base, horizontal Ricker 3 Hz on the block; NewmarkBeta + Newton (test 5, 1e-6, "
+                       f"nmax 20), dt={dt}", "nfree": m.nfree, "rtol": args.implicit_rtol,
+           "steps_timed": [skip + 1, skip + steps]}
+
+    def run(learned):
+        dm = DeviceModel(m)
+        nl = NewmarkBeta(dm, dt, cnvgtol=1e-6, nmax=20, cnvgtest=5, rtol=args.implicit_rtol, maxit=5000)
+        nl.Initialize()
+        info = None
+        if learned:
+            pc = LearnedPreconditioner(nl, channels=1, layers=args.gnn_layers, mfma=-1)
+            pc.use(True)
+            info = pc.info()
+        if skip:
+            nl.advance(1, skip)
+        dm.sync()
+        t = time.perf_counter()
+        nl.advance(skip + 1, steps)
+        el = (time.perf_counter() - t) / steps
+        r = {"s_per_step": el, "newton_iters_mean": float(nl.iters.mean()),
+             "cg_iters_per_solve": float(nl.cg_iters.sum() / max(nl.iters.sum(), 1))}
+        if info:
+            r.update({"layers": info["layers"], "mfma_bf16": bool(info["mfma"])})
+        print(f"# nonlinear {'learned FCG' if learned else 'Jacobi-PCG'}: {el * 1e3:.3f} ms/step, Newton "
+              f"{r['newton_iters_mean']:.2f}, CG per solve {r['cg_iters_per_solve']:.1f}", file=sys.stderr, flush=True)
+        dm.close()
+        return r
+
+    out["jacobi_pcg"] = run(False)
+    out["learned_fcg"] = run(True)
+    out["speedup_vs_jacobi"] = out["jacobi_pcg"]["s_per_step"] / out["learned_fcg"]["s_per_step"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +233,10 @@ def main():
     ap.add_argument("--gnn-layers", type=int, default=8)
     ap.add_argument("--gnn-epochs", type=int, default=30)
     ap.add_argument("--no-implicit", action="store_true")
+    ap.add_argument("--nonlinear-steps", type=int, default=20)
+    ap.add_argument("--nonlinear-skip", type=int, default=150,
+                    help="untimed steps before the timed window of the J2 soil-structure run")
+    ap.add_argument("--no-nonlinear", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="only run the device copy of MB megabytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -256,6 +306,8 @@ def main():
     dm.close()
     if not args.no_implicit:
         out["implicit"] = implicit_bench(args)
+    if not args.no_nonlinear:
+        out["nonlinear"] = nonlinear_bench(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, dt, args.cpu_steps)
         out["cpu_baseline"]["cores"] = 1

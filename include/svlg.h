@@ -191,12 +191,14 @@ int svlg_solve(svlg_model *m, const double *b /*[nfree]*/, double *x /*[nfree]*/
  * SVL's flags (1 UnbalanceForce ... 5 RelativeIncrementalDisplacement ... 8 none). */
 int svlg_nl_init(svlg_model *m, double dt, double cg_rtol, int32_t cg_maxit, double cnvgtol, int32_t nmax,
                  int32_t cnvgtest);
-/* Advances nsteps steps from step index k0; recording as svlg_cd_steps; newton_iters[s]
- * (may be NULL) receives the Newton iteration count of step s and newton_res[s * nmax + i]
- * (may be NULL; entries past the last iteration are left as they are) the convergence-test
- * value of iteration i. State: svlg_nm_get_state. */
+/* Advances nsteps steps from step index k0; recording as svlg_cd_steps. Optional outputs
+ * (NULL to skip): newton_iters[s] = Newton iterations of step s; newton_res[s * nmax + i] =
+ * convergence-test value of its iteration i (entries past the last iteration are left as
+ * they are); cg_iters[s] = CG iterations summed over its Newton solves. The preconditioner
+ * is the one svlg_nm_set_precond selected (the learned one approximates the initial
+ * effective stiffness). State: svlg_nm_get_state. */
 int svlg_nl_steps(svlg_model *m, int32_t k0, int32_t nsteps, int32_t nrec, const int32_t *rec_dofs,
-                  double *hist, int32_t *newton_iters, double *newton_res);
+                  double *hist, int32_t *newton_iters, double *newton_res, int32_t *cg_iters);
 /* Stresses of the Gauss points of J2 element elem (lin*::GetStress): [ngp][3] (xx, yy, xy)
  * for Quad4 plane strain, [ngp][6] (xx, yy, zz, xy, yz, zx) for Hexa8. */
 int svlg_gp_stress(svlg_model *m, int32_t elem, double *stress);

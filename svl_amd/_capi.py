@@ -132,7 +132,7 @@ SIGNATURES = {
     "svlg_solve": (C.c_int, [C.c_void_p, f64p, f64p, C.c_double, C.c_int32, i32p, f64p]),
     "svlg_nm_set_precond": (C.c_int, [C.c_void_p, C.c_int32]),
     "svlg_nl_init": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.c_double, C.c_int32, C.c_int32]),
-    "svlg_nl_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, i32p, f64p, i32p, f64p]),
+    "svlg_nl_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, i32p, f64p, i32p, f64p, i32p]),
     "svlg_gp_stress": (C.c_int, [C.c_void_p, C.c_int32, f64p]),
     "svlg_gnn_init": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, f64p]),
     "svlg_gnn_info": (C.c_int, [C.c_void_p, i32p, f64p]),

@@ -124,6 +124,7 @@ struct Model {
 
     // Implicit (Extended)NewmarkBeta state and Jacobi / learned PCG work vectors.
     bool nm_ready = false;
+    bool nm_initial = false;          // operators use the initial stiffness of J2 elements
     double nm_dt = 0.0, nm_rtol = 1e-8;
     int nm_maxit = 1000, nm_precond = 0;
     long nm_steps_done = 0;
@@ -143,6 +144,7 @@ struct Model {
     DevBuf<int32_t> mat;              // [nelem] material type
     DevBuf<double> j2p;               // [nelem * 5] K, G, sigma_y, H, beta
     DevBuf<double> gp_state;          // [nelem * ngp * kJ2State] Gauss-point states
+    DevBuf<int32_t> j2_plastic;       // [nelem] 1: some Gauss point holds the elastoplastic tangent
     bool nl_ready = false;
     double nl_tol = 1e-6;
     int nl_nmax = 20, nl_test = 5;

@@ -894,6 +894,16 @@ static int ehat_bounds(Model &m, int iters, double *lmin, double *lmax) {
     return 0;
 }
 
+// The network approximates the inverse of the initial effective stiffness: while it is set
+// up or trained, J2 elements enter every operator with their initial tangent (the Newmark
+// steps that make training pairs then run on the linear surrogate).
+struct InitialOperator {
+    Model &m;
+    bool saved;
+    explicit InitialOperator(Model &mm) : m(mm), saved(mm.nm_initial) { m.nm_initial = true; }
+    ~InitialOperator() { m.nm_initial = saved; }
+};
+
 }  // namespace svlg
 
 using namespace svlg;
@@ -903,6 +913,7 @@ extern "C" {
 int svlg_gnn_init(svlg_model *h, int32_t channels, int32_t layers, int32_t mfma, const double *theta) {
     SVLG_CHECK(h, "null model");
     Model &m = h->m;
+    InitialOperator initial(m);
     SVLG_CHECK(m.nm_ready, "svlg_nm_init has not been called");
     SVLG_CHECK(channels >= 1 && channels <= kMaxC, "channels must be in [1, 4]");
     SVLG_CHECK(layers >= 2 && layers <= kMaxL, "layers must be in [2, 16]");
@@ -1102,6 +1113,7 @@ int svlg_gnn_train(svlg_model *h, int32_t nsteps, int32_t every, int32_t sweeps,
                    double *loss_hist, int32_t *accepted) {
     SVLG_CHECK(h, "null model");
     Model &m = h->m;
+    InitialOperator initial(m);
     SVLG_CHECK(m.gcn_ready, "svlg_gnn_init has not been called");
     SVLG_CHECK(nsteps > 0 && every > 0 && sweeps >= 0 && epochs >= 0 && lr > 0.0, "invalid training settings");
     if (int rc = gcn_train_buffers(m)) return rc;

@@ -67,6 +67,7 @@ struct OpArgs {
     const int32_t *mat;    // [nelem]
     const double *j2p;     // [nelem * 5]
     const double *state;   // [nelem * ngp * kJ2State]
+    const int32_t *plastic;  // [nelem] elastoplastic tangent somewhere in the element
     const double *Vm, *dUn;
 };
 
@@ -78,12 +79,11 @@ struct OpArgs {
 //          b = (((Fext - y) + C_diag V) + M (4/dt V + A)).
 //  NL: elements of J2 material use their Gauss-point states instead: the internal force from
 //      the stored stresses (OP_RHS) or the consistent tangent (OP_CG), each plus the Rayleigh
-//      term of the initial stiffness (needs staged coordinates, PATH 0 or 1).
+//      term of the initial stiffness (coordinates are staged on every path).
 template <int DIM, int PATH, int MODE, bool NL>
 __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
-    static_assert(!NL || PATH < 2, "J2 elements need staged coordinates");
     constexpr int NEN = Elem<DIM>::nen;
-    constexpr bool STAGE_X = PATH < 2;
+    constexpr bool STAGE_X = PATH < 2 || NL;  // J2 Gauss-point loops need the coordinates
     constexpr bool RHS = MODE == OP_RHS;
     extern __shared__ double smem[];
     __shared__ double red[3 * 17];
@@ -189,8 +189,10 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
         double u[NEN][DIM], f[NEN][DIM];
         bool j2 = false;
         if constexpr (NL) {
+            // Tangent operator: a J2 element whose points all hold the elastic tangent is the
+            // elastic element of its initial constants (lam, mu), taken on the fast path below.
             const int tag = a.teid[e0 + tid];
-            if (a.mat[tag] == SVLG_MAT_J2) {
+            if (a.mat[tag] == SVLG_MAT_J2 && (RHS || a.plastic[tag])) {
                 j2 = true;
                 const double *q5 = a.j2p + size_t(tag) * 5;
                 const J2Par jp{q5[0], q5[1], q5[2], q5[3], q5[4]};
@@ -472,18 +474,23 @@ static OpArgs op_args(const Model &m) {
     a.mat = m.mat.p;
     a.j2p = m.j2p.p;
     a.state = m.gp_state.p;
+    a.plastic = m.j2_plastic.p;
     return a;
 }
 
 static size_t op_lds(const Model &m, int mode) {
-    const int staged = (mode == OP_RHS ? 2 : 1) + (m.elem_path < 2 ? 1 : 0);
+    const int staged = (mode == OP_RHS ? 2 : 1) + (m.elem_path < 2 || m.has_j2 ? 1 : 0);
     return (size_t(staged) * m.tiles.max_stage * m.ndim + size_t(m.nen) * m.ndim * m.tiles.max_own) * sizeof(double);
 }
 
 template <int DIM, int MODE>
 static const void *op_fn_t(int path, bool nl) {
-    if (nl) return path == 1 ? reinterpret_cast<const void *>(&k_nm_op<DIM, 1, MODE, true>)
-                             : reinterpret_cast<const void *>(&k_nm_op<DIM, 0, MODE, true>);
+    if (nl) switch (path) {
+            case 3: return reinterpret_cast<const void *>(&k_nm_op<DIM, 3, MODE, true>);
+            case 2: return reinterpret_cast<const void *>(&k_nm_op<DIM, 2, MODE, true>);
+            case 1: return reinterpret_cast<const void *>(&k_nm_op<DIM, 1, MODE, true>);
+            default: return reinterpret_cast<const void *>(&k_nm_op<DIM, 0, MODE, true>);
+        }
     switch (path) {
         case 3: return reinterpret_cast<const void *>(&k_nm_op<DIM, 3, MODE, false>);
         case 2: return reinterpret_cast<const void *>(&k_nm_op<DIM, 2, MODE, false>);
@@ -492,11 +499,18 @@ static const void *op_fn_t(int path, bool nl) {
     }
 }
 
-static const void *op_fn(const Model &m, int mode) {
-    const bool nl = m.has_j2;
+static const void *op_fn_sel(const Model &m, int mode, bool nl) {
     if (m.ndim == 3)
         return mode == OP_CG ? op_fn_t<3, OP_CG>(m.elem_path, nl) : op_fn_t<3, OP_RHS>(m.elem_path, nl);
     return mode == OP_CG ? op_fn_t<2, OP_CG>(m.elem_path, nl) : op_fn_t<2, OP_RHS>(m.elem_path, nl);
+}
+
+static const void *op_fn(const Model &m, int mode) {
+    // J2 elements from their Gauss-point states, unless the initial (elastic) operator is
+    // asked for (training pairs of the learned preconditioner) or no states exist yet (before
+    // svlg_nl_init every point is at rest, where the tangent is the initial one).
+    const bool nl = m.has_j2 && !m.nm_initial && m.gp_state.p != nullptr && m.j2_plastic.p != nullptr;
+    return op_fn_sel(m, mode, nl);
 }
 
 static int launch_op(Model &m, int mode, const OpArgs &a) {
@@ -697,7 +711,11 @@ int svlg_nm_init(svlg_model *h, double dt, double rtol, int32_t maxit, int32_t p
     for (int mode : {OP_CG, OP_RHS}) {
         const size_t lds = op_lds(m, mode);
         if (lds > 64 * 1024)
-            SVLG_HIP(hipFuncSetAttribute(op_fn(m, mode), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+            for (bool nl : {false, true}) {
+                if (nl && !m.has_j2) continue;
+                SVLG_HIP(hipFuncSetAttribute(op_fn_sel(m, mode, nl), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             int(lds)));
+            }
     }
     m.nm_steps_done = 0;
     m.nm_ready = true;
@@ -709,7 +727,7 @@ int svlg_nm_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
     SVLG_CHECK(h, "null model");
     Model &m = h->m;
     SVLG_CHECK(m.nm_ready, "svlg_nm_init has not been called");
-    SVLG_CHECK(!m.has_j2, "models with J2 materials step with svlg_nl_steps (Newton-Raphson)");
+    SVLG_CHECK(!m.has_j2 || m.nm_initial, "models with J2 materials step with svlg_nl_steps (Newton-Raphson)");
     SVLG_CHECK(nsteps >= 0 && nrec >= 0, "negative count");
     hipStream_t s = m.stream;
     DevBuf<int32_t> rec;

@@ -415,7 +415,6 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
     // Element path of the fused kernel and, for parallelepipeds, the table of distinct
     // (J^-1, |J|) geometries (one entry for a uniform structured grid).
     int geo_opt = o ? o->geo_table : 0;
-    if (any_j2) geo_opt = -1;  // J2 Gauss-point loops need the staged coordinates
     m.elem_path = m.affine ? 1 : 0;
     std::vector<double> escale(ne, 1.0), gtab;
     std::vector<uint16_t> gid(ne, 0);

@@ -10,8 +10,8 @@
 //     (RP/10-Integrators/03-Newmark/NewmarkBeta.cpp:115-144, 198-235)
 // Per Newton iteration: the residual from the stored Gauss-point stresses (tile kernel), a
 // conjugate-gradient solve of the tangent system K_t + 4/dt^2 M + 2/dt C applied matrix-free
-// (Jacobi preconditioner of the initial stiffness), dU += du, the convergence test, then the
-// Gauss-point update at U + dU. The mass is lumped; C = am M + ak K_initial + dashpots.
+// (Jacobi preconditioner of the initial effective stiffness, or the learned preconditioner of
+// it in flexible CG), dU += du, the convergence test, then the Gauss-point update at U + dU. The mass is lumped; C = am M + ak K_initial + dashpots.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -33,21 +33,25 @@ static constexpr int kNlGrid = 256, kNlBlock = 512;
 // Gauss-point states at rest: zero stresses and strains, elastic tangent.
 template <int DIM>
 __global__ void k_j2_init(int ne, const int32_t *__restrict__ mat, const double *__restrict__ j2p,
-                          double *__restrict__ state) {
+                          double *__restrict__ state, int32_t *__restrict__ plastic) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne || mat[e] != SVLG_MAT_J2) return;
+    if (e >= ne) return;
+    plastic[e] = 0;
+    if (mat[e] != SVLG_MAT_J2) return;
     const double *q = j2p + size_t(e) * 5;
     const J2Par p{q[0], q[1], q[2], q[3], q[4]};
     for (int g = 0; g < Elem<DIM>::ngp; ++g) j2_init_state(p, state + (size_t(e) * Elem<DIM>::ngp + g) * kJ2State);
 }
 
 // lin*::UpdateState at U + dU for every J2 element: strain B (U + dU) at each Gauss point,
-// then the material update (element-parallel: states are per element).
+// then the material update (element-parallel: states are per element). plastic[e] = 1 when
+// the update left some Gauss point with the elastoplastic tangent, 0 when every point holds
+// the elastic one (the tangent operator then takes the element's elastic path).
 template <int DIM>
 __global__ void k_j2_update(int ne, const int32_t *__restrict__ conn, const double *__restrict__ X,
                             const double *__restrict__ U, const double *__restrict__ dU,
                             const int32_t *__restrict__ mat, const double *__restrict__ j2p, double th,
-                            double *__restrict__ state) {
+                            double *__restrict__ state, int32_t *__restrict__ plastic) {
     constexpr int NEN = Elem<DIM>::nen, NV = DIM == 2 ? 3 : 6;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne || mat[e] != SVLG_MAT_J2) return;
@@ -63,6 +67,7 @@ __global__ void k_j2_update(int ne, const int32_t *__restrict__ conn, const doub
             u[a][d] = U[nd + d] + dU[nd + d];
         }
     }
+    int any = 0;
 #pragma unroll 1
     for (int g = 0; g < Elem<DIM>::ngp; ++g) {
         double dn[NEN][DIM];
@@ -70,8 +75,11 @@ __global__ void k_j2_update(int ne, const int32_t *__restrict__ conn, const doub
         double ev[NV], et[6], e6[6];
         gp_strain<DIM>(dn, u, ev);
         j2_embed<DIM>(ev, et, e6);
-        j2_update_gp(p, et, state + (size_t(e) * Elem<DIM>::ngp + g) * kJ2State);
+        double *st = state + (size_t(e) * Elem<DIM>::ngp + g) * kJ2State;
+        j2_update_gp(p, et, st);
+        any |= st[26] != 0.0;
     }
+    plastic[e] = any;
 }
 
 // un = U + dU, w = V - 2/dt dU.
@@ -182,12 +190,15 @@ int svlg_nl_init(svlg_model *h, double dt, double cg_rtol, int32_t cg_maxit, dou
     m.nl_ready = false;
     if (m.has_j2) {
         SVLG_HIP(m.gp_state.alloc(size_t(m.nelem) * ngp * kJ2State));
+        SVLG_HIP(m.j2_plastic.alloc(m.nelem));
         SVLG_HIP(hipMemsetAsync(m.gp_state.p, 0, sizeof(double) * m.gp_state.n, s));
         const int nb = (m.nelem + 127) / 128;
         if (m.ndim == 3)
-            hipLaunchKernelGGL(k_j2_init<3>, dim3(nb), dim3(128), 0, s, m.nelem, m.mat.p, m.j2p.p, m.gp_state.p);
+            hipLaunchKernelGGL(k_j2_init<3>, dim3(nb), dim3(128), 0, s, m.nelem, m.mat.p, m.j2p.p, m.gp_state.p,
+                               m.j2_plastic.p);
         else
-            hipLaunchKernelGGL(k_j2_init<2>, dim3(nb), dim3(128), 0, s, m.nelem, m.mat.p, m.j2p.p, m.gp_state.p);
+            hipLaunchKernelGGL(k_j2_init<2>, dim3(nb), dim3(128), 0, s, m.nelem, m.mat.p, m.j2p.p, m.gp_state.p,
+                               m.j2_plastic.p);
     }
     for (DevBuf<double> *b : {&m.nl_dUa, &m.nl_un, &m.nl_w}) {
         SVLG_HIP(b->alloc(n));
@@ -205,7 +216,7 @@ int svlg_nl_init(svlg_model *h, double dt, double cg_rtol, int32_t cg_maxit, dou
 }
 
 int svlg_nl_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const int32_t *rec_dofs, double *hist,
-                  int32_t *newton_iters, double *newton_res) {
+                  int32_t *newton_iters, double *newton_res, int32_t *cg_iters) {
     SVLG_CHECK(h, "null model");
     Model &m = h->m;
     SVLG_CHECK(m.nl_ready, "svlg_nl_init has not been called");
@@ -232,7 +243,7 @@ int svlg_nl_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
     for (int st = 0; st < nsteps; ++st) {
         const int k = k0 + st;
         SVLG_HIP(hipMemsetAsync(m.nl_dUa.p, 0, sizeof(double) * n, s));
-        int it = 0;
+        int it = 0, cg_total = 0;
         double res = 0.0;
         do {
             hipLaunchKernelGGL(k_nl_prep, dim3(nb), dim3(256), 0, s, n, m.nm_dt, m.nU.p, m.nV.p, m.nl_dUa.p, m.nl_un.p,
@@ -241,8 +252,8 @@ int svlg_nl_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
             SVLG_HIP(hipMemsetAsync(m.ndU.p, 0, sizeof(double) * n, s));
             int cit = 0;
             double rel = 0.0;
-            if (int rc = cg_solve(m, m.cg_b.p, m.ndU.p, m.nm_rtol, m.nm_maxit, SVLG_PRECOND_JACOBI, &cit, &rel))
-                return rc;
+            if (int rc = cg_solve(m, m.cg_b.p, m.ndU.p, m.nm_rtol, m.nm_maxit, m.nm_precond, &cit, &rel)) return rc;
+            cg_total += cit;
             hipLaunchKernelGGL(k_nl_accum, dim3(kNlGrid), dim3(kNlBlock), 0, s, n, m.ndU.p, m.nl_dUa.p, m.cg_b.p,
                                m.nl_part.p);
             SVLG_HIP(hipMemcpyAsync(hp.data(), m.nl_part.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, s));
@@ -256,14 +267,15 @@ int svlg_nl_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
             if (m.has_j2) {
                 if (m.ndim == 3)
                     hipLaunchKernelGGL(k_j2_update<3>, dim3(neb), dim3(128), 0, s, m.nelem, m.conn.p, m.X.p, m.nU.p,
-                                       m.nl_dUa.p, m.mat.p, m.j2p.p, m.th, m.gp_state.p);
+                                       m.nl_dUa.p, m.mat.p, m.j2p.p, m.th, m.gp_state.p, m.j2_plastic.p);
                 else
                     hipLaunchKernelGGL(k_j2_update<2>, dim3(neb), dim3(128), 0, s, m.nelem, m.conn.p, m.X.p, m.nU.p,
-                                       m.nl_dUa.p, m.mat.p, m.j2p.p, m.th, m.gp_state.p);
+                                       m.nl_dUa.p, m.mat.p, m.j2p.p, m.th, m.gp_state.p, m.j2_plastic.p);
             }
             ++it;
         } while (res > m.nl_tol && it < m.nl_nmax);
         if (newton_iters) newton_iters[st] = it;
+        if (cg_iters) cg_iters[st] = cg_total;
         hipLaunchKernelGGL(k_nl_newmark, dim3(nb), dim3(256), 0, s, n, m.nm_dt, m.nl_dUa.p, m.jac.p, m.nU.p, m.nV.p,
                            m.nA.p);
         if (nrec > 0)

@@ -236,28 +236,29 @@ class NewmarkBeta(ExtendedNewmarkBeta):
                                            self.cnvgtest), "svlg_nl_init")
 
     def ComputeNewStep(self, k: int) -> bool:
-        it = np.zeros(1, np.int32)
+        it, cg = np.zeros(1, np.int32), np.zeros(1, np.int32)
         res = np.full((1, self.nmax), np.nan)
-        rc = capi.lib().svlg_nl_steps(self.mesh.h, int(k), 1, 0, None, None, capi.iptr(it), capi.fptr(res))
-        self.iters, self.residuals = it, res
+        rc = capi.lib().svlg_nl_steps(self.mesh.h, int(k), 1, 0, None, None, capi.iptr(it), capi.fptr(res),
+                                      capi.iptr(cg))
+        self.iters, self.residuals, self.cg_iters = it, res, cg
         return rc != 0
 
     def advance(self, k0: int, nsteps: int, rec_dofs=None) -> np.ndarray | None:
-        """nsteps steps from step index k0; Newton iteration counts land in ``iters`` and the
+        """nsteps steps from step index k0; Newton iteration counts land in ``iters``, the
         convergence-test value of every iteration in ``residuals`` [nsteps, nmax] (NaN past
-        the last iteration)."""
+        the last iteration) and the CG iterations of each step in ``cg_iters``."""
         it = np.zeros(max(nsteps, 1), np.int32)
+        cg = np.zeros(max(nsteps, 1), np.int32)
         res = np.full((max(nsteps, 1), self.nmax), np.nan)
         if rec_dofs is None or len(rec_dofs) == 0:
-            rc = capi.lib().svlg_nl_steps(self.mesh.h, int(k0), int(nsteps), 0, None, None, capi.iptr(it),
-                                          capi.fptr(res))
-            hist = None
+            rec, hist, nrec, hp = None, None, 0, None
         else:
             rec = np.ascontiguousarray(rec_dofs, dtype=np.int32)
             hist = np.zeros((nsteps, rec.size))
-            rc = capi.lib().svlg_nl_steps(self.mesh.h, int(k0), int(nsteps), int(rec.size), capi.iptr(rec),
-                                          capi.fptr(hist), capi.iptr(it), capi.fptr(res))
-        self.iters, self.residuals = it[:nsteps], res[:nsteps]
+            nrec, hp = int(rec.size), capi.fptr(hist)
+        rc = capi.lib().svlg_nl_steps(self.mesh.h, int(k0), int(nsteps), nrec, capi.iptr(rec) if nrec else None, hp,
+                                      capi.iptr(it), capi.fptr(res), capi.iptr(cg))
+        self.iters, self.residuals, self.cg_iters = it[:nsteps], res[:nsteps], cg[:nsteps]
         capi.check(rc, "svlg_nl_steps")
         return hist
 

@@ -137,6 +137,51 @@ def block3d_implicit(nx: int = 64, ny: int = 64, nz: int = 64, h: float = 1.0, n
     return m, dt, nt
 
 
+def soil_structure_j2(nx: int = 80, ny: int = 80, nz: int = 40, h: float = 2.0, nt: int = 1001,
+                      block=(10, 10, 8), seed: int = 20260305, vs_layers=(150.0, 250.0, 400.0, 600.0),
+                      nu: float = 0.3, rho: float = 2000.0, zeta: float = 0.05, f0: float = 3.0, t0: float = 0.4,
+                      Ap: float = 2.0e8, dt: float = 2e-3):
+    """Nonlinear soil-structure block: Hexa8 soil in len(vs_layers) equal layers of Plastic3DJ2
+    (top layer first; K, G from Vs and nu; sigma_y = sqrt(3) c with the cohesion c ~ U[50, 150]
+    kPa per layer from default_rng(seed); H = 0.1 G; beta = 1) around an embedded elastic block
+    of block = (bx, by, bz) elements at the surface centre (E = 30 GPa, nu = 0.2, rho = 2400).
+    Base fixed, Rayleigh damping zeta at 1 and 10 Hz, a horizontal Ricker point load (velocity
+    form) on the block's top centre. Returns (model, dt, nt)."""
+    Lx, Ly, Lz = nx * h, ny * h, nz * h
+    P0 = (-Lx / 2.0, -Ly / 2.0, -Lz)
+    coords, conn, bnd = make_domain_volume(nx, ny, nz, P0, (Lx / 2.0, -Ly / 2.0, -Lz),
+                                           (-Lx / 2.0, Ly / 2.0, -Lz), (-Lx / 2.0, -Ly / 2.0, 0.0))
+    ne = conn.shape[0]
+    kk, rem = np.divmod(np.arange(ne), nx * ny)
+    jj, ii = np.divmod(rem, nx)
+    nl = len(vs_layers)
+    layer = np.minimum((nz - 1 - kk) * nl // nz, nl - 1)  # 0 = top layer
+    rng = np.random.default_rng(seed)
+    coh = rng.uniform(50e3, 150e3, nl)
+    vs = np.asarray(vs_layers, dtype=np.float64)[layer]
+    G = rho * vs * vs
+    K = 2.0 * G * (1.0 + nu) / (3.0 * (1.0 - 2.0 * nu))
+    E = 2.0 * G * (1.0 + nu)
+    bx, by, bz = block
+    inb = ((ii >= (nx - bx) // 2) & (ii < (nx - bx) // 2 + bx) & (jj >= (ny - by) // 2) & (jj < (ny - by) // 2 + by)
+           & (kk >= nz - bz))
+    E = np.where(inb, 30.0e9, E)
+    nue = np.where(inb, 0.2, nu)
+    rhoe = np.where(inb, 2400.0, rho)
+    restrained = np.zeros(coords.shape, bool)
+    restrained[bnd["bottom"]] = True
+    am, ak = rayleigh(zeta, 1.0, 10.0)
+    m = Model(ndim=3, coords=coords, node_tags=tags_for(coords.shape[0]), elem_type=ELEM_HEXA8, conn=conn,
+              E=E, nu=nue, rho=rhoe, restrained=restrained, ray_am=np.full(ne, am), ray_ak=np.full(ne, ak),
+              name=f"soil_structure_{nx}x{ny}x{nz}")
+    soil = np.nonzero(~inb)[0]
+    m.set_j2(K[soil], G[soil], np.sqrt(3.0) * coh[layer[soil]], 0.1 * G[soil], 1.0, soil)
+    t = np.arange(nt) * dt
+    src = m.nearest_node((0.0, 0.0, 0.0))
+    m.add_point_load(src, [1.0, 0.0, 0.0], ricker_velocity(t, t0, f0, Ap))
+    return m, dt, nt
+
+
 def dofs_of(model: Model, node: int):
     """Free DOF indices of a node (restrained components are skipped)."""
     f = model.free_dof()[node]
