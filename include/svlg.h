@@ -152,6 +152,27 @@ int svlg_cd_get_state(svlg_model *m, double *U, double *V, double *A);
 /* Blocks until all work queued on the model's stream is done. */
 int svlg_sync(svlg_model *m);
 
+/* ---- Independent meshes advanced together (no SVL counterpart: SVL runs one model per
+ * process) --------------------------------------------------------------------------
+ * A model whose description is the disjoint union of nparts meshes (node ranges
+ * part_node_ptr[p]..part_node_ptr[p+1] in description order, sharing no element) is stepped
+ * by one fused launch per step over all of them; each part keeps its own dt (CentralDifference
+ * coefficients per DOF, CentralDifference.cpp:64-70) and reads sample k of its own load series. */
+int svlg_cd_init_batch(svlg_model *m, int32_t nparts, const int32_t *part_node_ptr /*[nparts+1]*/,
+                       const double *dts /*[nparts]*/, double ftol);
+/* Advances nsteps steps from step index k0 by replaying a captured hipGraph of graph_steps
+ * fused launches (a multiple of 3; captured on first use, reused while the size and the U ring
+ * slot match), the remainder eagerly. Asynchronous like svlg_cd_steps_async. */
+int svlg_cd_graph_steps(svlg_model *m, int32_t k0, int32_t nsteps, int32_t graph_steps);
+/* Autotuner of the node-tile size: builds the model once per candidate tile_nodes (other
+ * options from base, may be NULL), times `steps` CentralDifference steps with HIP events and
+ * returns the fastest candidate in best_tile_nodes and every candidate's time in us_per_step
+ * (may be NULL; -1 where the candidate could not be built). nparts/part_node_ptr/dts as
+ * svlg_cd_init_batch (nparts = 1: one mesh, dts[0]). */
+int svlg_autotune_tiles(const svlg_mesh_desc *desc, const svlg_exec_opts *base, const int32_t *cand, int32_t ncand,
+                        int32_t nparts, const int32_t *part_node_ptr, const double *dts, int32_t steps,
+                        int32_t *best_tile_nodes, double *us_per_step);
+
 /* ---- ExtendedNewmarkBeta + Linear + LinearSystem ---------------------------------
  * (10-Integrators/03-Newmark/ExtendedNewmarkBeta.cpp, gamma = 1/2, beta = 1/4, G = 0;
  * 09-Algorithms/01-Linear/Linear.cpp:22-56; 11-Solvers/LinearSystem.hpp:44-66.)

@@ -121,6 +121,15 @@ struct Model {
     DevBuf<double> keff, kforce;      // diag(M/dt^2 + C/2dt), diag(M/dt^2 - C/2dt)
     DevBuf<double> hist;              // recorder buffer
     DevBuf<int32_t> rec;              // recorded node-DOFs
+    // Independent meshes advanced together (one model whose parts share no node): each part
+    // keeps its own time step.
+    std::vector<double> h_dt_dof;     // [ndofn] time step of every DOF (empty: m.dt everywhere)
+    DevBuf<double> dt_dof;
+    // Graph-captured stepping: cd_graph_steps launches (a multiple of 3, so the U ring comes
+    // back to the captured slot) plus the advance of the device step index kdev.
+    hipGraphExec_t cd_graph = nullptr;
+    int cd_graph_steps = 0, cd_graph_ring = -1;
+    DevBuf<int> kdev;
 
     // Implicit (Extended)NewmarkBeta state and Jacobi / learned PCG work vectors.
     bool nm_ready = false;
@@ -179,6 +188,13 @@ struct Model {
     DevBuf<double> geo;               // [ngeo * ndim * ndim] inverse Jacobians of distinct geometries
     int elem_bf16 = 0;                // explicit element step on BF16 MFMA (single geometry, Hexa8)
     DevBuf<float> cd_kmat;            // its K_lambda, K_mu
+
+    Model() = default;
+    Model(const Model &) = delete;
+    Model &operator=(const Model &) = delete;
+    ~Model() {
+        if (cd_graph) (void)hipGraphExecDestroy(cd_graph);
+    }
 };
 
 }  // namespace svlg

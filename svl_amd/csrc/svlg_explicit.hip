@@ -24,9 +24,11 @@ namespace svlg {
 int launch_fint_colored(Model &m, const double *U, double ftol, double *F);
 int gather_free(Model &m, const double *dnode, double *dfree);
 
-__global__ void k_cd_coeffs(int n, double dt, const double *__restrict__ M, const double *__restrict__ C,
-                            const int32_t *__restrict__ fdof, double *__restrict__ keff,
-                            double *__restrict__ kforce, int *__restrict__ singular) {
+// keff = M/dt^2 + C/2dt, kforce = M/dt^2 - C/2dt per DOF (CentralDifference.cpp:64-70); dtv
+// (per DOF, may be null) gives every DOF its own time step (independent batched meshes).
+__global__ void k_cd_coeffs(int n, double dt0, const double *__restrict__ dtv, const double *__restrict__ M,
+                            const double *__restrict__ C, const int32_t *__restrict__ fdof,
+                            double *__restrict__ keff, double *__restrict__ kforce, int *__restrict__ singular) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (fdof[i] < 0) {
@@ -34,6 +36,7 @@ __global__ void k_cd_coeffs(int n, double dt, const double *__restrict__ M, cons
         kforce[i] = 0.0;
         return;
     }
+    const double dt = dtv ? dtv[i] : dt0;
     double k = 1.0 / dt / dt * M[i] + 1.0 / 2.0 / dt * C[i];
     keff[i] = k;
     kforce[i] = 1.0 / dt / dt * M[i] - 1.0 / 2.0 / dt * C[i];
@@ -94,6 +97,7 @@ struct FusedArgs {
     double th, ftol;
     int ntiles, ncolors, max_stage, max_own;
     const float *kmat;  // PATH 4: K_lambda, K_mu of the single element geometry
+    const int *kdev;    // non-null: load sample = *kdev + k (graph replays advance *kdev)
 };
 
 __device__ __forceinline__ int color_of(const int32_t *__restrict__ cp, int i) {
@@ -119,6 +123,7 @@ __global__ __launch_bounds__(512) void k_cd_fused(FusedArgs a) {
     extern __shared__ double smem[];
     __shared__ int32_t cps[65];
     const int nt = a.ntiles;
+    const int kstep = a.kdev ? *a.kdev + a.k : a.k;
     const int b = blockIdx.x, xcd = b & 7, j = b >> 3;
     const int q = nt >> 3, r = nt & 7;
     const int tile = xcd * q + min(xcd, r) + j;
@@ -292,7 +297,7 @@ __global__ __launch_bounds__(512) void k_cd_fused(FusedArgs a) {
                 for (int q2 = a.nl_ptr[node]; q2 < a.nl_ptr[node + 1]; ++q2) {
                     const int l = a.nl_idx[q2];
                     const double amp =
-                        (a.k >= 0 && a.k < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + a.k] : 0.0;
+                        (kstep >= 0 && kstep < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + kstep] : 0.0;
                     fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + c]);
                 }
                 a.Unext[g0 + i] = u + ((fe - fi) + a.kforce[g0 + i] * (u - up[k])) / kd[k];
@@ -314,6 +319,7 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
     constexpr int NB = 4;
     extern __shared__ double smem[];
     const int nt = a.ntiles;
+    const int kstep = a.kdev ? *a.kdev + a.k : a.k;
     const int b = blockIdx.x, xcd = b & 7, j = b >> 3;
     const int q = nt >> 3, r = nt & 7;
     const int tile = xcd * q + min(xcd, r) + j;
@@ -486,7 +492,8 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
             double fe = 0.0;
             for (int q2 = a.nl_ptr[node]; q2 < a.nl_ptr[node + 1]; ++q2) {
                 const int l = a.nl_idx[q2];
-                const double amp = (a.k >= 0 && a.k < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + a.k] : 0.0;
+                const double amp =
+                    (kstep >= 0 && kstep < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + kstep] : 0.0;
                 fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + d]);
             }
             a.Unext[g] = u + ((fe - fi) + a.kforce[g] * (u - up[d])) / kd[d];
@@ -528,9 +535,11 @@ int prepare_fused(Model &m) {
     return 0;
 }
 
-static int launch_fused(Model &m, int k, const double *Ucur, const double *Uprev, double *Unext) {
+static int launch_fused(Model &m, int k, const double *Ucur, const double *Uprev, double *Unext,
+                        const int *kdev = nullptr) {
     const TileSet &t = m.tiles;
     FusedArgs a;
+    a.kdev = kdev;
     a.own_ptr = t.own_ptr.p;
     a.nplain = t.nplain.p;
     a.halo_ptr = t.halo_ptr.p;
@@ -610,21 +619,22 @@ static int cd_one_step(Model &m, int k) {
     return 0;
 }
 
-}  // namespace svlg
-
-using namespace svlg;
-
-extern "C" {
-
-int svlg_cd_init(svlg_model *h, double dt, double ftol) {
-    SVLG_CHECK(h, "null model");
-    Model &m = h->m;
+// CentralDifference::Initialize (CentralDifference.cpp:36-71) with the per-DOF time steps
+// dtd (device DOF order) or one dt.
+static int cd_init_impl(Model &m, double dt, const std::vector<double> *dtd, double ftol) {
     SVLG_CHECK(dt > 0.0, "dt must be positive");
     SVLG_CHECK(m.lumped, "CentralDifference on the device needs MassFormulation = LUMPED");
     SVLG_CHECK(!m.has_ak, "CentralDifference on the device needs a diagonal C (Rayleigh ak must be 0)");
     SVLG_CHECK(!m.has_j2, "the explicit kernels restate elastic materials only; J2 models use svlg_nl_steps");
     m.dt = dt;
     m.ftol = ftol;
+    m.cd_ready = false;
+    if (m.cd_graph) {
+        (void)hipGraphExecDestroy(m.cd_graph);
+        m.cd_graph = nullptr;
+        m.cd_graph_steps = 0;
+        m.cd_graph_ring = -1;
+    }
     for (auto &b : m.U) {
         SVLG_HIP(b.alloc(m.ndofn));
         SVLG_HIP(hipMemsetAsync(b.p, 0, sizeof(double) * m.ndofn, m.stream));
@@ -635,12 +645,18 @@ int svlg_cd_init(svlg_model *h, double dt, double ftol) {
     SVLG_HIP(hipMemsetAsync(m.Fext.p, 0, sizeof(double) * m.ndofn, m.stream));
     SVLG_HIP(m.keff.alloc(m.ndofn));
     SVLG_HIP(m.kforce.alloc(m.ndofn));
+    if (dtd) {
+        m.h_dt_dof = *dtd;
+        SVLG_HIP(m.dt_dof.upload(dtd->data(), dtd->size(), m.stream));
+    } else {
+        m.h_dt_dof.clear();
+    }
     DevBuf<int> sing;
     SVLG_HIP(sing.alloc(1));
     SVLG_HIP(hipMemsetAsync(sing.p, 0, sizeof(int), m.stream));
     int nb = (m.ndofn + 255) / 256;
-    hipLaunchKernelGGL(k_cd_coeffs, dim3(nb), dim3(256), 0, m.stream, m.ndofn, dt, m.mdiag.p, m.cdiag.p, m.fdof.p,
-                       m.keff.p, m.kforce.p, sing.p);
+    hipLaunchKernelGGL(k_cd_coeffs, dim3(nb), dim3(256), 0, m.stream, m.ndofn, dt, dtd ? m.dt_dof.p : nullptr,
+                       m.mdiag.p, m.cdiag.p, m.fdof.p, m.keff.p, m.kforce.p, sing.p);
     SVLG_HIP(hipGetLastError());
     int hs = 0;
     SVLG_HIP(hipMemcpyAsync(&hs, sing.p, sizeof(int), hipMemcpyDeviceToHost, m.stream));
@@ -654,6 +670,74 @@ int svlg_cd_init(svlg_model *h, double dt, double ftol) {
     return 0;
 }
 
+__global__ void k_step_advance(int *kdev, int by) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *kdev += by;
+}
+
+__global__ void k_step_set(int *kdev, int k) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *kdev = k;
+}
+
+// Captures cd_graph_steps fused launches (load sample *kdev + s for launch s) and the advance
+// of *kdev into one executable graph, starting from the current U ring slot.
+static int cd_capture(Model &m, int steps) {
+    if (m.cd_graph) {
+        (void)hipGraphExecDestroy(m.cd_graph);
+        m.cd_graph = nullptr;
+    }
+    if (!m.kdev.p) SVLG_HIP(m.kdev.alloc(1));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    SVLG_HIP(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
+    int ring = m.ring;
+    for (int s = 0; s < steps; ++s) {
+        launch_fused(m, s, m.U[ring].p, m.U[(ring + 2) % 3].p, m.U[(ring + 1) % 3].p, m.kdev.p);
+        ring = (ring + 1) % 3;
+    }
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, m.stream, m.kdev.p, steps);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(m.stream, &g);
+    SVLG_HIP(ec);
+    const hipError_t ei = hipGraphInstantiate(&m.cd_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    SVLG_HIP(ei);
+    m.cd_graph_steps = steps;
+    m.cd_graph_ring = m.ring;
+    return 0;
+}
+
+}  // namespace svlg
+
+using namespace svlg;
+
+extern "C" {
+
+int svlg_cd_init(svlg_model *h, double dt, double ftol) {
+    SVLG_CHECK(h, "null model");
+    return cd_init_impl(h->m, dt, nullptr, ftol);
+}
+
+int svlg_cd_init_batch(svlg_model *h, int32_t nparts, const int32_t *part_node_ptr, const double *dts, double ftol) {
+    SVLG_CHECK(h && part_node_ptr && dts, "null argument");
+    Model &m = h->m;
+    SVLG_CHECK(nparts >= 1, "nparts must be positive");
+    SVLG_CHECK(part_node_ptr[0] == 0 && part_node_ptr[nparts] == m.nnodes, "part ranges must cover the nodes");
+    for (int p = 0; p < nparts; ++p) {
+        SVLG_CHECK(part_node_ptr[p + 1] >= part_node_ptr[p], "part ranges must be ascending");
+        SVLG_CHECK(dts[p] > 0.0, "dt must be positive");
+    }
+    // Per device DOF: the time step of the part that holds the node (description order).
+    std::vector<double> dtd(m.ndofn);
+    int part = 0;
+    std::vector<double> dt_node(m.nnodes);
+    for (int n = 0; n < m.nnodes; ++n) {
+        while (n >= part_node_ptr[part + 1]) ++part;
+        dt_node[n] = dts[part];
+    }
+    for (int p = 0; p < m.nnodes; ++p)
+        for (int c = 0; c < m.ndim; ++c) dtd[size_t(p) * m.ndim + c] = dt_node[m.perm[p]];
+    return cd_init_impl(m, dts[0], &dtd, ftol);
+}
+
 int svlg_cd_steps_async(svlg_model *h, int32_t k0, int32_t nsteps) {
     SVLG_CHECK(h, "null model");
     Model &m = h->m;
@@ -661,6 +745,70 @@ int svlg_cd_steps_async(svlg_model *h, int32_t k0, int32_t nsteps) {
     for (int s = 0; s < nsteps; ++s)
         if (int rc = cd_one_step(m, k0 + s)) return rc;
     SVLG_HIP(hipGetLastError());
+    return 0;
+}
+
+int svlg_cd_graph_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t graph_steps) {
+    SVLG_CHECK(h, "null model");
+    Model &m = h->m;
+    SVLG_CHECK(m.cd_ready, "svlg_cd_init has not been called");
+    SVLG_CHECK(nsteps >= 0, "negative count");
+    SVLG_CHECK(m.variant == 1 || m.variant == 3, "graph stepping needs the fused kernel (variant 1 or 3)");
+    SVLG_CHECK(graph_steps >= 3 && graph_steps % 3 == 0 && graph_steps <= 3000,
+               "graph_steps must be a multiple of 3 in [3, 3000]");
+    int done = 0;
+    if (nsteps >= graph_steps) {
+        if (!m.cd_graph || m.cd_graph_steps != graph_steps || m.cd_graph_ring != m.ring)
+            if (int rc = cd_capture(m, graph_steps)) return rc;
+        hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, m.stream, m.kdev.p, int(k0));
+        for (; done + graph_steps <= nsteps; done += graph_steps) SVLG_HIP(hipGraphLaunch(m.cd_graph, m.stream));
+        m.steps_done += done;  // the ring returns to its slot after every replay
+    }
+    for (; done < nsteps; ++done)
+        if (int rc = cd_one_step(m, k0 + done)) return rc;
+    SVLG_HIP(hipGetLastError());
+    return 0;
+}
+
+int svlg_autotune_tiles(const svlg_mesh_desc *desc, const svlg_exec_opts *base, const int32_t *cand, int32_t ncand,
+                        int32_t nparts, const int32_t *part_node_ptr, const double *dts, int32_t steps,
+                        int32_t *best_tile_nodes, double *us_per_step) {
+    SVLG_CHECK(desc && cand && dts && best_tile_nodes, "null argument");
+    SVLG_CHECK(ncand >= 1 && steps >= 1 && nparts >= 1, "empty search");
+    SVLG_CHECK(nparts == 1 || part_node_ptr, "part ranges missing");
+    double best = -1.0;
+    *best_tile_nodes = 0;
+    for (int c = 0; c < ncand; ++c) {
+        if (us_per_step) us_per_step[c] = -1.0;
+        svlg_exec_opts o{};
+        if (base) o = *base;
+        o.tile_nodes = cand[c];
+        svlg_model *h = nullptr;
+        if (svlg_model_create_ex(desc, &o, &h) != 0) continue;  // e.g. the tile does not fit the LDS
+        Model &m = h->m;
+        int rc = nparts > 1 ? svlg_cd_init_batch(h, nparts, part_node_ptr, dts, 1e-12) : svlg_cd_init(h, dts[0], 1e-12);
+        if (rc == 0 && (m.variant == 1 || m.variant == 3)) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+            for (int s = 0; ok && s < 3; ++s) ok = cd_one_step(m, 1 + s) == 0;
+            ok = ok && hipEventRecord(e0, m.stream) == hipSuccess;
+            for (int s = 0; ok && s < steps; ++s) ok = cd_one_step(m, 4 + s) == 0;
+            ok = ok && hipEventRecord(e1, m.stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+            float ms = 0.0f;
+            if (ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+                const double us = 1e3 * double(ms) / steps;
+                if (us_per_step) us_per_step[c] = us;
+                if (best < 0.0 || us < best) {
+                    best = us;
+                    *best_tile_nodes = cand[c];
+                }
+            }
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+        svlg_model_destroy(h);
+    }
+    SVLG_CHECK(best > 0.0, "no candidate tile size could be built and timed");
     return 0;
 }
 
@@ -708,10 +856,10 @@ int svlg_cd_get_state(svlg_model *h, double *U, double *V, double *A) {
     SVLG_HIP(hipMemcpyAsync(u2.data(), m.U[(m.ring + 1) % 3].p, sizeof(double) * m.ndofn, hipMemcpyDeviceToHost,
                             m.stream));
     SVLG_HIP(hipStreamSynchronize(m.stream));
-    const double dt = m.dt;
     for (int i = 0; i < m.ndofn; ++i) {
         int f = m.h_free[i];
         if (f < 0) continue;
+        const double dt = m.h_dt_dof.empty() ? m.dt : m.h_dt_dof[i];
         if (U) U[f] = u0[i];
         // CentralDifference::ComputeNewStep (:141-145) with U = U_{k-1}, Up = U_{k-2},
         // dU = U_k - U_{k-1}.

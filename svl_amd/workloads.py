@@ -182,6 +182,71 @@ def soil_structure_j2(nx: int = 80, ny: int = 80, nz: int = 40, h: float = 2.0, 
     return m, dt, nt
 
 
+def realization2d(nx: int, ny: int, seed: int, h: float = 1.0, nt: int = 3001, vs_mean: float = 300.0,
+                  sigma: float = 0.2, nu: float = 0.25, rho: float = 2000.0, f0: float | None = None,
+                  cfl: float = 0.5, x0: float = 0.0):
+    """One 2D plane-strain realization of the batched workload: lin2DQuad4 nx x ny elements of
+    h m with its left edge at x0, per-element Vs lognormal with mean vs_mean,
+    Vs = vs_mean exp(sigma z - sigma^2/2), z ~ N(0,1) from ``default_rng(seed)`` in element-tag
+    order (drawn after f0 ~ U[2, 8] Hz when f0 is None); Lysmer dashpots on left/right/bottom
+    (per node from the first adjacent element, rho V h th); a vertically incident SV plane
+    wave as Lysmer-consistent base forces 2 eta_s(node) v_in(t) in x on the bottom nodes, with
+    v_in the Ricker velocity waveform (t0 = 1.2/f0, unit amplitude); CentralDifference with
+    dt = cfl min_e(h / Vp_e). Returns (model, dt, nt)."""
+    rng = np.random.default_rng(seed)
+    if f0 is None:
+        f0 = float(rng.uniform(2.0, 8.0))
+    Lx, Ly = nx * h, ny * h
+    coords, conn, bnd = make_domain_area(nx, ny, (x0, -Ly), (x0 + Lx, -Ly), (x0, 0.0))
+    ne = conn.shape[0]
+    z = rng.standard_normal(ne)
+    vs = vs_mean * np.exp(sigma * z - 0.5 * sigma * sigma)
+    G = rho * vs * vs
+    E = 2.0 * G * (1.0 + nu)
+    vp = vs * math.sqrt(2.0 * (1.0 - nu) / (1.0 - 2.0 * nu))
+    th = 1.0
+    m = Model(ndim=2, coords=coords, node_tags=tags_for(coords.shape[0]), elem_type=ELEM_QUAD4, conn=conn,
+              E=E, nu=np.full(ne, nu), rho=np.full(ne, rho), thickness=th, name=f"realization2d_{nx}x{ny}_{seed}")
+    first = np.full(coords.shape[0], ne, dtype=np.int64)
+    flat = conn.reshape(-1)
+    np.minimum.at(first, flat, np.arange(flat.size) // conn.shape[1])
+
+    def eta(n, name, k):
+        e = first[n]
+        return rho * (vp[e] if k == normal_dir(2, name) else vs[e]) * h * th
+
+    dn, dd, de = lysmer_dashpots(2, bnd, ["left", "right", "bottom"], eta)
+    m.dash_node, m.dash_dir, m.dash_eta = dn, dd, de
+    dt = cfl * float(np.min(h / vp))
+    t = np.arange(nt) * dt
+    vin = ricker_velocity(t, 1.2 / f0, f0, 1.0)
+    base = np.sort(bnd["bottom"]).astype(np.int32)
+    m.load_node = base
+    m.load_dir = np.ascontiguousarray(np.tile([1.0, 0.0], (base.size, 1)))
+    m.load_factor = np.array([2.0 * eta(int(n), "bottom", 0) for n in base])
+    m.load_series = np.ascontiguousarray(np.tile(vin, (base.size, 1)))
+    return m, dt, nt
+
+
+def batch2d(nreal: int = 64, seed0: int = 20260304, nt: int = 3001, nx_range=(50, 400), ny_range=(50, 200),
+            gap: int = 8):
+    """The batched workload: realization i uses ``default_rng(seed0 + i)`` for nx ~ U{nx_range},
+    ny ~ U{ny_range}, f0 ~ U[2, 8] Hz and its Vs field (realization2d); realizations are laid
+    side by side along x with ``gap`` element widths between them. Returns (models, dts, nt)."""
+    models, dts = [], []
+    x0 = 0.0
+    for i in range(nreal):
+        rng = np.random.default_rng(seed0 + i)
+        nx = int(rng.integers(nx_range[0], nx_range[1] + 1))
+        ny = int(rng.integers(ny_range[0], ny_range[1] + 1))
+        f0 = float(rng.uniform(2.0, 8.0))
+        m, dt, _ = realization2d(nx, ny, seed0 + i, nt=nt, f0=f0, x0=x0)
+        models.append(m)
+        dts.append(dt)
+        x0 += nx + gap
+    return models, dts, nt
+
+
 def dofs_of(model: Model, node: int):
     """Free DOF indices of a node (restrained components are skipped)."""
     f = model.free_dof()[node]
