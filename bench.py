@@ -127,11 +127,14 @@ def implicit_bench(args):
     from svl_amd.integrators import DeviceModel, ExtendedNewmarkBeta, LearnedPreconditioner
     n = args.implicit_ne
     m, dt, nt = W.block3d_implicit(n, n, n, nt=max(501, 2 * args.implicit_steps + 10))
-    dm = DeviceModel(m)
+    # 330-node tiles give 1021 tiles on the 64^3 block: the operator (2 workgroups per CU) and the
+    # network layers (4 per CU) then run in whole rounds of resident workgroups.
+    dm = DeviceModel(m, tile_nodes=args.implicit_tile_nodes)
     nm = ExtendedNewmarkBeta(dm, dt, rtol=args.implicit_rtol, maxit=5000)
     out = {"workload": f"block3d lin3DHexa8 {n}x{n}x{n} h=1m, Vs~U[200,1000] (seed 20260302), Rayleigh 5% at "
                        f"1 and 10 Hz, fixed base, Ricker point load, ExtendedNewmarkBeta dt={dt}",
-           "nfree": m.nfree, "rtol": args.implicit_rtol, "steps": args.implicit_steps}
+           "nfree": m.nfree, "rtol": args.implicit_rtol, "steps": args.implicit_steps,
+           "tile_nodes": args.implicit_tile_nodes, "ntiles": dm.info()["ntiles"]}
 
     def timed(label):
         dm.sync()
@@ -281,6 +284,7 @@ def main():
     ap.add_argument("--implicit-steps", type=int, default=30)
     ap.add_argument("--implicit-ne", type=int, default=64, help="elements per side of the implicit block")
     ap.add_argument("--implicit-rtol", type=float, default=1e-8)
+    ap.add_argument("--implicit-tile-nodes", type=int, default=330)
     ap.add_argument("--gnn-channels", type=int, default=1)
     ap.add_argument("--gnn-layers", type=int, default=8)
     ap.add_argument("--gnn-epochs", type=int, default=30)

@@ -130,6 +130,20 @@ struct Model {
     hipGraphExec_t cd_graph = nullptr;
     int cd_graph_steps = 0, cd_graph_ring = -1;
     DevBuf<int> kdev;
+    // Captured pairs of CG iterations (even, odd) for the solution buffers the integrators
+    // reuse, keyed by (x, flexible, operator kernel); the iteration base lives in jdev.
+    struct CgGraph {
+        hipGraphExec_t exec;
+        const double *x;
+        int flexible;
+        const void *op;
+    };
+    std::vector<CgGraph> cg_graphs;
+    DevBuf<int> jdev;
+    void drop_cg_graphs() {
+        for (auto &g : cg_graphs) (void)hipGraphExecDestroy(g.exec);
+        cg_graphs.clear();
+    }
 
     // Implicit (Extended)NewmarkBeta state and Jacobi / learned PCG work vectors.
     bool nm_ready = false;
@@ -194,6 +208,7 @@ struct Model {
     Model &operator=(const Model &) = delete;
     ~Model() {
         if (cd_graph) (void)hipGraphExecDestroy(cd_graph);
+        drop_cg_graphs();
     }
 };
 

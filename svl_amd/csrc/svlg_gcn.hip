@@ -41,7 +41,6 @@ namespace svlg {
 int gather_free(Model &m, const double *dnode, double *dfree);
 int scatter_free(Model &m, const double *dfree, double *dnode);
 int keff_apply_node(Model &m, const double *x, double *y);
-int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int precond, int *iters, double *relres);
 
 static constexpr int kMaxC = 4;
 static constexpr int kMaxL = 16;
@@ -77,6 +76,7 @@ struct GcnArgs {
     double *z, *scal;
     CgParts cg;
     int iter, normalize;
+    const int *jdev;           // non-null: the iteration is *jdev + iter (graph replays)
 };
 
 __device__ __forceinline__ float mixw(const float *M, int C, int c, int k, int trans) {
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(512) void k_gcn_layer(GcnArgs a) {
         if (a.scal && a.scal[SC_DONE] != 0.0) return;
     }
     double sig = 1.0;
-    if constexpr (MODE == G_OUT) sig = gcn_sigma(a.cg, a.iter, a.normalize, a.ndofn, red);
+    if constexpr (MODE == G_OUT) sig = gcn_sigma(a.cg, iter_of(a.jdev, a.iter), a.normalize, a.ndofn, red);
     const int nt = a.ntiles;
     const int bI = blockIdx.x, xcd = bI & 7, j = bI >> 3;
     const int q = nt >> 3, r = nt & 7;
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(512) void k_gcn_layer(GcnArgs a) {
         acc0 = block_sum(acc0, red);
         acc1 = block_sum(acc1, red);
         if (tid == 0) {
-            double *t = a.cg.tp + size_t(a.iter & 1) * 3 * nt;
+            double *t = a.cg.tp + size_t(iter_of(a.jdev, a.iter) & 1) * 3 * nt;
             t[3 * tile] = acc0;
             t[3 * tile + 1] = acc1;
             t[3 * tile + 2] = 0.0;
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(512) void k_gcn_fast(GcnArgs a) {
     __shared__ double red[3 * 17];
     if (a.scal && a.scal[SC_DONE] != 0.0) return;
     double sig = 1.0;
-    if constexpr (OUT) sig = gcn_sigma(a.cg, a.iter, a.normalize, a.ndofn, red);
+    if constexpr (OUT) sig = gcn_sigma(a.cg, iter_of(a.jdev, a.iter), a.normalize, a.ndofn, red);
     const int nt = a.ntiles;
     const int bI = blockIdx.x, xcd = bI & 7, jb = bI >> 3;
     const int qq = nt >> 3, rr = nt & 7;
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(512) void k_gcn_fast(GcnArgs a) {
         acc0 = block_sum(acc0, red);
         acc1 = block_sum(acc1, red);
         if (tid == 0) {
-            double *t = a.cg.tp + size_t(a.iter & 1) * 3 * nt;
+            double *t = a.cg.tp + size_t(iter_of(a.jdev, a.iter) & 1) * 3 * nt;
             t[3 * tile] = acc0;
             t[3 * tile + 1] = acc1;
             t[3 * tile + 2] = 0.0;
@@ -419,10 +419,11 @@ __global__ void k_gcn_kimg(const float *__restrict__ kmat, uint4 *__restrict__ i
 __global__ __launch_bounds__(512) void k_gcn_first(int n, int C, int normalize, CgParts cg, int j,
                                                    const double *__restrict__ r, const double *__restrict__ s,
                                                    const float *__restrict__ c1, const double *scal,
-                                                   float *__restrict__ x0, float *__restrict__ h1) {
+                                                   float *__restrict__ x0, float *__restrict__ h1,
+                                                   const int *jdev) {
     __shared__ double red[3 * 17];
     if (normalize && scal[SC_DONE] != 0.0) return;
-    const double sig = gcn_sigma(cg, j, normalize, n, red);
+    const double sig = gcn_sigma(cg, iter_of(jdev, j), normalize, n, red);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float xv = float(s[i] * r[i] / sig);
         x0[i] = xv;
@@ -655,10 +656,13 @@ static int upload_theta(Model &m) {
 
 // Inference: z = P(r) (node layout). In the solver (normalize = 1) r is r_j of iteration j
 // with its partials in `cg`; the output layer writes the tp[j&1] partials.
-static int gcn_run(Model &m, const CgParts &cg, int j, const double *r, const double *q, double *z, int normalize) {
+// first = false: x0 and H^1 are already in gcn_x0 / gcn_h[0] (formed by the CG vector kernel).
+static int gcn_run(Model &m, const CgParts &cg, int j, const double *r, const double *q, double *z, int normalize,
+                   const int *jdev = nullptr, bool first = true) {
     const int n = m.ndofn, C = m.gcn_C, L = m.gcn_L;
-    hipLaunchKernelGGL(k_gcn_first, dim3(kVecGrid), dim3(kVecBlock), 0, m.stream, n, C, normalize, cg, j, r,
-                       m.gcn_s.p, theta_c(m, 1), m.scal.p, m.gcn_x0.p, m.gcn_h[0].p);
+    if (first)
+        hipLaunchKernelGGL(k_gcn_first, dim3(kVecGrid), dim3(kVecBlock), 0, m.stream, n, C, normalize, cg, j, r,
+                           m.gcn_s.p, theta_c(m, 1), m.scal.p, m.gcn_x0.p, m.gcn_h[0].p, jdev);
     GcnArgs a = gcn_args(m);
     int cur = 0;
     for (int l = 2; l <= L; ++l) {
@@ -677,6 +681,7 @@ static int gcn_run(Model &m, const CgParts &cg, int j, const double *r, const do
             a.z = z;
             a.cg = cg;
             a.iter = j;
+            a.jdev = jdev;
             a.normalize = normalize;
         }
         if (m.gcn_mfma && C == 1) {
@@ -693,8 +698,14 @@ static int gcn_run(Model &m, const CgParts &cg, int j, const double *r, const do
     return 0;
 }
 
-int gcn_precond(Model &m, const CgParts &c, int j, const double *r, const double *q, double *z) {
-    return gcn_run(m, c, j, r, q, z, 1);
+// In the solver the network runs unnormalised (it is linear in r, and FP32 / BF16 cover the
+// residual's range), so its input layer comes from the CG vector kernel with no global norm.
+int gcn_precond(Model &m, const CgParts &c, int j, const double *r, const double *q, double *z, const int *jdev) {
+    return gcn_run(m, c, j, r, q, z, 0, jdev, false);
+}
+
+GcnInput gcn_input_args(Model &m) {
+    return GcnInput{m.gcn_s.p, theta_c(m, 1), m.gcn_x0.p, m.gcn_h[0].p, m.gcn_C};
 }
 
 // Loss and gradient of a batch of pairs (node layout, device); grad is FP64 on the device,
@@ -710,7 +721,7 @@ static int gcn_loss_grad(Model &m, int np, const double *const *R, const double 
     for (int p = 0; p < np; ++p) {
         // Forward with stored states: H^l at gcn_H + l * plane (l = 1..L), Ehat H^(l-1) at gcn_Y + l * plane.
         hipLaunchKernelGGL(k_gcn_first, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, C, 0, CgParts{}, 0, R[p],
-                           m.gcn_s.p, theta_c(m, 1), m.scal.p, m.gcn_x0.p, m.gcn_H.p + plane);
+                           m.gcn_s.p, theta_c(m, 1), m.scal.p, m.gcn_x0.p, m.gcn_H.p + plane, nullptr);
         GcnArgs a = gcn_args(m);
         for (int l = 2; l <= L; ++l) {
             a.in = m.gcn_H.p + size_t(l - 1) * plane;
@@ -920,6 +931,7 @@ int svlg_gnn_init(svlg_model *h, int32_t channels, int32_t layers, int32_t mfma,
     const bool mfma_ok = m.ndim == 3 && m.elem_path >= 2 && m.ngeo == 1;
     SVLG_CHECK(!(mfma == 1 && !mfma_ok), "the MFMA element step needs a Hexa8 mesh with a single element geometry");
     m.gcn_ready = false;
+    m.drop_cg_graphs();  // the network buffers are reallocated below
     m.gcn_C = channels;
     m.gcn_L = layers;
     m.gcn_mfma = (mfma == 1 || (mfma < 0 && mfma_ok)) ? 1 : 0;

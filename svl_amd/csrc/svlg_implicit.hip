@@ -69,6 +69,7 @@ struct OpArgs {
     const double *state;   // [nelem * ngp * kJ2State]
     const int32_t *plastic;  // [nelem] elastoplastic tangent somewhere in the element
     const double *Vm, *dUn;
+    const int *jdev;       // non-null: the iteration is *jdev + iter (graph replays)
 };
 
 // One node tile of the effective-stiffness operator (one thread per tile element in the
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
             if (a.scal[SC_DONE] != 0.0) return;
             // Stopping test and beta of iteration j from the partials of the previous kernels;
             // scal holds r.z and p.q of iteration j-1 (left by the vector kernel).
-            const int j = a.iter;
+            const int j = iter_of(a.jdev, a.iter);
             double v[3];
             cg_vec(a.cg, j, red, v);
             const double rr = v[1], bb = v[2];
@@ -261,7 +262,7 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
     if constexpr (!RHS) {
         if (!a.reduce) return;
         const double sp = block_sum(acc, red);
-        if (tid == 0) a.cg.pq[size_t(a.iter & 1) * nt + tile] = sp;
+        if (tid == 0) a.cg.pq[size_t(iter_of(a.jdev, a.iter) & 1) * nt + tile] = sp;
     }
 }
 
@@ -277,7 +278,7 @@ __global__ void k_solver_reset(double *scal, double tol2, double maxit) {
 // into vp[0] and b.b into ip.
 __global__ __launch_bounds__(512) void k_cg_init(int n, CgParts c, const double *__restrict__ b,
                                                  const double *__restrict__ q, const double *__restrict__ jac,
-                                                 double *__restrict__ r, double *__restrict__ z) {
+                                                 double *__restrict__ r, double *__restrict__ z, GcnInput g) {
     __shared__ double red[17];
     double rz = 0.0, rr = 0.0, bb = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -287,6 +288,8 @@ __global__ __launch_bounds__(512) void k_cg_init(int n, CgParts c, const double 
             const double zi = jac[i] * ri;
             z[i] = zi;
             rz += ri * zi;
+        } else {
+            gcn_input(g, n, i, ri);
         }
         rr += ri * ri;
         bb += b[i] * b[i];
@@ -305,12 +308,13 @@ __global__ __launch_bounds__(512) void k_cg_init(int n, CgParts c, const double 
 // Iteration j: alpha_j = r_j.z_j / p_j.q_j; x += alpha p; r -= alpha q; for Jacobi also
 // z = jac r. Partials (r.z (Jacobi), r.r, b.b) of iteration j+1 into vp[(j+1)&1]; the first
 // workgroup leaves r_j.z_j and p_j.q_j in scal for the next operator kernel.
-__global__ __launch_bounds__(512) void k_cg_update(int n, int j, CgParts c, double *scal, double *__restrict__ x,
+__global__ __launch_bounds__(512) void k_cg_update(int n, int j0, CgParts c, double *scal, double *__restrict__ x,
                                                    const double *__restrict__ p, const double *__restrict__ q,
                                                    double *__restrict__ r, double *__restrict__ z,
-                                                   const double *__restrict__ jac) {
+                                                   const double *__restrict__ jac, const int *jdev, GcnInput g) {
     __shared__ double red[3 * 17];
     if (scal[SC_DONE] != 0.0) return;
+    const int j = iter_of(jdev, j0);
     const double pq = cg_pq(c, j, red);
     double rzj;
     if (c.flexible) {
@@ -337,6 +341,8 @@ __global__ __launch_bounds__(512) void k_cg_update(int n, int j, CgParts c, doub
             const double zi = jac[i] * ri;
             z[i] = zi;
             rz += ri * zi;
+        } else {
+            gcn_input(g, n, i, ri);
         }
         rr += ri * ri;
     }
@@ -553,8 +559,64 @@ static size_t cg_parts_size(const Model &m) { return 8 * size_t(m.tiles.ntiles) 
 // preconditioner (SVLG_PRECOND_LEARNED; beta = -z.q / p.q, Notay's FCG(1)). Per iteration:
 // one operator launch, one vector launch (plus the preconditioner's); the host polls the
 // done flag every 16 iterations.
-int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int precond, int *iters,
-             double *relres) {
+__global__ void k_iter_set(int *jdev, int j) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *jdev = j;
+}
+
+__global__ void k_iter_advance(int *jdev, int by) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *jdev += by;
+}
+
+// Launches of CG iteration j (jdev null: j is the iteration; else *jdev + j is, with j the
+// parity-fixed position inside a captured pair). p ping-pongs: even iterations stage
+// p = z + beta cg_p[1] into cg_p[0], odd ones the reverse.
+static int cg_iteration(Model &m, const CgParts &c, OpArgs &a, double *x, int j, const int *jdev) {
+    const int cur = (j & 1) ? 0 : 1;
+    a.iter = j;
+    a.jdev = jdev;
+    a.p_old = m.cg_p[cur].p;
+    a.p_new = m.cg_p[1 - cur].p;
+    if (int rc = launch_op(m, OP_CG, a)) return rc;
+    hipLaunchKernelGGL(k_cg_update, dim3(kVecGrid), dim3(kVecBlock), 0, m.stream, m.ndofn, j, c, m.scal.p, x,
+                       m.cg_p[1 - cur].p, m.cg_q.p, m.cg_r.p, m.cg_z.p, m.jac.p, jdev,
+                       c.flexible ? gcn_input_args(m) : GcnInput{});
+    if (c.flexible)
+        if (int rc = gcn_precond(m, c, j + 1, m.cg_r.p, m.cg_q.p, m.cg_z.p, jdev)) return rc;
+    return 0;
+}
+
+// The executable graph of one pair of iterations (*jdev, *jdev + 1) followed by *jdev += 2,
+// captured on first use for (x, flexible, operator kernel).
+static hipGraphExec_t cg_pair_graph(Model &m, const CgParts &c, const OpArgs &a0, double *x) {
+    const void *op = op_fn(m, OP_CG);
+    for (auto &g : m.cg_graphs)
+        if (g.x == x && g.flexible == c.flexible && g.op == op) return g.exec;
+    if (!m.jdev.p && m.jdev.alloc(1) != hipSuccess) return nullptr;
+    if (hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+    OpArgs a = a0;
+    int rc = 0;
+    for (int j = 0; j < 2 && rc == 0; ++j) rc = cg_iteration(m, c, a, x, j, m.jdev.p);
+    hipLaunchKernelGGL(k_iter_advance, dim3(1), dim3(64), 0, m.stream, m.jdev.p, 2);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(m.stream, &g);
+    if (ec != hipSuccess || rc != 0 || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    m.cg_graphs.push_back({exec, x, c.flexible, op});
+    return exec;
+}
+
+int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int precond, int *iters, double *relres,
+             bool graph) {
     hipStream_t s = m.stream;
     const int n = m.ndofn;
     const int flexible = precond == SVLG_PRECOND_LEARNED ? 1 : 0;
@@ -571,39 +633,50 @@ int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int p
     a.reduce = 0;
     if (int rc = launch_op(m, OP_CG, a)) return rc;
     hipLaunchKernelGGL(k_cg_init, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, c, b, m.cg_q.p, m.jac.p, m.cg_r.p,
-                       m.cg_z.p);
+                       m.cg_z.p, flexible ? gcn_input_args(m) : GcnInput{});
     if (flexible)
         if (int rc = gcn_precond(m, c, 0, m.cg_r.p, m.cg_q.p, m.cg_z.p)) return rc;
-    int cur = 1;  // p_old of the first iteration (beta = 0 there, so p = z)
-    double hs[SC_COUNT];
     a.z = m.cg_z.p;
     a.reduce = 1;
-    int it = 0;
-    for (; it < maxit; ++it) {
-        a.iter = it;
-        a.p_old = m.cg_p[cur].p;
-        a.p_new = m.cg_p[1 - cur].p;
-        if (int rc = launch_op(m, OP_CG, a)) return rc;
-        hipLaunchKernelGGL(k_cg_update, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, it, c, m.scal.p, x,
-                           m.cg_p[1 - cur].p, m.cg_q.p, m.cg_r.p, m.cg_z.p, m.jac.p);
-        if (flexible)
-            if (int rc = gcn_precond(m, c, it + 1, m.cg_r.p, m.cg_q.p, m.cg_z.p)) return rc;
-        cur = 1 - cur;
-        // The done flag is read back every iteration from one before the previous solve's
-        // count (successive time steps need nearly the same number), every 16 before that;
-        // iterations queued past convergence exit at once on the device.
-        const int expect = m.cg_last_iters[flexible];
-        if ((expect > 0 && it + 2 >= expect) || (it & 15) == 15) {
+    hipGraphExec_t pair = graph && maxit >= 2 ? cg_pair_graph(m, c, a, x) : nullptr;
+    if (pair) hipLaunchKernelGGL(k_iter_set, dim3(1), dim3(64), 0, s, m.jdev.p, 0);
+    double hs[SC_COUNT];
+    int launched = 0;  // iterations 0 .. launched-1 are queued
+    bool done = false;
+    // The solve that converges at iteration j sets DONE in the operator kernel of iteration
+    // j, so with the previous solve's count as the prediction (successive time steps need
+    // nearly the same number) the flag is first read back once that kernel is queued, then
+    // after every launch; every 16 iterations while there is no prediction. Iterations
+    // queued past convergence exit at once on the device.
+    const int expect = m.cg_last_iters[flexible];
+    while (launched < maxit && !done) {
+        const int before = launched;
+        if (pair && launched + 2 <= maxit) {
+            // The pair graph starts at an even iteration: *jdev tracks `launched`.
+            SVLG_HIP(hipGraphLaunch(pair, s));
+            launched += 2;
+        } else {
+            if (pair) pair = nullptr;  // odd remainder: eager from here on
+            if (int rc = cg_iteration(m, c, a, x, launched, nullptr)) return rc;
+            launched += 1;
+        }
+        if ((expect > 0 && launched >= expect + 1) || (before >> 4) != (launched >> 4) || launched >= maxit) {
             SVLG_HIP(hipMemcpyAsync(hs, m.scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
             SVLG_HIP(hipStreamSynchronize(s));
-            if (hs[SC_DONE] != 0.0) break;
+            done = hs[SC_DONE] != 0.0;
         }
     }
-    hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(256), 0, s, it < maxit ? it + 1 : maxit, c, m.scal.p);
-    hipLaunchKernelGGL(k_cg_zero_rhs, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, m.scal.p, x);
+    if (!done) {
+        // The stopping test on r_maxit, then the scalars.
+        hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(256), 0, s, launched, c, m.scal.p);
+        SVLG_HIP(hipGetLastError());
+        SVLG_HIP(hipMemcpyAsync(hs, m.scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
+        SVLG_HIP(hipStreamSynchronize(s));
+    }
+    // The poll that saw DONE holds the final scalars (written with the flag); the work
+    // queued after this call needs no further synchronisation.
+    if (hs[SC_BB] == 0.0) hipLaunchKernelGGL(k_cg_zero_rhs, dim3(kVecGrid), dim3(kVecBlock), 0, s, n, m.scal.p, x);
     SVLG_HIP(hipGetLastError());
-    SVLG_HIP(hipMemcpyAsync(hs, m.scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
-    SVLG_HIP(hipStreamSynchronize(s));
     if (iters) *iters = int(hs[SC_ITERS]);
     m.cg_last_iters[flexible] = int(hs[SC_ITERS]);
     const double rel = hs[SC_BB] > 0.0 ? std::sqrt(hs[SC_RR] / hs[SC_BB]) : 0.0;
@@ -659,6 +732,7 @@ int svlg_nm_init(svlg_model *h, double dt, double rtol, int32_t maxit, int32_t p
     const int n = m.ndofn;
     m.nm_ready = false;
     m.gcn_ready = false;
+    m.drop_cg_graphs();
     m.nm_dt = dt;
     m.nm_rtol = rtol;
     m.nm_maxit = maxit;
@@ -768,7 +842,7 @@ int svlg_nm_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
         // Keff dU = Feff; the previous increment is the initial guess.
         int it = 0;
         double rel = 0.0;
-        const int rc = cg_solve(m, m.cg_b.p, m.ndU.p, m.nm_rtol, m.nm_maxit, m.nm_precond, &it, &rel);
+        const int rc = cg_solve(m, m.cg_b.p, m.ndU.p, m.nm_rtol, m.nm_maxit, m.nm_precond, &it, &rel, true);
         if (iters) iters[st] = it;
         if (relres) relres[st] = rel;
         if (rc) return rc;

@@ -25,7 +25,6 @@
 namespace svlg {
 
 int gather_free(Model &m, const double *dnode, double *dfree);
-int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int precond, int *iters, double *relres);
 int launch_nl_rhs(Model &m, int k, const double *un, const double *w, const double *dU, double *b);
 
 static constexpr int kNlGrid = 256, kNlBlock = 512;
@@ -252,7 +251,8 @@ int svlg_nl_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
             SVLG_HIP(hipMemsetAsync(m.ndU.p, 0, sizeof(double) * n, s));
             int cit = 0;
             double rel = 0.0;
-            if (int rc = cg_solve(m, m.cg_b.p, m.ndU.p, m.nm_rtol, m.nm_maxit, m.nm_precond, &cit, &rel)) return rc;
+            if (int rc = cg_solve(m, m.cg_b.p, m.ndU.p, m.nm_rtol, m.nm_maxit, m.nm_precond, &cit, &rel, true))
+                return rc;
             cg_total += cit;
             hipLaunchKernelGGL(k_nl_accum, dim3(kNlGrid), dim3(kNlBlock), 0, s, n, m.ndU.p, m.nl_dUa.p, m.cg_b.p,
                                m.nl_part.p);

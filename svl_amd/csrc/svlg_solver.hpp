@@ -80,6 +80,10 @@ __device__ __forceinline__ void sum_records(const double *part, int count, int s
     __syncthreads();
 }
 
+// Iteration index of a kernel launch: the launch's own index, plus the device-resident base
+// *jdev when the launch belongs to a replayed graph of iterations (jdev non-null).
+__device__ __forceinline__ int iter_of(const int *jdev, int j) { return jdev ? *jdev + j : j; }
+
 // Partial sums of one solve. Iteration j works on r_j, z_j, p_j, q_j = Keff p_j. Every kernel
 // writes its per-workgroup partials into the half of a ping-pong pair picked by the parity of
 // the iteration it produces, and the kernels that need a total re-sum the partials in
@@ -96,6 +100,21 @@ struct CgParts {
     int ntiles, nvb, flexible;
 };
 
+// Input layer of the learned preconditioner, formed by the CG vector kernels of the flexible
+// solve from the residual they produce: x0 = s r and H^1 = x0 c_1^T (C channels, FP32).
+struct GcnInput {
+    const double *s;
+    const float *c1;
+    float *x0, *h1;
+    int C;
+};
+
+__device__ __forceinline__ void gcn_input(const GcnInput &g, int n, int i, double r) {
+    const float xv = float(g.s[i] * r);
+    g.x0[i] = xv;
+    for (int k = 0; k < g.C; ++k) g.h1[size_t(k) * n + i] = xv * g.c1[k];
+}
+
 // (r.z, r.r, b.b) of iteration j from the vector partials.
 __device__ __forceinline__ void cg_vec(const CgParts &c, int j, double *buf, double (&o)[3]) {
     sum_records<3>(c.vp + size_t(j & 1) * 3 * c.nvb, c.nvb, 3, buf, o);
@@ -109,8 +128,18 @@ __device__ __forceinline__ void cg_pre(const CgParts &c, int j, double *buf, dou
 }
 
 // Preconditioner hook of the flexible CG loop (svlg_gcn.hip): z = P(r) on the node-DOF
-// layout for iteration j (z = z_j; r = r_j; q = q_(j-1)); writes the tp[j&1] partials
-// (z.r, z.q per tile).
-int gcn_precond(Model &m, const CgParts &c, int j, const double *r, const double *q, double *z);
+// layout for iteration j (z = z_j; r = r_j; q = q_(j-1)) from the input layer the vector
+// kernel already formed (GcnInput); writes the tp[j&1] partials (z.r, z.q per tile). With
+// jdev the iteration is *jdev + j (graph replays).
+int gcn_precond(Model &m, const CgParts &c, int j, const double *r, const double *q, double *z,
+                const int *jdev = nullptr);
+// The input layer of the solver's preconditioner (x0, H^1 buffers and c_1).
+GcnInput gcn_input_args(Model &m);
+
+// Preconditioned CG (Jacobi) or flexible CG (learned) on Keff x = b from the given x, to
+// ||b - Keff x|| <= rtol ||b|| (svlg_implicit.hip). graph: replay captured pairs of
+// iterations (for solution buffers reused across calls, such as the integrators' increment).
+int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int precond, int *iters, double *relres,
+             bool graph = false);
 
 }  // namespace svlg
