@@ -65,6 +65,8 @@ class Model:
     load_series: np.ndarray = field(default_factory=lambda: np.zeros((0, 1), np.float64))
     massform_lumped: bool = True
     name: str = "model"
+    # Free-DOF numbers of a run file ([nnodes, ndim], -1 restrained); None: the Plain scheme.
+    free_dof_override: np.ndarray | None = None
 
     def __post_init__(self):
         self.coords = np.ascontiguousarray(self.coords, dtype=np.float64)
@@ -148,6 +150,8 @@ class Model:
         components in order, restrained components skipped. Lysmer copies are tagged
         after every mesh node and are fully restrained, so they take no free index.
         """
+        if self.free_dof_override is not None:
+            return np.ascontiguousarray(self.free_dof_override, dtype=np.int32)
         free = np.full((self.nnodes, self.ndim), -1, dtype=np.int32)
         mask = ~self.restrained
         free[mask] = np.arange(int(mask.sum()), dtype=np.int32)

@@ -1,0 +1,459 @@
+"""SVL run files (JSON) in, SVL NODE recorder files out, for the models the device path runs.
+
+Reader (``read_run``) of the run-file schema that SVL's driver parses
+(RP/12-Utilities/Driver.hpp): ``Global`` (ndim, massform :1993-2004), ``Nodes`` (ndof, freedof,
+totaldof, coords :304-345), ``Materials`` (Elastic2DPlaneStrain, Elastic3DLinear,
+PlasticPlaneStrainJ2 / Plastic3DJ2 with K, G, rho, h, Sy, beta, Viscous1DLinear :567-760),
+``Elements`` (lin2DQuad4 :1185-1193, lin3DHexa8 :1272-1279, ZeroLength1D :1072-1078),
+``Dampings`` (Free / Rayleigh with an element list :1362-1454), ``Loads`` (PointLoad,
+CONCENTRATED, CONSTANT / TIMESERIES with a node list and a series file whose first entry is
+the number of samples :1461-1560), ``Combinations`` (:1929-1975), ``Simulations`` (solver,
+algorithm, integrator, analysis, combo :1747-1850) and ``Recorders`` (:1858-1927).
+Continuum nodes become mesh nodes in ascending tag order; a ZeroLength1D element with a
+Viscous1DLinear material between a mesh node and a fully restrained node is a Lysmer dashpot;
+the free-DOF numbering is taken from the file. Anything else (other elements or materials,
+support motion, constraints, point masses, body or DRM loads) is refused with ValueError:
+it is not on the device path.
+
+Writer (``write_run``) of the same schema from a ``Model`` (the inverse, with a restrained
+copy node per dashpot as setLysmerDomain makes them, PP/Method/Builder.py:1086-1131), so a
+model built here can be handed to SVL's own driver.
+
+NODE recorder (``write_node_recorder``): SVL's text format (RP/12-Utilities/Recorder.cpp:
+73-106 header, 239-268 rows): ``nIDs nDofs ntotal nsteps``, then per node ``tag ndof totaldof...``,
+then one row per step k = 1..nt-1 with every DOF of every listed node in scientific notation
+with ``ndps`` digits, each value followed by a space.
+"""
+from __future__ import annotations
+
+import json
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .mesh import ELEM_HEXA8, ELEM_QUAD4, MAT_J2, Model
+
+_CONTINUUM = {"lin2dquad4": (ELEM_QUAD4, 2, 4), "lin3dhexa8": (ELEM_HEXA8, 3, 8)}
+_ELASTIC = {"elastic2dplanestrain": 2, "elastic3dlinear": 3}
+_J2 = {"plasticplanestrainj2": 2, "plastic3dj2": 3}
+
+
+@dataclass
+class NodeRecorder:
+    file: str
+    response: str          # DISP, VEL, ACCEL
+    tags: list             # SVL node tags
+    ndps: int
+
+
+@dataclass
+class SvlRun:
+    model: Model
+    node_tags: np.ndarray            # SVL tag of every mesh node (model node order)
+    totaldof: np.ndarray             # [nnodes, ndim] SVL total-DOF numbers of the mesh nodes
+    ntotal: int
+    integrator: str                  # CENTRALDIFFERENCE, EXTENDEDNEWMARK, NEWMARK
+    algorithm: str                   # LINEAR, NEWTON
+    dt: float
+    nt: int
+    cnvgtol: float = 1e-6
+    nmax: int = 50
+    cnvgtest: int = 4
+    combo_name: str = "Combo"
+    recorders: list = field(default_factory=list)
+    skipped_recorders: list = field(default_factory=list)
+
+
+def _lc(s):
+    return str(s).lower()
+
+
+def _series_file(path, base):
+    p = path if os.path.isabs(path) else os.path.join(base, path)
+    with open(p) as f:
+        vals = f.read().split()
+    n = int(float(vals[0]))
+    return np.asarray([float(v) for v in vals[1:1 + n]], dtype=np.float64)
+
+
+def read_run(path: str) -> SvlRun:
+    """Reads an SVL run file into a device-path model (see the module docstring)."""
+    base = os.path.dirname(os.path.abspath(path))
+    with open(path) as f:
+        js = json.load(f)
+    g = js.get("Global", {})
+    ndim = int(g.get("ndim", 0))
+    lumped = _lc(g.get("massform", "CONSISTENT")) == "lumped"
+    for key in ("Supports", "Constraints", "Masses", "Sections"):
+        if js.get(key):
+            raise ValueError(f"run file has {key}: not on the device path")
+    mats = {int(t): v for t, v in js.get("Materials", {}).items()}
+    nodes = {int(t): v for t, v in js.get("Nodes", {}).items()}
+    elems = sorted(((int(t), v) for t, v in js.get("Elements", {}).items()), key=lambda x: x[0])
+    cont, dash = [], []
+    etype = None
+    for tag, e in elems:
+        name = _lc(e["name"])
+        at = e.get("attributes", {})
+        if name in _CONTINUUM:
+            t, d, nen = _CONTINUUM[name]
+            if etype is None:
+                etype = t
+            elif etype != t:
+                raise ValueError("run file mixes continuum element types")
+            if d != ndim or len(e["conn"]) != nen:
+                raise ValueError(f"element {tag}: {e['name']} does not match ndim {ndim}")
+            if _lc(at.get("rule", "GAUSS")) != "gauss" or int(at.get("np", nen)) != nen:
+                raise ValueError(f"element {tag}: only the {nen}-point Gauss rule is on the device path")
+            cont.append((tag, e))
+        elif name == "zerolength1d":
+            dash.append((tag, e))
+        else:
+            raise ValueError(f"element {tag}: {e['name']} is not on the device path")
+    if not cont:
+        raise ValueError("run file has no lin2DQuad4 / lin3DHexa8 elements")
+    mesh_tags = sorted({int(n) for _, e in cont for n in e["conn"]})
+    index = {t: i for i, t in enumerate(mesh_tags)}
+    coords = np.array([[float(c) for c in nodes[t]["coords"][:ndim]] for t in mesh_tags])
+    freedof = np.array([[int(v) for v in nodes[t]["freedof"][:ndim]] for t in mesh_tags], dtype=np.int32)
+    totaldof = np.array([[int(v) for v in nodes[t]["totaldof"][:ndim]] for t in mesh_tags], dtype=np.int32)
+    conn = np.array([[index[int(n)] for n in e["conn"]] for _, e in cont], dtype=np.int32)
+    ne = len(cont)
+    E, nu, rho = np.zeros(ne), np.zeros(ne), np.zeros(ne)
+    matt = np.zeros(ne, np.int32)
+    j2 = np.zeros((ne, 5))
+    th = 1.0
+    for i, (tag, e) in enumerate(cont):
+        at = e["attributes"]
+        if etype == ELEM_QUAD4:
+            th = float(at.get("th", 1.0)) if i == 0 else th
+            if float(at.get("th", 1.0)) != th:
+                raise ValueError("lin2DQuad4 elements with different thicknesses")
+        m = mats[int(at["material"])]
+        mname, ma = _lc(m["name"]), m.get("attributes", {})
+        if mname in _ELASTIC and _ELASTIC[mname] == ndim:
+            E[i], nu[i], rho[i] = float(ma.get("E", 0.0)), float(ma.get("nu", 0.0)), float(ma.get("rho", 0.0))
+        elif mname in _J2 and _J2[mname] == ndim:
+            K, G = float(ma.get("K", 0.0)), float(ma.get("G", 0.0))
+            matt[i] = MAT_J2
+            j2[i] = [K, G, float(ma.get("Sy", 0.0)), float(ma.get("h", 0.0)), float(ma.get("beta", 0.0))]
+            E[i] = 9.0 * K * G / (3.0 * K + G)
+            nu[i] = (3.0 * K - 2.0 * G) / (2.0 * (3.0 * K + G))
+            rho[i] = float(ma.get("rho", 0.0))
+        else:
+            raise ValueError(f"element {tag}: material {m['name']} is not on the device path")
+    dn, dd, de = [], [], []
+    for tag, e in dash:
+        a, b = (int(n) for n in e["conn"])
+        at = e["attributes"]
+        m = mats[int(at["material"])]
+        if _lc(m["name"]) != "viscous1dlinear":
+            raise ValueError(f"ZeroLength1D {tag}: only Viscous1DLinear dashpots are on the device path")
+        pairs = [(a, b), (b, a)]
+        mesh_end = [p for p in pairs if p[0] in index and all(int(v) == -1 for v in nodes[p[1]]["freedof"])]
+        if not mesh_end:
+            raise ValueError(f"ZeroLength1D {tag}: a dashpot must join a mesh node to a fully restrained node")
+        dn.append(index[mesh_end[0][0]])
+        dd.append(int(at["dir"]))
+        de.append(float(m["attributes"]["eta"]))
+    model = Model(ndim=ndim, coords=coords, node_tags=np.asarray(mesh_tags, dtype=np.int64), elem_type=etype,
+                  conn=conn, E=E, nu=nu, rho=rho, thickness=th, restrained=freedof < 0, mat_type=matt, j2=j2,
+                  dash_node=np.asarray(dn, np.int32), dash_dir=np.asarray(dd, np.int32),
+                  dash_eta=np.asarray(de, np.float64), massform_lumped=lumped,
+                  name=os.path.splitext(os.path.basename(path))[0])
+    model.free_dof_override = freedof
+    nfree = int(freedof.max()) + 1 if (freedof >= 0).any() else 0
+    if sorted(freedof[freedof >= 0].tolist()) != list(range(nfree)):
+        raise ValueError("the free DOFs of the mesh nodes must be numbered 0..nfree-1")
+    # Dampings (per continuum element; dashpot elements have no Rayleigh term).
+    am, ak = np.zeros(ne), np.zeros(ne)
+    etag_index = {t: i for i, (t, _) in enumerate(cont)}
+    for _, d in sorted(js.get("Dampings", {}).items(), key=lambda x: int(x[0])):
+        nm, at = _lc(d["name"]), d.get("attributes", {})
+        if nm not in ("free", "rayleigh"):
+            raise ValueError(f"damping {d['name']} is not on the device path")
+        for t in at.get("list", []):
+            if int(t) in etag_index:
+                i = etag_index[int(t)]
+                am[i] = float(at.get("am", 0.0)) if nm == "rayleigh" else 0.0
+                ak[i] = float(at.get("ak", 0.0)) if nm == "rayleigh" else 0.0
+    model.ray_am, model.ray_ak = am, ak
+    # Simulation.
+    sim = js["Simulations"]
+    sa = sim["attributes"]
+    integ = _lc(sa["integrator"]["name"]).upper()
+    alg = sa["algorithm"]
+    run = SvlRun(model=model, node_tags=np.asarray(mesh_tags), totaldof=totaldof,
+                 ntotal=int(g.get("ntotal", 0)), integrator=integ, algorithm=_lc(alg["name"]).upper(),
+                 dt=float(sa["integrator"].get("dt", 0.0)), nt=int(sa["analysis"].get("nt", 1)),
+                 cnvgtol=float(alg.get("cnvgtol", 1e-6)), nmax=int(alg.get("nstep", 50)),
+                 cnvgtest=int(alg.get("cnvgtest", 4)))
+    if _lc(sa["analysis"]["name"]) != "dynamic":
+        raise ValueError("only DYNAMIC analyses are on the device path")
+    # Loads of the simulation's combination.
+    combo = js.get("Combinations", {}).get(str(sim["combo"]), {})
+    run.combo_name = combo.get("name", "Combo")
+    ca = combo.get("attributes", {})
+    loads = js.get("Loads", {})
+    series, rows = [], []
+    for lt, fac in zip(ca.get("load", []), ca.get("factor", [])):
+        ld = loads[str(lt)]
+        if _lc(ld["name"]) != "pointload":
+            raise ValueError(f"load {lt}: {ld['name']} is not on the device path")
+        at = ld["attributes"]
+        if _lc(at.get("type")) != "concentrated":
+            raise ValueError(f"load {lt}: only CONCENTRATED point loads are on the device path")
+        fn = _lc(at.get("name"))
+        if fn == "timeseries":
+            s = _series_file(at["file"], base)
+        elif fn == "constant":
+            s = np.full(run.nt, float(at["mag"]))
+        else:
+            raise ValueError(f"load {lt}: pattern {at.get('name')} is not on the device path")
+        direction = [float(v) for v in at["dir"]][:ndim]
+        seen = set()
+        for n in at["list"]:
+            if int(n) in seen:  # SVL assigns per node: a repeated node counts once
+                continue
+            seen.add(int(n))
+            rows.append((index[int(n)], direction, float(fac)))
+            series.append(s)
+    if rows:
+        nts = max(len(s) for s in series)
+        model.load_node = np.asarray([r[0] for r in rows], np.int32)
+        model.load_dir = np.asarray([r[1] for r in rows], np.float64)
+        model.load_factor = np.asarray([r[2] for r in rows], np.float64)
+        model.load_series = np.zeros((len(rows), nts))
+        for i, s in enumerate(series):
+            model.load_series[i, :len(s)] = s
+    # NODE recorders (others are listed as skipped).
+    for _, r in sorted(js.get("Recorders", {}).items(), key=lambda x: int(x[0])):
+        if _lc(r["name"]) == "node" and _lc(r.get("resp", "")) in ("disp", "vel", "accel"):
+            run.recorders.append(NodeRecorder(r["file"], r["resp"].upper(), [int(t) for t in r["list"]],
+                                              int(r.get("ndps", 8))))
+        else:
+            run.skipped_recorders.append(r)
+    return run
+
+
+def write_run(model: Model, path: str, integrator: str, dt: float, nt: int, algorithm: str = "LINEAR",
+              cnvgtol: float = 1e-6, nmax: int = 50, cnvgtest: int = 4, recorders=(), series_dir: str | None = None):
+    """Writes `model` as an SVL run file (tags: mesh nodes 1..n, dashpot partner nodes after
+    them; continuum elements 1..ne, dashpots after them; one material per distinct parameter
+    set; one TIMESERIES PointLoad per model load with its series in a text file next to the
+    run file). recorders: (file, resp, [mesh node indices], ndps). Returns the node tags."""
+    nd = model.ndim
+    out = {"Global": {"ndim": nd, "massform": "LUMPED" if model.massform_lumped else "CONSISTENT",
+                      "update": "RESTART"}}
+    nn, ne = model.nnodes, model.nelem
+    fd = model.free_dof()
+    nodes = {}
+    for i in range(nn):
+        tot = [i * nd + c for c in range(nd)]
+        nodes[str(i + 1)] = {"ndof": nd, "freedof": [int(v) for v in fd[i]], "totaldof": tot,
+                             "coords": [float(c) for c in model.coords[i]]}
+    mats, mat_of = {}, {}
+
+    def material(key, name, attrs):
+        if key not in mat_of:
+            mat_of[key] = len(mats) + 1
+            mats[str(mat_of[key])] = {"name": name, "attributes": attrs}
+        return mat_of[key]
+
+    elems = {}
+    ename = "lin2DQuad4" if model.elem_type == ELEM_QUAD4 else "lin3DHexa8"
+    for e in range(ne):
+        if model.mat_type[e] == MAT_J2:
+            K, G, sy, H, beta = (float(v) for v in model.j2[e])
+            mname = "PlasticPlaneStrainJ2" if nd == 2 else "Plastic3DJ2"
+            mt = material(("j2", K, G, sy, H, beta, float(model.rho[e])), mname,
+                          {"K": K, "G": G, "rho": float(model.rho[e]), "h": H, "Sy": sy, "beta": beta})
+        else:
+            mname = "Elastic2DPlaneStrain" if nd == 2 else "Elastic3DLinear"
+            mt = material(("el", float(model.E[e]), float(model.nu[e]), float(model.rho[e])), mname,
+                          {"E": float(model.E[e]), "nu": float(model.nu[e]), "rho": float(model.rho[e])})
+        at = {"material": mt, "rule": "GAUSS", "np": 4 if nd == 2 else 8}
+        if nd == 2:
+            at["th"] = float(model.thickness)
+        elems[str(e + 1)] = {"name": ename, "conn": [int(c) + 1 for c in model.conn[e]], "attributes": at}
+    ntot = nn * nd
+    for k in range(model.dash_node.size):
+        tag = nn + k + 1
+        n = int(model.dash_node[k])
+        nodes[str(tag)] = {"ndof": nd, "freedof": [-1] * nd, "totaldof": [ntot + k * nd + c for c in range(nd)],
+                           "coords": [float(c) for c in model.coords[n]]}
+        mt = material(("visc", float(model.dash_eta[k])), "Viscous1DLinear", {"eta": float(model.dash_eta[k])})
+        elems[str(ne + k + 1)] = {"name": "ZeroLength1D", "conn": [n + 1, tag],
+                                  "attributes": {"material": mt, "dir": int(model.dash_dir[k])}}
+    out["Global"]["ntotal"] = ntot + model.dash_node.size * nd
+    out["Global"]["nfree"] = model.nfree
+    out["Nodes"], out["Materials"], out["Elements"] = nodes, mats, elems
+    damp = {}
+    am = np.zeros(ne) if model.ray_am is None else np.asarray(model.ray_am)
+    ak = np.zeros(ne) if model.ray_ak is None else np.asarray(model.ray_ak)
+    groups = {}
+    for e in range(ne):
+        groups.setdefault((float(am[e]), float(ak[e])), []).append(e + 1)
+    for i, ((a, b), lst) in enumerate(sorted(groups.items())):
+        damp[str(i + 1)] = ({"name": "Free", "attributes": {"list": lst}} if a == 0.0 and b == 0.0 else
+                            {"name": "Rayleigh", "attributes": {"am": a, "ak": b, "list": lst}})
+    out["Dampings"] = damp
+    base = series_dir or os.path.dirname(os.path.abspath(path))
+    os.makedirs(base, exist_ok=True)
+    loads, lids, facs = {}, [], []
+    stem = os.path.splitext(os.path.basename(path))[0]
+    for l in range(model.load_node.size):
+        fname = os.path.join(base, f"{stem}_load{l + 1}.txt")
+        s = model.load_series[l]
+        with open(fname, "w") as f:
+            f.write(f"{s.size}\n")
+            f.write("\n".join(repr(float(v)) for v in s))
+            f.write("\n")
+        loads[str(l + 1)] = {"name": "PointLoad", "attributes": {
+            "type": "CONCENTRATED", "name": "TIMESERIES", "file": fname, "list": [int(model.load_node[l]) + 1],
+            "dir": [float(v) for v in model.load_dir[l]]}}
+        lids.append(l + 1)
+        facs.append(float(model.load_factor[l]))
+    out["Loads"] = loads
+    out["Combinations"] = {"1": {"name": "Combo", "attributes": {"load": lids, "factor": facs}}}
+    out["Simulations"] = {"combo": 1, "attributes": {
+        "solver": {"name": "EIGEN", "update": "OFF"},
+        "algorithm": {"name": algorithm, "cnvgtol": cnvgtol, "nstep": nmax, "cnvgtest": cnvgtest},
+        "integrator": {"name": integrator, "dt": dt, "mtol": 1e-12, "ktol": 1e-12, "ftol": 1e-12},
+        "analysis": {"name": "DYNAMIC", "nt": nt}}}
+    out["Recorders"] = {str(i + 1): {"name": "NODE", "file": f, "resp": r, "ndps": int(p), "nsamp": 1,
+                                     "list": [int(n) + 1 for n in lst]}
+                        for i, (f, r, lst, p) in enumerate(recorders)}
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    return np.arange(1, nn + 1)
+
+
+def _fmt(v: float, ndps: int) -> str:
+    return f"{v:.{ndps}e}"
+
+
+def write_node_recorder(path: str, rec: NodeRecorder, node_tags, totaldof, ntotal: int, nt: int,
+                        values: np.ndarray):
+    """values[k-1, i, c]: response of component c of the recorder's node i after step k."""
+    ndof = totaldof.shape[1]
+    tags = list(node_tags)
+    rows = [tags.index(t) for t in rec.tags]
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as f:
+        f.write(f"{len(rec.tags)} {len(rec.tags) * ndof} {ntotal} {nt}\n")
+        for t, r in zip(rec.tags, rows):
+            f.write(f"{t} {ndof}" + "".join(f" {int(d)}" for d in totaldof[r]) + "\n")
+        for k in range(values.shape[0]):
+            f.write("".join(_fmt(float(v), rec.ndps) + " " for v in values[k].reshape(-1)) + "\n")
+
+
+def read_node_recorder(path: str):
+    """Parses a NODE recorder file: (tags, totaldof [n, ndof], ntotal, nt, values [rows, n*ndof])."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    nids, ndofs, ntotal, nt = (int(v) for v in lines[0].split())
+    tags, tot = [], []
+    for line in lines[1:1 + nids]:
+        p = line.split()
+        tags.append(int(p[0]))
+        tot.append([int(v) for v in p[2:2 + int(p[1])]])
+    vals = np.array([[float(v) for v in line.split()] for line in lines[1 + nids:] if line.strip()])
+    return tags, np.asarray(tot), ntotal, nt, vals.reshape(-1, ndofs) if vals.size else np.zeros((0, ndofs))
+
+
+def run(svl: SvlRun, out_dir: str | None = None, json_path: str | None = None, cg_rtol: float = 1e-12,
+        learned: bool = False) -> dict:
+    """Runs the simulation of a run file on the device (DynamicAnalysis k = 1..nt-1) and writes
+    its NODE recorders into out_dir (default: SVL's <run dir>/../Solution/<combo>/). Returns
+    {file: values}. VEL and ACCEL are formed from the recorded displacement history by the
+    integrator's own update formulas (CentralDifference.cpp:141-148, ExtendedNewmarkBeta.cpp:
+    141-150); the NODE recorder is written for every step (SVL writes it on every call)."""
+    from .integrators import CentralDifference, DeviceModel, ExtendedNewmarkBeta, LearnedPreconditioner, NewmarkBeta
+    m = svl.model
+    nd = m.ndim
+    fd = m.free_dof()
+    needed = sorted({t for r in svl.recorders for t in r.tags})
+    tags = list(svl.node_tags)
+    rec_dofs, where = [], {}
+    for t in needed:
+        for c in range(nd):
+            f = int(fd[tags.index(t)][c])
+            if f >= 0:
+                where[(t, c)] = len(rec_dofs)
+                rec_dofs.append(f)
+    dm = DeviceModel(m)
+    steps = svl.nt - 1
+    if svl.integrator == "CENTRALDIFFERENCE":
+        integ = CentralDifference(dm, svl.dt)
+    elif svl.algorithm == "NEWTON":
+        if svl.integrator != "NEWMARK":
+            raise ValueError("the device Newton path is NewmarkBeta's (consistent residual)")
+        integ = NewmarkBeta(dm, svl.dt, cnvgtol=svl.cnvgtol, nmax=svl.nmax, cnvgtest=svl.cnvgtest, rtol=cg_rtol)
+    elif svl.integrator in ("NEWMARK", "EXTENDEDNEWMARK"):
+        integ = ExtendedNewmarkBeta(dm, svl.dt, rtol=cg_rtol)
+    else:
+        dm.close()
+        raise ValueError(f"integrator {svl.integrator} is not on the device path")
+    integ.Initialize()
+    if learned and not isinstance(integ, CentralDifference):
+        LearnedPreconditioner(integ, channels=1, layers=8, mfma=-1).use(True)
+    U = integ.advance(1, steps, rec_dofs) if rec_dofs else (integ.advance(1, steps) or np.zeros((steps, 0)))
+    dm.close()
+    U = np.asarray(U).reshape(steps, len(rec_dofs))
+    Up = np.vstack([np.zeros((1, U.shape[1])), U[:-1]])
+    dt = svl.dt
+    if svl.integrator == "CENTRALDIFFERENCE":
+        Upp = np.vstack([np.zeros((2, U.shape[1])), U[:-2]])[:steps]
+        dU = U - Up
+        V = 1.0 / 2.0 / dt * ((Up + dU) - Upp)
+        A = 1.0 / dt / dt * ((dU - Up) + Upp)
+    else:
+        V, A = np.zeros_like(U), np.zeros_like(U)
+        v, a = np.zeros(U.shape[1]), np.zeros(U.shape[1])
+        for k in range(steps):
+            du = U[k] - Up[k]
+            a = 4.0 / dt / dt * du - 4.0 / dt * v - a
+            v = 2.0 / dt * du - v
+            V[k], A[k] = v, a
+    resp = {"DISP": U, "VEL": V, "ACCEL": A}
+    base = out_dir
+    if base is None:
+        root = os.path.dirname(os.path.abspath(json_path)) if json_path else "."
+        base = os.path.join(root, "..", "Solution", svl.combo_name)
+    out = {}
+    for r in svl.recorders:
+        vals = np.zeros((steps, len(r.tags), nd))
+        for i, t in enumerate(r.tags):
+            for c in range(nd):
+                j = where.get((t, c))
+                if j is not None:
+                    vals[:, i, c] = resp[r.response][:, j]
+        p = os.path.join(base, r.file)
+        write_node_recorder(p, r, svl.node_tags, svl.totaldof, svl.ntotal, svl.nt, vals)
+        out[p] = vals
+    return out
+
+
+def main(argv=None):
+    """Command line: ``python -m svl_amd.svlfile RUN.json [--out DIR] [--learned]`` runs an
+    SVL run file on the GPU and writes its NODE recorders (default: SVL's
+    <run dir>/../Solution/<combo>/)."""
+    import argparse
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("file")
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--cg-rtol", type=float, default=1e-12)
+    ap.add_argument("--learned", action="store_true", help="learned preconditioner for the implicit solves")
+    a = ap.parse_args(argv)
+    r = read_run(a.file)
+    files = run(r, out_dir=a.out, json_path=a.file, cg_rtol=a.cg_rtol, learned=a.learned)
+    for p in files:
+        print(p)
+    for s in r.skipped_recorders:
+        print(f"skipped recorder {s.get('name')} ({s.get('file')}): not written by the device path")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,64 @@
+"""GPU: SVL run files through the device path, NODE recorders out (svl_amd.svlfile.run).
+
+A model is written as an SVL run file, read back, run on the device by the integrator and
+algorithm the file names, and its NODE recorder files (DISP, VEL, ACCEL) are parsed and
+compared with the oracle's histories of the original model: relative L2 <= 1e-10 for DISP
+(CG at 1e-13 on the implicit paths), <= 1e-8 for VEL / ACCEL (formed from the displacement
+history by the integrator's update formulas, so differences of nearly equal displacements
+enter), restrained components exactly 0.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from svl_amd import svlfile as S
+from svl_amd import workloads as W
+
+from _meshes import block_model
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def case(kind):
+    if kind == "cd":
+        m, dt, nt = W.layer2d(16, 8, nt=200)
+        return m, dt, 200, "CENTRALDIFFERENCE", "LINEAR", oracle.CENTRAL_DIFFERENCE, {}
+    if kind == "newmark":
+        m, dt, nt, src = block_model(4, 4, 3)
+        m.ray_am = np.full(m.nelem, 0.5)
+        m.ray_ak = np.full(m.nelem, 1e-3)
+        return m, dt, 120, "EXTENDEDNEWMARK", "LINEAR", oracle.EXTENDED_NEWMARK, {}
+    m, dt, nt = W.soil_structure_j2(8, 8, 4, nt=41, block=(2, 2, 2), Ap=2e7, t0=0.08, f0=8.0)
+    return m, dt, 41, "NEWMARK", "NEWTON", oracle.NEWMARK, dict(algorithm=oracle.NEWTON, nmax=20, cnvgtest=5,
+                                                                   cnvgtol=1e-6)
+
+
+@pytest.mark.parametrize("kind", ["cd", "newmark", "newton"])
+def test_run_file_through_device(tmp_path, kind):
+    m, dt, nt, integ, alg, ointeg, okw = case(kind)
+    nodes = [0, m.nnodes // 3, m.nnodes - 1, int(m.load_node[0])]
+    recs = [("U.out", "DISP", nodes, 12), ("V.out", "VEL", nodes, 12), ("A.out", "ACCEL", nodes, 12)]
+    p = str(tmp_path / "run" / "model.json")
+    import os
+    os.makedirs(os.path.dirname(p))
+    S.write_run(m, p, integ, dt, nt, alg, cnvgtol=1e-6, nmax=20, cnvgtest=5, recorders=recs)
+    r = S.read_run(p)
+    out_dir = str(tmp_path / "Solution")
+    S.run(r, out_dir=out_dir, cg_rtol=1e-13)
+    fd = m.free_dof()
+    cols = [int(fd[n][c]) for n in nodes for c in range(m.ndim)]
+    free_cols = [f for f in cols if f >= 0]
+    ref = oracle.run(m, ointeg, dt, nt, free_cols, want_va=True, **okw)
+    for fname, key, tol in (("U.out", "U", 1e-10), ("V.out", "V", 1e-8), ("A.out", "A", 1e-8)):
+        tags, tot, ntotal, ntr, vals = S.read_node_recorder(os.path.join(out_dir, fname))
+        assert tags == [n + 1 for n in nodes] and ntr == nt and vals.shape == (nt - 1, len(cols))
+        restrained = [i for i, f in enumerate(cols) if f < 0]
+        assert np.all(vals[:, restrained] == 0.0)
+        free = [i for i, f in enumerate(cols) if f >= 0]
+        err = rel(vals[:, free], ref[key])
+        print(f"{kind} {fname}: rel L2 {err:.2e}")
+        assert err < tol
