@@ -78,7 +78,7 @@ def _lib():
         L.oracle_last_error.restype = C.c_char_p
         L.oracle_run.restype = C.c_int
         L.oracle_run.argtypes = [C.POINTER(MeshDesc), C.POINTER(OracleOpts), C.c_int32, i32p, f64p, f64p, f64p,
-                                 f64p, f64p, i32p, f64p]
+                                 f64p, f64p, i32p, f64p, C.c_int32, i32p, f64p]
         L.oracle_element_matrices.restype = C.c_int
         L.oracle_element_matrices.argtypes = [C.POINTER(MeshDesc), C.c_int32, f64p, f64p, f64p]
         L.oracle_internal_force.restype = C.c_int
@@ -105,11 +105,14 @@ def _f(a):
 
 
 def run(model, integrator: int, dt: float, nt: int, rec_dofs=(), ftol: float = 1e-12, want_va: bool = False,
-        algorithm: int = LINEAR, nmax: int = 20, cnvgtest: int = 5, cnvgtol: float = 1e-6):
+        algorithm: int = LINEAR, nmax: int = 20, cnvgtest: int = 5, cnvgtol: float = 1e-6, reac_tot=()):
     """Runs k = 1..nt-1; returns dict with U (and V, A) histories [nt-1, nrec],
     the final free-DOF displacement, the time-loop wall time in seconds and, for Newton,
     the iterations of every step and the convergence-test value of every iteration
-    ("residuals", [nt-1, nmax], NaN past the last iteration)."""
+    ("residuals", [nt-1, nmax], NaN past the last iteration). reac_tot: total DOF indices
+    (mesh node n, component c: n*ndim + c; the restrained partner node of the dashpots of mesh
+    node n follows the mesh nodes in order of first appearance) whose reaction history lands
+    in "R"."""
     from svl_amd._capi import make_desc
     desc, keep = make_desc(model)
     rec = np.ascontiguousarray(rec_dofs, dtype=np.int32)
@@ -122,12 +125,16 @@ def run(model, integrator: int, dt: float, nt: int, rec_dofs=(), ftol: float = 1
     o = OracleOpts(integrator, nt, dt, 1e-12, 1e-12, ftol, algorithm, nmax, cnvgtest, 0, cnvgtol)
     its = np.zeros(max(ns, 1), np.int32)
     res = np.full((max(ns, 1), max(nmax, 1)), np.nan) if algorithm == NEWTON else None
+    rt = np.ascontiguousarray(reac_tot, dtype=np.int32)
+    hr = np.zeros((max(ns, 1), rt.size)) if rt.size else None
     _chk(_lib().oracle_run(C.byref(desc), C.byref(o), rec.size, rec.ctypes.data_as(C.POINTER(C.c_int32)), _f(hu),
                            _f(hv), _f(ha), C.byref(secs), _f(uf), its.ctypes.data_as(C.POINTER(C.c_int32)),
-                           _f(res)), "run")
+                           _f(res), int(rt.size), rt.ctypes.data_as(C.POINTER(C.c_int32)), _f(hr)), "run")
     out = {"U": hu, "final_U": uf, "seconds": secs.value, "iters": its[:ns]}
     if res is not None:
         out["residuals"] = res[:ns]
+    if hr is not None:
+        out["R"] = hr[:ns]
     if want_va:
         out["V"] = hv
         out["A"] = ha

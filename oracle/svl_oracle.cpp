@@ -839,6 +839,43 @@ struct Assembler {
         }
         // Dashpots: Viscous1DLinear::GetStress() is 0, so every entry is filtered out.
     }
+    // Assembler::ComputeDynamicInternalForceVector (Assembler.cpp:273-287, 596-619) for the
+    // reactions: every element with a fixed node (Element::HasFixedNode, Element.cpp:53-64)
+    // adds Fint_e + C_e V_e + M_e A_e (lin3DHexa8.cpp:416-437, lin2DQuad4 alike), unfiltered;
+    // a dashpot adds C_e V_e (ZeroLength1D.cpp:257-277, its Fint and mass are zero).
+    void fdyn(const std::vector<double> &V, const std::vector<double> &A, std::vector<double> &F) const {
+        F.assign(m->ntotal, 0.0);
+        auto fixed = [&](int node) {
+            for (int c = 0; c < m->dim; ++c)
+                if (m->freedof[size_t(node) * m->dim + c] < 0) return true;
+            return false;
+        };
+        int dofs[24];
+        double f[24];
+        const int nd = m->nen * m->dim;
+        for (auto &e : m->elems) {
+            bool any = false;
+            for (int a = 0; a < m->nen; ++a) any = any || fixed(e.nodes[a]);
+            if (!any) continue;
+            elem_dofs(e, dofs);
+            element_fint(*m, e, f);
+            Mat Ce = element_damping(*m, e), Me = element_mass(*m, e);
+            for (int i = 0; i < nd; ++i) {
+                double cv = 0.0, ma = 0.0;
+                for (int j = 0; j < nd; ++j) {
+                    cv += Ce(i, j) * V[dofs[j]];
+                    ma += Me(i, j) * A[dofs[j]];
+                }
+                F[dofs[i]] += (f[i] + cv) + ma;
+            }
+        }
+        for (auto &dp : m->dash) {
+            const int di = dp.node * m->dim + dp.dir, dj = dp.partner * m->dim + dp.dir;
+            const double w = -V[di] + V[dj];  // a^T V with a = [-e_dir, +e_dir]
+            F[di] += dp.eta * (-1.0) * w;
+            F[dj] += dp.eta * w;
+        }
+    }
     // Assembler::ComputeExternalForceVector (Assembler.cpp:291-350, 485).
     void fext(int k, std::vector<double> &F) const {
         F.assign(m->ntotal, 0.0);
@@ -946,7 +983,8 @@ const char *oracle_last_error(void) { return g_err.c_str(); }
 // free DOF indices. `seconds` (optional) receives the wall time of the time loop only.
 int oracle_run(const svlg_mesh_desc *d, const oracle_opts *o, int32_t nrec, const int32_t *rec_dofs,
                double *hist_U, double *hist_V, double *hist_A, double *seconds, double *final_U,
-               int32_t *newton_iters, double *newton_res) {
+               int32_t *newton_iters, double *newton_res, int32_t nreac, const int32_t *reac_tot,
+               double *hist_R) {
     try {
         Model m = build(d);
         Assembler as{&m, o->mtol, o->ktol, o->ftol};
@@ -1084,6 +1122,18 @@ int oracle_run(const svlg_mesh_desc *d, const oracle_opts *o, int32_t nrec, cons
             // DynamicAnalysis::UpdateDomain: commit (elastic: newStrain = Strain).
             commit_states(m);
             int row = k - 1;
+            if (nreac > 0 && hist_R) {
+                // Integrator::ComputeReactionForce (CentralDifference.cpp:156-171, Newmark
+                // alike): R = Fdyn - Fext - Fbar at the fixed nodes (DynamicAnalysis.cpp:130-150).
+                std::vector<double> Fd, Fe;
+                as.fdyn(V, A, Fd);
+                as.fext(k, Fe);
+                for (int r = 0; r < nreac; ++r) {
+                    const int td = reac_tot[r];
+                    if (td < 0 || td >= m.ntotal) { g_err = "reaction dof out of range"; return 2; }
+                    hist_R[size_t(row) * nreac + r] = (Fd[td] - Fe[td]) - 0.0;
+                }
+            }
             for (int r = 0; r < nrec; ++r) {
                 int td = rec_tot[r];
                 if (hist_U) hist_U[size_t(row) * nrec + r] = U[td];
