@@ -152,6 +152,14 @@ int svlg_cd_get_state(svlg_model *m, double *U, double *V, double *A);
 /* Blocks until all work queued on the model's stream is done. */
 int svlg_sync(svlg_model *m);
 
+/* Reactions at the mesh nodes nodes[0..n) (description order) for the current state of the
+ * integrator initialised last, after step k (DynamicAnalysis::UpdateDomain,
+ * DynamicAnalysis.cpp:130-150 -> Integrator::ComputeReactionForce, CentralDifference.cpp:156-171):
+ * R = Fint + C V + M A - Fext(k) assembled over the node's elements
+ * (Assembler::ComputeDynamicInternalForceVector, Assembler.cpp:273-287), R [n*ndim]. The
+ * restrained partner node of a dashpot carries -eta V of its mesh node. */
+int svlg_reactions(svlg_model *m, int32_t n, const int32_t *nodes, int32_t k, double *R);
+
 /* ---- Independent meshes advanced together (no SVL counterpart: SVL runs one model per
  * process) --------------------------------------------------------------------------
  * A model whose description is the disjoint union of nparts meshes (node ranges

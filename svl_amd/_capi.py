@@ -125,6 +125,7 @@ SIGNATURES = {
     "svlg_cd_steps_async": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "svlg_cd_get_state": (C.c_int, [C.c_void_p, f64p, f64p, f64p]),
     "svlg_sync": (C.c_int, [C.c_void_p]),
+    "svlg_reactions": (C.c_int, [C.c_void_p, C.c_int32, i32p, C.c_int32, f64p]),
     "svlg_cd_init_batch": (C.c_int, [C.c_void_p, C.c_int32, i32p, f64p, C.c_double]),
     "svlg_cd_graph_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "svlg_autotune_tiles": (C.c_int, [C.POINTER(MeshDesc), C.c_void_p, i32p, C.c_int32, C.c_int32, i32p, f64p,

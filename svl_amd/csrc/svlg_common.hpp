@@ -112,6 +112,7 @@ struct Model {
 
     // Central difference state.
     bool cd_ready = false;
+    int active = 0;                   // integrator of the last init: 1 CentralDifference, 2 Newmark
     double dt = 0.0, ftol = 1e-12;
     int ring = 0;                     // slot of U_k (current)
     long steps_done = 0;

@@ -667,6 +667,7 @@ static int cd_init_impl(Model &m, double dt, const std::vector<double> *dtd, dou
     m.ring = 0;
     m.steps_done = 0;
     m.cd_ready = true;
+    m.active = 1;
     return 0;
 }
 

@@ -793,6 +793,7 @@ int svlg_nm_init(svlg_model *h, double dt, double rtol, int32_t maxit, int32_t p
     }
     m.nm_steps_done = 0;
     m.nm_ready = true;
+    m.active = 2;
     return 0;
 }
 

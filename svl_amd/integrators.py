@@ -68,6 +68,15 @@ class DeviceModel:
         return int(capi.lib().svlg_model_stream(self.h) or 0)
 
     # ---------------------------------------------------------------- Assembler
+    def reactions(self, nodes, k: int) -> np.ndarray:
+        """Reactions [len(nodes), ndim] at mesh nodes after step k for the integrator
+        initialised last (Integrator::ComputeReactionForce restricted to these nodes)."""
+        q = np.ascontiguousarray(nodes, dtype=np.int32)
+        R = np.zeros((q.size, self.ndim))
+        capi.check(capi.lib().svlg_reactions(self.h, int(q.size), capi.iptr(q), int(k), capi.fptr(R)),
+                   "svlg_reactions")
+        return R
+
     def ComputeMassMatrix(self) -> np.ndarray:
         """Lumped mass diagonal in free DOFs (Assembler::ComputeMassMatrix)."""
         out = np.zeros(self.nfree)

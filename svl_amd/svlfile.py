@@ -63,6 +63,8 @@ class SvlRun:
     combo_name: str = "Combo"
     recorders: list = field(default_factory=list)
     skipped_recorders: list = field(default_factory=list)
+    # Restrained nodes outside the mesh (dashpot partners): tag -> (totaldof, [(mesh node, dir, eta)]).
+    partners: dict = field(default_factory=dict)
 
 
 def _lc(s):
@@ -144,6 +146,7 @@ def read_run(path: str) -> SvlRun:
         else:
             raise ValueError(f"element {tag}: material {m['name']} is not on the device path")
     dn, dd, de = [], [], []
+    partner_of = {}
     for tag, e in dash:
         a, b = (int(n) for n in e["conn"])
         at = e["attributes"]
@@ -157,6 +160,9 @@ def read_run(path: str) -> SvlRun:
         dn.append(index[mesh_end[0][0]])
         dd.append(int(at["dir"]))
         de.append(float(m["attributes"]["eta"]))
+        pt = mesh_end[0][1]
+        partner_of.setdefault(pt, ([int(v) for v in nodes[pt]["totaldof"][:ndim]], []))[1].append(
+            (dn[-1], dd[-1], de[-1]))
     model = Model(ndim=ndim, coords=coords, node_tags=np.asarray(mesh_tags, dtype=np.int64), elem_type=etype,
                   conn=conn, E=E, nu=nu, rho=rho, thickness=th, restrained=freedof < 0, mat_type=matt, j2=j2,
                   dash_node=np.asarray(dn, np.int32), dash_dir=np.asarray(dd, np.int32),
@@ -189,6 +195,7 @@ def read_run(path: str) -> SvlRun:
                  dt=float(sa["integrator"].get("dt", 0.0)), nt=int(sa["analysis"].get("nt", 1)),
                  cnvgtol=float(alg.get("cnvgtol", 1e-6)), nmax=int(alg.get("nstep", 50)),
                  cnvgtest=int(alg.get("cnvgtest", 4)))
+    run.partners = partner_of
     if _lc(sa["analysis"]["name"]) != "dynamic":
         raise ValueError("only DYNAMIC analyses are on the device path")
     # Loads of the simulation's combination.
@@ -229,7 +236,7 @@ def read_run(path: str) -> SvlRun:
             model.load_series[i, :len(s)] = s
     # NODE recorders (others are listed as skipped).
     for _, r in sorted(js.get("Recorders", {}).items(), key=lambda x: int(x[0])):
-        if _lc(r["name"]) == "node" and _lc(r.get("resp", "")) in ("disp", "vel", "accel"):
+        if _lc(r["name"]) == "node" and _lc(r.get("resp", "")) in ("disp", "vel", "accel", "reaction"):
             run.recorders.append(NodeRecorder(r["file"], r["resp"].upper(), [int(t) for t in r["list"]],
                                               int(r.get("ndps", 8))))
         else:
@@ -242,7 +249,9 @@ def write_run(model: Model, path: str, integrator: str, dt: float, nt: int, algo
     """Writes `model` as an SVL run file (tags: mesh nodes 1..n, dashpot partner nodes after
     them; continuum elements 1..ne, dashpots after them; one material per distinct parameter
     set; one TIMESERIES PointLoad per model load with its series in a text file next to the
-    run file). recorders: (file, resp, [mesh node indices], ndps). Returns the node tags."""
+    run file). recorders: (file, resp, [node tags], ndps); mesh node i has tag i + 1, the
+    restrained copy of the j-th dashpot node (order of first appearance) tag nnodes + j + 1.
+    Returns the mesh node tags."""
     nd = model.ndim
     out = {"Global": {"ndim": nd, "massform": "LUMPED" if model.massform_lumped else "CONSISTENT",
                       "update": "RESTART"}}
@@ -278,15 +287,19 @@ def write_run(model: Model, path: str, integrator: str, dt: float, nt: int, algo
             at["th"] = float(model.thickness)
         elems[str(e + 1)] = {"name": ename, "conn": [int(c) + 1 for c in model.conn[e]], "attributes": at}
     ntot = nn * nd
+    partner = {}
     for k in range(model.dash_node.size):
-        tag = nn + k + 1
         n = int(model.dash_node[k])
-        nodes[str(tag)] = {"ndof": nd, "freedof": [-1] * nd, "totaldof": [ntot + k * nd + c for c in range(nd)],
-                           "coords": [float(c) for c in model.coords[n]]}
+        if n not in partner:  # one restrained copy per dashpot node (setLysmerDomain)
+            partner[n] = nn + len(partner) + 1
+            tag = partner[n]
+            nodes[str(tag)] = {"ndof": nd, "freedof": [-1] * nd,
+                               "totaldof": [ntot + (tag - nn - 1) * nd + c for c in range(nd)],
+                               "coords": [float(c) for c in model.coords[n]]}
         mt = material(("visc", float(model.dash_eta[k])), "Viscous1DLinear", {"eta": float(model.dash_eta[k])})
-        elems[str(ne + k + 1)] = {"name": "ZeroLength1D", "conn": [n + 1, tag],
+        elems[str(ne + k + 1)] = {"name": "ZeroLength1D", "conn": [n + 1, partner[n]],
                                   "attributes": {"material": mt, "dir": int(model.dash_dir[k])}}
-    out["Global"]["ntotal"] = ntot + model.dash_node.size * nd
+    out["Global"]["ntotal"] = ntot + len(partner) * nd
     out["Global"]["nfree"] = model.nfree
     out["Nodes"], out["Materials"], out["Elements"] = nodes, mats, elems
     damp = {}
@@ -323,7 +336,7 @@ def write_run(model: Model, path: str, integrator: str, dt: float, nt: int, algo
         "integrator": {"name": integrator, "dt": dt, "mtol": 1e-12, "ktol": 1e-12, "ftol": 1e-12},
         "analysis": {"name": "DYNAMIC", "nt": nt}}}
     out["Recorders"] = {str(i + 1): {"name": "NODE", "file": f, "resp": r, "ndps": int(p), "nsamp": 1,
-                                     "list": [int(n) + 1 for n in lst]}
+                                     "list": [int(n) for n in lst]}
                         for i, (f, r, lst, p) in enumerate(recorders)}
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
@@ -336,7 +349,9 @@ def _fmt(v: float, ndps: int) -> str:
 
 def write_node_recorder(path: str, rec: NodeRecorder, node_tags, totaldof, ntotal: int, nt: int,
                         values: np.ndarray):
-    """values[k-1, i, c]: response of component c of the recorder's node i after step k."""
+    """values[k-1, i, c]: response of component c of the recorder's node i after step k;
+    node_tags / totaldof list every node the recorder may name."""
+    totaldof = np.asarray(totaldof)
     ndof = totaldof.shape[1]
     tags = list(node_tags)
     rows = [tags.index(t) for t in rec.tags]
@@ -369,13 +384,25 @@ def run(svl: SvlRun, out_dir: str | None = None, json_path: str | None = None, c
     its NODE recorders into out_dir (default: SVL's <run dir>/../Solution/<combo>/). Returns
     {file: values}. VEL and ACCEL are formed from the recorded displacement history by the
     integrator's own update formulas (CentralDifference.cpp:141-148, ExtendedNewmarkBeta.cpp:
-    141-150); the NODE recorder is written for every step (SVL writes it on every call)."""
+    141-150); REACTION from the device at the fixed mesh nodes (svlg_reactions) and as -eta V
+    at dashpot partner nodes, 0 elsewhere; the NODE recorder is written for every step (SVL
+    writes it on every call)."""
     from .integrators import CentralDifference, DeviceModel, ExtendedNewmarkBeta, LearnedPreconditioner, NewmarkBeta
     m = svl.model
     nd = m.ndim
     fd = m.free_dof()
-    needed = sorted({t for r in svl.recorders for t in r.tags})
     tags = list(svl.node_tags)
+    mesh = set(tags)
+    needed = {t for r in svl.recorders for t in r.tags if t in mesh}
+    for r in svl.recorders:  # a partner's reaction is -eta V of its mesh node
+        if r.response == "REACTION":
+            for t in r.tags:
+                for (n, _, _) in svl.partners.get(t, (None, []))[1]:
+                    needed.add(tags[n])
+    unknown = [t for r in svl.recorders for t in r.tags if t not in mesh and t not in svl.partners]
+    if unknown:
+        raise ValueError(f"recorder nodes {unknown[:5]} are not in the model")
+    needed = sorted(needed)
     rec_dofs, where = [], {}
     for t in needed:
         for c in range(nd):
@@ -383,6 +410,9 @@ def run(svl: SvlRun, out_dir: str | None = None, json_path: str | None = None, c
             if f >= 0:
                 where[(t, c)] = len(rec_dofs)
                 rec_dofs.append(f)
+    # Reactions are set only at fixed nodes (DynamicAnalysis.cpp:140-150); elsewhere 0.
+    fixed_mesh = sorted({t for r in svl.recorders if r.response == "REACTION" for t in r.tags
+                         if t in mesh and bool(m.restrained[tags.index(t)].any())})
     dm = DeviceModel(m)
     steps = svl.nt - 1
     if svl.integrator == "CENTRALDIFFERENCE":
@@ -399,7 +429,18 @@ def run(svl: SvlRun, out_dir: str | None = None, json_path: str | None = None, c
     integ.Initialize()
     if learned and not isinstance(integ, CentralDifference):
         LearnedPreconditioner(integ, channels=1, layers=8, mfma=-1).use(True)
-    U = integ.advance(1, steps, rec_dofs) if rec_dofs else (integ.advance(1, steps) or np.zeros((steps, 0)))
+    Rm = np.zeros((steps, len(fixed_mesh), nd))
+    if fixed_mesh:
+        # Step by step: the reactions of each step's state.
+        qn = [tags.index(t) for t in fixed_mesh]
+        rows = []
+        for k in range(1, steps + 1):
+            h = integ.advance(k, 1, rec_dofs) if rec_dofs else None
+            rows.append(np.asarray(h).reshape(-1) if h is not None else np.zeros(0))
+            Rm[k - 1] = dm.reactions(qn, k)
+        U = np.asarray(rows).reshape(steps, len(rec_dofs))
+    else:
+        U = integ.advance(1, steps, rec_dofs) if rec_dofs else (integ.advance(1, steps) or np.zeros((steps, 0)))
     dm.close()
     U = np.asarray(U).reshape(steps, len(rec_dofs))
     Up = np.vstack([np.zeros((1, U.shape[1])), U[:-1]])
@@ -422,16 +463,29 @@ def run(svl: SvlRun, out_dir: str | None = None, json_path: str | None = None, c
     if base is None:
         root = os.path.dirname(os.path.abspath(json_path)) if json_path else "."
         base = os.path.join(root, "..", "Solution", svl.combo_name)
+    all_tags = tags + list(svl.partners)
+    all_tot = np.vstack([svl.totaldof] + [np.asarray([v[0]]) for v in svl.partners.values()])
     out = {}
     for r in svl.recorders:
         vals = np.zeros((steps, len(r.tags), nd))
         for i, t in enumerate(r.tags):
+            if r.response == "REACTION":
+                if t in fixed_mesh:
+                    vals[:, i, :] = Rm[:, fixed_mesh.index(t), :]
+                elif t in svl.partners:
+                    for (n, c, eta) in svl.partners[t][1]:
+                        j = where.get((tags[n], c))
+                        if j is not None:
+                            vals[:, i, c] += -eta * V[:, j]
+                continue
+            if t not in mesh:
+                continue  # restrained partner node: its motion is zero
             for c in range(nd):
                 j = where.get((t, c))
                 if j is not None:
                     vals[:, i, c] = resp[r.response][:, j]
         p = os.path.join(base, r.file)
-        write_node_recorder(p, r, svl.node_tags, svl.totaldof, svl.ntotal, svl.nt, vals)
+        write_node_recorder(p, r, all_tags, all_tot, svl.ntotal, svl.nt, vals)
         out[p] = vals
     return out
 

@@ -5,7 +5,8 @@ algorithm the file names, and its NODE recorder files (DISP, VEL, ACCEL) are par
 compared with the oracle's histories of the original model: relative L2 <= 1e-10 for DISP
 (CG at 1e-13 on the implicit paths), <= 1e-8 for VEL / ACCEL (formed from the displacement
 history by the integrator's update formulas, so differences of nearly equal displacements
-enter), restrained components exactly 0.
+enter), <= 1e-9 for REACTION at fixed mesh nodes and dashpot partner nodes, restrained
+components of the motion exactly 0.
 """
 import numpy as np
 import pytest
@@ -41,7 +42,18 @@ def case(kind):
 def test_run_file_through_device(tmp_path, kind):
     m, dt, nt, integ, alg, ointeg, okw = case(kind)
     nodes = [0, m.nnodes // 3, m.nnodes - 1, int(m.load_node[0])]
-    recs = [("U.out", "DISP", nodes, 12), ("V.out", "VEL", nodes, 12), ("A.out", "ACCEL", nodes, 12)]
+    ntags = [n + 1 for n in nodes]
+    recs = [("U.out", "DISP", ntags, 12), ("V.out", "VEL", ntags, 12), ("A.out", "ACCEL", ntags, 12)]
+    # Reactions: fixed mesh nodes, and the partner nodes of the first dashpot nodes.
+    fixed_nodes = list(np.nonzero(m.restrained.any(1))[0][:6])
+    order = []
+    for n in m.dash_node:
+        if int(n) not in order:
+            order.append(int(n))
+    partner_idx = list(range(min(4, len(order))))
+    rtags = [int(n) + 1 for n in fixed_nodes] + [m.nnodes + j + 1 for j in partner_idx]
+    if rtags:
+        recs.append(("R.out", "REACTION", rtags, 12))
     p = str(tmp_path / "run" / "model.json")
     import os
     os.makedirs(os.path.dirname(p))
@@ -52,10 +64,18 @@ def test_run_file_through_device(tmp_path, kind):
     fd = m.free_dof()
     cols = [int(fd[n][c]) for n in nodes for c in range(m.ndim)]
     free_cols = [f for f in cols if f >= 0]
-    ref = oracle.run(m, ointeg, dt, nt, free_cols, want_va=True, **okw)
+    reac_tot = ([int(n) * m.ndim + c for n in fixed_nodes for c in range(m.ndim)]
+                + [(m.nnodes + j) * m.ndim + c for j in partner_idx for c in range(m.ndim)])
+    ref = oracle.run(m, ointeg, dt, nt, free_cols, want_va=True, reac_tot=reac_tot, **okw)
+    if rtags:
+        tags, tot, ntotal, ntr, vals = S.read_node_recorder(os.path.join(out_dir, "R.out"))
+        assert tags == rtags and vals.shape == (nt - 1, len(reac_tot))
+        err = rel(vals, ref["R"])
+        print(f"{kind} R.out: rel L2 {err:.2e}")
+        assert err < 1e-9
     for fname, key, tol in (("U.out", "U", 1e-10), ("V.out", "V", 1e-8), ("A.out", "A", 1e-8)):
         tags, tot, ntotal, ntr, vals = S.read_node_recorder(os.path.join(out_dir, fname))
-        assert tags == [n + 1 for n in nodes] and ntr == nt and vals.shape == (nt - 1, len(cols))
+        assert tags == ntags and ntr == nt and vals.shape == (nt - 1, len(cols))
         restrained = [i for i, f in enumerate(cols) if f < 0]
         assert np.all(vals[:, restrained] == 0.0)
         free = [i for i, f in enumerate(cols) if f >= 0]

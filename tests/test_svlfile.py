@@ -37,7 +37,7 @@ def test_write_read_roundtrip(tmp_path, case):
     m, dt, nt, integ, alg = models()[case]
     p = str(tmp_path / "model.json")
     S.write_run(m, p, integ, dt, nt, alg, cnvgtol=1e-7, nmax=17, cnvgtest=5,
-                recorders=[("U.out", "DISP", [0, 3], 6)])
+                recorders=[("U.out", "DISP", [1, 4], 6)])
     r = S.read_run(p)
     q = r.model
     assert (r.integrator, r.algorithm, r.dt, r.nt, r.nmax, r.cnvgtest, r.cnvgtol) == (integ, alg, dt, nt, 17, 5, 1e-7)
