@@ -275,7 +275,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--ne", type=str, default="100,100,50", help="hexa elements nx,ny,nz")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-nodes", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
