@@ -87,10 +87,10 @@ def copy_calibration(nbytes: int, reps: int = 20):
             "copy_GBs": 2 * nbytes * reps / (ms / 1e3) / 1e9}
 
 
-def traffic_from_profile(variant: int, tile_nodes: int):
-    """HBM bytes per launch of the fused step kernel from the committed rocprofv3 PMC
-    summary (profiles/halfspace3d_fused_step.md), if it covers this configuration."""
-    path = os.path.join(ROOT, "profiles", "halfspace3d_fused_step.md")
+def traffic_from_profile(variant: int, tile_nodes: int, name: str = "halfspace3d_fused_step.md"):
+    """HBM bytes per launch of the fused step kernel from a committed rocprofv3 PMC summary
+    under profiles/, if it covers this configuration."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     vals = {}
@@ -99,7 +99,7 @@ def traffic_from_profile(variant: int, tile_nodes: int):
             k, _, v = line.partition(":")
             vals[k.strip()] = v.strip()
     try:
-        if int(vals["traffic_variant"]) == variant and int(vals["traffic_tile_nodes"]) == tile_nodes:
+        if int(vals.get("traffic_variant", variant)) == variant and int(vals["traffic_tile_nodes"]) == tile_nodes:
             return float(vals["traffic_bytes_per_launch"])
     except (KeyError, ValueError):
         return None
@@ -261,6 +261,7 @@ def batched_bench(args):
            "dof_steps_per_s": u.nfree * capi_steps / wall, "us_per_step": step_s * 1e6,
            "graph_steps": g, "tile_nodes": best, "autotune_us_per_step": times, "kernel_variant": info["variant"],
            "alg_bytes_per_step": alg, "achieved_GBs": alg / step_s / 1e9,
+           "traffic": traffic_from_profile(info["variant"], best, "batch2d_fused_step.md"),
            "roofline_frac": alg / step_s / 1e9 / HBM_PEAK_GBS, "build_s": build_s, "autotune_s": tune_s}
     print(f"# batched: {len(models)} realizations, {u.nfree} DOF, {step_s * 1e6:.1f} us/step, "
           f"{out['dof_steps_per_s']:.3e} DOF-steps/s, {out['achieved_GBs']:.0f} GB/s, tile {best}",
