@@ -70,6 +70,7 @@ struct OpArgs {
     const int32_t *plastic;  // [nelem] elastoplastic tangent somewhere in the element
     const double *Vm, *dUn;
     const int *jdev;       // non-null: the iteration is *jdev + iter (graph replays)
+    int nelem;             // elements (the Gauss-point state stride)
 };
 
 // One node tile of the effective-stiffness operator (one thread per tile element in the
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
                         u[c][d] = RHS ? sV[lc[c] * DIM + d] : sU[lc[c] * DIM + d];
                     }
                 const double cR = RHS ? c2 : a.coef[e0 + tid] - 1.0;
-                j2_elem_force<DIM>(x, u, jp, a.state + size_t(tag) * Elem<DIM>::ngp * kJ2State, cR, !RHS, a.th, f);
+                j2_elem_force<DIM>(x, u, jp, a.state, size_t(a.nelem), tag, cR, !RHS, a.th, f);
             }
         }
         if (!j2) {
@@ -481,6 +482,7 @@ static OpArgs op_args(const Model &m) {
     a.j2p = m.j2p.p;
     a.state = m.gp_state.p;
     a.plastic = m.j2_plastic.p;
+    a.nelem = m.nelem;
     return a;
 }
 

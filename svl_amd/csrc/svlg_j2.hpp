@@ -2,9 +2,11 @@
 // PlasticPlaneStrainJ2 (RP/02-Materials/02-NonLinear/Plastic3DJ2.cpp:206-258,
 // PlasticPlaneStrainJ2.cpp:227-279).
 //
-// State of a Gauss point, kJ2State doubles (tensor components [11, 22, 33, 12, 23, 31]):
+// State of a Gauss point, kJ2State fields (tensor components [11, 22, 33, 12, 23, 31]):
 //   [0,6) plastic strain, [6,12) back stress, [12] alpha, [13,19) stress,
 //   [19,25) unit normal n of the last plastic update, [25] a1, [26] a2,
+// stored field-major: field f of Gauss point g of element e at ((f * ngp + g) * nelem + e), so
+// the threads of neighbouring elements read neighbouring addresses (J2Ref below).
 // where the consistent tangent of the last update is
 //   C eps = K tr(eps) (1,1,1,0,0,0) + a1 Id eps - a2 (n . eps) n,
 //   Id eps = (eps_i - tr/3 for i < 3, eps_i / 2 for shears), eps engineering,
@@ -26,16 +28,37 @@ struct J2Par {
     double K, G, sy, H, beta;
 };
 
-__device__ __host__ inline void j2_init_state(const J2Par &p, double *st) {
+// The kJ2State fields of one Gauss point: st[f] = p[f * stride].
+template <typename T>
+struct J2RefT {
+    T *p;
+    size_t stride;
+    __device__ __host__ T &operator[](int f) const { return p[size_t(f) * stride]; }
+};
+using J2Ref = J2RefT<double>;
+using J2CRef = J2RefT<const double>;
+
+__device__ __host__ inline J2Ref j2_at(double *state, size_t nelem, int ngp, int e, int g) {
+    return J2Ref{state + size_t(g) * nelem + e, size_t(ngp) * nelem};
+}
+__device__ __host__ inline J2CRef j2_at(const double *state, size_t nelem, int ngp, int e, int g) {
+    return J2CRef{state + size_t(g) * nelem + e, size_t(ngp) * nelem};
+}
+
+__device__ __host__ inline void j2_init_state(const J2Par &p, J2Ref st) {
     for (int i = 0; i < kJ2State; ++i) st[i] = 0.0;
     st[25] = 2.0 * p.G;
 }
 
 // One UpdateState(strain, 1) with the tensor strain et.
-__device__ inline void j2_update_gp(const J2Par &p, const double (&et)[6], double *st) {
-    double *ep = st, *q = st + 6, *sig = st + 13, *nn = st + 19;
+__device__ inline void j2_update_gp(const J2Par &p, const double (&et)[6], J2Ref st) {
     const double tr = et[0] + et[1] + et[2];
-    double strial[6], xi[6];
+    double ep[6], q[6], strial[6], xi[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        ep[i] = st[i];
+        q[i] = st[6 + i];
+    }
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
         const double one = i < 3 ? 1.0 : 0.0;
@@ -50,8 +73,8 @@ __device__ inline void j2_update_gp(const J2Par &p, const double (&et)[6], doubl
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
             const double one = i < 3 ? 1.0 : 0.0;
-            sig[i] = p.K * tr * one + strial[i];
-            nn[i] = 0.0;
+            st[13 + i] = p.K * tr * one + strial[i];
+            st[19 + i] = 0.0;
         }
         st[25] = 2.0 * p.G;
         st[26] = 0.0;
@@ -63,10 +86,10 @@ __device__ inline void j2_update_gp(const J2Par &p, const double (&et)[6], doubl
     for (int i = 0; i < 6; ++i) {
         const double one = i < 3 ? 1.0 : 0.0;
         const double n = xi[i] / nx;
-        q[i] += 2.0 / 3.0 * (1.0 - p.beta) * p.H * dg * n;
-        ep[i] += dg * n;
-        sig[i] = p.K * tr * one + strial[i] - 2.0 * p.G * dg * n;
-        nn[i] = n;
+        st[6 + i] = q[i] + 2.0 / 3.0 * (1.0 - p.beta) * p.H * dg * n;
+        st[i] = ep[i] + dg * n;
+        st[13 + i] = p.K * tr * one + strial[i] - 2.0 * p.G * dg * n;
+        st[19 + i] = n;
     }
     const double c1 = 1.0 / (1.0 + p.H / p.G / 3.0);
     const double c2 = 4.0 * p.G * p.G * dg / nx;
@@ -75,9 +98,11 @@ __device__ inline void j2_update_gp(const J2Par &p, const double (&et)[6], doubl
 }
 
 // sigma' = C_t eps for an engineering strain eps (6 components) with the tangent of st.
-__device__ inline void j2_tangent_apply(const J2Par &p, const double *st, const double (&e)[6], double (&s)[6]) {
+__device__ inline void j2_tangent_apply(const J2Par &p, J2CRef st, const double (&e)[6], double (&s)[6]) {
     const double tr = e[0] + e[1] + e[2];
-    const double *n = st + 19;
+    double n[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) n[j] = st[19 + j];
     double ne = 0.0;
 #pragma unroll
     for (int j = 0; j < 6; ++j) ne += n[j] * e[j];
@@ -201,8 +226,8 @@ __device__ __forceinline__ void gp_btsig(const double (&dn)[Elem<DIM>::nen][DIM]
 // Rayleigh operand (V - 2/dt dU for RHS).
 template <int DIM>
 __device__ inline void j2_elem_force(const double (&x)[Elem<DIM>::nen][DIM], const double (&u)[Elem<DIM>::nen][DIM],
-                                     const J2Par &p, const double *st, double c, bool tangent, double th,
-                                     double (&f)[Elem<DIM>::nen][DIM]) {
+                                     const J2Par &p, const double *state, size_t nelem, int e, double c,
+                                     bool tangent, double th, double (&f)[Elem<DIM>::nen][DIM]) {
     constexpr int NEN = Elem<DIM>::nen, NV = DIM == 2 ? 3 : 6, NGP = Elem<DIM>::ngp;
 #pragma unroll
     for (int a = 0; a < NEN; ++a)
@@ -216,7 +241,7 @@ __device__ inline void j2_elem_force(const double (&x)[Elem<DIM>::nen][DIM], con
         gp_strain<DIM>(dn, u, ev);
         j2_embed<DIM>(ev, et, e6);
         j2_initial_apply(p, e6, i6);
-        const double *sg = st + size_t(g) * kJ2State;
+        const J2CRef sg = j2_at(state, nelem, NGP, e, g);
         if (tangent) {
             j2_tangent_apply(p, sg, e6, s6);
 #pragma unroll

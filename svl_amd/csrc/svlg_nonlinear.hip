@@ -39,46 +39,64 @@ __global__ void k_j2_init(int ne, const int32_t *__restrict__ mat, const double 
     if (mat[e] != SVLG_MAT_J2) return;
     const double *q = j2p + size_t(e) * 5;
     const J2Par p{q[0], q[1], q[2], q[3], q[4]};
-    for (int g = 0; g < Elem<DIM>::ngp; ++g) j2_init_state(p, state + (size_t(e) * Elem<DIM>::ngp + g) * kJ2State);
+    for (int g = 0; g < Elem<DIM>::ngp; ++g) j2_init_state(p, j2_at(state, size_t(ne), Elem<DIM>::ngp, e, g));
 }
 
 // lin*::UpdateState at U + dU for every J2 element: strain B (U + dU) at each Gauss point,
-// then the material update (element-parallel: states are per element). plastic[e] = 1 when
-// the update left some Gauss point with the elastoplastic tangent, 0 when every point holds
-// the elastic one (the tangent operator then takes the element's elastic path).
+// then the material update. One thread per (element, Gauss point): a workgroup holds
+// kJ2Block / ngp elements, thread t works on Gauss point t / epb of element t % epb, so the
+// field-major states are read and written contiguously. plastic[e] = 1 when the update left
+// some Gauss point of e with the elastoplastic tangent, 0 when every point holds the elastic
+// one (the tangent operator then takes the element's elastic path).
+constexpr int kJ2Block = 256;
+
 template <int DIM>
-__global__ void k_j2_update(int ne, const int32_t *__restrict__ conn, const double *__restrict__ X,
-                            const double *__restrict__ U, const double *__restrict__ dU,
-                            const int32_t *__restrict__ mat, const double *__restrict__ j2p, double th,
-                            double *__restrict__ state, int32_t *__restrict__ plastic) {
-    constexpr int NEN = Elem<DIM>::nen, NV = DIM == 2 ? 3 : 6;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ne || mat[e] != SVLG_MAT_J2) return;
-    const double *q = j2p + size_t(e) * 5;
-    const J2Par p{q[0], q[1], q[2], q[3], q[4]};
-    double x[NEN][DIM], u[NEN][DIM];
+__global__ __launch_bounds__(kJ2Block) void k_j2_update(int ne, const int32_t *__restrict__ conn,
+                                                        const double *__restrict__ X, const double *__restrict__ U,
+                                                        const double *__restrict__ dU,
+                                                        const int32_t *__restrict__ mat,
+                                                        const double *__restrict__ j2p, double th,
+                                                        double *__restrict__ state, int32_t *__restrict__ plastic) {
+    constexpr int NEN = Elem<DIM>::nen, NV = DIM == 2 ? 3 : 6, NGP = Elem<DIM>::ngp;
+    constexpr int EPB = kJ2Block / NGP;
+    __shared__ int flag[EPB];
+    const int el = threadIdx.x % EPB, g = threadIdx.x / EPB;
+    const int e = blockIdx.x * EPB + el;
+    if (threadIdx.x < EPB) flag[threadIdx.x] = 0;
+    __syncthreads();
+    const bool j2 = e < ne && mat[e] == SVLG_MAT_J2;
+    if (j2) {
+        const double *q = j2p + size_t(e) * 5;
+        const J2Par p{q[0], q[1], q[2], q[3], q[4]};
+        double x[NEN][DIM], u[NEN][DIM];
 #pragma unroll
-    for (int a = 0; a < NEN; ++a) {
-        const size_t nd = size_t(conn[size_t(e) * NEN + a]) * DIM;
+        for (int a = 0; a < NEN; ++a) {
+            const size_t nd = size_t(conn[size_t(e) * NEN + a]) * DIM;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            x[a][d] = X[nd + d];
-            u[a][d] = U[nd + d] + dU[nd + d];
+            for (int d = 0; d < DIM; ++d) {
+                x[a][d] = X[nd + d];
+                u[a][d] = U[nd + d] + dU[nd + d];
+            }
         }
-    }
-    int any = 0;
-#pragma unroll 1
-    for (int g = 0; g < Elem<DIM>::ngp; ++g) {
         double dn[NEN][DIM];
         (void)gp_dn<DIM>(x, g, th, dn);
         double ev[NV], et[6], e6[6];
         gp_strain<DIM>(dn, u, ev);
         j2_embed<DIM>(ev, et, e6);
-        double *st = state + (size_t(e) * Elem<DIM>::ngp + g) * kJ2State;
+        const J2Ref st = j2_at(state, size_t(ne), NGP, e, g);
         j2_update_gp(p, et, st);
-        any |= st[26] != 0.0;
+        if (st[26] != 0.0) atomicOr(&flag[el], 1);
     }
-    plastic[e] = any;
+    __syncthreads();
+    if (g == 0 && j2) plastic[e] = flag[el];
+}
+
+// The six stress fields of every Gauss point of element e into out[ngp][6].
+__global__ void k_gp_stress(const double *__restrict__ state, size_t nelem, int ngp, int e, double *__restrict__ out) {
+    const int t = threadIdx.x;
+    if (t >= ngp * 6) return;
+    const int g = t / 6, i = t % 6;
+    out[t] = j2_at(state, nelem, ngp, e, g)[13 + i];
 }
 
 // un = U + dU, w = V - 2/dt dU.
@@ -237,7 +255,8 @@ int svlg_nl_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
         SVLG_HIP(rec.upload(rn.data(), rn.size(), s));
         SVLG_HIP(hbuf.alloc(size_t(nrec) * nsteps));
     }
-    const int nb = (n + 255) / 256, neb = (m.nelem + 127) / 128;
+    const int epb = kJ2Block / (m.ndim == 3 ? 8 : 4);  // elements per k_j2_update workgroup
+    const int nb = (n + 255) / 256, neb = (m.nelem + epb - 1) / epb;
     std::vector<double> hp(4 * kNlGrid);
     for (int st = 0; st < nsteps; ++st) {
         const int k = k0 + st;
@@ -266,11 +285,11 @@ int svlg_nl_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
             // UpdateStatesIncrements(U + dU).
             if (m.has_j2) {
                 if (m.ndim == 3)
-                    hipLaunchKernelGGL(k_j2_update<3>, dim3(neb), dim3(128), 0, s, m.nelem, m.conn.p, m.X.p, m.nU.p,
-                                       m.nl_dUa.p, m.mat.p, m.j2p.p, m.th, m.gp_state.p, m.j2_plastic.p);
+                    hipLaunchKernelGGL(k_j2_update<3>, dim3(neb), dim3(kJ2Block), 0, s, m.nelem, m.conn.p, m.X.p,
+                                       m.nU.p, m.nl_dUa.p, m.mat.p, m.j2p.p, m.th, m.gp_state.p, m.j2_plastic.p);
                 else
-                    hipLaunchKernelGGL(k_j2_update<2>, dim3(neb), dim3(128), 0, s, m.nelem, m.conn.p, m.X.p, m.nU.p,
-                                       m.nl_dUa.p, m.mat.p, m.j2p.p, m.th, m.gp_state.p, m.j2_plastic.p);
+                    hipLaunchKernelGGL(k_j2_update<2>, dim3(neb), dim3(kJ2Block), 0, s, m.nelem, m.conn.p, m.X.p,
+                                       m.nU.p, m.nl_dUa.p, m.mat.p, m.j2p.p, m.th, m.gp_state.p, m.j2_plastic.p);
             }
             ++it;
         } while (res > m.nl_tol && it < m.nl_nmax);
@@ -296,14 +315,15 @@ int svlg_gp_stress(svlg_model *h, int32_t elem, double *stress) {
     SVLG_CHECK(m.nl_ready && m.has_j2, "no Gauss-point states (svlg_nl_init on a model with J2 elements)");
     SVLG_CHECK(elem >= 0 && elem < m.nelem, "element index out of range");
     const int ngp = m.ndim == 3 ? 8 : 4, nv = m.ndim == 3 ? 6 : 3;
-    std::vector<double> st(size_t(ngp) * kJ2State);
-    SVLG_HIP(hipMemcpyAsync(st.data(), m.gp_state.p + size_t(elem) * ngp * kJ2State, sizeof(double) * st.size(),
-                            hipMemcpyDeviceToHost, m.stream));
+    DevBuf<double> buf;
+    SVLG_HIP(buf.alloc(size_t(ngp) * 6));
+    hipLaunchKernelGGL(k_gp_stress, dim3(1), dim3(64), 0, m.stream, m.gp_state.p, size_t(m.nelem), ngp, elem, buf.p);
+    std::vector<double> st(size_t(ngp) * 6);
+    SVLG_HIP(hipMemcpyAsync(st.data(), buf.p, sizeof(double) * st.size(), hipMemcpyDeviceToHost, m.stream));
     SVLG_HIP(hipStreamSynchronize(m.stream));
     const int idx2[3] = {1, 2, 4};
     for (int g = 0; g < ngp; ++g)
-        for (int i = 0; i < nv; ++i)
-            stress[g * nv + i] = st[size_t(g) * kJ2State + 13 + (m.ndim == 3 ? i : idx2[i])];
+        for (int i = 0; i < nv; ++i) stress[g * nv + i] = st[size_t(g) * 6 + (m.ndim == 3 ? i : idx2[i])];
     return 0;
 }
 

@@ -34,6 +34,7 @@ struct ReacArgs {
     double th;
     const int32_t *mat;
     const double *j2p, *state;  // null state: J2 elements as their initial elastic material
+    int nelem;
     double *R;                  // [nq * DIM]
 };
 
@@ -84,7 +85,7 @@ __global__ void k_reactions(ReacArgs a) {
         if (a.state && a.mat[e] == SVLG_MAT_J2) {
             const double *q5 = a.j2p + size_t(e) * 5;
             const J2Par jp{q5[0], q5[1], q5[2], q5[3], q5[4]};
-            j2_elem_force<DIM>(x, v, jp, a.state + size_t(e) * Elem<DIM>::ngp * kJ2State, ak, false, a.th, f);
+            j2_elem_force<DIM>(x, v, jp, a.state, size_t(a.nelem), e, ak, false, a.th, f);
         } else if constexpr (DIM == 3) {
             hexa_fint(x, u, a.lam[e], a.mu[e], f);
         } else {
@@ -167,6 +168,7 @@ int svlg_reactions(svlg_model *h, int32_t n, const int32_t *nodes, int32_t k, do
     a.mat = m.mat.p;
     a.j2p = m.j2p.p;
     a.state = (m.active == 2 && m.has_j2 && m.nl_ready) ? m.gp_state.p : nullptr;
+    a.nelem = m.nelem;
     a.R = rb.p;
     const int nb = (n + 63) / 64;
     if (m.ndim == 3)
