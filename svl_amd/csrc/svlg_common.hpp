@@ -187,6 +187,7 @@ struct Model {
     DevBuf<double> gcn_s;             // D^-1/2 (0 on restrained DOFs)
     DevBuf<float> gcn_sf;             // the same in FP32
     DevBuf<double> gcn_elam, gcn_emu; // per tile element (lambda, mu) * (1 + 2 ak/dt)
+    DevBuf<float> gcn_elamf, gcn_emuf, gcn_dgnf;  // FP32 copies read by the one-channel layer kernel
     DevBuf<float> gcn_kmat;           // [2][24][24] K_lambda, K_mu of the single geometry class (MFMA path)
     DevBuf<uint4> gcn_kimg;           // [3][64] per-lane BF16 A operands of the MFMA element step
     DevBuf<float> gcn_x0;             // s * r

@@ -77,6 +77,7 @@ struct GcnArgs {
     CgParts cg;
     int iter, normalize;
     const int *jdev;           // non-null: the iteration is *jdev + iter (graph replays)
+    const float *elamf, *emuf, *dgnf;  // FP32 copies of elam, emu, dgn (one-channel kernel)
 };
 
 __device__ __forceinline__ float mixw(const float *M, int C, int c, int k, int trans) {
@@ -283,10 +284,11 @@ __global__ __launch_bounds__(512) void k_gcn_fast(GcnArgs a) {
     const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
     const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
     const int MS = a.max_stage, MO = a.max_own;
-    // LDS: corner slots [24][MO] + one trash row, staged operand [MS * 3] + one zero cell.
+    // LDS: corner slots [24][MO] + one trash cell (every masked-out write lands there),
+    // staged operand [MS * 3] + one zero cell.
     float *sIn = reinterpret_cast<float *>(smem);
     const int trash = 24 * MO;
-    float *sX = sIn + size_t(25) * MO;
+    float *sX = sIn + size_t(24) * MO + 1;
     const int zcell = 3 * MS;
     const int tid = threadIdx.x, B = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6, nw = B >> 6, n16 = lane & 15, g = lane >> 4;
@@ -304,8 +306,8 @@ __global__ __launch_bounds__(512) void k_gcn_fast(GcnArgs a) {
     for (int r = 0; r < MAXR; ++r) {
         const int el = min(wave * 16 + r * nw * 16 + n16, max(ne - 1, 0));
         pk[r] = *reinterpret_cast<const uint4 *>(a.lconn + size_t(e0 + el) * 8);
-        lamr[r] = float(a.elam[e0 + el]);
-        mur[r] = float(a.emu[e0 + el]);
+        lamr[r] = a.elamf[e0 + el];
+        mur[r] = a.emuf[e0 + el];
     }
     const size_t g0 = size_t(o0) * 3;
     for (int i = tid; i < nown * 3; i += B) sX[i] = a.sf[g0 + i] * a.in[g0 + i];
@@ -370,7 +372,7 @@ __global__ __launch_bounds__(512) void k_gcn_fast(GcnArgs a) {
             const size_t gd = g0 + size_t(tid) * 3 + d;
             const float sg = a.sf[gd];
             const float xh = sX[tid * 3 + d];
-            const float y = sg * (fi + float(a.dgn[gd]) * xh);
+            const float y = sg * (fi + a.dgnf[gd] * xh);
             const float o = am * a.in[gd] + bm * y + cm * a.x0[gd];
             if constexpr (OUT) {
                 const double zv = sig * a.s[gd] * double(o * a.w[0]);
@@ -576,8 +578,10 @@ static size_t gcn_lds(const Model &m) {
     return b;
 }
 
+// Corner slots [24][max_own] + one trash cell, staged operand [max_stage * 3] + one zero cell
+// (below 40 KB for the default tiles, so four workgroups fit a CU).
 static size_t gcn_fast_lds(const Model &m) {
-    return size_t(25) * m.tiles.max_own * sizeof(float) + (size_t(m.tiles.max_stage) * 3 + 1) * sizeof(float);
+    return (size_t(24) * m.tiles.max_own + 1 + size_t(m.tiles.max_stage) * 3 + 1) * sizeof(float);
 }
 
 template <int DIM, int PATH, bool MFMA>
@@ -619,6 +623,9 @@ static GcnArgs gcn_args(const Model &m) {
     a.egeo = t.elem_geo.p;
     a.elam = m.gcn_elam.p;
     a.emu = m.gcn_emu.p;
+    a.elamf = m.gcn_elamf.p;
+    a.emuf = m.gcn_emuf.p;
+    a.dgnf = m.gcn_dgnf.p;
     a.geo = m.geo.p;
     a.X = m.X.p;
     a.th = m.th;
@@ -955,6 +962,15 @@ int svlg_gnn_init(svlg_model *h, int32_t channels, int32_t layers, int32_t mfma,
         }
         SVLG_HIP(m.gcn_elam.upload(el.data(), ne, s));
         SVLG_HIP(m.gcn_emu.upload(em.data(), ne, s));
+        std::vector<float> elf(el.begin(), el.end()), emf(em.begin(), em.end());
+        SVLG_HIP(m.gcn_elamf.upload(elf.data(), ne, s));
+        SVLG_HIP(m.gcn_emuf.upload(emf.data(), ne, s));
+        std::vector<double> dg(n);
+        SVLG_HIP(hipMemcpyAsync(dg.data(), m.dgn.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        SVLG_HIP(hipStreamSynchronize(s));
+        std::vector<float> dgf(dg.begin(), dg.end());
+        SVLG_HIP(m.gcn_dgnf.upload(dgf.data(), size_t(n), s));
+        SVLG_HIP(hipStreamSynchronize(s));  // the host vectors above end with this block
     }
     if (m.gcn_mfma) {
         if (int rc = build_hexa_kmat(m, m.gcn_kmat)) return rc;
