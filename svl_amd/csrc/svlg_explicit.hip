@@ -463,15 +463,18 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
                     quad_fint(x, u, lam, mu, a.th, f);
             }
         }
+        // Branch-free scatter: corners of nodes this tile does not own write to the trash
+        // cell after the slots.
+        const int trash = NEN * DIM * MO;
 #pragma unroll
-        for (int c = 0; c < NEN; ++c)
-            if (lc[c] < nown) {
+        for (int c = 0; c < NEN; ++c) {
+            const bool own = lc[c] < nown;
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) {
-                    const double v = f[c][d];
-                    sIn[(c * DIM + d) * MO + lc[c]] = fabs(v) > a.ftol ? v : 0.0;
-                }
+            for (int d = 0; d < DIM; ++d) {
+                const double v = f[c][d];
+                sIn[own ? (c * DIM + d) * MO + lc[c] : trash] = fabs(v) > a.ftol ? v : 0.0;
             }
+        }
     }
     __syncthreads();
     if (!has_node) return;

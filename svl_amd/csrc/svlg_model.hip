@@ -482,8 +482,8 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         // <= 512 threads covers every tile element and owned node; colour phases otherwise.
         const int need = std::max(ht.max_elem, ht.max_own);
         const int cblock = std::max(64, (need + 63) / 64 * 64);
-        const size_t corner_lds =
-            (size_t(staged_arrays * ht.max_stage) * ndim + size_t(nen) * ndim * ht.max_own) * sizeof(double);
+        const size_t corner_lds =  // + one trash cell for the branch-free corner scatter
+            (size_t(staged_arrays * ht.max_stage) * ndim + size_t(nen) * ndim * ht.max_own + 1) * sizeof(double);
         const bool corner_ok = corner_unique && cblock <= 512 && corner_lds <= 160 * 1024;
         if (req_variant == 0) m.variant = corner_ok ? 3 : 1;
         else m.variant = req_variant;
