@@ -270,6 +270,48 @@ def batched_bench(args):
     return out
 
 
+def drm_bench(args, us_per_step_plain):
+    """The headline half-space loaded through a domain reduction layer (synthetic SV
+    plane-wave signals) instead of the point source: per step one DRM element-force launch
+    (two masked Gauss-point element forces per DRM element) before the fused step, which adds
+    the DRM forces at the layer's nodes. Reported beside the plain step time of the same mesh."""
+    from svl_amd import workloads as W
+    from svl_amd.integrators import CentralDifference, DeviceModel
+    nx, ny, nz = (int(v) for v in args.ne.split(","))
+    t0 = time.perf_counter()
+    m, dt, nt = W.halfspace3d_drm(nx, ny, nz, h=2.0, nt=args.drm_steps + args.drm_warmup + 2)
+    build_s = time.perf_counter() - t0
+    dm = DeviceModel(m, tile_nodes=args.tile_nodes, block=args.block, variant=args.variant)
+    cd = CentralDifference(dm, dt)
+    cd.Initialize()
+    ev = HipEvents()
+    stream = dm.stream()
+    cd.advance_async(1, args.drm_warmup)
+    dm.sync()
+    e0, e1 = ev.event(), ev.event()
+    t = time.perf_counter()
+    ev.record(e0, stream)
+    cd.advance_async(1 + args.drm_warmup, args.drm_steps)
+    ev.record(e1, stream)
+    dm.sync()
+    wall = time.perf_counter() - t
+    step_s = ev.elapsed_ms(e0, e1) / 1e3 / args.drm_steps
+    # Per step the DRM launch reads the signal rows (72 B per DRM node), the element data of
+    # the DRM elements (8 corners x (4 B connectivity + 24 B coordinates), lambda, mu, ak) and
+    # writes 8 x 3 doubles per element that the fused step reads back.
+    drm_bytes = 72 * m.drm_node.size + (8 * 28 + 24) * m.drm_elem.size + 2 * 192 * m.drm_elem.size
+    out = {"workload": f"halfspace3d lin3DHexa8 {nx}x{ny}x{nz} h=2m, Lysmer, DRM box 0.8 L (open at the surface), "
+                       f"synthetic SV Ricker signals, CentralDifference",
+           "ndrm_elem": int(m.drm_elem.size), "ndrm_node": int(m.drm_node.size), "nfree": m.nfree,
+           "steps": args.drm_steps, "us_per_step": step_s * 1e6, "us_per_step_point_source": us_per_step_plain,
+           "drm_overhead_us": step_s * 1e6 - us_per_step_plain, "dof_steps_per_s": m.nfree * args.drm_steps / wall,
+           "drm_alg_bytes_per_step": drm_bytes, "build_s": build_s}
+    print(f"# drm: {m.drm_elem.size} DRM elements, {step_s * 1e6:.2f} us/step "
+          f"(point source {us_per_step_plain:.2f})", file=sys.stderr, flush=True)
+    dm.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -299,6 +341,9 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=30)
     ap.add_argument("--batch-graph-steps", type=int, default=30)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--drm-steps", type=int, default=300)
+    ap.add_argument("--drm-warmup", type=int, default=30)
+    ap.add_argument("--no-drm", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="only run the device copy of MB megabytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -372,6 +417,8 @@ def main():
         out["nonlinear"] = nonlinear_bench(args)
     if not args.no_batch:
         out["batched"] = batched_bench(args)
+    if not args.no_drm:
+        out["drm"] = drm_bench(args, step_s * 1e6)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, dt, args.cpu_steps)
         out["cpu_baseline"]["cores"] = 1

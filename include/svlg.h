@@ -66,6 +66,17 @@ typedef struct svlg_mesh_desc {
     const double *load_dir;   /* [nload*ndim] direction vector */
     const double *load_factor;/* [nload] combination factor */
     const double *load_series;/* [nload*load_nt] amplitude time series */
+    /* Domain reduction (ElementLoad GENERALWAVE: ELEMENTLOAD_DOMAIN_REDUCTION, Assembler.cpp:
+     * 460-478; lin*::ComputeDomainReductionForces, lin2DQuad4.cpp:564-614; node signals as
+     * Driver.hpp:1669-1720 reads them). ndrm_elem = 0: none. */
+    int32_t ndrm_elem;
+    const int32_t *drm_elem;  /* [ndrm_elem] continuum element index */
+    int32_t ndrm_node;
+    const int32_t *drm_node;  /* [ndrm_node] mesh node; every node of a DRM element is one */
+    const int32_t *drm_cond;  /* [ndrm_node] 1 = exterior node (its signal enters negated) */
+    int32_t drm_nt;           /* rows of every node signal (row k at step k) */
+    const double *drm_signal; /* [ndrm_node*drm_nt*3*ndim] rows (U, V, A components) */
+    double drm_factor;        /* combination factor of the DRM load */
 } svlg_mesh_desc;
 
 typedef struct svlg_model svlg_model;

@@ -28,6 +28,10 @@
 //     ExtendedNewmarkBeta.cpp:123-154, 208-245; NewmarkBeta.cpp:115-144, 198-235);
 //   * Linear algorithm and state scatter (RP/09-Algorithms/01-Linear/Linear.cpp:22-56,
 //     RP/09-Algorithms/Algorithm.cpp:18-56);
+//   * Domain reduction load: masked element M, C, K times the node motion rows, exterior
+//     signals negated on input (RP/04-Elements/06-Quadrilateral/lin2DQuad4.cpp:563-614,
+//     RP/04-Elements/10-Hexahedron/lin3DHexa8.cpp:660-718, RP/07-Assembler/Assembler.cpp
+//     :460-478, RP/12-Utilities/Driver.hpp:1669-1720); rows past the signal read as zero;
 //   * DynamicAnalysis loop and index convention (RP/08-Analysis/02-Dynamic/
 //     DynamicAnalysis.cpp:25-64): step k uses load sample k, rows written at k-1.
 //   * EigenSolver with update OFF (RP/11-Solvers/01-Direct/EigenSolver.cpp:19-59):
@@ -376,6 +380,13 @@ struct Model {
     int nload = 0, load_nt = 0;
     std::vector<int> load_node;
     std::vector<double> load_dir, load_factor, load_series;
+    // Domain reduction load: element list, per mesh node its slot in the node signals
+    // (-1: no DRM node) and exterior flag, signals [slot][nt][3*dim] with the exterior
+    // ones already negated (Driver.hpp:1669-1720).
+    std::vector<int> drm_elem, drm_slot, drm_ext;
+    int drm_nt = 0;
+    std::vector<double> drm_sig;
+    double drm_factor = 1.0;
     GaussCache gc;
 
     void element_X(const Continuum &e, double *X) const {
@@ -755,6 +766,29 @@ Model build(const svlg_mesh_desc *d) {
     m.load_dir.assign(d->load_dir, d->load_dir + size_t(d->nload) * m.dim);
     m.load_factor.assign(d->load_factor, d->load_factor + d->nload);
     m.load_series.assign(d->load_series, d->load_series + size_t(d->nload) * d->load_nt);
+    if (d->ndrm_elem > 0) {
+        const int nf = 3 * m.dim;
+        m.drm_factor = d->drm_factor;
+        m.drm_nt = d->drm_nt;
+        m.drm_elem.assign(d->drm_elem, d->drm_elem + d->ndrm_elem);
+        m.drm_slot.assign(d->nnodes, -1);
+        m.drm_ext.assign(d->nnodes, 0);
+        m.drm_sig.assign(size_t(d->ndrm_node) * d->drm_nt * nf, 0.0);
+        for (int i = 0; i < d->ndrm_node; ++i) {
+            const int n = d->drm_node[i];
+            if (n < 0 || n >= d->nnodes || m.drm_slot[n] >= 0) throw std::runtime_error("bad DRM node list");
+            m.drm_slot[n] = i;
+            m.drm_ext[n] = d->drm_cond[i] != 0;
+            const double sg = d->drm_cond[i] ? -1.0 : 1.0;
+            for (size_t q = 0; q < size_t(d->drm_nt) * nf; ++q)
+                m.drm_sig[size_t(i) * d->drm_nt * nf + q] = sg * d->drm_signal[size_t(i) * d->drm_nt * nf + q];
+        }
+        for (int e : m.drm_elem) {
+            if (e < 0 || e >= d->nelem) throw std::runtime_error("DRM element out of range");
+            for (int a = 0; a < m.nen; ++a)
+                if (m.drm_slot[m.elems[e].nodes[a]] < 0) throw std::runtime_error("DRM element node without a signal");
+        }
+    }
     return m;
 }
 
@@ -886,6 +920,44 @@ struct Assembler {
             int n = m->load_node[l];
             for (int c = 0; c < m->dim; ++c) tmp[size_t(n) * m->dim + c] = amp * m->load_dir[size_t(l) * m->dim + c];
             for (int i = 0; i < m->ntotal; ++i) F[i] += m->load_factor[l] * tmp[i];
+        }
+        if (!m->drm_elem.empty()) {
+            // ELEMENTLOAD_DOMAIN_REDUCTION (Assembler.cpp:460-478): per listed element
+            // lin*::ComputeDomainReductionForces (lin2DQuad4.cpp:564-614, lin3DHexa8.cpp:
+            // 660-718): M, C, K with every entry (i, j) of equal exterior flags zeroed, times
+            // the node motion rows k (U, V, A); summed into the load's vector, then factored.
+            std::fill(tmp.begin(), tmp.end(), 0.0);
+            const int nd = m->nen * m->dim, nf = 3 * m->dim;
+            for (int eid : m->drm_elem) {
+                const Continuum &e = m->elems[eid];
+                Mat Me = element_mass(*m, e), Ce = element_damping(*m, e), Ke = element_stiffness(*m, e, true);
+                double Uo[24], Vo[24], Ao[24];
+                int cond[24], dofs[24];
+                elem_dofs(e, dofs);
+                for (int a = 0; a < m->nen; ++a) {
+                    const int n = e.nodes[a], sl = m->drm_slot[n];
+                    for (int c = 0; c < m->dim; ++c) {
+                        double u = 0.0, v = 0.0, ac = 0.0;
+                        if (k < m->drm_nt) {
+                            const double *row = &m->drm_sig[(size_t(sl) * m->drm_nt + k) * nf];
+                            u = row[c]; v = row[m->dim + c]; ac = row[2 * m->dim + c];
+                        }
+                        Uo[a * m->dim + c] = u; Vo[a * m->dim + c] = v; Ao[a * m->dim + c] = ac;
+                        cond[a * m->dim + c] = m->drm_ext[n];
+                    }
+                }
+                for (int i = 0; i < nd; ++i) {
+                    double ma = 0.0, cv = 0.0, ku = 0.0;
+                    for (int j = 0; j < nd; ++j) {
+                        if (cond[i] == cond[j]) continue;
+                        ma += Me(i, j) * Ao[j];
+                        cv += Ce(i, j) * Vo[j];
+                        ku += Ke(i, j) * Uo[j];
+                    }
+                    tmp[dofs[i]] += (ma + cv) + ku;
+                }
+            }
+            for (int i = 0; i < m->ntotal; ++i) F[i] += m->drm_factor * tmp[i];
         }
     }
 };

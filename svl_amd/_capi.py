@@ -48,6 +48,14 @@ class MeshDesc(C.Structure):
         ("load_dir", f64p),
         ("load_factor", f64p),
         ("load_series", f64p),
+        ("ndrm_elem", C.c_int32),
+        ("drm_elem", i32p),
+        ("ndrm_node", C.c_int32),
+        ("drm_node", i32p),
+        ("drm_cond", i32p),
+        ("drm_nt", C.c_int32),
+        ("drm_signal", f64p),
+        ("drm_factor", C.c_double),
     ]
 
 
@@ -100,6 +108,16 @@ def make_desc(m: Model):
     d.load_dir = _p(arr(m.load_dir, np.float64), f64p)
     d.load_factor = _p(arr(m.load_factor, np.float64), f64p)
     d.load_series = _p(arr(m.load_series, np.float64), f64p)
+    if m.drm_elem is not None and len(m.drm_elem):
+        sig = arr(m.drm_signal, np.float64)
+        d.ndrm_elem = int(len(m.drm_elem))
+        d.drm_elem = _p(arr(m.drm_elem, np.int32), i32p)
+        d.ndrm_node = int(len(m.drm_node))
+        d.drm_node = _p(arr(m.drm_node, np.int32), i32p)
+        d.drm_cond = _p(arr(m.drm_cond, np.int32), i32p)
+        d.drm_nt = int(sig.shape[1])
+        d.drm_signal = _p(sig, f64p)
+        d.drm_factor = float(m.drm_factor)
     return d, keep
 
 

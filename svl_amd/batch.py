@@ -22,6 +22,8 @@ def union(models) -> tuple[Model, np.ndarray, np.ndarray]:
         if (m.ndim, m.elem_type, m.thickness, m.massform_lumped) != (m0.ndim, m0.elem_type, m0.thickness,
                                                                      m0.massform_lumped):
             raise ValueError("models of a batch must share dimension, element type, thickness and mass form")
+        if m.drm_elem is not None and len(m.drm_elem):
+            raise ValueError("a batch takes models without a domain reduction load")
     node_ptr = np.zeros(len(models) + 1, np.int32)
     free_ptr = np.zeros(len(models) + 1, np.int32)
     for i, m in enumerate(models):

@@ -16,6 +16,7 @@
 #include <cstring>
 
 #include "svlg_common.hpp"
+#include "svlg_drm.hpp"
 #include "svlg_elem.hpp"
 #include "svlg_mfma.hpp"
 
@@ -98,6 +99,7 @@ struct FusedArgs {
     int ntiles, ncolors, max_stage, max_own;
     const float *kmat;  // PATH 4: K_lambda, K_mu of the single element geometry
     const int *kdev;    // non-null: load sample = *kdev + k (graph replays advance *kdev)
+    DrmRef drm;         // domain reduction forces of this step (null ptr: none)
 };
 
 __device__ __forceinline__ int color_of(const int32_t *__restrict__ cp, int i) {
@@ -300,6 +302,7 @@ __global__ __launch_bounds__(512) void k_cd_fused(FusedArgs a) {
                         (kstep >= 0 && kstep < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + kstep] : 0.0;
                     fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + c]);
                 }
+                if (a.drm.ptr) fe += drm_node_force<DIM>(a.drm, node, c);
                 a.Unext[g0 + i] = u + ((fe - fi) + a.kforce[g0 + i] * (u - up[k])) / kd[k];
             }
         }
@@ -499,6 +502,7 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
                     (kstep >= 0 && kstep < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + kstep] : 0.0;
                 fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + d]);
             }
+            if (a.drm.ptr) fe += drm_node_force<DIM>(a.drm, node, d);
             a.Unext[g] = u + ((fe - fi) + a.kforce[g] * (u - up[d])) / kd[d];
         }
     }
@@ -573,6 +577,8 @@ static int launch_fused(Model &m, int k, const double *Ucur, const double *Uprev
     a.max_own = t.max_own;
     a.egeo = t.elem_geo.p;
     a.geo = m.geo.p;
+    a.drm = drm_ref(m);
+    if (int rc = drm_forces(m, k, kdev)) return rc;
     dim3 grid(t.ntiles), block(t.block);
 #define SVLG_LAUNCH_FUSED(D, P)                                                                        \
     do {                                                                                              \
@@ -604,7 +610,7 @@ static int cd_one_step(Model &m, int k) {
     double *Uprev = m.U[(m.ring + 2) % 3].p;
     double *Unext = m.U[(m.ring + 1) % 3].p;
     if (m.variant == 1 || m.variant == 3) {
-        launch_fused(m, k, Ucur, Uprev, Unext);
+        if (int rc = launch_fused(m, k, Ucur, Uprev, Unext)) return rc;
         m.ring = (m.ring + 1) % 3;
         m.steps_done++;
         return 0;
@@ -613,6 +619,8 @@ static int cd_one_step(Model &m, int k) {
         hipLaunchKernelGGL(k_loads, dim3(1), dim3(64), 0, m.stream, k, m.nload, m.load_nt, m.ndim,
                            (int)m.h_load_dofs.size(), m.load_dofs.p, m.load_node.p, m.load_dir.p, m.load_factor.p,
                            m.load_series.p, m.Fext.p);
+    if (int rc = drm_forces(m, k, nullptr)) return rc;
+    if (int rc = drm_add_fext(m, m.Fext.p)) return rc;
     if (int rc = launch_fint_colored(m, Ucur, m.ftol, m.F.p)) return rc;
     int nb = (m.ndofn + 255) / 256;
     hipLaunchKernelGGL(k_cd_update, dim3(nb), dim3(256), 0, m.stream, m.ndofn, Ucur, Uprev, Unext, m.F.p, m.Fext.p,
@@ -692,14 +700,18 @@ static int cd_capture(Model &m, int steps) {
     if (!m.kdev.p) SVLG_HIP(m.kdev.alloc(1));
     SVLG_HIP(hipStreamSynchronize(m.stream));
     SVLG_HIP(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
-    int ring = m.ring;
-    for (int s = 0; s < steps; ++s) {
-        launch_fused(m, s, m.U[ring].p, m.U[(ring + 2) % 3].p, m.U[(ring + 1) % 3].p, m.kdev.p);
+    int ring = m.ring, rc = 0;
+    for (int s = 0; s < steps && rc == 0; ++s) {
+        rc = launch_fused(m, s, m.U[ring].p, m.U[(ring + 2) % 3].p, m.U[(ring + 1) % 3].p, m.kdev.p);
         ring = (ring + 1) % 3;
     }
-    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, m.stream, m.kdev.p, steps);
+    if (rc == 0) hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, m.stream, m.kdev.p, steps);
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(m.stream, &g);
+    if (rc != 0) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
     SVLG_HIP(ec);
     const hipError_t ei = hipGraphInstantiate(&m.cd_graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);

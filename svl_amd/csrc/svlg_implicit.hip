@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "svlg_common.hpp"
+#include "svlg_drm.hpp"
 #include "svlg_elem.hpp"
 #include "svlg_j2.hpp"
 #include "svlg_solver.hpp"
@@ -71,6 +72,7 @@ struct OpArgs {
     const double *Vm, *dUn;
     const int *jdev;       // non-null: the iteration is *jdev + iter (graph replays)
     int nelem;             // elements (the Gauss-point state stride)
+    DrmRef drm;            // OP_RHS: domain reduction forces of step k (null ptr: none)
 };
 
 // One node tile of the effective-stiffness operator (one thread per tile element in the
@@ -246,6 +248,7 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
                         (a.k >= 0 && a.k < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + a.k] : 0.0;
                     fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + d]);
                 }
+                if (a.drm.ptr) fe += drm_node_force<DIM>(a.drm, node, d);
                 const double v = sV[tid * DIM + d];
                 // Newton residual (Vm set): the staged V is V - 2/dt dU, the mass term uses
                 // the step's V and dU.
@@ -711,6 +714,8 @@ int launch_nl_rhs(Model &m, int k, const double *un, const double *w, const doub
     a.load_series = m.load_series.p;
     a.load_nt = m.load_nt;
     a.k = k;
+    a.drm = drm_ref(m);
+    if (int rc = drm_forces(m, k, nullptr)) return rc;
     return launch_op(m, OP_RHS, a);
 }
 
@@ -841,6 +846,8 @@ int svlg_nm_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
     for (int st = 0; st < nsteps; ++st) {
         // Feff (ExtendedNewmarkBeta::ComputeEffectiveForce with the Linear algorithm's dU = 0).
         a.k = k0 + st;
+        a.drm = drm_ref(m);
+        if (int rc = drm_forces(m, a.k, nullptr)) return rc;
         if (int rc = launch_op(m, OP_RHS, a)) return rc;
         // Keff dU = Feff; the previous increment is the initial guess.
         int it = 0;

@@ -14,6 +14,7 @@
 #include <stdexcept>
 
 #include "svlg_common.hpp"
+#include "svlg_drm.hpp"
 #include "svlg_elem.hpp"
 #include "svlg_tiles.hpp"
 
@@ -270,6 +271,11 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         for (int l = 0; l < d->nload; ++l)
             SVLG_CHECK(d->load_node[l] >= 0 && d->load_node[l] < nn, "load node out of range");
     }
+    if (d->ndrm_elem > 0) {
+        SVLG_CHECK(d->drm_elem && d->drm_node && d->drm_cond && d->drm_signal, "incomplete DRM arrays");
+        for (int i = 0; i < d->ndrm_elem; ++i)
+            SVLG_CHECK(d->drm_elem[i] >= 0 && d->drm_elem[i] < ne, "DRM element out of range");
+    }
     int ndev = 0;
     SVLG_CHECK(hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0, "no HIP device available");
 
@@ -320,6 +326,8 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
             if (d->free_dof[size_t(n) * ndim + c] < 0) special[n] = 1;
     for (int i = 0; i < d->ndash; ++i) special[d->dash_node[i]] = 1;
     for (int l = 0; l < d->nload; ++l) special[d->load_node[l]] = 1;
+    for (int i = 0; i < std::max(d->ndrm_elem, 0); ++i)
+        for (int a = 0; a < nen; ++a) special[d->conn[size_t(d->drm_elem[i]) * nen + a]] = 1;
     for (int e = 0; e < ne; ++e)
         if (am[e] != 0.0)
             for (int a = 0; a < nen; ++a) special[d->conn[size_t(e) * nen + a]] = 1;
@@ -615,6 +623,7 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
             return fail(11);
         }
     }
+    if (int rc = drm_setup(m, d)) return fail(rc);
     if (int rc = prepare_fused(m)) return fail(rc);
     *out = h;
     return 0;
@@ -637,6 +646,10 @@ int svlg_plan_tiles(const svlg_mesh_desc *d, int32_t tile_nodes, int64_t *stats)
             if (d->free_dof[size_t(n) * ndim + c] < 0) special[n] = 1;
     for (int i = 0; i < d->ndash; ++i) special[d->dash_node[i]] = 1;
     for (int l = 0; l < d->nload; ++l) special[d->load_node[l]] = 1;
+    for (int i = 0; i < std::max(d->ndrm_elem, 0); ++i) {
+        SVLG_CHECK(d->drm_elem && d->drm_elem[i] >= 0 && d->drm_elem[i] < ne, "DRM element out of range");
+        for (int a = 0; a < nen; ++a) special[d->conn[size_t(d->drm_elem[i]) * nen + a]] = 1;
+    }
     HostTiles t;
     try {
         make_tiles(ndim, nn, ne, nen, d->coords, d->conn, color.data(), ncolors, special.data(),

@@ -67,6 +67,13 @@ class Model:
     name: str = "model"
     # Free-DOF numbers of a run file ([nnodes, ndim], -1 restrained); None: the Plain scheme.
     free_dof_override: np.ndarray | None = None
+    # Domain reduction (ElementLoad GENERALWAVE): DRM elements, their nodes with the exterior
+    # flag, and per node the signal rows [nt, 3*ndim] (U, V, A components, as in SVL's files).
+    drm_elem: np.ndarray | None = None
+    drm_node: np.ndarray | None = None
+    drm_cond: np.ndarray | None = None
+    drm_signal: np.ndarray | None = None     # [n_drm_node, nt, 3*ndim]
+    drm_factor: float = 1.0
 
     def __post_init__(self):
         self.coords = np.ascontiguousarray(self.coords, dtype=np.float64)

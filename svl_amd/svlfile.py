@@ -12,8 +12,9 @@ algorithm, integrator, analysis, combo :1747-1850) and ``Recorders`` (:1858-1927
 Continuum nodes become mesh nodes in ascending tag order; a ZeroLength1D element with a
 Viscous1DLinear material between a mesh node and a fully restrained node is a Lysmer dashpot;
 the free-DOF numbering is taken from the file. Anything else (other elements or materials,
-support motion, constraints, point masses, body or DRM loads) is refused with ValueError:
-it is not on the device path.
+support motion, constraints, point masses, body loads) is refused with ValueError: it is
+not on the device path. ``ElementLoad`` GENERALWAVE (domain reduction, :1669-1720) is read with
+its per-node signal files.
 
 Writer (``write_run``) of the same schema from a ``Model`` (the inverse, with a restrained
 copy node per dashpot as setLysmerDomain makes them, PP/Method/Builder.py:1086-1131), so a
@@ -77,6 +78,37 @@ def _series_file(path, base):
         vals = f.read().split()
     n = int(float(vals[0]))
     return np.asarray([float(v) for v in vals[1:1 + n]], dtype=np.float64)
+
+
+def _read_drm(model: Model, at: dict, etag_index: dict, index: dict, base: str, factor: float):
+    """ElementLoad GENERALWAVE (Driver.hpp:1669-1720): the listed elements, and per node of
+    those elements (ascending tag) a file named by the load's pattern with '$' replaced by the
+    node tag, holding ``nt nFields cond`` and nt rows of U, V, A components."""
+    elems = []
+    for t in at["list"]:
+        if int(t) not in etag_index:
+            raise ValueError(f"DRM element {t} is not a continuum element")
+        elems.append(etag_index[int(t)])
+    elems = np.asarray(elems, np.int32)
+    tags = sorted({int(model.node_tags[n]) for n in model.conn[elems].reshape(-1)})
+    nf = 3 * model.ndim
+    sig, cond = [], []
+    for t in tags:
+        fname = str(at["file"]).replace("$", str(t), 1)
+        p = fname if os.path.isabs(fname) else os.path.join(base, fname)
+        with open(p) as f:
+            vals = f.read().split()
+        nt, nfields, c = int(vals[0]), int(vals[1]), int(vals[2])
+        if nfields != nf:
+            raise ValueError(f"DRM node {t}: {nfields} fields, the device path reads U, V, A ({nf})")
+        sig.append(np.asarray([float(v) for v in vals[3:3 + nt * nf]], np.float64).reshape(nt, nf))
+        cond.append(1 if c else 0)
+    nt = max(s.shape[0] for s in sig)
+    signal = np.zeros((len(tags), nt, nf))
+    for i, s in enumerate(sig):
+        signal[i, :s.shape[0]] = s
+    from .drm import attach
+    attach(model, elems, np.asarray([index[t] for t in tags], np.int32), np.asarray(cond, np.int32), signal, factor)
 
 
 def read_run(path: str) -> SvlRun:
@@ -206,9 +238,14 @@ def read_run(path: str) -> SvlRun:
     series, rows = [], []
     for lt, fac in zip(ca.get("load", []), ca.get("factor", [])):
         ld = loads[str(lt)]
+        at = ld["attributes"]
+        if _lc(ld["name"]) == "elementload" and _lc(at.get("type")) == "generalwave":
+            if model.drm_elem is not None:
+                raise ValueError(f"load {lt}: one domain reduction load per combination is on the device path")
+            _read_drm(model, at, etag_index, index, base, float(fac))
+            continue
         if _lc(ld["name"]) != "pointload":
             raise ValueError(f"load {lt}: {ld['name']} is not on the device path")
-        at = ld["attributes"]
         if _lc(at.get("type")) != "concentrated":
             raise ValueError(f"load {lt}: only CONCENTRATED point loads are on the device path")
         fn = _lc(at.get("name"))
@@ -328,6 +365,21 @@ def write_run(model: Model, path: str, integrator: str, dt: float, nt: int, algo
             "dir": [float(v) for v in model.load_dir[l]]}}
         lids.append(l + 1)
         facs.append(float(model.load_factor[l]))
+    if model.drm_elem is not None and len(model.drm_elem):
+        # ElementLoad GENERALWAVE with one signal file per DRM node ('$' = node tag).
+        pattern = os.path.join(base, f"{stem}_drm_$.txt")
+        for i, n in enumerate(model.drm_node):
+            sg = model.drm_signal[i]
+            with open(pattern.replace("$", str(int(n) + 1)), "w") as f:
+                f.write(f"{sg.shape[0]} {sg.shape[1]} {int(model.drm_cond[i])}\n")
+                for row in sg:
+                    f.write(" ".join(repr(float(v)) for v in row) + "\n")
+        lid = len(loads) + 1
+        loads[str(lid)] = {"name": "ElementLoad", "attributes": {
+            "type": "GENERALWAVE", "name": "TIMESERIES", "file": pattern,
+            "list": [int(e) + 1 for e in model.drm_elem]}}
+        lids.append(lid)
+        facs.append(float(model.drm_factor))
     out["Loads"] = loads
     out["Combinations"] = {"1": {"name": "Combo", "attributes": {"load": lids, "factor": facs}}}
     out["Simulations"] = {"combo": 1, "attributes": {

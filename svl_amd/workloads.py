@@ -20,6 +20,8 @@
   mass, vertical Ricker f0 = 5 Hz (t0 = 0.25 s, Ap = 1e6 N) at the top centre node,
   ExtendedNewmarkBeta with dt = 2e-3 s. At 64 x 64 x 64 this is the 811,200-DOF block
   of 500 implicit steps.
+* ``halfspace3d_drm`` -- the half-space driven through a domain reduction layer instead of
+  the point source (synthetic SV plane-wave signals).
 
 Lysmer convention (Builder.py:1086-1131, per-node generalisation): a boundary node's
 dashpot on the boundary-normal direction takes rho Vp A, the tangential ones rho Vs A,
@@ -104,6 +106,38 @@ def halfspace3d(nx: int = 100, ny: int = 100, nz: int = 50, h: float = 2.0, nt: 
     t = np.arange(nt) * dt
     src = m.nearest_node((0.0, 0.0, 0.0))
     m.add_point_load(src, [0.0, 0.0, -1.0], ricker_velocity(t, t0, f0, Ap))
+    return m, dt, nt
+
+
+def halfspace3d_drm(nx: int = 100, ny: int = 100, nz: int = 50, h: float = 2.0, nt: int = 331,
+                    box_frac: float = 0.8, f0: float = 3.0, t0: float = 0.4, amp: float = 1e-3, **kw):
+    """The half-space of ``halfspace3d`` without its point source, loaded instead through a
+    domain reduction layer: the box |x|, |y| <= box_frac L/2, z >= -box_frac Lz (open at the
+    free surface) selected as SVL's pre-processor does, with synthetic node signals of a
+    vertically incident SV Ricker pulse (x displacement amp r(t - t0 - (z + Lz)/vs) plus its
+    surface reflection, vs = 150 m/s; V and A by finite differences in time). The signals
+    are timing input, not the layered medium's exact free field. Returns (model, dt, nt)."""
+    from .drm import attach, drm_box
+    m, dt, nt = halfspace3d(nx, ny, nz, h=h, nt=nt, **kw)
+    m.load_node = np.zeros(0, np.int32)
+    m.load_dir, m.load_factor, m.load_series = np.zeros((0, 3)), np.zeros(0), np.zeros((0, 1))
+    Lx, Ly, Lz = nx * h, ny * h, nz * h
+    elems, nodes, cond = drm_box(m, (0.0, 0.0, 0.0), (box_frac * Lx / 2, box_frac * Ly / 2, box_frac * Lz))
+    t = np.arange(nt) * dt
+    z = m.coords[nodes, 2]
+    vs = 150.0
+
+    def ricker(tau):
+        a = (math.pi * f0 * tau) ** 2
+        return (1.0 - 2.0 * a) * np.exp(-a)
+
+    u = amp * (ricker(t[None, :] - t0 - (z[:, None] + Lz) / vs) + ricker(t[None, :] - t0 - (Lz - z[:, None]) / vs))
+    v = np.gradient(u, dt, axis=1)
+    a = np.gradient(v, dt, axis=1)
+    sig = np.zeros((nodes.size, nt, 9))
+    sig[:, :, 0], sig[:, :, 3], sig[:, :, 6] = u, v, a
+    attach(m, elems, nodes, cond, sig)
+    m.name = f"halfspace3d_drm_{nx}x{ny}x{nz}"
     return m, dt, nt
 
 

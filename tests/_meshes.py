@@ -1,7 +1,7 @@
 """Small meshes shared by the GPU parity tests (no GPU needed to build them)."""
 import numpy as np
 
-from svl_amd.mesh import ELEM_HEXA8, Model, make_domain_volume, ricker_velocity, tags_for
+from svl_amd.mesh import ELEM_HEXA8, ELEM_QUAD4, Model, make_domain_area, make_domain_volume, ricker_velocity, tags_for
 
 
 def block_model(nx, ny, nz, transform=None, distort=0.0, seed=0):
@@ -52,3 +52,31 @@ def graded(m):
         c[:, d] = new[np.searchsorted(vals, c[:, d])]
     m.coords = np.ascontiguousarray(c)
     return m
+
+
+def drm_quad_model(nt, dt, half=(3.5, 2.5)):
+    """Quad4 grid with every side fixed, a source outside a DRM box (center, half sides)."""
+    coords, conn, bnd = make_domain_area(16, 12, (0, 0), (16, 0), (0, 12))
+    restrained = np.zeros(coords.shape, bool)
+    for k in ("left", "right", "top", "bottom"):
+        restrained[bnd[k]] = True
+    rng = np.random.default_rng(3)
+    m = Model(ndim=2, coords=coords, node_tags=tags_for(coords.shape[0]), elem_type=ELEM_QUAD4, conn=conn,
+              E=rng.uniform(0.8e8, 1.2e8, conn.shape[0]), nu=0.3, rho=2000.0, restrained=restrained)
+    src = int(np.argmin(np.linalg.norm(coords - np.array([2.0, 10.0]), axis=1)))
+    m.add_point_load(src, [1.0, -0.5], ricker_velocity(np.arange(nt) * dt, 0.03, 30.0, 1e6))
+    return m, ((8.0, 6.0), half)
+
+
+def drm_hexa_model(nt, dt, n=(8, 8, 6)):
+    """Hexa8 block with every side fixed, a source outside a DRM box (center, half sides)."""
+    coords, conn, bnd = make_domain_volume(*n, (0, 0, 0), (n[0], 0, 0), (0, n[1], 0), (0, 0, n[2]))
+    restrained = np.zeros(coords.shape, bool)
+    for k in bnd:
+        restrained[bnd[k]] = True
+    m = Model(ndim=3, coords=coords, node_tags=tags_for(coords.shape[0]), elem_type=ELEM_HEXA8, conn=conn,
+              E=1e8, nu=0.25, rho=2000.0, restrained=restrained)
+    src = int(np.argmin(np.linalg.norm(coords - np.array([1.0, 1.0, n[2] - 1.0]), axis=1)))
+    m.add_point_load(src, [0.3, 0.5, -1.0], ricker_velocity(np.arange(nt) * dt, 0.02, 40.0, 1e6))
+    c = np.array(n, float) / 2
+    return m, ((c[0], c[1] + 0.5, c[2]), (n[0] / 5.3, n[1] / 5.3, n[2] / 4))

@@ -139,3 +139,46 @@ def test_node_recorder_format(tmp_path):
     tags, tot, ntotal, nt, v = S.read_node_recorder(p)
     assert tags == [7, 3] and ntotal == 9 and nt == 4
     assert np.allclose(v, vals.reshape(3, 4), rtol=1e-5, atol=1e-9)
+
+
+def test_drm_roundtrip(tmp_path):
+    from svl_amd import drm
+    from _meshes import drm_quad_model
+    dt, nt = 2e-3, 30
+    m, box = drm_quad_model(nt, dt)
+    rec = list(range(m.nfree))
+    bg = oracle.run(m, oracle.EXTENDED_NEWMARK, dt, nt, rec, want_va=True)
+    m.load_node = m.load_node[:0]
+    m.load_dir, m.load_factor, m.load_series = np.zeros((0, 2)), np.zeros(0), np.zeros((0, 1))
+    elems, nodes, cond = drm.drm_box(m, *box)
+    elems = elems[::-1].copy()  # SVL keeps the list order of the load's elements
+    drm.attach(m, elems, nodes, cond, drm.signals_from_history(m, nodes, nt, bg["U"], bg["V"], bg["A"], 1), 0.75)
+    p = str(tmp_path / "drm.json")
+    S.write_run(m, p, "EXTENDEDNEWMARKBETA", dt, nt)
+    js = json.load(open(p))
+    ld = js["Loads"]["1"]
+    assert ld["name"] == "ElementLoad" and ld["attributes"]["type"] == "GENERALWAVE"
+    assert js["Combinations"]["1"]["attributes"]["factor"] == [0.75]
+    q = S.read_run(p).model
+    assert np.array_equal(q.drm_elem, elems) and np.array_equal(q.drm_node, nodes)
+    assert np.array_equal(q.drm_cond, cond) and np.array_equal(q.drm_signal, m.drm_signal)
+    assert q.drm_factor == 0.75
+    a = oracle.run(m, oracle.EXTENDED_NEWMARK, dt, nt, rec)["U"]
+    b = oracle.run(q, oracle.EXTENDED_NEWMARK, dt, nt, rec)["U"]
+    assert np.abs(a).max() > 0 and np.array_equal(a, b)
+
+
+def test_handwritten_generalwave(tmp_path):
+    # Node files as SVL's driver reads them: "nt nFields cond", then nt rows.
+    for tag, c in ((1, 1), (2, 0), (3, 0), (4, 1)):
+        rows = [[tag + 0.1 * k + 0.01 * f for f in range(6)] for k in range(3)]
+        (tmp_path / f"drm{tag}.txt").write_text(f"3 6 {c}\n" + "\n".join(" ".join(map(str, r)) for r in rows))
+
+    def edit(js):
+        js["Loads"]["3"] = {"name": "ElementLoad", "attributes": {"type": "GENERALWAVE", "name": "TIMESERIES",
+                                                                  "file": str(tmp_path / "drm$.txt"), "list": [1]}}
+        js["Combinations"]["1"]["attributes"] = {"load": [1, 3], "factor": [2.0, 1.5]}
+    m = S.read_run(handwritten(tmp_path, edit)).model
+    assert list(m.drm_elem) == [0] and list(m.drm_node) == [0, 1, 2, 3] and list(m.drm_cond) == [1, 0, 0, 1]
+    assert m.drm_factor == 1.5 and m.drm_signal.shape == (4, 3, 6)
+    assert m.drm_signal[3, 2, 5] == 4 + 0.2 + 0.05  # stored as read; the exterior sign flips on the device
