@@ -107,16 +107,12 @@ struct Model {
     DevBuf<int32_t> load_dofs;
     DevBuf<int32_t> nl_ptr, nl_idx;   // node -> loads acting on it (ascending load index)
 
-    // Domain reduction load (ELEMENTLOAD_DOMAIN_REDUCTION). Signals are time-major
-    // [nt][ndrm_node][3*ndim] with the exterior ones negated; the element forces of the
-    // current step land in drm_fe [ndrm][nen*ndim]; drm_ptr/drm_idx list, per device node,
-    // its (DRM element j, corner a) entries as j*nen + a in ascending j.
+    // Domain reduction load (ELEMENTLOAD_DOMAIN_REDUCTION): the factored nodal forces of every
+    // signal row, drm_F [nt][slot][ndim] (slots in ascending device node order), the slot of
+    // every device node (-1 outside the layer) and the device node of every slot.
     int ndrm = 0, ndrm_node = 0, drm_nt = 0;
-    double drm_factor = 1.0;
-    DevBuf<int32_t> drm_elem, drm_cslot, drm_ext;
-    DevBuf<double> drm_sig, drm_fe;
-    DevBuf<int32_t> drm_ptr, drm_idx;
-    DevBuf<int32_t> drm_dnode;        // [ndrm_node] device node of every signal slot
+    DevBuf<double> drm_F;
+    DevBuf<int32_t> drm_slot, drm_dnode;
 
     // Device node numbering (tile order): perm[device] = original, inv[original] = device.
     std::vector<int32_t> perm, inv;

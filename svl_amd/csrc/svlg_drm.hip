@@ -10,10 +10,17 @@
 // With the lumped mass every mass entry couples a DOF with itself, so M' = 0, and with
 // Rayleigh damping C = am M + ak K, C' = ak K'. Then F_e = K' (U_0 + ak V_0), and with
 // P_b, P_x the selections of interior and exterior DOFs, K' = P_x K P_b + P_b K P_x:
-// exterior rows of K (P_b w), interior rows of K (P_x w). One thread per DRM element
-// evaluates the two internal forces with the Gauss-point element kernel and keeps the
-// matching rows; the step kernels add the corners of a node in element-list order.
+// exterior rows of K (P_b w), interior rows of K (P_x w).
+//
+// The signals are fixed inputs and the DRM elements are elastic, so the nodal forces of
+// every step depend on nothing the time loop computes. The set-up evaluates all of them
+// once, a chunk of steps per launch pair (two threads per (step, DRM element) for the two
+// masked element forces; one thread per (step, DRM node) for the sum of its corners in
+// element-list order times the load factor), and keeps them as F[k][slot][dim] (slots in
+// device node order); the signals are then released. A step kernel adds F[k][slot] at a
+// DRM node where the point loads enter Fext: no launch in the time loop.
 #include <algorithm>
+#include <numeric>
 #include <vector>
 
 #include "svlg_drm.hpp"
@@ -22,64 +29,94 @@
 namespace svlg {
 
 struct DrmArgs {
-    int n, nt, nnode, k;
-    const int *kdev;
+    int64_t nthreads;
+    int n, nnode, k0;
     const int32_t *elem, *cslot, *ext, *conn;
     const double *X, *lam, *mu, *ak, *sig;
     double th;
     double *fe;
 };
 
-template <int DIM>
+// Thread t: step k0 + t / (2 n), DRM element j, side. Side 0 evaluates K (P_b w) and keeps
+// the exterior rows, side 1 evaluates K (P_x w) and keeps the interior rows (disjoint rows
+// of fe). AFFINE: every element is a parallelepiped (parallelogram), so the one-point modal
+// form of the element force is exact; otherwise the Gauss-point loop.
+template <int DIM, bool AFFINE>
 __global__ __launch_bounds__(64) void k_drm_forces(DrmArgs a) {
     constexpr int NEN = Elem<DIM>::nen, NF = 3 * DIM;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= a.n) return;
-    const int ks = a.kdev ? *a.kdev + a.k : a.k;
-    const bool have = ks >= 0 && ks < a.nt;
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= a.nthreads) return;
+    const int64_t kk = t / (2 * a.n);
+    const int r = int(t - kk * 2 * a.n), j = r >> 1, side = r & 1;
     const int e = a.elem[j];
     const double ak = a.ak[e];
-    double x[NEN][DIM], wb[NEN][DIM], wx[NEN][DIM], f1[NEN][DIM], f2[NEN][DIM];
-    bool ext[NEN];
+    double x[NEN][DIM], w[NEN][DIM], f[NEN][DIM];
+    bool keep[NEN];
 #pragma unroll
     for (int c = 0; c < NEN; ++c) {
         const size_t nd = size_t(a.conn[size_t(e) * NEN + c]) * DIM;
         const int slot = a.cslot[size_t(j) * NEN + c];
-        ext[c] = a.ext[slot] != 0;
-        const double *row = a.sig + (size_t(have ? ks : 0) * a.nnode + slot) * NF;
+        const bool ext = a.ext[slot] != 0;
+        const bool staged = side == 0 ? !ext : ext;
+        keep[c] = !staged;
+        const double *row = a.sig + ((size_t(a.k0) + size_t(kk)) * a.nnode + slot) * NF;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
             x[c][d] = a.X[nd + d];
-            const double w = have ? row[d] + ak * row[DIM + d] : 0.0;
-            wb[c][d] = ext[c] ? 0.0 : w;
-            wx[c][d] = ext[c] ? w : 0.0;
+            w[c][d] = staged ? row[d] + ak * row[DIM + d] : 0.0;
         }
     }
     if constexpr (DIM == 3) {
-        hexa_fint(x, wb, a.lam[e], a.mu[e], f1);
-        hexa_fint(x, wx, a.lam[e], a.mu[e], f2);
+        if constexpr (AFFINE) hexa_affine_fint(x, w, a.lam[e], a.mu[e], f);
+        else hexa_fint(x, w, a.lam[e], a.mu[e], f);
     } else {
-        quad_fint(x, wb, a.lam[e], a.mu[e], a.th, f1);
-        quad_fint(x, wx, a.lam[e], a.mu[e], a.th, f2);
+        if constexpr (AFFINE) quad_affine_fint(x, w, a.lam[e], a.mu[e], a.th, f);
+        else quad_fint(x, w, a.lam[e], a.mu[e], a.th, f);
     }
+    double *o = a.fe + (size_t(kk) * a.n + j) * NEN * DIM;
 #pragma unroll
     for (int c = 0; c < NEN; ++c)
+        if (keep[c])
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) a.fe[(size_t(j) * NEN + c) * DIM + d] = ext[c] ? f1[c][d] : f2[c][d];
+            for (int d = 0; d < DIM; ++d) o[c * DIM + d] = f[c][d];
+}
+
+// Thread t: step k0 + t / ns, slot s = t % ns: factor * (its corners in element-list order).
+template <int DIM>
+__global__ __launch_bounds__(64) void k_drm_gather(int64_t nthreads, int ns, int nen, int n, int k0,
+                                                   const int32_t *__restrict__ ptr, const int32_t *__restrict__ idx,
+                                                   const double *__restrict__ fe, double factor,
+                                                   double *__restrict__ F) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nthreads) return;
+    const int64_t kk = t / ns;
+    const int s = int(t - kk * ns);
+    const double *base = fe + size_t(kk) * n * nen * DIM;
+    double acc[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) acc[d] = 0.0;
+    for (int q = ptr[s]; q < ptr[s + 1]; ++q) {
+        const size_t o = size_t(idx[q]) * DIM;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) acc[d] += base[o + d];
+    }
+    double *out = F + ((size_t(k0) + size_t(kk)) * ns + s) * DIM;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) out[d] = factor * acc[d];
 }
 
 template <int DIM>
-__global__ void k_drm_add(int nnode, const int32_t *__restrict__ dnode, const int32_t *__restrict__ nl_ptr,
+__global__ void k_drm_add(int ns, int k, const int32_t *__restrict__ dnode, const int32_t *__restrict__ nl_ptr,
                           DrmRef r, double *__restrict__ Fext) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nnode) return;
+    if (s >= ns) return;
     const int node = dnode[s];
     const bool loaded = nl_ptr[node + 1] > nl_ptr[node];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         const size_t g = size_t(node) * DIM + d;
         const double base = loaded ? Fext[g] : 0.0;
-        Fext[g] = base + drm_node_force<DIM>(r, node, d);
+        Fext[g] = base + drm_force(r, s, k, DIM, d);
     }
 }
 
@@ -87,18 +124,27 @@ int drm_setup(Model &m, const svlg_mesh_desc *d) {
     m.ndrm = 0;
     if (d->ndrm_elem <= 0) return 0;
     const int nen = m.nen, ndim = m.ndim, nf = 3 * ndim;
-    const int ne = d->ndrm_elem, nnode = d->ndrm_node, nt = d->drm_nt;
+    const int ne = d->ndrm_elem, ns = d->ndrm_node, nt = d->drm_nt;
     SVLG_CHECK(d->drm_elem && d->drm_node && d->drm_cond && d->drm_signal, "DRM arrays missing");
-    SVLG_CHECK(nnode > 0 && nt > 0, "a DRM load needs nodes and at least one signal row");
-    std::vector<int32_t> slot(m.nnodes, -1);
-    for (int i = 0; i < nnode; ++i) {
+    SVLG_CHECK(ns > 0 && nt > 0, "a DRM load needs nodes and at least one signal row");
+    // Internal slots in ascending device node order (coalesced reads in the step kernels).
+    std::vector<int32_t> order(ns);
+    std::iota(order.begin(), order.end(), 0);
+    for (int i = 0; i < ns; ++i) {
         const int n = d->drm_node[i];
         SVLG_CHECK(n >= 0 && n < m.nnodes, "DRM node out of range");
-        SVLG_CHECK(slot[n] < 0, "DRM node listed twice");
-        slot[n] = i;
     }
-    std::vector<int32_t> cslot(size_t(ne) * nen), ext(nnode), dnode(nnode);
-    std::vector<int32_t> cnt(m.nnodes + 1, 0);
+    std::sort(order.begin(), order.end(),
+              [&](int a, int b) { return m.inv[d->drm_node[a]] < m.inv[d->drm_node[b]]; });
+    std::vector<int32_t> slot(m.nnodes, -1), dnode(ns), ext(ns);
+    for (int s = 0; s < ns; ++s) {
+        const int n = d->drm_node[order[s]];
+        SVLG_CHECK(slot[n] < 0, "DRM node listed twice");
+        slot[n] = s;
+        dnode[s] = m.inv[n];
+        ext[s] = d->drm_cond[order[s]] != 0;
+    }
+    std::vector<int32_t> cslot(size_t(ne) * nen), cnt(ns + 1, 0);
     for (int j = 0; j < ne; ++j) {
         const int e = d->drm_elem[j];
         SVLG_CHECK(e >= 0 && e < m.nelem, "DRM element out of range");
@@ -108,79 +154,90 @@ int drm_setup(Model &m, const svlg_mesh_desc *d) {
             const int n = d->conn[size_t(e) * nen + a];
             SVLG_CHECK(slot[n] >= 0, "a node of a DRM element has no signal");
             cslot[size_t(j) * nen + a] = slot[n];
-            cnt[m.inv[n] + 1]++;
+            cnt[slot[n] + 1]++;
         }
     }
-    for (int i = 0; i < nnode; ++i) {
-        ext[i] = d->drm_cond[i] != 0;
-        dnode[i] = m.inv[d->drm_node[i]];
-    }
-    for (int n = 0; n < m.nnodes; ++n) cnt[n + 1] += cnt[n];
-    std::vector<int32_t> idx(cnt[m.nnodes]), fill(cnt.begin(), cnt.end() - 1);
+    for (int s = 0; s < ns; ++s) cnt[s + 1] += cnt[s];
+    std::vector<int32_t> idx(cnt[ns]), fill(cnt.begin(), cnt.end() - 1);
     for (int j = 0; j < ne; ++j)
-        for (int a = 0; a < nen; ++a) idx[fill[m.inv[d->conn[size_t(d->drm_elem[j]) * nen + a]]]++] = j * nen + a;
+        for (int a = 0; a < nen; ++a) idx[fill[cslot[size_t(j) * nen + a]]++] = j * nen + a;
+    std::vector<int32_t> nslot(m.nnodes, -1);
+    for (int s = 0; s < ns; ++s) nslot[dnode[s]] = s;
     // Time-major signals, exterior ones negated (SVL negates them when it reads the files).
-    std::vector<double> sig(size_t(nt) * nnode * nf);
-    for (int i = 0; i < nnode; ++i) {
-        const double sg = ext[i] ? -1.0 : 1.0;
+    std::vector<double> sig(size_t(nt) * ns * nf);
+    for (int s = 0; s < ns; ++s) {
+        const int i = order[s];
+        const double sg = ext[s] ? -1.0 : 1.0;
         for (int k = 0; k < nt; ++k)
             for (int f = 0; f < nf; ++f)
-                sig[(size_t(k) * nnode + i) * nf + f] = sg * d->drm_signal[(size_t(i) * nt + k) * nf + f];
+                sig[(size_t(k) * ns + s) * nf + f] = sg * d->drm_signal[(size_t(i) * nt + k) * nf + f];
     }
-    hipStream_t s = m.stream;
-    SVLG_HIP(m.drm_elem.upload(d->drm_elem, ne, s));
-    SVLG_HIP(m.drm_cslot.upload(cslot.data(), cslot.size(), s));
-    SVLG_HIP(m.drm_ext.upload(ext.data(), ext.size(), s));
-    SVLG_HIP(m.drm_dnode.upload(dnode.data(), dnode.size(), s));
-    SVLG_HIP(m.drm_ptr.upload(cnt.data(), cnt.size(), s));
-    SVLG_HIP(m.drm_idx.upload(idx.data(), idx.size(), s));
-    SVLG_HIP(m.drm_sig.upload(sig.data(), sig.size(), s));
-    SVLG_HIP(m.drm_fe.alloc(size_t(ne) * nen * ndim));
-    SVLG_HIP(hipMemsetAsync(m.drm_fe.p, 0, sizeof(double) * size_t(ne) * nen * ndim, s));
-    SVLG_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+    hipStream_t st = m.stream;
+    DevBuf<int32_t> delem, dcslot, dext, dptr, didx;
+    DevBuf<double> dsig, fe;
+    SVLG_HIP(delem.upload(d->drm_elem, ne, st));
+    SVLG_HIP(dcslot.upload(cslot.data(), cslot.size(), st));
+    SVLG_HIP(dext.upload(ext.data(), ext.size(), st));
+    SVLG_HIP(dptr.upload(cnt.data(), cnt.size(), st));
+    SVLG_HIP(didx.upload(idx.data(), idx.size(), st));
+    SVLG_HIP(dsig.upload(sig.data(), sig.size(), st));
+    SVLG_HIP(m.drm_F.alloc(size_t(nt) * ns * ndim));
+    // Steps per pass: element forces of at most 2^25 doubles (256 MiB) at a time.
+    const size_t per_step = size_t(ne) * nen * ndim;
+    const int chunk = int(std::max<size_t>(1, std::min<size_t>(size_t(nt), (size_t(1) << 25) / per_step)));
+    SVLG_HIP(fe.alloc(size_t(chunk) * per_step));
+    SVLG_HIP(hipMemsetAsync(fe.p, 0, sizeof(double) * size_t(chunk) * per_step, st));
+    for (int k0 = 0; k0 < nt; k0 += chunk) {
+        const int kc = std::min(chunk, nt - k0);
+        DrmArgs a{};
+        a.nthreads = int64_t(kc) * 2 * ne;
+        a.n = ne;
+        a.nnode = ns;
+        a.k0 = k0;
+        a.elem = delem.p;
+        a.cslot = dcslot.p;
+        a.ext = dext.p;
+        a.conn = m.conn.p;
+        a.X = m.X.p;
+        a.lam = m.lam.p;
+        a.mu = m.mu.p;
+        a.ak = m.ak.p;
+        a.sig = dsig.p;
+        a.th = m.th;
+        a.fe = fe.p;
+        const unsigned nb = unsigned((a.nthreads + 63) / 64);
+        const int64_t ng64 = int64_t(kc) * ns;
+        const unsigned ng = unsigned((ng64 + 63) / 64);
+        if (ndim == 3) {
+            if (m.affine) hipLaunchKernelGGL((k_drm_forces<3, true>), dim3(nb), dim3(64), 0, st, a);
+            else hipLaunchKernelGGL((k_drm_forces<3, false>), dim3(nb), dim3(64), 0, st, a);
+            hipLaunchKernelGGL(k_drm_gather<3>, dim3(ng), dim3(64), 0, st, ng64, ns, nen, ne, k0, dptr.p, didx.p,
+                               fe.p, d->drm_factor, m.drm_F.p);
+        } else {
+            if (m.affine) hipLaunchKernelGGL((k_drm_forces<2, true>), dim3(nb), dim3(64), 0, st, a);
+            else hipLaunchKernelGGL((k_drm_forces<2, false>), dim3(nb), dim3(64), 0, st, a);
+            hipLaunchKernelGGL(k_drm_gather<2>, dim3(ng), dim3(64), 0, st, ng64, ns, nen, ne, k0, dptr.p, didx.p,
+                               fe.p, d->drm_factor, m.drm_F.p);
+        }
+        SVLG_HIP(hipGetLastError());
+    }
+    SVLG_HIP(m.drm_slot.upload(nslot.data(), nslot.size(), st));
+    SVLG_HIP(m.drm_dnode.upload(dnode.data(), dnode.size(), st));
+    SVLG_HIP(hipStreamSynchronize(st));  // the host vectors and the temporaries go out of scope
     m.ndrm = ne;
-    m.ndrm_node = nnode;
+    m.ndrm_node = ns;
     m.drm_nt = nt;
-    m.drm_factor = d->drm_factor;
     return 0;
 }
 
-int drm_forces(Model &m, int k, const int *kdev) {
-    if (m.ndrm == 0) return 0;
-    DrmArgs a{};
-    a.n = m.ndrm;
-    a.nt = m.drm_nt;
-    a.nnode = m.ndrm_node;
-    a.k = k;
-    a.kdev = kdev;
-    a.elem = m.drm_elem.p;
-    a.cslot = m.drm_cslot.p;
-    a.ext = m.drm_ext.p;
-    a.conn = m.conn.p;
-    a.X = m.X.p;
-    a.lam = m.lam.p;
-    a.mu = m.mu.p;
-    a.ak = m.ak.p;
-    a.sig = m.drm_sig.p;
-    a.th = m.th;
-    a.fe = m.drm_fe.p;
-    const int nb = (m.ndrm + 63) / 64;
-    if (m.ndim == 3)
-        hipLaunchKernelGGL(k_drm_forces<3>, dim3(nb), dim3(64), 0, m.stream, a);
-    else
-        hipLaunchKernelGGL(k_drm_forces<2>, dim3(nb), dim3(64), 0, m.stream, a);
-    SVLG_HIP(hipGetLastError());
-    return 0;
-}
-
-int drm_add_fext(Model &m, double *Fext) {
+int drm_add_fext(Model &m, int k, double *Fext) {
     if (m.ndrm == 0) return 0;
     const int nb = (m.ndrm_node + 63) / 64;
     if (m.ndim == 3)
-        hipLaunchKernelGGL(k_drm_add<3>, dim3(nb), dim3(64), 0, m.stream, m.ndrm_node, m.drm_dnode.p, m.nl_ptr.p,
+        hipLaunchKernelGGL(k_drm_add<3>, dim3(nb), dim3(64), 0, m.stream, m.ndrm_node, k, m.drm_dnode.p, m.nl_ptr.p,
                            drm_ref(m), Fext);
     else
-        hipLaunchKernelGGL(k_drm_add<2>, dim3(nb), dim3(64), 0, m.stream, m.ndrm_node, m.drm_dnode.p, m.nl_ptr.p,
+        hipLaunchKernelGGL(k_drm_add<2>, dim3(nb), dim3(64), 0, m.stream, m.ndrm_node, k, m.drm_dnode.p, m.nl_ptr.p,
                            drm_ref(m), Fext);
     SVLG_HIP(hipGetLastError());
     return 0;

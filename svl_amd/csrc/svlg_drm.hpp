@@ -5,35 +5,33 @@
 
 namespace svlg {
 
-// What a step kernel needs to add the DRM forces of the current step at a node: the
-// per-node list of (DRM element, corner) entries and the element forces of the step.
+// The DRM nodal forces of every step, F[k][slot][dim] (factored), and the slot of every
+// device node (-1 outside the DRM layer). F null: no DRM load.
 struct DrmRef {
-    const int32_t *ptr, *idx;  // null ptr: no DRM load
-    const double *fe;
-    double factor;
+    const int32_t *slot;
+    const double *F;
+    int nt, ns;
 };
 
-// factor * sum over the node's DRM element corners (ascending element order) of component d.
-template <int DIM>
-__device__ __forceinline__ double drm_node_force(const DrmRef &r, int node, int d) {
-    double s = 0.0;
-    for (int q = r.ptr[node]; q < r.ptr[node + 1]; ++q) s += r.fe[size_t(r.idx[q]) * DIM + d];
-    return r.factor * s;
-}
-
 inline DrmRef drm_ref(const Model &m) {
-    if (m.ndrm == 0) return DrmRef{nullptr, nullptr, nullptr, 0.0};
-    return DrmRef{m.drm_ptr.p, m.drm_idx.p, m.drm_fe.p, m.drm_factor};
+    if (m.ndrm == 0) return DrmRef{nullptr, nullptr, 0, 0};
+    return DrmRef{m.drm_slot.p, m.drm_F.p, m.drm_nt, m.ndrm_node};
 }
 
-// Uploads the DRM load of the description (after the device node numbering exists).
+// Component d of the DRM force of slot s at step k (zero past the signals).
+__device__ __forceinline__ double drm_force(const DrmRef &r, int s, int k, int dim, int d) {
+    return (k >= 0 && k < r.nt) ? r.F[(size_t(k) * r.ns + s) * dim + d] : 0.0;
+}
+
+// Slot of device node `node` (-1: none, or no DRM load).
+__device__ __forceinline__ int drm_slot(const DrmRef &r, int node) { return r.F ? r.slot[node] : -1; }
+
+// Uploads the DRM load of the description (after the device node numbering exists) and
+// evaluates its nodal forces for every signal row.
 int drm_setup(Model &m, const svlg_mesh_desc *d);
 
-// Element forces of load sample k (kdev non-null: sample *kdev + k) into m.drm_fe.
-int drm_forces(Model &m, int k, const int *kdev);
-
-// Fext += DRM forces at the DRM nodes (paths that read a dense Fext). Loaded nodes keep
-// what the load kernel assigned this step; the others are overwritten.
-int drm_add_fext(Model &m, double *Fext);
+// Fext += DRM forces of step k at the DRM nodes (paths that read a dense Fext). Loaded nodes
+// keep what the load kernel assigned this step; the others are overwritten.
+int drm_add_fext(Model &m, int k, double *Fext);
 
 }  // namespace svlg

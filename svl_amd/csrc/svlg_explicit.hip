@@ -99,7 +99,7 @@ struct FusedArgs {
     int ntiles, ncolors, max_stage, max_own;
     const float *kmat;  // PATH 4: K_lambda, K_mu of the single element geometry
     const int *kdev;    // non-null: load sample = *kdev + k (graph replays advance *kdev)
-    DrmRef drm;         // domain reduction forces of this step (null ptr: none)
+    DrmRef drm;         // domain reduction forces of every step (F null: none)
 };
 
 __device__ __forceinline__ int color_of(const int32_t *__restrict__ cp, int i) {
@@ -302,7 +302,8 @@ __global__ __launch_bounds__(512) void k_cd_fused(FusedArgs a) {
                         (kstep >= 0 && kstep < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + kstep] : 0.0;
                     fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + c]);
                 }
-                if (a.drm.ptr) fe += drm_node_force<DIM>(a.drm, node, c);
+                const int ds = drm_slot(a.drm, node);
+                if (ds >= 0) fe += drm_force(a.drm, ds, kstep, DIM, c);
                 a.Unext[g0 + i] = u + ((fe - fi) + a.kforce[g0 + i] * (u - up[k])) / kd[k];
             }
         }
@@ -482,6 +483,7 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
     __syncthreads();
     if (!has_node) return;
     const int npl = a.nplain[tile];
+    const int ds = tid >= npl ? drm_slot(a.drm, o0 + tid) : -1;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         double fi = 0.0;
@@ -502,7 +504,7 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
                     (kstep >= 0 && kstep < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + kstep] : 0.0;
                 fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + d]);
             }
-            if (a.drm.ptr) fe += drm_node_force<DIM>(a.drm, node, d);
+            if (ds >= 0) fe += drm_force(a.drm, ds, kstep, DIM, d);
             a.Unext[g] = u + ((fe - fi) + a.kforce[g] * (u - up[d])) / kd[d];
         }
     }
@@ -578,7 +580,6 @@ static int launch_fused(Model &m, int k, const double *Ucur, const double *Uprev
     a.egeo = t.elem_geo.p;
     a.geo = m.geo.p;
     a.drm = drm_ref(m);
-    if (int rc = drm_forces(m, k, kdev)) return rc;
     dim3 grid(t.ntiles), block(t.block);
 #define SVLG_LAUNCH_FUSED(D, P)                                                                        \
     do {                                                                                              \
@@ -619,8 +620,7 @@ static int cd_one_step(Model &m, int k) {
         hipLaunchKernelGGL(k_loads, dim3(1), dim3(64), 0, m.stream, k, m.nload, m.load_nt, m.ndim,
                            (int)m.h_load_dofs.size(), m.load_dofs.p, m.load_node.p, m.load_dir.p, m.load_factor.p,
                            m.load_series.p, m.Fext.p);
-    if (int rc = drm_forces(m, k, nullptr)) return rc;
-    if (int rc = drm_add_fext(m, m.Fext.p)) return rc;
+    if (int rc = drm_add_fext(m, k, m.Fext.p)) return rc;
     if (int rc = launch_fint_colored(m, Ucur, m.ftol, m.F.p)) return rc;
     int nb = (m.ndofn + 255) / 256;
     hipLaunchKernelGGL(k_cd_update, dim3(nb), dim3(256), 0, m.stream, m.ndofn, Ucur, Uprev, Unext, m.F.p, m.Fext.p,

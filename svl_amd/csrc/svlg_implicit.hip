@@ -72,7 +72,7 @@ struct OpArgs {
     const double *Vm, *dUn;
     const int *jdev;       // non-null: the iteration is *jdev + iter (graph replays)
     int nelem;             // elements (the Gauss-point state stride)
-    DrmRef drm;            // OP_RHS: domain reduction forces of step k (null ptr: none)
+    DrmRef drm;            // OP_RHS: domain reduction forces of every step (F null: none)
 };
 
 // One node tile of the effective-stiffness operator (one thread per tile element in the
@@ -248,7 +248,8 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
                         (a.k >= 0 && a.k < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + a.k] : 0.0;
                     fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + d]);
                 }
-                if (a.drm.ptr) fe += drm_node_force<DIM>(a.drm, node, d);
+                const int ds = drm_slot(a.drm, node);
+                if (ds >= 0) fe += drm_force(a.drm, ds, a.k, DIM, d);
                 const double v = sV[tid * DIM + d];
                 // Newton residual (Vm set): the staged V is V - 2/dt dU, the mass term uses
                 // the step's V and dU.
@@ -715,7 +716,6 @@ int launch_nl_rhs(Model &m, int k, const double *un, const double *w, const doub
     a.load_nt = m.load_nt;
     a.k = k;
     a.drm = drm_ref(m);
-    if (int rc = drm_forces(m, k, nullptr)) return rc;
     return launch_op(m, OP_RHS, a);
 }
 
@@ -847,7 +847,6 @@ int svlg_nm_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
         // Feff (ExtendedNewmarkBeta::ComputeEffectiveForce with the Linear algorithm's dU = 0).
         a.k = k0 + st;
         a.drm = drm_ref(m);
-        if (int rc = drm_forces(m, a.k, nullptr)) return rc;
         if (int rc = launch_op(m, OP_RHS, a)) return rc;
         // Keff dU = Feff; the previous increment is the initial guess.
         int it = 0;

@@ -36,7 +36,7 @@ struct ReacArgs {
     const int32_t *mat;
     const double *j2p, *state;  // null state: J2 elements as their initial elastic material
     int nelem;
-    DrmRef drm;                 // domain reduction forces of step k (null ptr: none)
+    DrmRef drm;                 // domain reduction forces of every step (F null: none)
     double *R;                  // [nq * DIM]
 };
 
@@ -107,7 +107,8 @@ __global__ void k_reactions(ReacArgs a) {
             const double amp = (a.k >= 0 && a.k < a.load_nt) ? a.load_series[size_t(l) * a.load_nt + a.k] : 0.0;
             fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + d]);
         }
-        if (a.drm.ptr) fe += drm_node_force<DIM>(a.drm, node, d);
+        const int ds = drm_slot(a.drm, node);
+        if (ds >= 0) fe += drm_force(a.drm, ds, a.k, DIM, d);
         a.R[size_t(q) * DIM + d] = ((r[d] + a.cdiag[g] * vv) + a.mdiag[g] * aa) - fe;
     }
 }
@@ -173,7 +174,6 @@ int svlg_reactions(svlg_model *h, int32_t n, const int32_t *nodes, int32_t k, do
     a.state = (m.active == 2 && m.has_j2 && m.nl_ready) ? m.gp_state.p : nullptr;
     a.nelem = m.nelem;
     a.drm = drm_ref(m);
-    if (int rc = drm_forces(m, k, nullptr)) return rc;
     a.R = rb.p;
     const int nb = (n + 63) / 64;
     if (m.ndim == 3)
