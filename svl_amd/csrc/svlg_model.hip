@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <numeric>
 #include <stdexcept>
 
 #include "svlg_common.hpp"
@@ -502,6 +503,29 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         if (m.variant == 3) {
             t.block = cblock;
             t.lds_bytes = corner_lds;
+            // Corner slots make the sums independent of the element order, so the tile's
+            // elements go in the order of their first corner's staged slot instead of by
+            // colour: neighbouring lanes then touch neighbouring LDS slots (fewer bank
+            // conflicts in the gathers and the corner-slot writes). The colour offsets are
+            // not used by the corner-slot kernels.
+            std::vector<int32_t> ord, id2;
+            std::vector<uint16_t> lc2;
+            for (int ti = 0; ti < ht.ntiles; ++ti) {
+                const int e0 = ht.elem_ptr[ti], e1 = ht.elem_ptr[ti + 1];
+                ord.resize(e1 - e0);
+                std::iota(ord.begin(), ord.end(), e0);
+                std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
+                    return ht.elem_lconn[size_t(a) * nen] < ht.elem_lconn[size_t(b) * nen];
+                });
+                id2.resize(e1 - e0);
+                lc2.resize(size_t(e1 - e0) * nen);
+                for (int i = 0; i < e1 - e0; ++i) {
+                    id2[i] = ht.elem_id[ord[i]];
+                    for (int a = 0; a < nen; ++a) lc2[size_t(i) * nen + a] = ht.elem_lconn[size_t(ord[i]) * nen + a];
+                }
+                std::copy(id2.begin(), id2.end(), ht.elem_id.begin() + e0);
+                std::copy(lc2.begin(), lc2.end(), ht.elem_lconn.begin() + size_t(e0) * nen);
+            }
         } else {
             t.lds_bytes = size_t(staged_arrays * ht.max_stage + ht.max_own) * ndim * sizeof(double);
         }
