@@ -406,7 +406,7 @@ def main():
                    "ntiles": info["ntiles"], "lds_bytes": info["lds_bytes"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profile(info["variant"], args.tile_nodes or 320),
+                     "traffic": traffic_from_profile(info["variant"], args.tile_nodes or 160),
                      "kernel": "k_cd_corner / k_cd_fused: one launch = one CentralDifference step",
                      "avg_launch_us": step_s * 1e6,
                      "alg_bytes_per_step": bytes_step},

@@ -84,7 +84,7 @@ typedef struct svlg_model svlg_model;
 /* Execution options (no counterpart in SVL): node-tile size of the fused explicit
  * kernel, workgroup size, and the explicit kernel variant. Zero fields take defaults. */
 typedef struct svlg_exec_opts {
-    int32_t tile_nodes;   /* max nodes owned by one workgroup (default 320 in 3D, 448 in 2D) */
+    int32_t tile_nodes;   /* max nodes owned by one workgroup (default 160 in 3D, 448 in 2D) */
     int32_t block;        /* threads per workgroup of the colour-phase fused kernel (default 512) */
     int32_t variant;      /* 0 auto: 3 when the mesh is corner-unique and a tile fits one round,
                            * else 1; 1 fused node tiles with colour phases, 2 element-colored

@@ -321,6 +321,8 @@ template <int DIM, int PATH>
 __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
     constexpr int NEN = Elem<DIM>::nen;
     constexpr int NB = 4;
+    // Corner-slot planes per pass: the MFMA element step writes all corners at once.
+    constexpr int NPASS = PATH == 4 ? 1 : 2, NH = NEN / NPASS;
     extern __shared__ double smem[];
     const int nt = a.ntiles;
     const int kstep = a.kdev ? *a.kdev + a.k : a.k;
@@ -424,75 +426,102 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
                 }
             }
         }
-        for (int i = tid; i < NEN * DIM * MO; i += B) sIn[i] = 0.0;
+        for (int i = tid; i < NH * DIM * MO; i += B) sIn[i] = 0.0;
     }
     __syncthreads();
+    double fi[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) fi[d] = 0.0;
     if constexpr (PATH == 4) {
         // BF16 MFMA element step (single element geometry), FP64 slots and update.
         bf16x8 Ak[3];
         mfma_load_kblocks(a.kmat, Ak);
         mfma_hexa_elements<double>(a.lconn, a.elam, a.emu, e0, ne, nown, sU, sIn, MO, Ak, a.ftol);
-    } else if (has_elem) {
-        double u[NEN][DIM], f[NEN][DIM];
+        __syncthreads();
+        if (has_node)
 #pragma unroll
-        for (int c = 0; c < NEN; ++c)
+            for (int d = 0; d < DIM; ++d)
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) u[c][d] = sU[lc[c] * DIM + d];
-        if constexpr (PATH >= 2) {
-            const double *g = a.geo + size_t(gid) * (DIM * DIM);
-            double Lam[DIM][DIM];
-#pragma unroll
-            for (int r2 = 0; r2 < DIM; ++r2)
-#pragma unroll
-                for (int c2 = 0; c2 < DIM; ++c2) Lam[r2][c2] = g[r2 * DIM + c2];
-            if constexpr (DIM == 3)
-                hexa_modal_force<PATH == 3>(Lam, lam, mu, u, f);
-            else
-                quad_modal_force<PATH == 3>(Lam, lam, mu, u, f);
-        } else {
-            double x[NEN][DIM];
+                for (int c = 0; c < NEN; ++c) fi[d] += sIn[(c * DIM + d) * MO + tid];
+    } else {
+        double f[NEN][DIM];
+        if (has_elem) {
+            double u[NEN][DIM];
 #pragma unroll
             for (int c = 0; c < NEN; ++c)
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) x[c][d] = sX[lc[c] * DIM + d];
-            if constexpr (DIM == 3) {
-                if constexpr (PATH == 1)
-                    hexa_affine_fint(x, u, lam, mu, f);
+                for (int d = 0; d < DIM; ++d) u[c][d] = sU[lc[c] * DIM + d];
+            if constexpr (PATH >= 2) {
+                const double *g = a.geo + size_t(gid) * (DIM * DIM);
+                double Lam[DIM][DIM];
+#pragma unroll
+                for (int r2 = 0; r2 < DIM; ++r2)
+#pragma unroll
+                    for (int c2 = 0; c2 < DIM; ++c2) Lam[r2][c2] = g[r2 * DIM + c2];
+                if constexpr (DIM == 3)
+                    hexa_modal_force<PATH == 3>(Lam, lam, mu, u, f);
                 else
-                    hexa_fint(x, u, lam, mu, f);
+                    quad_modal_force<PATH == 3>(Lam, lam, mu, u, f);
             } else {
-                if constexpr (PATH == 1)
-                    quad_affine_fint(x, u, lam, mu, a.th, f);
-                else
-                    quad_fint(x, u, lam, mu, a.th, f);
+                double x[NEN][DIM];
+#pragma unroll
+                for (int c = 0; c < NEN; ++c)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) x[c][d] = sX[lc[c] * DIM + d];
+                if constexpr (DIM == 3) {
+                    if constexpr (PATH == 1)
+                        hexa_affine_fint(x, u, lam, mu, f);
+                    else
+                        hexa_fint(x, u, lam, mu, f);
+                } else {
+                    if constexpr (PATH == 1)
+                        quad_affine_fint(x, u, lam, mu, a.th, f);
+                    else
+                        quad_fint(x, u, lam, mu, a.th, f);
+                }
             }
         }
-        // Branch-free scatter: corners of nodes this tile does not own write to the trash
-        // cell after the slots.
-        const int trash = NEN * DIM * MO;
+        // Two passes of NH corners through NH slot planes: elements write corners
+        // pass*NH .. pass*NH+NH-1 (branch-free: corners of nodes this tile does not own go to
+        // the trash cell after the slots), then every owned node adds its planes in corner
+        // order and clears them for the next pass. Same summation order as one pass over
+        // all corners, half the LDS.
+        const int trash = NH * DIM * MO;
 #pragma unroll
-        for (int c = 0; c < NEN; ++c) {
-            const bool own = lc[c] < nown;
+        for (int pass = 0; pass < NPASS; ++pass) {
+            if (pass > 0) __syncthreads();
+            if (has_elem)
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                const double v = f[c][d];
-                sIn[own ? (c * DIM + d) * MO + lc[c] : trash] = fabs(v) > a.ftol ? v : 0.0;
-            }
+                for (int c = 0; c < NH; ++c) {
+                    const int cc = pass * NH + c;
+                    const bool own = lc[cc] < nown;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        const double v = f[cc][d];
+                        sIn[own ? (c * DIM + d) * MO + lc[cc] : trash] = fabs(v) > a.ftol ? v : 0.0;
+                    }
+                }
+            __syncthreads();
+            if (has_node)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+#pragma unroll
+                    for (int c = 0; c < NH; ++c) {
+                        double &slot = sIn[(c * DIM + d) * MO + tid];
+                        fi[d] += slot;
+                        if (pass + 1 < NPASS) slot = 0.0;
+                    }
         }
     }
-    __syncthreads();
     if (!has_node) return;
     const int npl = a.nplain[tile];
     const int ds = tid >= npl ? drm_slot(a.drm, o0 + tid) : -1;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        double fi = 0.0;
-#pragma unroll
-        for (int c = 0; c < NEN; ++c) fi += sIn[(c * DIM + d) * MO + tid];
         const double u = sU[tid * DIM + d];
         const size_t g = g0 + size_t(tid) * DIM + d;
         if (tid < npl) {
-            a.Unext[g] = u + ((0.0 - fi) + kd[d] * (u - up[d])) / kd[d];
+            a.Unext[g] = u + ((0.0 - fi[d]) + kd[d] * (u - up[d])) / kd[d];
         } else if (kd[d] == 0.0) {
             a.Unext[g] = 0.0;
         } else {
@@ -505,7 +534,7 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
                 fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + d]);
             }
             if (ds >= 0) fe += drm_force(a.drm, ds, kstep, DIM, d);
-            a.Unext[g] = u + ((fe - fi) + a.kforce[g] * (u - up[d])) / kd[d];
+            a.Unext[g] = u + ((fe - fi[d]) + a.kforce[g] * (u - up[d])) / kd[d];
         }
     }
 }
@@ -536,6 +565,14 @@ static const void *fused_kernel(const Model &m) {
 
 int prepare_fused(Model &m) {
     if (m.variant != 1 && m.variant != 3) return 0;
+    if (m.variant == 3) {
+        // Staged U (and X unless the geometry table is used), the corner-slot planes of one
+        // pass (half the corners; all of them for the MFMA element step) and the trash cell.
+        const int staged = m.elem_path >= 2 ? 1 : 2;
+        const int planes = m.elem_bf16 ? m.nen : m.nen / 2;
+        m.tiles.lds_bytes = (size_t(staged) * m.tiles.max_stage * m.ndim + size_t(planes) * m.ndim * m.tiles.max_own + 1) *
+                            sizeof(double);
+    }
     SVLG_CHECK(m.tiles.block <= 512, "fused kernel block must be at most 512 threads");
     SVLG_CHECK(m.tiles.lds_bytes <= 160 * 1024, "node tile needs more than 160 KB of LDS; use smaller tiles");
     if (m.tiles.lds_bytes > 64 * 1024)

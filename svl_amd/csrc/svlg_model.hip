@@ -296,7 +296,7 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
     m.ndofn = nn * ndim;
     m.th = d->thickness;
     m.lumped = d->lumped != 0;
-    int tile_nodes = (o && o->tile_nodes > 0) ? o->tile_nodes : (ndim == 3 ? 320 : 448);
+    int tile_nodes = (o && o->tile_nodes > 0) ? o->tile_nodes : (ndim == 3 ? 160 : 448);
     int block = (o && o->block > 0) ? o->block : 512;
     const int req_variant = (o && o->variant > 0) ? o->variant : 0;
     if (req_variant < 0 || req_variant > 3) { set_error("variant must be 0 (auto), 1, 2 or 3"); return fail(11); }
@@ -677,7 +677,7 @@ int svlg_plan_tiles(const svlg_mesh_desc *d, int32_t tile_nodes, int64_t *stats)
     HostTiles t;
     try {
         make_tiles(ndim, nn, ne, nen, d->coords, d->conn, color.data(), ncolors, special.data(),
-                   tile_nodes > 0 ? tile_nodes : (ndim == 3 ? 320 : 448), t);
+                   tile_nodes > 0 ? tile_nodes : (ndim == 3 ? 160 : 448), t);
     } catch (const std::exception &ex) {
         set_error(std::string("tiling failed: ") + ex.what());
         return 11;
