@@ -9,6 +9,7 @@ import numpy as np
 
 from . import _capi as capi
 from .integrators import DeviceModel
+from .drm import attach
 from .mesh import Model, tags_for
 
 
@@ -22,8 +23,9 @@ def union(models) -> tuple[Model, np.ndarray, np.ndarray]:
         if (m.ndim, m.elem_type, m.thickness, m.massform_lumped) != (m0.ndim, m0.elem_type, m0.thickness,
                                                                      m0.massform_lumped):
             raise ValueError("models of a batch must share dimension, element type, thickness and mass form")
-        if m.drm_elem is not None and len(m.drm_elem):
-            raise ValueError("a batch takes models without a domain reduction load")
+    drm_parts = [m for m in models if m.drm_elem is not None and len(m.drm_elem)]
+    if len({m.drm_factor for m in drm_parts}) > 1:
+        raise ValueError("the domain reduction loads of a batch must share one factor")
     node_ptr = np.zeros(len(models) + 1, np.int32)
     free_ptr = np.zeros(len(models) + 1, np.int32)
     for i, m in enumerate(models):
@@ -54,10 +56,28 @@ def union(models) -> tuple[Model, np.ndarray, np.ndarray]:
               dash_node=np.concatenate([m.dash_node + node_ptr[i] for i, m in enumerate(models)]).astype(np.int32),
               dash_dir=cat(lambda m: m.dash_dir).astype(np.int32), dash_eta=cat(lambda m: m.dash_eta),
               load_node=np.concatenate([m.load_node + node_ptr[i] for i, m in enumerate(models)]).astype(np.int32),
-              load_dir=np.concatenate([m.load_dir.reshape(-1, m0.ndim) for m in models if m.load_node.size]),
+              load_dir=(np.concatenate([m.load_dir.reshape(-1, m0.ndim) for m in models if m.load_node.size])
+                        if any(m.load_node.size for m in models) else np.zeros((0, m0.ndim))),
               load_factor=cat(lambda m: m.load_factor),
               load_series=np.concatenate(series) if series else np.zeros((0, 1)),
               massform_lumped=m0.massform_lumped, name=f"batch_{len(models)}")
+    if drm_parts:
+        # One domain reduction load over every part's layer (parts share no node), element
+        # and node indices offset per part, signals padded with zero rows.
+        elem_off = np.concatenate([[0], np.cumsum([m.nelem for m in models])]).astype(np.int64)
+        nts = max(m.drm_signal.shape[1] for m in drm_parts)
+        el, nd, cd, sg = [], [], [], []
+        for i, m in enumerate(models):
+            if m.drm_elem is None or not len(m.drm_elem):
+                continue
+            el.append(np.asarray(m.drm_elem, np.int64) + elem_off[i])
+            nd.append(np.asarray(m.drm_node, np.int64) + node_ptr[i])
+            cd.append(np.asarray(m.drm_cond, np.int32))
+            pad = np.zeros((m.drm_node.size, nts, m.drm_signal.shape[2]))
+            pad[:, :m.drm_signal.shape[1]] = m.drm_signal
+            sg.append(pad)
+        attach(u, np.concatenate(el), np.concatenate(nd), np.concatenate(cd), np.concatenate(sg),
+               drm_parts[0].drm_factor)
     return u, node_ptr, free_ptr
 
 

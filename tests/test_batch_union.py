@@ -40,3 +40,30 @@ def test_realizations_follow_their_seeds():
     # Base forces: 2 eta_s of the bottom dashpot in x, one per bottom node.
     m = a[0]
     assert np.all(m.load_dir[:, 0] == 1.0) and np.all(m.load_factor > 0)
+
+
+def test_union_carries_domain_reduction_loads():
+    """Two DRM-loaded Quad4 models in one union: the oracle's run of the union equals the runs
+    of the parts (one DRM load over both layers, indices offset per part)."""
+    import oracle
+    from svl_amd import drm
+    from _meshes import drm_quad_model
+    dt, nt = 1e-3, 60
+    parts = []
+    for scale in (1.0, 0.8):
+        m, box = drm_quad_model(nt, dt)
+        m.E = m.E * scale
+        bg = oracle.run(m, oracle.CENTRAL_DIFFERENCE, dt, nt, list(range(m.nfree)))
+        m.load_node = m.load_node[:0]
+        m.load_dir, m.load_factor, m.load_series = np.zeros((0, 2)), np.zeros(0), np.zeros((0, 1))
+        e, n, c = drm.drm_box(m, *box)
+        drm.attach(m, e, n, c, drm.signals_from_history(m, n, nt - 10 * len(parts), bg["U"], lag=2), 1.5)
+        parts.append(m)
+    u, nptr, fptr = B.union(parts)
+    assert u.drm_elem.size == sum(p.drm_elem.size for p in parts)
+    assert u.drm_signal.shape[1] == nt
+    whole = oracle.run(u, oracle.CENTRAL_DIFFERENCE, dt, nt, list(range(u.nfree)))["U"]
+    for i, p in enumerate(parts):
+        own = oracle.run(p, oracle.CENTRAL_DIFFERENCE, dt, nt, list(range(p.nfree)))["U"]
+        assert np.abs(own).max() > 0
+        assert np.array_equal(whole[:, fptr[i]:fptr[i + 1]], own)
