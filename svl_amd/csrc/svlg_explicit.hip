@@ -317,7 +317,10 @@ __global__ __launch_bounds__(512) void k_cd_fused(FusedArgs a) {
 // owned node then adds its NEN slots in corner order. Two barriers per tile, no color
 // phases, and every global load that does not depend on staged data is issued before
 // the first barrier.
-template <int DIM, int PATH>
+// MOC > 0: every tile owns at most MOC nodes; the slot planes then have the compile-time
+// stride MOC + 1 with column MOC as the trash column, so that every slot address is one
+// per-corner base plus an immediate offset.
+template <int DIM, int PATH, int MOC = 0>
 __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
     constexpr int NEN = Elem<DIM>::nen;
     constexpr int NB = 4;
@@ -333,10 +336,10 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
     const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
     const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
     constexpr bool STAGE_X = PATH < 2;
-    const int MO = a.max_own;
+    const int MO = MOC > 0 ? MOC + 1 : a.max_own;
     double *sU = smem;
     double *sX = smem + size_t(a.max_stage) * DIM;
-    double *sIn = smem + size_t((STAGE_X ? 2 : 1) * a.max_stage) * DIM;  // [NEN*DIM][max_own]
+    double *sIn = smem + size_t((STAGE_X ? 2 : 1) * a.max_stage) * DIM;  // [NH*DIM][MO]
     const int tid = threadIdx.x, B = blockDim.x;
     const int nU = nown * DIM, nH = nh * DIM;
     const size_t g0 = size_t(o0) * DIM;
@@ -495,10 +498,15 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
                 for (int c = 0; c < NH; ++c) {
                     const int cc = pass * NH + c;
                     const bool own = lc[cc] < nown;
+                    const int col = own ? lc[cc] : MOC;
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) {
                         const double v = f[cc][d];
-                        sIn[own ? (c * DIM + d) * MO + lc[cc] : trash] = fabs(v) > a.ftol ? v : 0.0;
+                        const double fv = fabs(v) > a.ftol ? v : 0.0;
+                        if constexpr (MOC > 0)
+                            sIn[(c * DIM + d) * MO + col] = fv;
+                        else
+                            sIn[own ? (c * DIM + d) * MO + lc[cc] : trash] = fv;
                     }
                 }
             __syncthreads();
@@ -545,9 +553,17 @@ static const void *fused_fn(bool corner) {
                   : reinterpret_cast<const void *>(&k_cd_fused<DIM, PATH>);
 }
 
+// Owned-node bound of the compile-time slot stride (the default 3D tile size).
+static constexpr int kMocHexa = 160;
+
+static bool use_moc(const Model &m) {
+    return m.variant == 3 && !m.elem_bf16 && m.ndim == 3 && m.elem_path == 3 && m.tiles.max_own <= kMocHexa;
+}
+
 static const void *fused_kernel(const Model &m) {
     const bool cr = m.variant == 3;
     if (m.elem_bf16) return reinterpret_cast<const void *>(&k_cd_corner<3, 4>);
+    if (use_moc(m)) return reinterpret_cast<const void *>(&k_cd_corner<3, 3, kMocHexa>);
     if (m.ndim == 3)
         switch (m.elem_path) {
             case 3: return fused_fn<3, 3>(cr);
@@ -570,8 +586,11 @@ int prepare_fused(Model &m) {
         // pass (half the corners; all of them for the MFMA element step) and the trash cell.
         const int staged = m.elem_path >= 2 ? 1 : 2;
         const int planes = m.elem_bf16 ? m.nen : m.nen / 2;
-        m.tiles.lds_bytes = (size_t(staged) * m.tiles.max_stage * m.ndim + size_t(planes) * m.ndim * m.tiles.max_own + 1) *
-                            sizeof(double);
+        m.tiles.lds_bytes =
+            use_moc(m) ? (size_t(staged) * m.tiles.max_stage * m.ndim + size_t(planes) * m.ndim * (kMocHexa + 1)) *
+                             sizeof(double)
+                       : (size_t(staged) * m.tiles.max_stage * m.ndim + size_t(planes) * m.ndim * m.tiles.max_own + 1) *
+                             sizeof(double);
     }
     SVLG_CHECK(m.tiles.block <= 512, "fused kernel block must be at most 512 threads");
     SVLG_CHECK(m.tiles.lds_bytes <= 160 * 1024, "node tile needs more than 160 KB of LDS; use smaller tiles");
@@ -628,6 +647,8 @@ static int launch_fused(Model &m, int k, const double *Ucur, const double *Uprev
     a.kmat = m.cd_kmat.p;
     if (m.elem_bf16) {
         hipLaunchKernelGGL((k_cd_corner<3, 4>), grid, block, t.lds_bytes, m.stream, a);
+    } else if (use_moc(m)) {
+        hipLaunchKernelGGL((k_cd_corner<3, 3, kMocHexa>), grid, block, t.lds_bytes, m.stream, a);
     } else if (m.ndim == 3) {
         if (m.elem_path == 3) SVLG_LAUNCH_FUSED(3, 3);
         else if (m.elem_path == 2) SVLG_LAUNCH_FUSED(3, 2);

@@ -200,3 +200,22 @@ def test_batched_parts_with_own_dt_match_oracle():
     hist = cd.advance(1, nt - 1, rec_dofs=np.arange(u.nfree, dtype=np.int32))
     for i in range(len(parts)):
         assert rel_l2(hist[:, fptr[i]:fptr[i + 1]], refs[i]) <= 1e-10
+
+
+def test_short_signal_and_factor_match_oracle():
+    """Signals shorter than the run (rows past the end read as zero) and a load factor."""
+    dt, nt = 1e-3, 200
+    m, box = drm_quad_model(nt, dt)
+    m.ray_am = np.full(m.nelem, 0.3)
+    rec = list(range(m.nfree))
+    bg = oracle.run(m, oracle.CENTRAL_DIFFERENCE, dt, nt, rec)
+    r = reduced(m, box, nt // 2, bg["U"], lag=2)
+    r.drm_factor = 1.5
+    assert r.drm_signal.shape[1] == nt // 2
+    ref = oracle.run(r, oracle.CENTRAL_DIFFERENCE, dt, nt, rec)
+    dm = DeviceModel(r)
+    cd = CentralDifference(dm, dt)
+    cd.Initialize()
+    hist = cd.advance(1, nt - 1, rec)
+    assert np.abs(ref["U"][nt // 2:]).max() > 0
+    assert rel_l2(hist, ref["U"]) <= 1e-10
