@@ -268,7 +268,10 @@ __global__ __launch_bounds__(512) void k_gcn_layer(GcnArgs a) {
 // 16-bit staged slot of corner 2w + h from the packed connectivity (four 32-bit words).
 __device__ __forceinline__ int packed_half(uint32_t w, int h) { return int((w >> (16 * h)) & 0xffffu); }
 
-template <bool OUT>
+// MOC > 0: every tile owns at most MOC nodes; the slot planes then have the compile-time
+// stride MOC + 1 with column MOC as the trash column (one base address per corner, immediate
+// plane offsets).
+template <bool OUT, int MOC = 0>
 __global__ __launch_bounds__(512) void k_gcn_fast(GcnArgs a) {
     constexpr int MAXR = 4;  // element rounds per wave: tiles hold at most 512 elements
     extern __shared__ double smem[];
@@ -283,9 +286,9 @@ __global__ __launch_bounds__(512) void k_gcn_fast(GcnArgs a) {
     const int o0 = a.own_ptr[tile], nown = a.own_ptr[tile + 1] - o0;
     const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
     const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
-    const int MS = a.max_stage, MO = a.max_own;
-    // LDS: corner slots [24][MO] + one trash cell (every masked-out write lands there),
-    // staged operand [MS * 3] + one zero cell.
+    const int MS = a.max_stage, MO = MOC > 0 ? MOC + 1 : a.max_own;
+    // LDS: corner slots [24][MO] + one trash cell (every masked-out write lands there; with
+    // MOC the trash column), staged operand [MS * 3] + one zero cell.
     float *sIn = reinterpret_cast<float *>(smem);
     const int trash = 24 * MO;
     float *sX = sIn + size_t(24) * MO + 1;
@@ -356,8 +359,13 @@ __global__ __launch_bounds__(512) void k_gcn_fast(GcnArgs a) {
         const bool w0 = valid && s0 < nown, w1 = valid && s1 < nown;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            sIn[w0 ? (c0 * 3 + i) * MO + s0 : trash] = lamr[r] * l0[i] + mur[r] * m0[i];
-            sIn[w1 ? (c1 * 3 + i) * MO + s1 : trash] = lamr[r] * l1[i] + mur[r] * m1[i];
+            if constexpr (MOC > 0) {
+                sIn[(c0 * 3 + i) * MO + (w0 ? s0 : MOC)] = lamr[r] * l0[i] + mur[r] * m0[i];
+                sIn[(c1 * 3 + i) * MO + (w1 ? s1 : MOC)] = lamr[r] * l1[i] + mur[r] * m1[i];
+            } else {
+                sIn[w0 ? (c0 * 3 + i) * MO + s0 : trash] = lamr[r] * l0[i] + mur[r] * m0[i];
+                sIn[w1 ? (c1 * 3 + i) * MO + s1 : trash] = lamr[r] * l1[i] + mur[r] * m1[i];
+            }
         }
     }
     __syncthreads();
@@ -578,10 +586,33 @@ static size_t gcn_lds(const Model &m) {
     return b;
 }
 
-// Corner slots [24][max_own] + one trash cell, staged operand [max_stage * 3] + one zero cell
+// Owned-node bounds of the compile-time slot strides of the one-channel layer kernel (tiles
+// of up to 168 and up to 336 owned nodes); 0: the stride read from the tile set.
+static int gcn_moc(const Model &m) {
+    const int mo = m.tiles.max_own;
+    return mo <= 168 ? 168 : mo <= 336 ? 336 : 0;
+}
+
+// Corner slots [24][stride] + one trash cell, staged operand [max_stage * 3] + one zero cell
 // (below 40 KB for the default tiles, so four workgroups fit a CU).
 static size_t gcn_fast_lds(const Model &m) {
-    return (size_t(24) * m.tiles.max_own + 1 + size_t(m.tiles.max_stage) * 3 + 1) * sizeof(float);
+    const int moc = gcn_moc(m);
+    const size_t stride = moc ? moc + 1 : m.tiles.max_own;
+    return (size_t(24) * stride + 1 + size_t(m.tiles.max_stage) * 3 + 1) * sizeof(float);
+}
+
+static const void *gcn_fast_fn(const Model &m, bool out) {
+    switch (gcn_moc(m)) {
+        case 168:
+            return out ? reinterpret_cast<const void *>(&k_gcn_fast<true, 168>)
+                       : reinterpret_cast<const void *>(&k_gcn_fast<false, 168>);
+        case 336:
+            return out ? reinterpret_cast<const void *>(&k_gcn_fast<true, 336>)
+                       : reinterpret_cast<const void *>(&k_gcn_fast<false, 336>);
+        default:
+            return out ? reinterpret_cast<const void *>(&k_gcn_fast<true>)
+                       : reinterpret_cast<const void *>(&k_gcn_fast<false>);
+    }
 }
 
 template <int DIM, int PATH, bool MFMA>
@@ -694,8 +725,7 @@ static int gcn_run(Model &m, const CgParts &cg, int j, const double *r, const do
         if (m.gcn_mfma && C == 1) {
             a.kmat = reinterpret_cast<const float *>(m.gcn_kimg.p);
             void *args[] = {&a};
-            const void *fn = mode == G_OUT ? reinterpret_cast<const void *>(&k_gcn_fast<true>)
-                                           : reinterpret_cast<const void *>(&k_gcn_fast<false>);
+            const void *fn = gcn_fast_fn(m, mode == G_OUT);
             SVLG_HIP(hipLaunchKernel(fn, dim3(m.tiles.ntiles), dim3(m.tiles.block), args, gcn_fast_lds(m), m.stream));
         } else if (int rc = launch_gcn(m, mode, m.gcn_mfma != 0, a)) {
             return rc;
@@ -996,8 +1026,7 @@ int svlg_gnn_init(svlg_model *h, int32_t channels, int32_t layers, int32_t mfma,
     }
     if (int rc = upload_theta(m)) return rc;
     if (gcn_fast_lds(m) > 64 * 1024)
-        for (const void *fn : {reinterpret_cast<const void *>(&k_gcn_fast<true>),
-                               reinterpret_cast<const void *>(&k_gcn_fast<false>)})
+        for (const void *fn : {gcn_fast_fn(m, true), gcn_fast_fn(m, false)})
             SVLG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(gcn_fast_lds(m))));
     const size_t lds = gcn_lds(m);
     if (lds > 64 * 1024)
