@@ -84,7 +84,10 @@ struct OpArgs {
 //  NL: elements of J2 material use their Gauss-point states instead: the internal force from
 //      the stored stresses (OP_RHS) or the consistent tangent (OP_CG), each plus the Rayleigh
 //      term of the initial stiffness (coordinates are staged on every path).
-template <int DIM, int PATH, int MODE, bool NL>
+// MOC > 0: every tile owns at most MOC nodes; the slot planes then have the compile-time
+// stride MOC + 1 with column MOC as the trash column (branch-free corner writes, immediate
+// plane offsets).
+template <int DIM, int PATH, int MODE, bool NL, int MOC = 0>
 __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
     constexpr int NEN = Elem<DIM>::nen;
     constexpr bool STAGE_X = PATH < 2 || NL;  // J2 Gauss-point loops need the coordinates
@@ -132,7 +135,7 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
     const int o0 = a.own_ptr[tile], nown = a.own_ptr[tile + 1] - o0;
     const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
     const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
-    const int MS = a.max_stage, MO = a.max_own;
+    const int MS = a.max_stage, MO = MOC > 0 ? MOC + 1 : a.max_own;
     double *sU = smem;
     double *sV = smem + size_t(MS) * DIM;
     double *sX = smem + size_t((RHS ? 2 : 1) * MS) * DIM;
@@ -222,12 +225,21 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
                 }
             tile_elem_force<DIM, PATH>(a.geo, a.th, lc, sX, lam, mu, gid, u, f);
         }
+        if constexpr (MOC > 0) {
 #pragma unroll
-        for (int c = 0; c < NEN; ++c)
-            if (lc[c] < nown) {
+            for (int c = 0; c < NEN; ++c) {
+                const int col = lc[c] < nown ? lc[c] : MOC;
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) sIn[(c * DIM + d) * MO + lc[c]] = f[c][d];
+                for (int d = 0; d < DIM; ++d) sIn[(c * DIM + d) * MO + col] = f[c][d];
             }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NEN; ++c)
+                if (lc[c] < nown) {
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) sIn[(c * DIM + d) * MO + lc[c]] = f[c][d];
+                }
+        }
     }
     __syncthreads();
     double acc = 0.0;
@@ -490,9 +502,19 @@ static OpArgs op_args(const Model &m) {
     return a;
 }
 
+// Compile-time slot stride of the CG operator on axis-aligned Hexa8 tiles of up to 168 or 336
+// owned nodes (0: the stride read from the tile set).
+static int op_moc(const Model &m, int mode) {
+    if (mode != OP_CG || m.ndim != 3 || m.elem_path != 3) return 0;
+    const int mo = m.tiles.max_own;
+    return mo <= 168 ? 168 : mo <= 336 ? 336 : 0;
+}
+
 static size_t op_lds(const Model &m, int mode) {
     const int staged = (mode == OP_RHS ? 2 : 1) + (m.elem_path < 2 || m.has_j2 ? 1 : 0);
-    return (size_t(staged) * m.tiles.max_stage * m.ndim + size_t(m.nen) * m.ndim * m.tiles.max_own) * sizeof(double);
+    const int moc = op_moc(m, mode);
+    const size_t stride = moc ? moc + 1 : m.tiles.max_own;
+    return (size_t(staged) * m.tiles.max_stage * m.ndim + size_t(m.nen) * m.ndim * stride) * sizeof(double);
 }
 
 template <int DIM, int MODE>
@@ -512,6 +534,15 @@ static const void *op_fn_t(int path, bool nl) {
 }
 
 static const void *op_fn_sel(const Model &m, int mode, bool nl) {
+    switch (op_moc(m, mode)) {
+        case 168:
+            return nl ? reinterpret_cast<const void *>(&k_nm_op<3, 3, OP_CG, true, 168>)
+                      : reinterpret_cast<const void *>(&k_nm_op<3, 3, OP_CG, false, 168>);
+        case 336:
+            return nl ? reinterpret_cast<const void *>(&k_nm_op<3, 3, OP_CG, true, 336>)
+                      : reinterpret_cast<const void *>(&k_nm_op<3, 3, OP_CG, false, 336>);
+        default: break;
+    }
     if (m.ndim == 3)
         return mode == OP_CG ? op_fn_t<3, OP_CG>(m.elem_path, nl) : op_fn_t<3, OP_RHS>(m.elem_path, nl);
     return mode == OP_CG ? op_fn_t<2, OP_CG>(m.elem_path, nl) : op_fn_t<2, OP_RHS>(m.elem_path, nl);
