@@ -350,18 +350,38 @@ __global__ __launch_bounds__(512) void k_cg_update(int n, int j0, CgParts c, dou
         scal[SC_ALPHA] = alpha;
     }
     double rz = 0.0, rr = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        x[i] += alpha * p[i];
-        const double ri = r[i] - alpha * q[i];
-        r[i] = ri;
-        if (!c.flexible) {
-            const double zi = jac[i] * ri;
-            z[i] = zi;
-            rz += ri * zi;
-        } else {
-            gcn_input(g, n, i, ri);
+    // Grid-stride loop, four strides per pass with every load issued first (the per-thread
+    // summation order is that of the plain grid-stride loop).
+    constexpr int NB = 4;
+    const int S = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x + threadIdx.x; base < n; base += NB * S) {
+        double xv[NB], pv[NB], qv[NB], rv[NB], jv[NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const int i = base + k * S;
+            const bool in = i < n;
+            xv[k] = in ? x[i] : 0.0;
+            pv[k] = in ? p[i] : 0.0;
+            qv[k] = in ? q[i] : 0.0;
+            rv[k] = in ? r[i] : 0.0;
+            jv[k] = (in && !c.flexible) ? jac[i] : 0.0;
         }
-        rr += ri * ri;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const int i = base + k * S;
+            if (i >= n) break;
+            x[i] = xv[k] + alpha * pv[k];
+            const double ri = rv[k] - alpha * qv[k];
+            r[i] = ri;
+            if (!c.flexible) {
+                const double zi = jv[k] * ri;
+                z[i] = zi;
+                rz += ri * zi;
+            } else {
+                gcn_input(g, n, i, ri);
+            }
+            rr += ri * ri;
+        }
     }
     rz = block_sum(rz, red);
     rr = block_sum(rr, red);
