@@ -14,11 +14,12 @@
 //
 // The signals are fixed inputs and the DRM elements are elastic, so the nodal forces of
 // every step depend on nothing the time loop computes. The set-up evaluates all of them
-// once, a chunk of steps per launch pair (two threads per (step, DRM element) for the two
-// masked element forces; one thread per (step, DRM node) for the sum of its corners in
-// element-list order times the load factor), and keeps them as F[k][slot][dim] (slots in
-// device node order); the signals are then released. A step kernel adds F[k][slot] at a
-// DRM node where the point loads enter Fext: no launch in the time loop.
+// once, a chunk of signal rows per pass: the rows of the chunk are copied from the caller's
+// array (no host-side copy), then two threads per (row, DRM element) form the two masked
+// element forces and one thread per (row, DRM node) the sum of its corners in element-list
+// order times the load factor. The result stays as F[k][slot][dim] (slots in device node
+// order; a third of the signals' size); a step kernel adds F[k][slot] at a DRM node where
+// the point loads enter Fext: no launch in the time loop.
 #include <algorithm>
 #include <numeric>
 #include <vector>
@@ -30,9 +31,9 @@ namespace svlg {
 
 struct DrmArgs {
     int64_t nthreads;
-    int n, nnode, k0;
-    const int32_t *elem, *cslot, *ext, *conn;
-    const double *X, *lam, *mu, *ak, *sig;
+    int n, kc;
+    const int32_t *elem, *cslot, *ext, *uidx, *conn;
+    const double *X, *lam, *mu, *ak, *sig;  // sig: rows k0 .. k0+kc-1 of every node, [node][kc][3 dim]
     double th;
     double *fe;
 };
@@ -59,11 +60,13 @@ __global__ __launch_bounds__(64) void k_drm_forces(DrmArgs a) {
         const bool ext = a.ext[slot] != 0;
         const bool staged = side == 0 ? !ext : ext;
         keep[c] = !staged;
-        const double *row = a.sig + ((size_t(a.k0) + size_t(kk)) * a.nnode + slot) * NF;
+        // Exterior signals enter negated, as SVL negates them when it reads the node files.
+        const double sg = ext ? -1.0 : 1.0;
+        const double *row = a.sig + (size_t(a.uidx[slot]) * a.kc + size_t(kk)) * NF;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
             x[c][d] = a.X[nd + d];
-            w[c][d] = staged ? row[d] + ak * row[DIM + d] : 0.0;
+            w[c][d] = staged ? sg * row[d] + ak * (sg * row[DIM + d]) : 0.0;
         }
     }
     if constexpr (DIM == 3) {
@@ -163,40 +166,37 @@ int drm_setup(Model &m, const svlg_mesh_desc *d) {
         for (int a = 0; a < nen; ++a) idx[fill[cslot[size_t(j) * nen + a]]++] = j * nen + a;
     std::vector<int32_t> nslot(m.nnodes, -1);
     for (int s = 0; s < ns; ++s) nslot[dnode[s]] = s;
-    // Time-major signals, exterior ones negated (SVL negates them when it reads the files).
-    std::vector<double> sig(size_t(nt) * ns * nf);
-    for (int s = 0; s < ns; ++s) {
-        const int i = order[s];
-        const double sg = ext[s] ? -1.0 : 1.0;
-        for (int k = 0; k < nt; ++k)
-            for (int f = 0; f < nf; ++f)
-                sig[(size_t(k) * ns + s) * nf + f] = sg * d->drm_signal[(size_t(i) * nt + k) * nf + f];
-    }
     hipStream_t st = m.stream;
-    DevBuf<int32_t> delem, dcslot, dext, dptr, didx;
+    DevBuf<int32_t> delem, dcslot, dext, duidx, dptr, didx;
     DevBuf<double> dsig, fe;
     SVLG_HIP(delem.upload(d->drm_elem, ne, st));
     SVLG_HIP(dcslot.upload(cslot.data(), cslot.size(), st));
     SVLG_HIP(dext.upload(ext.data(), ext.size(), st));
+    SVLG_HIP(duidx.upload(order.data(), order.size(), st));
     SVLG_HIP(dptr.upload(cnt.data(), cnt.size(), st));
     SVLG_HIP(didx.upload(idx.data(), idx.size(), st));
-    SVLG_HIP(dsig.upload(sig.data(), sig.size(), st));
     SVLG_HIP(m.drm_F.alloc(size_t(nt) * ns * ndim));
-    // Steps per pass: element forces of at most 2^25 doubles (256 MiB) at a time.
-    const size_t per_step = size_t(ne) * nen * ndim;
+    // Signal rows per pass: element forces and signal rows of at most 2^25 doubles (256 MiB)
+    // each at a time. The caller's signals ([node][nt][3 dim]) are copied a block of rows per
+    // pass, with no host-side copy.
+    const size_t per_step = std::max(size_t(ne) * nen * ndim, size_t(ns) * nf);
     const int chunk = int(std::max<size_t>(1, std::min<size_t>(size_t(nt), (size_t(1) << 25) / per_step)));
-    SVLG_HIP(fe.alloc(size_t(chunk) * per_step));
-    SVLG_HIP(hipMemsetAsync(fe.p, 0, sizeof(double) * size_t(chunk) * per_step, st));
+    SVLG_HIP(fe.alloc(size_t(chunk) * ne * nen * ndim));
+    SVLG_HIP(hipMemsetAsync(fe.p, 0, sizeof(double) * size_t(chunk) * ne * nen * ndim, st));
+    SVLG_HIP(dsig.alloc(size_t(chunk) * ns * nf));
     for (int k0 = 0; k0 < nt; k0 += chunk) {
         const int kc = std::min(chunk, nt - k0);
+        SVLG_HIP(hipMemcpy2DAsync(dsig.p, sizeof(double) * size_t(kc) * nf, d->drm_signal + size_t(k0) * nf,
+                                  sizeof(double) * size_t(nt) * nf, sizeof(double) * size_t(kc) * nf, size_t(ns),
+                                  hipMemcpyHostToDevice, st));
         DrmArgs a{};
         a.nthreads = int64_t(kc) * 2 * ne;
         a.n = ne;
-        a.nnode = ns;
-        a.k0 = k0;
+        a.kc = kc;
         a.elem = delem.p;
         a.cslot = dcslot.p;
         a.ext = dext.p;
+        a.uidx = duidx.p;
         a.conn = m.conn.p;
         a.X = m.X.p;
         a.lam = m.lam.p;
