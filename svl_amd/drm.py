@@ -41,11 +41,14 @@ def signals_from_history(m: Model, nodes, nt: int, U, V=None, A=None, lag: int =
     state of step k - 1. Restrained DOFs read zero."""
     nd = m.ndim
     fd = m.free_dof()
+    nodes = np.asarray(nodes, dtype=np.int64)
     if cols is not None:
         pos = np.full(m.nfree, -1, np.int64)
         pos[np.asarray(cols, dtype=np.int64)] = np.arange(len(cols))
+        need = fd[nodes]
+        if np.any((need >= 0) & (pos[np.maximum(need, 0)] < 0)):
+            raise ValueError("every free DOF of the signal nodes needs a history column")
         fd = np.where(fd >= 0, pos[np.maximum(fd, 0)], -1)
-    nodes = np.asarray(nodes, dtype=np.int64)
     sig = np.zeros((nodes.size, nt, 3 * nd))
     for f, H in enumerate((U, V, A)):
         if H is None:
