@@ -82,3 +82,34 @@ def test_run_file_through_device(tmp_path, kind):
         err = rel(vals[:, free], ref[key])
         print(f"{kind} {fname}: rel L2 {err:.2e}")
         assert err < tol
+
+
+def test_drm_run_file_through_device(tmp_path):
+    """A DRM-loaded model (ElementLoad GENERALWAVE, one signal file per node) written, read
+    and run on the device: DISP against the oracle's run of the original model."""
+    import os
+    from svl_amd import drm
+    from _meshes import drm_quad_model
+    dt, nt = 2e-3, 60
+    m, box = drm_quad_model(nt, dt)
+    m.ray_am = np.full(m.nelem, 0.4)
+    m.ray_ak = np.full(m.nelem, 3e-4)
+    bg = oracle.run(m, oracle.EXTENDED_NEWMARK, dt, nt, list(range(m.nfree)), want_va=True)
+    m.load_node = m.load_node[:0]
+    m.load_dir, m.load_factor, m.load_series = np.zeros((0, 2)), np.zeros(0), np.zeros((0, 1))
+    e, n, c = drm.drm_box(m, *box)
+    drm.attach(m, e, n, c, drm.signals_from_history(m, n, nt, bg["U"], bg["V"], bg["A"], 1))
+    nodes = [int(n[0]), int(n[-1]), m.nnodes // 2]
+    ntags = [k + 1 for k in nodes]
+    p = str(tmp_path / "run" / "model.json")
+    os.makedirs(os.path.dirname(p))
+    S.write_run(m, p, "EXTENDEDNEWMARK", dt, nt, recorders=[("U.out", "DISP", ntags, 14)])
+    out_dir = str(tmp_path / "Solution")
+    S.run(S.read_run(p), out_dir=out_dir, cg_rtol=1e-13)
+    fd = m.free_dof()
+    cols = [int(fd[k][d]) for k in nodes for d in range(2)]
+    ref = oracle.run(m, oracle.EXTENDED_NEWMARK, dt, nt, [f for f in cols if f >= 0])
+    tags, tot, ntotal, ntr, vals = S.read_node_recorder(os.path.join(out_dir, "U.out"))
+    free = [i for i, f in enumerate(cols) if f >= 0]
+    assert np.abs(ref["U"]).max() > 0
+    assert rel(vals[:, free], ref["U"]) < 1e-10
