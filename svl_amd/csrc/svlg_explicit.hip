@@ -553,17 +553,29 @@ static const void *fused_fn(bool corner) {
                   : reinterpret_cast<const void *>(&k_cd_fused<DIM, PATH>);
 }
 
-// Owned-node bound of the compile-time slot stride (the default 3D tile size).
-static constexpr int kMocHexa = 160;
+// Owned-node bound of the compile-time slot stride of the corner-slot step on axis-aligned
+// meshes (0: the stride read from the tile set): 160 (the default 3D tile size) for Hexa8,
+// 192 and 448 (the batched and the default 2D tile sizes) for Quad4.
+static int use_moc(const Model &m) {
+    if (m.variant != 3 || m.elem_bf16 || m.elem_path != 3) return 0;
+    const int mo = m.tiles.max_own;
+    if (m.ndim == 3) return mo <= 160 ? 160 : 0;
+    return mo <= 192 ? 192 : mo <= 448 ? 448 : 0;
+}
 
-static bool use_moc(const Model &m) {
-    return m.variant == 3 && !m.elem_bf16 && m.ndim == 3 && m.elem_path == 3 && m.tiles.max_own <= kMocHexa;
+static const void *moc_kernel(const Model &m) {
+    switch (m.ndim * 1000 + use_moc(m)) {
+        case 3160: return reinterpret_cast<const void *>(&k_cd_corner<3, 3, 160>);
+        case 2192: return reinterpret_cast<const void *>(&k_cd_corner<2, 3, 192>);
+        case 2448: return reinterpret_cast<const void *>(&k_cd_corner<2, 3, 448>);
+        default: return nullptr;
+    }
 }
 
 static const void *fused_kernel(const Model &m) {
     const bool cr = m.variant == 3;
     if (m.elem_bf16) return reinterpret_cast<const void *>(&k_cd_corner<3, 4>);
-    if (use_moc(m)) return reinterpret_cast<const void *>(&k_cd_corner<3, 3, kMocHexa>);
+    if (use_moc(m)) return moc_kernel(m);
     if (m.ndim == 3)
         switch (m.elem_path) {
             case 3: return fused_fn<3, 3>(cr);
@@ -587,7 +599,7 @@ int prepare_fused(Model &m) {
         const int staged = m.elem_path >= 2 ? 1 : 2;
         const int planes = m.elem_bf16 ? m.nen : m.nen / 2;
         m.tiles.lds_bytes =
-            use_moc(m) ? (size_t(staged) * m.tiles.max_stage * m.ndim + size_t(planes) * m.ndim * (kMocHexa + 1)) *
+            use_moc(m) ? (size_t(staged) * m.tiles.max_stage * m.ndim + size_t(planes) * m.ndim * (use_moc(m) + 1)) *
                              sizeof(double)
                        : (size_t(staged) * m.tiles.max_stage * m.ndim + size_t(planes) * m.ndim * m.tiles.max_own + 1) *
                              sizeof(double);
@@ -648,7 +660,8 @@ static int launch_fused(Model &m, int k, const double *Ucur, const double *Uprev
     if (m.elem_bf16) {
         hipLaunchKernelGGL((k_cd_corner<3, 4>), grid, block, t.lds_bytes, m.stream, a);
     } else if (use_moc(m)) {
-        hipLaunchKernelGGL((k_cd_corner<3, 3, kMocHexa>), grid, block, t.lds_bytes, m.stream, a);
+        void *args[] = {&a};
+        SVLG_HIP(hipLaunchKernel(moc_kernel(m), grid, block, args, t.lds_bytes, m.stream));
     } else if (m.ndim == 3) {
         if (m.elem_path == 3) SVLG_LAUNCH_FUSED(3, 3);
         else if (m.elem_path == 2) SVLG_LAUNCH_FUSED(3, 2);
