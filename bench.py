@@ -98,6 +98,9 @@ def traffic_from_profile(variant: int, tile_nodes: int, name: str = "halfspace3d
         if line.startswith("traffic_"):
             k, _, v = line.partition(":")
             vals[k.strip()] = v.strip()
+    # Several tile sizes: one "traffic_bytes_t<tile nodes>: <bytes>" line each.
+    if f"traffic_bytes_t{tile_nodes}" in vals and int(vals.get("traffic_variant", variant)) == variant:
+        return float(vals[f"traffic_bytes_t{tile_nodes}"])
     try:
         if int(vals.get("traffic_variant", variant)) == variant and int(vals["traffic_tile_nodes"]) == tile_nodes:
             return float(vals["traffic_bytes_per_launch"])
@@ -231,8 +234,11 @@ def batched_bench(args):
     u, nptr, fptr = B.union(models)
     build_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    best, times = B.autotune_tile_nodes(u, nptr, dts, candidates=(96, 128, 160, 192, 256, 320, 448, 640),
-                                        steps=20)
+    if args.batch_tile_nodes > 0:
+        best, times = args.batch_tile_nodes, {}
+    else:
+        best, times = B.autotune_tile_nodes(u, nptr, dts, candidates=(96, 128, 160, 192, 256, 320, 448, 640),
+                                            steps=20)
     tune_s = time.perf_counter() - t0
     dm = DeviceModel(u, tile_nodes=best)
     info = dm.info()
@@ -342,6 +348,7 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=30)
     ap.add_argument("--batch-graph-steps", type=int, default=30)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-tile-nodes", type=int, default=0, help="fixed node-tile size (0: autotune)")
     ap.add_argument("--drm-steps", type=int, default=300)
     ap.add_argument("--drm-warmup", type=int, default=30)
     ap.add_argument("--no-drm", action="store_true")
