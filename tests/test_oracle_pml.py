@@ -1,0 +1,163 @@
+"""Pins the numpy restatement of PML2DQuad4 and of the mixed-unknown models around it
+(oracle/pml2d.py). SVL ships no output for its PML cases (F11 and J12 hold inputs only), so
+the pins are mathematical and physical:
+
+* symmetry of the element mass, damping and stiffness matrices;
+* without stretching (R = 1: ax = ay = 1, bx = by = 0) on the unit square the stiffness is zero
+  and every block has its closed form: rho/36 [4 2 1 2 ...] for the displacement mass, the
+  compliance constants times the same pattern for the stress unknowns, and the integrals of
+  dN_j/dx N_k (entries of +-1/6, +-1/12) coupling displacements and stresses in the damping;
+* with a constant stretching (m = 0) the three matrices scale by ax ay, ax by + ay bx and
+  bx by block by block;
+* the lin2DQuad4 matrices agree with the C++ oracle's (themselves pinned by closed forms);
+* physics: a 1 x 100 soil column driven at the top (SVL's F11 layout: 10 PML elements below
+  90 elastic ones, joined by EQUAL constraints, consistent mass, ExtendedNewmarkBeta) follows
+  a 400-element column — free of reflections within the window — far more closely than the
+  same 90 elements on a fixed base. A sign or coefficient mistake in the layer breaks it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import pml2d as P
+from svl_amd.mesh import ELEM_QUAD4, Model, tags_for
+
+UNIT = np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]])
+PAT = np.array([[4, 2, 1, 2], [2, 4, 2, 1], [1, 2, 4, 2], [2, 1, 2, 4]]) / 36.0
+
+
+def test_pml_matrices_symmetric():
+    X = np.array([[0.1, 0.0], [1.2, 0.1], [1.0, 1.3], [-0.1, 0.9]])
+    M, C, K = P.pml_matrices(X, 5e7, 0.25, 2000.0, (1.0, 2.0, 5.0, 1e-5, 0.5, 10.0, 0.0, -1.0))
+    for A in (M, C, K):
+        assert np.abs(A - A.T).max() <= 1e-12 * np.abs(A).max()
+        assert np.abs(A).max() > 0
+
+
+def test_closed_forms_without_stretching():
+    E, nu, rho = 5e7, 0.25, 2000.0
+    lam, mu = P.lame(E, nu)
+    M, C, K = P.pml_matrices(UNIT, E, nu, rho, (1.0, 2.0, 5.0, 1.0, 0.0, 0.0, 0.0, 1.0))
+    assert np.abs(K).max() == 0.0
+    np.testing.assert_allclose(M[0::5, 0::5], rho * PAT, rtol=1e-14)
+    np.testing.assert_allclose(M[2::5, 2::5], -(lam + 2 * mu) / (4 * mu * (lam + mu)) * PAT, rtol=1e-14)
+    np.testing.assert_allclose(M[4::5, 4::5], -PAT / mu, rtol=1e-14)
+    np.testing.assert_allclose(M[2::5, 3::5], lam / (4 * mu * (lam + mu)) * PAT, rtol=1e-14)
+    # int dN_j/dx N_k and int dN_j/dy N_k over the unit square, node order (0,0) (1,0) (1,1)
+    # (0,1): dN/dx = (-(1-y), 1-y, y, -y) and dN/dy = (-(1-x), -x, x, 1-x), hence entries of
+    # +-1/6 where N_j and N_k share the factor in the other coordinate and +-1/12 elsewhere.
+    Nx = np.array([[-2, -2, -1, -1], [2, 2, 1, 1], [1, 1, 2, 2], [-1, -1, -2, -2]], float) / 12.0
+    Ny = np.array([[-2, -1, -1, -2], [-1, -2, -2, -1], [1, 2, 2, 1], [2, 1, 1, 2]], float) / 12.0
+    for (i, j), B in (((0, 2), Nx), ((0, 4), Ny), ((1, 3), Ny), ((1, 4), Nx)):
+        np.testing.assert_allclose(C[i::5, j::5], B, rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(C[j::5, i::5], B.T, rtol=1e-13, atol=1e-15)
+    assert np.abs(C[0::5, 0::5]).max() == 0.0 and np.abs(C[2::5, 2::5]).max() == 0.0
+
+
+def test_constant_stretching_scales_blocks():
+    E, nu, rho = 5e7, 0.25, 2000.0
+    lam, mu = P.lame(E, nu)
+    p = (1.0, 0.0, 5.0, 1e-3, 0.0, 0.0, 1.0, 1.0)   # m = 0: f = 1 everywhere
+    ax, ay, bx, by = P.stretch(0.3, 0.4, rho, mu, lam, p)
+    assert ax == ay and bx == by and bx > 0
+    M, C, K = P.pml_matrices(UNIT, E, nu, rho, p)
+    M0, C0, K0 = P.pml_matrices(UNIT, E, nu, rho, (1.0, 0.0, 5.0, 1.0, 0.0, 0.0, 1.0, 1.0))
+    diag = np.zeros((20, 20), bool)
+    for a in range(5):
+        for b in range(5):
+            if (a < 2) == (b < 2):
+                diag[a::5, b::5] = True
+    np.testing.assert_allclose(M[diag], ax * ay * M0[diag], rtol=1e-13, atol=1e-18)
+    np.testing.assert_allclose(C[diag], (ax * by + ay * bx) * M0[diag], rtol=1e-13, atol=1e-18)
+    np.testing.assert_allclose(K[diag], bx * by * M0[diag], rtol=1e-13, atol=1e-18)
+    np.testing.assert_allclose(C[~diag], ax * C0[~diag], rtol=1e-13, atol=1e-18)
+    np.testing.assert_allclose(K[~diag], bx * C0[~diag], rtol=1e-13, atol=1e-18)
+
+
+@pytest.mark.parametrize("lumped", [False, True])
+def test_quad_matrices_match_oracle(lumped):
+    X = np.array([[0.1, 0.0], [1.2, 0.1], [1.0, 1.3], [-0.1, 0.9]])
+    m = Model(ndim=2, coords=X, node_tags=tags_for(4), elem_type=ELEM_QUAD4, conn=np.array([[0, 1, 2, 3]]),
+              E=np.array([3e7]), nu=np.array([0.3]), rho=np.array([1800.0]), thickness=0.7)
+    m.massform_lumped = lumped
+    K, M, _ = oracle.element_matrices(m, 0)
+    Kp, Mp = P.quad_matrices(X, 3e7, 0.3, 1800.0, 0.7, lumped)
+    np.testing.assert_allclose(Kp, K, rtol=1e-12, atol=1e-12 * np.abs(K).max())
+    np.testing.assert_allclose(Mp, M, rtol=1e-12, atol=1e-12 * np.abs(M).max())
+
+
+def ricker(t, t0=0.3, f0=5.0):
+    a = (math.pi * f0 * (t - t0)) ** 2
+    return (1 - 2 * a) * np.exp(-a)
+
+
+def column(n_reg, n_pml, y_base, R=1e-5, L=5.0):
+    """1 x n column of unit squares: n_pml PML elements (5 unknowns per node) from y_base, then
+    n_reg elastic ones; ux fixed everywhere; the two regions joined by EQUAL constraints on uy.
+    Without PML the base is fixed."""
+    E, nu, rho = 5e7, 0.25, 2000.0
+    coords, ndof, fixed, elems, equal = [], [], [], [], []
+    p = (1.0, 2.0, float(L), R, 0.5, y_base + n_pml, 0.0, -1.0)
+    if n_pml:
+        for j in range(n_pml + 1):
+            for i in range(2):
+                coords.append([float(i), y_base + j])
+                ndof.append(5)
+                fixed.append([True, j == 0, False, False, False])
+        for j in range(n_pml):
+            b = 2 * j
+            elems.append(("pml", (b, b + 1, b + 3, b + 2), E, nu, rho, p))
+    r0 = len(coords)
+    ybot = y_base + n_pml
+    for j in range(n_reg + 1):
+        for i in range(2):
+            coords.append([float(i), ybot + j])
+            ndof.append(2)
+            fixed.append([True, j == 0 and not n_pml])
+    for j in range(n_reg):
+        b = r0 + 2 * j
+        elems.append(("quad", (b, b + 1, b + 3, b + 2), E, nu, rho, (1.0,)))
+    if n_pml:
+        top = 2 * n_pml
+        equal = [(top, 1, r0, 1, 1.0), (top + 1, 1, r0 + 1, 1, 1.0)]
+    m = P.MixedModel(np.array(coords), ndof, fixed, elems, equal, lumped=False)
+    return m, len(coords) - 2
+
+
+def test_pml_column_absorbs():
+    dt, nt = 0.01, 200
+    s = 1e5 * ricker(np.arange(nt) * dt)
+    runs = {}
+    for name, (n_reg, n_pml, y0) in {"pml": (90, 10, 0.0), "fixed": (90, 0, 10.0), "long": (400, 0, -300.0)}.items():
+        m, top = column(n_reg, n_pml, y0)
+        m.loads = [(top, 1, 1.0, s), (top + 1, 1, 1.0, s)]
+        runs[name] = m.run_newmark(dt, nt, [(top, 1)])[:, 0]
+    ref = runs["long"]
+    e_pml = np.linalg.norm(runs["pml"] - ref) / np.linalg.norm(ref)
+    e_fix = np.linalg.norm(runs["fixed"] - ref) / np.linalg.norm(ref)
+    print(f"PML column error {e_pml:.3e}, fixed-base column error {e_fix:.3e}")
+    assert np.all(np.isfinite(runs["pml"]))
+    assert e_fix > 0.3
+    assert e_pml < 0.1 * e_fix
+
+
+def test_pml_reflection_coefficient():
+    """A wave crossing a continuous PML of depth L twice (fixed end) returns with amplitude
+    exp(-2/vp int_0^L b) = R for the profile b0 (d/L)^m, b0 = (m+1) vp ln(1/R) / (2L); the free
+    surface doubles it, so the top displacement differs from an unbounded column's by 2R of its
+    peak. Checked at R = 0.1 and 0.01 (a 10-element layer, 1 m elements, 10 Hz Ricker pulse);
+    below that the mesh's own reflection dominates."""
+    dt, nt = 0.005, 200
+    s = 1e5 * ricker(np.arange(nt) * dt, t0=0.15, f0=10.0)
+    m, top = column(300, 0, -260.0)
+    m.loads = [(top, 1, 1.0, s), (top + 1, 1, 1.0, s)]
+    ref = m.run_newmark(dt, nt, [(top, 1)])[:, 0]
+    for R in (1e-1, 1e-2):
+        m, top = column(40, 10, 0.0, R=R, L=10.0)
+        m.loads = [(top, 1, 1.0, s), (top + 1, 1, 1.0, s)]
+        u = m.run_newmark(dt, nt, [(top, 1)])[:, 0]
+        ratio = np.abs(u - ref).max() / np.abs(ref).max() / (2 * R)
+        print(f"R={R}: reflected / (2R incident) = {ratio:.3f}")
+        assert 0.9 < ratio < 1.25
