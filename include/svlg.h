@@ -281,6 +281,54 @@ int svlg_gnn_train(svlg_model *m, int32_t nsteps, int32_t every, int32_t sweeps,
  * device sustains. `dst` and `src` are device pointers. */
 int svlg_stream_copy(double *dst, const double *src, int64_t n, void *stream);
 
+/* ---- Perfectly matched layer, 2D (PML2DQuad4 + lin2DQuad4 + EQUAL constraints) -------
+ * A mixed model: lin2DQuad4 nodes carry 2 unknowns (ux, uy), PML2DQuad4 nodes 5 (ux, uy and
+ * the stress-like xx, yy, xy; 04-Elements/06-Quadrilateral/PML2DQuad4.cpp:11-12, 233-466),
+ * the two regions tied by EQUAL constraints (06-Mesh/Mesh.cpp:328-383). Integrated with
+ * ExtendedNewmarkBeta + Linear (10-Integrators/ExtendedNewmarkBeta.cpp:123-154, 208-245; the
+ * 2D element has no history matrix, so G and Ubar vanish). The effective matrix is symmetric
+ * and indefinite (negative stress blocks): the device assembles it once, scales it
+ * symmetrically by |diag|^-1/2 and solves with MINRES where SVL factorises it with Eigen
+ * (11-Solvers/EigenSolver.cpp). */
+#define SVLG_CLASS_LIN2DQUAD4 0
+#define SVLG_CLASS_PML2DQUAD4 1
+
+typedef struct svlg_mixed_desc {
+    int32_t nnodes;
+    const double *coords;      /* [nnodes][2] */
+    const int32_t *dof_ptr;    /* [nnodes+1] first total DOF of each node (2 or 5 per node) */
+    const int32_t *dof_free;   /* [ntotal] free DOF of each total DOF, -1 when restrained */
+    const double *dof_factor;  /* [ntotal] its entry of T: 1, or the factor of an EQUAL slave */
+    int32_t nelem;
+    const int32_t *elem_class; /* [nelem] SVLG_CLASS_* */
+    const int32_t *conn;       /* [nelem][4] node indices, counter-clockwise */
+    const double *E, *nu, *rho, *th; /* [nelem] plane strain material and thickness */
+    const double *pml;         /* [nelem][7] m, L, R, x0, y0, nx, ny (read for PML elements) */
+    int32_t lumped;            /* lin2DQuad4 mass: 1 row-sum lumped, 0 consistent; the PML
+                                  mass is always consistent (PML2DQuad4.cpp:286) */
+    int32_t nload, load_nt;    /* POINTLOAD_CONCENTRATED_DYNAMIC on total DOFs */
+    const int32_t *load_dof;   /* [nload] */
+    const double *load_factor; /* [nload] */
+    const double *load_series; /* [nload][load_nt] value of step k at column k */
+} svlg_mixed_desc;
+
+typedef struct svlg_pml svlg_pml;
+
+/* Computes the element matrices, assembles M, C, K and Keff = K + 4/dt^2 M + 2/dt C over the
+ * free DOFs (T^T A T) on the device. Replaces Assembler::ComputeMassMatrix / DampingMatrix /
+ * StiffnessMatrix (Assembler.hpp:50-181) and ExtendedNewmarkBeta::Initialize for this model. */
+int svlg_pml_create(const svlg_mixed_desc *desc, double dt, double rtol, int32_t maxit, svlg_pml **out);
+int svlg_pml_destroy(svlg_pml *p);
+/* info4 = nfree, ntotal, nonzeros of the free-space matrices, element-matrix contributions. */
+int svlg_pml_info(svlg_pml *p, int64_t *info4);
+/* Steps k0 .. k0+nsteps-1 (ExtendedNewmarkBeta::ComputeNewStep, load sample k of step k).
+ * Records U, V, A (each pointer may be NULL) of the total DOFs rec_dofs after every step, row
+ * s for step k0+s, and the MINRES iterations per step. Stop when a solve misses rtol within
+ * maxit iterations. */
+int svlg_pml_steps(svlg_pml *p, int32_t k0, int32_t nsteps, int32_t nrec, const int32_t *rec_dofs,
+                   double *histU, double *histV, double *histA, int32_t *iters);
+int svlg_pml_get_state(svlg_pml *p, double *U, double *V, double *A /* [nfree] each */);
+
 #ifdef __cplusplus
 }
 #endif

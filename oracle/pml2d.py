@@ -15,7 +15,7 @@ dense models:
 * assembly in element order, the total-to-free map with EQUAL constraints (a slave unknown
   takes the master's free number times the factor, RP/06-Mesh/Mesh.cpp:328-383) and
   ExtendedNewmarkBeta with the Linear algorithm and G = 0 (ExtendedNewmarkBeta.cpp:123-154,
-  208-245), solved densely.
+  208-245), solved densely after a symmetric diagonal scaling (keff_solver).
 
 Pins: tests/test_oracle_pml.py (symmetry, closed forms without stretching, constant-stretching
 scaling, the lin2DQuad4 matrices against the C++ oracle, a soil column whose PML-truncated
@@ -174,22 +174,38 @@ class MixedModel:
             K[ix] += Ke
         return off, T, M, C, K
 
-    def run_newmark(self, dt, nt, rec):
-        """ExtendedNewmarkBeta (G = 0) + Linear, steps k = 1..nt-1 with load sample k; returns
-        the total-DOF histories of `rec` [nt-1, len(rec)] (row k-1 after step k)."""
+    def keff_matrix(self, dt):
+        off, T, M, C, K = self.assemble()
+        return T.T @ (K + 4 / dt ** 2 * M + 2 / dt * C) @ T
+
+    def keff_solver(self, dt):
+        """Dense solve of Keff x = b after symmetric scaling by |diag|^-1/2. Keff mixes
+        rho/dt^2 on the displacements with compliance/dt^2 on the stress unknowns, eleven
+        orders of magnitude apart: LU of the raw matrix (condition ~5e12 on SVL's F11 column)
+        loses ~1e-11 per solve, the scaled one (condition ~20) keeps ~1e-16."""
+        Keff = self.keff_matrix(dt)
+        S = 1.0 / np.sqrt(np.abs(np.diag(Keff)))
+        As = Keff * S[:, None] * S[None, :]
+        return lambda b: S * np.linalg.solve(As, S * b)
+
+    def run_newmark(self, dt, nt, rec, k0=1, uva=False):
+        """ExtendedNewmarkBeta (G = 0) + Linear, steps k = k0..nt-1 with load sample k; returns
+        the total-DOF histories of `rec` [nt-k0, len(rec)] (row k-k0 after step k), or the
+        (U, V, A) histories when `uva`."""
         off, T, M, C, K = self.assemble()
         ntot = T.shape[0]
-        Keff = T.T @ (K + 4 / dt ** 2 * M + 2 / dt * C) @ T
+        solve = self.keff_solver(dt)
         U, V, A = np.zeros(ntot), np.zeros(ntot), np.zeros(ntot)
-        out = np.zeros((nt - 1, len(rec)))
-        for k in range(1, nt):
+        idx = [off[n] + a for n, a in rec]
+        out = np.zeros((3, nt - k0, len(rec)))
+        for k in range(k0, nt):
             F = np.zeros(ntot)
             for n, a, fac, series in self.loads:
                 F[off[n] + a] += fac * (series[k] if k < len(series) else 0.0)
             Feff = T.T @ (F - K @ U + C @ V + M @ (4 / dt * V + A))
-            dU = T @ np.linalg.solve(Keff, Feff)
+            dU = T @ solve(Feff)
             U = U + dU
             A = 4 / dt ** 2 * dU - 4 / dt * V - A
             V = 2 / dt * dU - V
-            out[k - 1] = U[[off[n] + a for n, a in rec]]
-        return out
+            out[:, k - k0] = U[idx], V[idx], A[idx]
+        return out if uva else out[0]

@@ -165,6 +165,11 @@ SIGNATURES = {
     "svlg_gnn_loss_grad": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p, f64p, f64p]),
     "svlg_gnn_train": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, f64p, i32p]),
     "svlg_stream_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "svlg_pml_create": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.POINTER(C.c_void_p)]),
+    "svlg_pml_destroy": (C.c_int, [C.c_void_p]),
+    "svlg_pml_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "svlg_pml_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, i32p, f64p, f64p, f64p, i32p]),
+    "svlg_pml_get_state": (C.c_int, [C.c_void_p, f64p, f64p, f64p]),
 }
 
 _LIB = None

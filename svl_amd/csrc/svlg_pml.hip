@@ -1,0 +1,651 @@
+// svlg_pml.hip -- 2D perfectly matched layer on the device: PML2DQuad4 and lin2DQuad4
+// elements on nodes of 5 or 2 unknowns, tied by EQUAL constraints, integrated with
+// ExtendedNewmarkBeta + Linear (the 2D element has no history matrix: G = 0, no Ubar).
+//
+// Set-up: one thread per element-matrix entry computes M, C and K (20 x 20, node-major,
+// quads in the leading 8 x 8); the free-space matrices T^T A T are summed per nonzero in
+// element order from a host-built contribution list (pure index work), and Keff is scaled
+// symmetrically by |diag|^-1/2. A step is one fused right-hand-side pass over the three
+// matrices, a MINRES solve of the scaled system (three kernels per iteration, the scalar
+// recurrence carried in device memory, convergence polled every few iterations) and the
+// Newmark update. Keff is indefinite (the stress blocks are negative definite), so CG does
+// not apply; SVL factorises it with Eigen (11-Solvers/EigenSolver.cpp).
+#include "svlg_common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+namespace svlg {
+namespace {
+
+constexpr int LD = 20;              // element-matrix leading dimension (4 nodes x 5 unknowns)
+constexpr int EMAT = LD * LD;
+constexpr int VB = 256;             // block of the vector and row kernels
+constexpr int MAXB = 1024;          // blocks of a reduction (partials per pass)
+constexpr int POLL = 8;             // MINRES iterations between convergence checks
+constexpr double GA = 0.577350269189626;  // two-point Gauss abscissa (GaussQuadrature.cpp:303-316)
+
+struct MrState {
+    double beta, cp, sp, c, s, eta, beta1;
+    int32_t done, iters, fail, pad;
+};
+
+struct EmatArgs {
+    int nelem, lumped;
+    const int32_t *cls, *conn;
+    const double *X, *E, *nu, *rho, *th, *pml;
+    double *out;  // [nelem][3][EMAT]: M, C, K
+};
+
+// Entry (i, j) of M, C and K of element e, summed over the 2 x 2 Gauss points in SVL's order
+// (r fastest). PML2DQuad4: PML2DQuad4.cpp:233-466 with the stretching factors of :655-693
+// (ax, ay multiply the mass-like blocks in M and the displacement-stress coupling in C;
+// bx, by the same blocks in K and the coupling in K; ax by + ay bx the mass-like blocks in
+// C). lin2DQuad4: plane-strain B^T D B and the consistent or row-sum mass (lin2DQuad4.cpp:243-318).
+__global__ void __launch_bounds__(VB) k_pml_emat(EmatArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)a.nelem * EMAT) return;
+    const int e = (int)(t / EMAT), ij = (int)(t % EMAT), i = ij / LD, j = ij % LD;
+    double *Me = a.out + (size_t)e * 3 * EMAT, *Ce = Me + EMAT, *Ke = Ce + EMAT;
+    const bool pml = a.cls[e] == SVLG_CLASS_PML2DQUAD4;
+    const int nun = pml ? 5 : 2;
+    if (i >= 4 * nun || j >= 4 * nun) {
+        Me[ij] = 0.0; Ce[ij] = 0.0; Ke[ij] = 0.0;
+        return;
+    }
+    const int na = i / nun, p = i % nun, nb = j / nun, q = j % nun;
+    double x[4], y[4];
+    for (int c = 0; c < 4; ++c) {
+        const int n = a.conn[4 * e + c];
+        x[c] = a.X[2 * n];
+        y[c] = a.X[2 * n + 1];
+    }
+    const double E = a.E[e], nu = a.nu[e], rho = a.rho[e], th = a.th[e];
+    const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = E / (2.0 * (1.0 + nu));
+    double mv = 0.0, cv = 0.0, kv = 0.0;
+    for (int g = 0; g < 4; ++g) {
+        const double r = (g & 1) ? GA : -GA, s = (g & 2) ? GA : -GA;
+        const double N[4] = {0.25 * (1 - r) * (1 - s), 0.25 * (1 + r) * (1 - s), 0.25 * (1 + r) * (1 + s),
+                             0.25 * (1 - r) * (1 + s)};
+        const double Nr[4] = {-0.25 * (1 - s), 0.25 * (1 - s), 0.25 * (1 + s), -0.25 * (1 + s)};
+        const double Ns[4] = {-0.25 * (1 - r), -0.25 * (1 + r), 0.25 * (1 + r), 0.25 * (1 - r)};
+        double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
+        for (int c = 0; c < 4; ++c) {
+            J00 += Nr[c] * x[c]; J01 += Nr[c] * y[c];
+            J10 += Ns[c] * x[c]; J11 += Ns[c] * y[c];
+        }
+        const double det = J00 * J11 - J01 * J10;
+        const double w = th * fabs(det);
+        const double nxa = (J11 * Nr[na] - J01 * Ns[na]) / det, nya = (-J10 * Nr[na] + J00 * Ns[na]) / det;
+        const double nxb = (J11 * Nr[nb] - J01 * Ns[nb]) / det, nyb = (-J10 * Nr[nb] + J00 * Ns[nb]) / det;
+        if (!pml) {
+            const double l2 = lam + 2.0 * mu;
+            double kk;
+            if (p == 0 && q == 0) kk = nxa * l2 * nxb + nya * mu * nyb;
+            else if (p == 0) kk = nxa * lam * nyb + nya * mu * nxb;
+            else if (q == 0) kk = nya * lam * nxb + nxa * mu * nyb;
+            else kk = nya * l2 * nyb + nxa * mu * nxb;
+            kv += w * kk;
+            if (a.lumped) {
+                if (i == j) mv += rho * N[na] * w;
+            } else if (p == q) {
+                mv += rho * N[na] * N[nb] * w;
+            }
+            continue;
+        }
+        double xg = 0, yg = 0;
+        for (int c = 0; c < 4; ++c) {
+            xg += N[c] * x[c];
+            yg += N[c] * y[c];
+        }
+        const double *pp = a.pml + 7 * (size_t)e;  // m, L, R, x0, y0, nx, ny
+        const double m = pp[0], L = pp[1], R = pp[2];
+        const double vp = sqrt((lam + 2.0 * mu) / rho);
+        const double a0 = (m + 1.0) * (L / 10.0) / 2.0 / L * log(1.0 / R);
+        const double b0 = (m + 1.0) * vp / 2.0 / L * log(1.0 / R);
+        const double fx = pow((xg - pp[3]) * pp[5] / L, m), fy = pow((yg - pp[4]) * pp[6] / L, m);
+        const double ax = 1.0 + a0 * fx, ay = 1.0 + a0 * fy, bx = b0 * fx, by = b0 * fy;
+        if ((p < 2) == (q < 2)) {
+            // Mass-like blocks: rho on the displacements, minus the plane-strain compliance on
+            // the stress unknowns (xx, yy coupled through lambda, xy through 1/mu).
+            double bv = 0.0;
+            if (p == q) bv = p < 2 ? rho : (p < 4 ? -(lam + 2.0 * mu) / (4.0 * mu * (lam + mu)) : -1.0 / mu);
+            else if (p >= 2 && q >= 2 && p < 4 && q < 4) bv = lam / (4.0 * mu * (lam + mu));
+            const double nn = bv * N[na] * N[nb] * w;
+            mv += ax * ay * nn;
+            cv += (ax * by + ay * bx) * nn;
+            kv += bx * by * nn;
+        } else {
+            // Displacement-stress coupling: u_x with xx (d/dx) and xy (d/dy), u_y with yy (d/dy)
+            // and xy (d/dx); the derivative falls on the displacement node's shape function.
+            const int pu = p < 2 ? p : q, ps = p < 2 ? q : p;
+            const double ndu_x = p < 2 ? nxa : nxb, ndu_y = p < 2 ? nya : nyb;
+            const double ns = p < 2 ? N[nb] : N[na];
+            if ((pu == 0 && ps == 2) || (pu == 1 && ps == 4)) {
+                const double v = ndu_x * ns * w;
+                cv += ay * v;
+                kv += by * v;
+            } else if ((pu == 0 && ps == 4) || (pu == 1 && ps == 3)) {
+                const double v = ndu_y * ns * w;
+                cv += ax * v;
+                kv += bx * v;
+            }
+        }
+    }
+    Me[ij] = mv;
+    Ce[ij] = cv;
+    Ke[ij] = kv;
+}
+
+// One thread per free-space nonzero: its contributions (element-matrix offset, product of
+// the two T entries) in element order, as Assembler sums its triplets.
+__global__ void __launch_bounds__(VB) k_pml_assemble(int64_t nnz, const int64_t *cptr, const int64_t *cidx,
+                                                     const double *cfac, const double *emat, double c0, double c1,
+                                                     double *vM, double *vC, double *vK, double *vE) {
+    for (int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; z < nnz; z += (int64_t)gridDim.x * blockDim.x) {
+        double m = 0.0, c = 0.0, k = 0.0;
+        for (int64_t t = cptr[z]; t < cptr[z + 1]; ++t) {
+            const double *b = emat + cidx[t];
+            const double f = cfac[t];
+            m += f * b[0];
+            c += f * b[EMAT];
+            k += f * b[2 * EMAT];
+        }
+        vM[z] = m;
+        vC[z] = c;
+        vK[z] = k;
+        vE[z] = k + c0 * m + c1 * c;
+    }
+}
+
+__global__ void __launch_bounds__(VB) k_pml_scale(int n, const int32_t *rowptr, const int32_t *col, const double *vE,
+                                                  double *S, int32_t *bad) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double d = 0.0;
+        for (int z = rowptr[i]; z < rowptr[i + 1]; ++z)
+            if (col[z] == i) d = vE[z];
+        if (d == 0.0) {
+            bad[0] = 1;
+            S[i] = 1.0;
+        } else {
+            S[i] = 1.0 / sqrt(fabs(d));
+        }
+    }
+}
+
+__global__ void __launch_bounds__(VB) k_pml_scaled(int n, const int32_t *rowptr, const int32_t *col, const double *vE,
+                                                   const double *S, double *vA) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int z = rowptr[i]; z < rowptr[i + 1]; ++z) vA[z] = S[i] * vE[z] * S[col[z]];
+}
+
+// Block sum of v into part[blockIdx.x] (deterministic tree).
+__device__ inline void block_sum(double v, double *part) {
+    __shared__ double sh[VB];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = VB / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Sum of the nb partials, the same value and order in every block.
+__device__ inline double reduce_parts(const double *part, int nb) {
+    __shared__ double sh[VB];
+    __shared__ double res;
+    double v = 0.0;
+    for (int t = threadIdx.x; t < nb; t += VB) v += part[t];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = VB / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) res = sh[0];
+    __syncthreads();
+    return res;
+}
+
+struct RhsArgs {
+    int n, k, nt;
+    const int32_t *rowptr, *col, *lptr, *lrow;
+    const double *vM, *vC, *vK, *lfac, *series, *S, *U, *V, *A;
+    double c2;
+    double *r, *part;
+};
+
+// Scaled effective load S T^T (F_k - K U + C V + M (4/dt V + A)) (ExtendedNewmarkBeta.cpp:208-245
+// with G = 0) and the partial sums of its square.
+__global__ void __launch_bounds__(VB) k_pml_rhs(RhsArgs a) {
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        double b = 0.0;
+        for (int z = a.rowptr[i]; z < a.rowptr[i + 1]; ++z) {
+            const int c = a.col[z];
+            b += -a.vK[z] * a.U[c] + a.vC[z] * a.V[c] + a.vM[z] * (a.c2 * a.V[c] + a.A[c]);
+        }
+        if (a.k < a.nt)
+            for (int l = a.lptr[i]; l < a.lptr[i + 1]; ++l) b += a.lfac[l] * a.series[(size_t)a.lrow[l] * a.nt + a.k];
+        const double r = a.S[i] * b;
+        a.r[i] = r;
+        acc += r * r;
+    }
+    block_sum(acc, a.part);
+}
+
+__global__ void __launch_bounds__(VB) k_mr_init(int n, int nb, const double *part, const double *r, double *v,
+                                                double *vprev, double *w, double *wprev, double *x, double rtol,
+                                                MrState *st) {
+    const double beta1 = sqrt(reduce_parts(part, nb));
+    const double inv = beta1 > 0.0 ? 1.0 / beta1 : 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        v[i] = r[i] * inv;
+        vprev[i] = 0.0;
+        w[i] = 0.0;
+        wprev[i] = 0.0;
+        x[i] = 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        MrState s{};
+        s.beta = 0.0; s.cp = 1.0; s.sp = 0.0; s.c = 1.0; s.s = 0.0; s.eta = beta1; s.beta1 = beta1;
+        s.done = beta1 == 0.0 ? 1 : 0;
+        s.iters = 0;
+        s.fail = 0;
+        st[0] = s;
+        st[1] = s;
+    }
+}
+
+// MINRES (Paige-Saunders) iteration, part 1: Av = A v and the partials of v.Av.
+__global__ void __launch_bounds__(VB) k_mr_spmv(int n, const int32_t *rowptr, const int32_t *col, const double *vA,
+                                                const double *v, double *Av, double *part, const MrState *st) {
+    if (st->done) return;
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int z = rowptr[i]; z < rowptr[i + 1]; ++z) s += vA[z] * v[col[z]];
+        Av[i] = s;
+        acc += v[i] * s;
+    }
+    block_sum(acc, part);
+}
+
+// Part 2: the Lanczos vector Av - alpha v - beta vprev (in Av) and the partials of its square.
+__global__ void __launch_bounds__(VB) k_mr_lanczos(int n, int nb, const double *part1, const double *v,
+                                                   const double *vprev, double *Av, double *part2, const MrState *st) {
+    if (st->done) return;
+    const double alpha = reduce_parts(part1, nb), beta = st->beta;
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double u = Av[i] - alpha * v[i] - beta * vprev[i];
+        Av[i] = u;
+        acc += u * u;
+    }
+    block_sum(acc, part2);
+}
+
+// Part 3: the Givens rotations, the new search direction (into wprev's buffer), the iterate,
+// and the normalised Lanczos vector (into vprev's buffer); the host then swaps the pairs.
+// Block 0 writes the scalar state of the next iteration into the other slot.
+__global__ void __launch_bounds__(VB) k_mr_update(int n, int nb, const double *part1, const double *part2,
+                                                  const double *v, double *vprev, const double *Av, const double *w,
+                                                  double *wprev, double *x, double rtol, const MrState *st,
+                                                  MrState *nxt) {
+    const MrState s = *st;
+    if (s.done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = s;
+        return;
+    }
+    const double alpha = reduce_parts(part1, nb), bn = sqrt(reduce_parts(part2, nb));
+    const double eps = s.sp * s.beta, dtmp = s.cp * s.beta;
+    const double delta = s.c * dtmp + s.s * alpha, gt = -s.s * dtmp + s.c * alpha;
+    const double gam = hypot(gt, bn);
+    if (gam == 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            MrState f = s;
+            f.done = 1;
+            f.fail = 1;
+            *nxt = f;
+        }
+        return;
+    }
+    const double cn = gt / gam, sn = bn / gam, step = cn * s.eta, ig = 1.0 / gam;
+    const double ib = bn > 0.0 ? 1.0 / bn : 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double wn = (v[i] - delta * w[i] - eps * wprev[i]) * ig;
+        x[i] += step * wn;
+        wprev[i] = wn;
+        vprev[i] = Av[i] * ib;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        MrState t = s;
+        t.beta = bn; t.cp = s.c; t.sp = s.s; t.c = cn; t.s = sn; t.eta = -sn * s.eta;
+        t.iters = s.iters + 1;
+        t.done = (fabs(t.eta) <= rtol * s.beta1 || bn == 0.0) ? 1 : 0;
+        *nxt = t;
+    }
+}
+
+// dU = S x, then ExtendedNewmarkBeta's update (ExtendedNewmarkBeta.cpp:208-245):
+// U += dU, A = 4/dt^2 dU - 4/dt V - A, V = 2/dt dU - V.
+__global__ void __launch_bounds__(VB) k_pml_update(int n, const double *S, const double *x, double c0, double c1,
+                                                   double c2, double *U, double *V, double *A) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double du = S[i] * x[i], v = V[i];
+        U[i] += du;
+        A[i] = c0 * du - c2 * v - A[i];
+        V[i] = c1 * du - v;
+    }
+}
+
+// Total-DOF values (T u) of the recorded DOFs into row s of the histories.
+__global__ void __launch_bounds__(VB) k_pml_record(int nrec, const int32_t *rec, const int32_t *dfree,
+                                                   const double *dfac, const double *U, const double *V,
+                                                   const double *A, double *hU, double *hV, double *hA) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrec) return;
+    const int d = rec[r], f = dfree[d];
+    const double g = f >= 0 ? dfac[d] : 0.0;
+    hU[r] = f >= 0 ? g * U[f] : 0.0;
+    hV[r] = f >= 0 ? g * V[f] : 0.0;
+    hA[r] = f >= 0 ? g * A[f] : 0.0;
+}
+
+int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(MAXB, (n + VB - 1) / VB)); }
+
+}  // namespace
+}  // namespace svlg
+
+using namespace svlg;
+
+struct svlg_pml {
+    hipStream_t stream = nullptr;
+    int nfree = 0, ntot = 0, nnodes = 0, nelem = 0, nb = 1, nt = 0, maxit = 0;
+    int64_t nnz = 0, ncontrib = 0;
+    double dt = 0, rtol = 0, c0 = 0, c1 = 0, c2 = 0;
+    DevBuf<int32_t> rowptr, col, lptr, lrow, dfree;
+    DevBuf<double> dfac, lfac, series;
+    DevBuf<double> vM, vC, vK, vA, S, U, V, A, r, x, Av, vb0, vb1, wb0, wb1, part0, part1, part2;
+    DevBuf<MrState> st;
+    MrState *hst = nullptr;  // pinned copy of the polled state
+    ~svlg_pml() {
+        if (hst) (void)hipHostFree(hst);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+int svlg_pml_create(const svlg_mixed_desc *d, double dt, double rtol, int32_t maxit, svlg_pml **out) {
+    SVLG_CHECK(d && out, "null argument");
+    *out = nullptr;
+    SVLG_CHECK(d->nnodes > 0 && d->nelem > 0 && d->coords && d->dof_ptr && d->dof_free && d->dof_factor &&
+                   d->elem_class && d->conn && d->E && d->nu && d->rho && d->th,
+               "incomplete mixed-model description");
+    SVLG_CHECK(dt > 0.0 && rtol > 0.0 && maxit > 0, "dt, rtol and maxit must be positive");
+    const int nn = d->nnodes, ne = d->nelem;
+    const int ntot = d->dof_ptr[nn];
+    for (int n = 0; n < nn; ++n) {
+        const int k = d->dof_ptr[n + 1] - d->dof_ptr[n];
+        SVLG_CHECK(k == 2 || k == 5, "every node carries 2 or 5 unknowns");
+    }
+    int nfree = 0;
+    for (int t = 0; t < ntot; ++t) {
+        SVLG_CHECK(d->dof_free[t] >= -1, "free DOF numbers are -1 or non-negative");
+        nfree = std::max(nfree, d->dof_free[t] + 1);
+    }
+    SVLG_CHECK(nfree > 0, "the model has no free DOF");
+    bool any_pml = false;
+    for (int e = 0; e < ne; ++e) {
+        const int cls = d->elem_class[e];
+        SVLG_CHECK(cls == SVLG_CLASS_LIN2DQUAD4 || cls == SVLG_CLASS_PML2DQUAD4, "unknown element class");
+        const int nun = cls == SVLG_CLASS_PML2DQUAD4 ? 5 : 2;
+        any_pml |= cls == SVLG_CLASS_PML2DQUAD4;
+        for (int c = 0; c < 4; ++c) {
+            const int n = d->conn[4 * e + c];
+            SVLG_CHECK(n >= 0 && n < nn, "element connectivity out of range");
+            SVLG_CHECK(d->dof_ptr[n + 1] - d->dof_ptr[n] == nun,
+                       "a node's unknown count does not match its element class");
+        }
+        SVLG_CHECK(d->E[e] > 0.0 && d->rho[e] > 0.0 && d->nu[e] > -1.0 && d->nu[e] < 0.5 && d->th[e] > 0.0,
+                   "element material or thickness out of range");
+    }
+    if (any_pml) {
+        SVLG_CHECK(d->pml != nullptr, "PML elements need their parameters");
+        for (int e = 0; e < ne; ++e)
+            if (d->elem_class[e] == SVLG_CLASS_PML2DQUAD4)
+                SVLG_CHECK(d->pml[7 * e + 1] > 0.0 && d->pml[7 * e + 2] > 0.0 && d->pml[7 * e + 2] <= 1.0,
+                           "PML depth L must be positive and 0 < R <= 1");
+    }
+    SVLG_CHECK(d->nload == 0 || (d->load_dof && d->load_factor && d->load_series && d->load_nt > 0),
+               "loads need DOFs, factors and series");
+    for (int l = 0; l < d->nload; ++l) SVLG_CHECK(d->load_dof[l] >= 0 && d->load_dof[l] < ntot, "load DOF out of range");
+
+    // Contributions of every element-matrix entry to the free-space matrices, in element order.
+    struct Contrib { int64_t key, idx; double fac; };
+    std::vector<Contrib> cl;
+    cl.reserve((size_t)ne * 100);
+    for (int e = 0; e < ne; ++e) {
+        const int nun = d->elem_class[e] == SVLG_CLASS_PML2DQUAD4 ? 5 : 2;
+        int fr[LD];
+        double fa[LD];
+        for (int a = 0; a < 4; ++a)
+            for (int p = 0; p < nun; ++p) {
+                const int t = d->dof_ptr[d->conn[4 * e + a]] + p;
+                fr[a * nun + p] = d->dof_free[t];
+                fa[a * nun + p] = d->dof_factor[t];
+            }
+        for (int i = 0; i < 4 * nun; ++i) {
+            if (fr[i] < 0) continue;
+            for (int j = 0; j < 4 * nun; ++j) {
+                if (fr[j] < 0) continue;
+                cl.push_back({(int64_t)fr[i] * nfree + fr[j], (int64_t)e * 3 * EMAT + i * LD + j, fa[i] * fa[j]});
+            }
+        }
+    }
+    std::stable_sort(cl.begin(), cl.end(), [](const Contrib &a, const Contrib &b) { return a.key < b.key; });
+    std::vector<int32_t> rowptr(nfree + 1, 0), col;
+    std::vector<int64_t> cptr, cidx(cl.size());
+    std::vector<double> cfac(cl.size());
+    for (size_t t = 0; t < cl.size(); ++t) {
+        if (t == 0 || cl[t].key != cl[t - 1].key) {
+            cptr.push_back((int64_t)t);
+            col.push_back((int32_t)(cl[t].key % nfree));
+            rowptr[cl[t].key / nfree + 1] += 1;
+        }
+        cidx[t] = cl[t].idx;
+        cfac[t] = cl[t].fac;
+    }
+    cptr.push_back((int64_t)cl.size());
+    std::partial_sum(rowptr.begin(), rowptr.end(), rowptr.begin());
+    SVLG_CHECK(col.size() < (size_t)INT32_MAX, "too many nonzeros");
+    // Loads per free row: (series row, load factor x T entry), in load order.
+    std::vector<std::pair<int, int>> lr;
+    for (int l = 0; l < d->nload; ++l)
+        if (d->dof_free[d->load_dof[l]] >= 0) lr.push_back({d->dof_free[d->load_dof[l]], l});
+    std::stable_sort(lr.begin(), lr.end(), [](auto &a, auto &b) { return a.first < b.first; });
+    std::vector<int32_t> lptr(nfree + 1, 0), lrow;
+    std::vector<double> lfac;
+    for (auto &[f, l] : lr) {
+        lptr[f + 1] += 1;
+        lrow.push_back(l);
+        lfac.push_back(d->load_factor[l] * d->dof_factor[d->load_dof[l]]);
+    }
+    std::partial_sum(lptr.begin(), lptr.end(), lptr.begin());
+
+    auto *p = new svlg_pml();
+    auto fail = [&](int rc) { delete p; return rc; };
+    p->nfree = nfree; p->ntot = ntot; p->nnodes = nn; p->nelem = ne;
+    p->nnz = (int64_t)col.size(); p->ncontrib = (int64_t)cl.size();
+    p->dt = dt; p->rtol = rtol; p->maxit = maxit;
+    p->c0 = 4.0 / (dt * dt); p->c1 = 2.0 / dt; p->c2 = 4.0 / dt;
+    p->nb = blocks_for(nfree);
+    p->nt = d->nload ? d->load_nt : 0;
+    if (hipStreamCreate(&p->stream) != hipSuccess) { set_error("hipStreamCreate failed"); return fail(10); }
+    if (hipHostMalloc((void **)&p->hst, sizeof(MrState)) != hipSuccess) { set_error("hipHostMalloc failed"); return fail(10); }
+    hipStream_t s = p->stream;
+    DevBuf<int32_t> cls, conn, bad;
+    DevBuf<double> X, E, nu, rho, th, pml, emat, cf;
+    DevBuf<int64_t> cp, ci;
+    std::vector<double> pml_h((size_t)7 * ne, 0.0);
+    if (any_pml) std::copy(d->pml, d->pml + (size_t)7 * ne, pml_h.begin());
+#define UP(buf, src, cnt)                                                              \
+    if ((buf).upload((src), (cnt), s) != hipSuccess) {                                 \
+        set_error("device allocation/upload failed for " #buf);                         \
+        return fail(10);                                                               \
+    }
+    UP(cls, d->elem_class, ne);
+    UP(conn, d->conn, (size_t)4 * ne);
+    UP(X, d->coords, (size_t)2 * nn);
+    UP(E, d->E, ne);
+    UP(nu, d->nu, ne);
+    UP(rho, d->rho, ne);
+    UP(th, d->th, ne);
+    UP(pml, pml_h.data(), pml_h.size());
+    UP(cp, cptr.data(), cptr.size());
+    UP(ci, cidx.data(), cidx.size());
+    UP(cf, cfac.data(), cfac.size());
+    UP(p->rowptr, rowptr.data(), rowptr.size());
+    UP(p->col, col.data(), col.size());
+    UP(p->lptr, lptr.data(), lptr.size());
+    UP(p->lrow, lrow.data(), lrow.size());
+    UP(p->lfac, lfac.data(), lfac.size());
+    UP(p->dfree, d->dof_free, ntot);
+    UP(p->dfac, d->dof_factor, ntot);
+    if (p->nt) UP(p->series, d->load_series, (size_t)d->nload * p->nt);
+#undef UP
+    int32_t zero = 0;
+    DevBuf<double> *vecs[] = {&p->S, &p->U, &p->V, &p->A, &p->r, &p->x, &p->Av, &p->vb0, &p->vb1, &p->wb0, &p->wb1};
+    for (DevBuf<double> *b : vecs)
+        if (b->alloc(nfree) != hipSuccess) { set_error("device allocation failed"); return fail(10); }
+    DevBuf<double> *mats[] = {&p->vM, &p->vC, &p->vK, &p->vA};
+    for (DevBuf<double> *b : mats)
+        if (b->alloc(p->nnz) != hipSuccess) { set_error("device allocation failed"); return fail(10); }
+    if (p->part0.alloc(MAXB) != hipSuccess || p->part1.alloc(MAXB) != hipSuccess ||
+        p->part2.alloc(MAXB) != hipSuccess || p->st.alloc(2) != hipSuccess || emat.alloc((size_t)ne * 3 * EMAT) != hipSuccess ||
+        bad.upload(&zero, 1, s) != hipSuccess) {
+        set_error("device allocation failed");
+        return fail(10);
+    }
+    DevBuf<double> vE;
+    if (vE.alloc(p->nnz) != hipSuccess) { set_error("device allocation failed"); return fail(10); }
+    const int64_t nent = (int64_t)ne * EMAT;
+    EmatArgs ea{ne, d->lumped ? 1 : 0, cls.p, conn.p, X.p, E.p, nu.p, rho.p, th.p, pml.p, emat.p};
+    hipLaunchKernelGGL(k_pml_emat, dim3((unsigned)((nent + VB - 1) / VB)), dim3(VB), 0, s, ea);
+    hipLaunchKernelGGL(k_pml_assemble, dim3(blocks_for(p->nnz)), dim3(VB), 0, s, p->nnz, cp.p, ci.p, cf.p, emat.p,
+                       p->c0, p->c1, p->vM.p, p->vC.p, p->vK.p, vE.p);
+    hipLaunchKernelGGL(k_pml_scale, dim3(p->nb), dim3(VB), 0, s, nfree, p->rowptr.p, p->col.p, vE.p, p->S.p, bad.p);
+    hipLaunchKernelGGL(k_pml_scaled, dim3(p->nb), dim3(VB), 0, s, nfree, p->rowptr.p, p->col.p, vE.p, p->S.p, p->vA.p);
+    if (hipGetLastError() != hipSuccess) { set_error("PML set-up launch failed"); return fail(10); }
+    int32_t hbad = 0;
+    if (hipMemcpyAsync(&hbad, bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        set_error("PML set-up failed");
+        return fail(10);
+    }
+    if (hbad) { set_error("Keff has a zero diagonal entry"); return fail(11); }
+    const size_t nbytes = sizeof(double) * nfree;
+    if (hipMemsetAsync(p->U.p, 0, nbytes, s) != hipSuccess || hipMemsetAsync(p->V.p, 0, nbytes, s) != hipSuccess ||
+        hipMemsetAsync(p->A.p, 0, nbytes, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        set_error("PML state initialisation failed");
+        return fail(10);
+    }
+    *out = p;
+    return 0;
+}
+
+int svlg_pml_destroy(svlg_pml *p) {
+    if (p) {
+        (void)hipStreamSynchronize(p->stream);
+        delete p;
+    }
+    return 0;
+}
+
+int svlg_pml_info(svlg_pml *p, int64_t *info4) {
+    SVLG_CHECK(p && info4, "null argument");
+    info4[0] = p->nfree;
+    info4[1] = p->ntot;
+    info4[2] = p->nnz;
+    info4[3] = p->ncontrib;
+    return 0;
+}
+
+int svlg_pml_steps(svlg_pml *p, int32_t k0, int32_t nsteps, int32_t nrec, const int32_t *rec_dofs, double *histU,
+                   double *histV, double *histA, int32_t *iters) {
+    SVLG_CHECK(p, "null handle");
+    SVLG_CHECK(k0 >= 0 && nsteps >= 0 && nrec >= 0, "negative step or record count");
+    SVLG_CHECK(nrec == 0 || rec_dofs, "null record list");
+    for (int r = 0; r < nrec; ++r) SVLG_CHECK(rec_dofs[r] >= 0 && rec_dofs[r] < p->ntot, "recorded DOF out of range");
+    hipStream_t s = p->stream;
+    const int n = p->nfree, nb = p->nb;
+    DevBuf<int32_t> rec;
+    DevBuf<double> hist;
+    const size_t hsz = (size_t)nsteps * nrec;
+    if (nrec) {
+        SVLG_HIP(rec.upload(rec_dofs, nrec, s));
+        SVLG_HIP(hist.alloc(3 * hsz));
+    }
+    RhsArgs ra{n, 0, p->nt, p->rowptr.p, p->col.p, p->lptr.p, p->lrow.p, p->vM.p, p->vC.p, p->vK.p, p->lfac.p,
+               p->series.p, p->S.p, p->U.p, p->V.p, p->A.p, p->c2, p->r.p, p->part0.p};
+    for (int st = 0; st < nsteps; ++st) {
+        ra.k = k0 + st;
+        hipLaunchKernelGGL(k_pml_rhs, dim3(nb), dim3(VB), 0, s, ra);
+        double *v = p->vb0.p, *vp = p->vb1.p, *w = p->wb0.p, *wp = p->wb1.p;
+        hipLaunchKernelGGL(k_mr_init, dim3(nb), dim3(VB), 0, s, n, nb, p->part0.p, p->r.p, v, vp, w, wp, p->x.p,
+                           p->rtol, p->st.p);
+        // Batches of POLL iterations; once converged the remaining launches of a batch return
+        // at once. The state the last launched iteration wrote sits in slot j & 1.
+        for (int j = 0;;) {
+            for (int q = 0; q < POLL; ++q, ++j) {
+                const MrState *cur = p->st.p + (j & 1);
+                MrState *nxt = p->st.p + ((j + 1) & 1);
+                hipLaunchKernelGGL(k_mr_spmv, dim3(nb), dim3(VB), 0, s, n, p->rowptr.p, p->col.p, p->vA.p, v, p->Av.p,
+                                   p->part1.p, cur);
+                hipLaunchKernelGGL(k_mr_lanczos, dim3(nb), dim3(VB), 0, s, n, nb, p->part1.p, v, vp, p->Av.p,
+                                   p->part2.p, cur);
+                hipLaunchKernelGGL(k_mr_update, dim3(nb), dim3(VB), 0, s, n, nb, p->part1.p, p->part2.p, v, vp,
+                                   p->Av.p, w, wp, p->x.p, p->rtol, cur, nxt);
+                std::swap(v, vp);
+                std::swap(w, wp);
+            }
+            SVLG_HIP(hipGetLastError());
+            SVLG_HIP(hipMemcpyAsync(p->hst, p->st.p + (j & 1), sizeof(MrState), hipMemcpyDeviceToHost, s));
+            SVLG_HIP(hipStreamSynchronize(s));
+            SVLG_CHECK(!p->hst->fail, "MINRES broke down (singular effective matrix)");
+            if (p->hst->done) break;
+            SVLG_CHECK(p->hst->iters < p->maxit, "MINRES did not converge within maxit iterations");
+        }
+        SVLG_CHECK(p->hst->iters <= p->maxit, "MINRES did not converge within maxit iterations");
+        if (iters) iters[st] = p->hst->iters;
+        hipLaunchKernelGGL(k_pml_update, dim3(nb), dim3(VB), 0, s, n, p->S.p, p->x.p, p->c0, p->c1, p->c2, p->U.p,
+                           p->V.p, p->A.p);
+        if (nrec)
+            hipLaunchKernelGGL(k_pml_record, dim3((nrec + VB - 1) / VB), dim3(VB), 0, s, nrec, rec.p, p->dfree.p,
+                               p->dfac.p, p->U.p, p->V.p, p->A.p, hist.p + (size_t)st * nrec,
+                               hist.p + hsz + (size_t)st * nrec, hist.p + 2 * hsz + (size_t)st * nrec);
+        SVLG_HIP(hipGetLastError());
+    }
+    if (nrec) {
+        double *dst[3] = {histU, histV, histA};
+        for (int c = 0; c < 3; ++c)
+            if (dst[c]) SVLG_HIP(hipMemcpyAsync(dst[c], hist.p + c * hsz, sizeof(double) * hsz, hipMemcpyDeviceToHost, s));
+    }
+    SVLG_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int svlg_pml_get_state(svlg_pml *p, double *U, double *V, double *A) {
+    SVLG_CHECK(p, "null handle");
+    const size_t nbytes = sizeof(double) * p->nfree;
+    if (U) SVLG_HIP(hipMemcpyAsync(U, p->U.p, nbytes, hipMemcpyDeviceToHost, p->stream));
+    if (V) SVLG_HIP(hipMemcpyAsync(V, p->V.p, nbytes, hipMemcpyDeviceToHost, p->stream));
+    if (A) SVLG_HIP(hipMemcpyAsync(A, p->A.p, nbytes, hipMemcpyDeviceToHost, p->stream));
+    SVLG_HIP(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+}  // extern "C"

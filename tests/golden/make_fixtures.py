@@ -18,6 +18,10 @@ Cases (SVL case name, what it pins):
   F03  1 lin2DQuad4, PlasticPlaneStrainJ2, massless, Newton + Newmark dt=0.001, 999 steps
   F07  100 lin2DQuad4 J2 soil column, Rayleigh, 2 base dashpots, Newton + Newmark
        (first 4000 of 9999 steps kept)
+  F11  10 PML2DQuad4 + 90 lin2DQuad4 soil column, EQUAL constraints, Newmark dt=0.01,
+       1000 steps (the case ships no reference history: only its load series is kept)
+
+Cases can be named after the path to rewrite only those (e.g. ``... /path/to/SVL F11``).
 """
 from __future__ import annotations
 
@@ -37,6 +41,7 @@ CASES = {
     "J05": ("J05-DY_Lin_3DSoilColumn_Elastic_Hexa8", "ricker.in"),
     "F03": ("F03-DY_Lin_2DPointLoad_J2PStrain_Quad4", "ricker.in"),
     "F07": ("F07-DY_Lin_2DSoilColumn_J2PStrain_Quad4", "ricker-SoilColumnPlasticPlaneStrainJ2.in"),
+    "F11": ("F11-DY_Lin_2DPMLSoilColumn_ElasticPStrain_Quad4", "ricker-SoilColumnElasticPlaneStrainPML.txt"),
 }
 KEEP_ROWS = {"F07": 4000}
 
@@ -55,9 +60,11 @@ def series_file(text: str) -> np.ndarray:
     return v[1:1 + nt]
 
 
-def main(svl_root: str):
+def main(svl_root: str, only=()):
     root = Path(svl_root) / "03-Validations" / "01-Debugging"
     for key, (case, load_file) in CASES.items():
+        if only and key not in only:
+            continue
         with zipfile.ZipFile(root / f"{case}.zip") as z:
             load = series_file(read_member(z, f"{case}/{load_file}"))
             out = {"load": load}
@@ -76,4 +83,4 @@ def main(svl_root: str):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference", tuple(sys.argv[2:]))

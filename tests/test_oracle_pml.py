@@ -161,3 +161,20 @@ def test_pml_reflection_coefficient():
         ratio = np.abs(u - ref).max() / np.abs(ref).max() / (2 * R)
         print(f"R={R}: reflected / (2R incident) = {ratio:.3f}")
         assert 0.9 < ratio < 1.25
+
+
+def test_dense_solve_is_accurate():
+    """The oracle's Keff solve against long-double iterative refinement on the F11 column: the
+    stress and displacement unknowns differ by eleven orders of magnitude in Keff, so a
+    solve that skipped the scaling would miss by ~1e-11 and fail here."""
+    m, _ = column(90, 10, 0.0)
+    dt = 0.01
+    K = m.keff_matrix(dt)
+    solve = m.keff_solver(dt)
+    b = np.random.default_rng(3).standard_normal(K.shape[0]) * np.sqrt(np.abs(np.diag(K)))
+    x = solve(b)
+    KL, xl = K.astype(np.longdouble), x.astype(np.longdouble)
+    for _ in range(3):
+        xl = xl + solve((b.astype(np.longdouble) - KL @ xl).astype(np.float64)).astype(np.longdouble)
+    xt = xl.astype(np.float64)
+    assert np.linalg.norm(x - xt) / np.linalg.norm(xt) < 1e-14
