@@ -319,6 +319,45 @@ def drm_bench(args, us_per_step_plain):
     return out
 
 
+def pml_bench(args):
+    """2D half-space wrapped by a 2D PML (PML2DQuad4 + lin2DQuad4, EQUAL constraints) under a
+    surface Ricker load, ExtendedNewmarkBeta with a scaled MINRES solve per step. Per MINRES
+    iteration the algorithmic traffic is 12 B per nonzero (value + column) and 116 B per
+    free DOF (row pointer, the gathered vector, and the three-term vector updates)."""
+    import numpy as np
+    from svl_amd.pml import PmlNewmark, halfspace_box
+    nx, ny, npml = (int(v) for v in args.pml_ne.split(","))
+    dt = 0.005
+    nt = args.pml_steps + args.pml_warmup + 2
+    tt = np.arange(nt) * dt
+    a = (np.pi * 10.0 * (tt - 0.15)) ** 2
+    t0 = time.perf_counter()
+    m = halfspace_box(nx, ny, npml, load_series=(1 - 2 * a) * np.exp(-a))
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pm = PmlNewmark(m, dt)
+    create_s = time.perf_counter() - t0
+    info = pm.info()
+    pm.advance(1, args.pml_warmup)
+    t0 = time.perf_counter()
+    _, _, _, its = pm.advance(1 + args.pml_warmup, args.pml_steps)
+    wall = time.perf_counter() - t0
+    us_step = wall / args.pml_steps * 1e6
+    iters = float(its.mean())
+    it_bytes = 12 * info["nnz"] + 116 * info["nfree"]
+    out = {"workload": f"2D half-space lin2DQuad4 {nx}x{ny} h=1m wrapped by PML2DQuad4 {npml} elements thick "
+                       f"(left, right, bottom), EQUAL ties, Ricker surface load, ExtendedNewmarkBeta dt={dt}",
+           "nelem": int(len(m.conn)), **info, "steps": args.pml_steps, "us_per_step": us_step,
+           "minres_iters_mean": iters, "us_per_minres_iter": us_step / max(iters, 1.0),
+           "alg_bytes_per_minres_iter": it_bytes,
+           "achieved_GBs_per_iter": it_bytes / (us_step / max(iters, 1.0) * 1e-6) / 1e9,
+           "build_s": build_s, "model_create_s": create_s}
+    print(f"# pml: {info['nfree']} free DOFs, {us_step:.0f} us/step, {iters:.1f} MINRES iterations",
+          file=sys.stderr, flush=True)
+    del pm
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -352,6 +391,10 @@ def main():
     ap.add_argument("--drm-steps", type=int, default=300)
     ap.add_argument("--drm-warmup", type=int, default=30)
     ap.add_argument("--no-drm", action="store_true")
+    ap.add_argument("--pml-ne", type=str, default="300,150,15", help="PML leg: nx,ny,PML elements")
+    ap.add_argument("--pml-steps", type=int, default=40)
+    ap.add_argument("--pml-warmup", type=int, default=5)
+    ap.add_argument("--no-pml", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="only run the device copy of MB megabytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -427,6 +470,8 @@ def main():
         out["batched"] = batched_bench(args)
     if not args.no_drm:
         out["drm"] = drm_bench(args, step_s * 1e6)
+    if not args.no_pml:
+        out["pml2d"] = pml_bench(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, dt, args.cpu_steps)
         out["cpu_baseline"]["cores"] = 1

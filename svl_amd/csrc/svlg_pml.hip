@@ -24,6 +24,7 @@ constexpr int EMAT = LD * LD;
 constexpr int VB = 256;             // block of the vector and row kernels
 constexpr int MAXB = 1024;          // blocks of a reduction (partials per pass)
 constexpr int POLL = 8;             // MINRES iterations between convergence checks
+constexpr int LPR = 8;              // lanes per matrix row in the row kernels
 constexpr double GA = 0.577350269189626;  // two-point Gauss abscissa (GaussQuadrature.cpp:303-316)
 
 struct MrState {
@@ -221,17 +222,25 @@ struct RhsArgs {
 // with G = 0) and the partial sums of its square.
 __global__ void __launch_bounds__(VB) k_pml_rhs(RhsArgs a) {
     double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x % LPR;
+    const int64_t nthr = (int64_t)a.n * LPR;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t - lane < nthr;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t / LPR);
         double b = 0.0;
-        for (int z = a.rowptr[i]; z < a.rowptr[i + 1]; ++z) {
+        for (int z = a.rowptr[i] + lane; z < a.rowptr[i + 1]; z += LPR) {
             const int c = a.col[z];
             b += -a.vK[z] * a.U[c] + a.vC[z] * a.V[c] + a.vM[z] * (a.c2 * a.V[c] + a.A[c]);
         }
-        if (a.k < a.nt)
-            for (int l = a.lptr[i]; l < a.lptr[i + 1]; ++l) b += a.lfac[l] * a.series[(size_t)a.lrow[l] * a.nt + a.k];
-        const double r = a.S[i] * b;
-        a.r[i] = r;
-        acc += r * r;
+        for (int o = LPR / 2; o > 0; o >>= 1) b += __shfl_xor(b, o, LPR);
+        if (lane == 0) {
+            if (a.k < a.nt)
+                for (int l = a.lptr[i]; l < a.lptr[i + 1]; ++l)
+                    b += a.lfac[l] * a.series[(size_t)a.lrow[l] * a.nt + a.k];
+            const double r = a.S[i] * b;
+            a.r[i] = r;
+            acc += r * r;
+        }
     }
     block_sum(acc, a.part);
 }
@@ -259,25 +268,35 @@ __global__ void __launch_bounds__(VB) k_mr_init(int n, int nb, const double *par
     }
 }
 
-// MINRES (Paige-Saunders) iteration, part 1: Av = A v and the partials of v.Av.
+// MINRES (Paige-Saunders) iteration, part 1: Av = A v and the partials of v.Av; LPR lanes
+// per row stride through its nonzeros and combine by shuffles.
+
 __global__ void __launch_bounds__(VB) k_mr_spmv(int n, const int32_t *rowptr, const int32_t *col, const double *vA,
                                                 const double *v, double *Av, double *part, const MrState *st) {
     if (st->done) return;
     double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x % LPR;
+    const int64_t nthr = (int64_t)n * LPR;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t - lane < nthr;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t / LPR);
         double s = 0.0;
-        for (int z = rowptr[i]; z < rowptr[i + 1]; ++z) s += vA[z] * v[col[z]];
-        Av[i] = s;
-        acc += v[i] * s;
+        if (i < n)
+            for (int z = rowptr[i] + lane; z < rowptr[i + 1]; z += LPR) s += vA[z] * v[col[z]];
+        for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
+        if (lane == 0 && i < n) {
+            Av[i] = s;
+            acc += v[i] * s;
+        }
     }
     block_sum(acc, part);
 }
 
 // Part 2: the Lanczos vector Av - alpha v - beta vprev (in Av) and the partials of its square.
-__global__ void __launch_bounds__(VB) k_mr_lanczos(int n, int nb, const double *part1, const double *v,
+__global__ void __launch_bounds__(VB) k_mr_lanczos(int n, int nbs, const double *part1, const double *v,
                                                    const double *vprev, double *Av, double *part2, const MrState *st) {
     if (st->done) return;
-    const double alpha = reduce_parts(part1, nb), beta = st->beta;
+    const double alpha = reduce_parts(part1, nbs), beta = st->beta;
     double acc = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double u = Av[i] - alpha * v[i] - beta * vprev[i];
@@ -290,7 +309,7 @@ __global__ void __launch_bounds__(VB) k_mr_lanczos(int n, int nb, const double *
 // Part 3: the Givens rotations, the new search direction (into wprev's buffer), the iterate,
 // and the normalised Lanczos vector (into vprev's buffer); the host then swaps the pairs.
 // Block 0 writes the scalar state of the next iteration into the other slot.
-__global__ void __launch_bounds__(VB) k_mr_update(int n, int nb, const double *part1, const double *part2,
+__global__ void __launch_bounds__(VB) k_mr_update(int n, int nbs, int nb, const double *part1, const double *part2,
                                                   const double *v, double *vprev, const double *Av, const double *w,
                                                   double *wprev, double *x, double rtol, const MrState *st,
                                                   MrState *nxt) {
@@ -299,7 +318,7 @@ __global__ void __launch_bounds__(VB) k_mr_update(int n, int nb, const double *p
         if (blockIdx.x == 0 && threadIdx.x == 0) *nxt = s;
         return;
     }
-    const double alpha = reduce_parts(part1, nb), bn = sqrt(reduce_parts(part2, nb));
+    const double alpha = reduce_parts(part1, nbs), bn = sqrt(reduce_parts(part2, nb));
     const double eps = s.sp * s.beta, dtmp = s.cp * s.beta;
     const double delta = s.c * dtmp + s.s * alpha, gt = -s.s * dtmp + s.c * alpha;
     const double gam = hypot(gt, bn);
@@ -363,7 +382,7 @@ using namespace svlg;
 
 struct svlg_pml {
     hipStream_t stream = nullptr;
-    int nfree = 0, ntot = 0, nnodes = 0, nelem = 0, nb = 1, nt = 0, maxit = 0;
+    int nfree = 0, ntot = 0, nnodes = 0, nelem = 0, nb = 1, nbs = 1, nt = 0, maxit = 0;
     int64_t nnz = 0, ncontrib = 0;
     double dt = 0, rtol = 0, c0 = 0, c1 = 0, c2 = 0;
     DevBuf<int32_t> rowptr, col, lptr, lrow, dfree;
@@ -371,11 +390,51 @@ struct svlg_pml {
     DevBuf<double> vM, vC, vK, vA, S, U, V, A, r, x, Av, vb0, vb1, wb0, wb1, part0, part1, part2;
     DevBuf<MrState> st;
     MrState *hst = nullptr;  // pinned copy of the polled state
+    int last_iters = 0;              // MINRES iterations of the previous solve
+    hipGraphExec_t batch = nullptr;  // POLL iterations, captured once (the buffer pairs swap
+                                     // every iteration, so an even batch ends where it began)
     ~svlg_pml() {
+        if (batch) (void)hipGraphExecDestroy(batch);
         if (hst) (void)hipHostFree(hst);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
+
+static_assert(POLL % 2 == 0, "a batch must end with the buffer pairs where it began");
+
+// POLL MINRES iterations starting from slot 0 and the buffer pairs (vb0, vb1), (wb0, wb1).
+static void launch_batch(svlg_pml *p, hipStream_t s) {
+    const int n = p->nfree, nb = p->nb, nbs = p->nbs;
+    double *v = p->vb0.p, *vp = p->vb1.p, *w = p->wb0.p, *wp = p->wb1.p;
+    for (int j = 0; j < POLL; ++j) {
+        const MrState *cur = p->st.p + (j & 1);
+        MrState *nxt = p->st.p + ((j + 1) & 1);
+        hipLaunchKernelGGL(k_mr_spmv, dim3(nbs), dim3(VB), 0, s, n, p->rowptr.p, p->col.p, p->vA.p, v, p->Av.p,
+                           p->part1.p, cur);
+        hipLaunchKernelGGL(k_mr_lanczos, dim3(nb), dim3(VB), 0, s, n, nbs, p->part1.p, v, vp, p->Av.p, p->part2.p,
+                           cur);
+        hipLaunchKernelGGL(k_mr_update, dim3(nb), dim3(VB), 0, s, n, nbs, nb, p->part1.p, p->part2.p, v, vp, p->Av.p,
+                           w, wp, p->x.p, p->rtol, cur, nxt);
+        std::swap(v, vp);
+        std::swap(w, wp);
+    }
+}
+
+// Captures launch_batch into a graph; on any capture error the caller launches eagerly.
+static void capture_batch(svlg_pml *p) {
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return;
+    launch_batch(p, p->stream);
+    const hipError_t ec = hipStreamEndCapture(p->stream, &g);
+    if (ec != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return;
+    }
+    if (hipGraphInstantiate(&p->batch, g, nullptr, nullptr, 0) != hipSuccess) p->batch = nullptr;
+    (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+}
 
 extern "C" {
 
@@ -483,6 +542,7 @@ int svlg_pml_create(const svlg_mixed_desc *d, double dt, double rtol, int32_t ma
     p->dt = dt; p->rtol = rtol; p->maxit = maxit;
     p->c0 = 4.0 / (dt * dt); p->c1 = 2.0 / dt; p->c2 = 4.0 / dt;
     p->nb = blocks_for(nfree);
+    p->nbs = blocks_for((int64_t)nfree * LPR);
     p->nt = d->nload ? d->load_nt : 0;
     if (hipStreamCreate(&p->stream) != hipSuccess) { set_error("hipStreamCreate failed"); return fail(10); }
     if (hipHostMalloc((void **)&p->hst, sizeof(MrState)) != hipSuccess) { set_error("hipHostMalloc failed"); return fail(10); }
@@ -589,31 +649,27 @@ int svlg_pml_steps(svlg_pml *p, int32_t k0, int32_t nsteps, int32_t nrec, const 
         SVLG_HIP(rec.upload(rec_dofs, nrec, s));
         SVLG_HIP(hist.alloc(3 * hsz));
     }
+    if (!p->batch) capture_batch(p);
     RhsArgs ra{n, 0, p->nt, p->rowptr.p, p->col.p, p->lptr.p, p->lrow.p, p->vM.p, p->vC.p, p->vK.p, p->lfac.p,
                p->series.p, p->S.p, p->U.p, p->V.p, p->A.p, p->c2, p->r.p, p->part0.p};
     for (int st = 0; st < nsteps; ++st) {
         ra.k = k0 + st;
-        hipLaunchKernelGGL(k_pml_rhs, dim3(nb), dim3(VB), 0, s, ra);
-        double *v = p->vb0.p, *vp = p->vb1.p, *w = p->wb0.p, *wp = p->wb1.p;
-        hipLaunchKernelGGL(k_mr_init, dim3(nb), dim3(VB), 0, s, n, nb, p->part0.p, p->r.p, v, vp, w, wp, p->x.p,
-                           p->rtol, p->st.p);
+        hipLaunchKernelGGL(k_pml_rhs, dim3(p->nbs), dim3(VB), 0, s, ra);
+        hipLaunchKernelGGL(k_mr_init, dim3(nb), dim3(VB), 0, s, n, p->nbs, p->part0.p, p->r.p, p->vb0.p, p->vb1.p,
+                           p->wb0.p, p->wb1.p, p->x.p, p->rtol, p->st.p);
         // Batches of POLL iterations; once converged the remaining launches of a batch return
-        // at once. The state the last launched iteration wrote sits in slot j & 1.
-        for (int j = 0;;) {
-            for (int q = 0; q < POLL; ++q, ++j) {
-                const MrState *cur = p->st.p + (j & 1);
-                MrState *nxt = p->st.p + ((j + 1) & 1);
-                hipLaunchKernelGGL(k_mr_spmv, dim3(nb), dim3(VB), 0, s, n, p->rowptr.p, p->col.p, p->vA.p, v, p->Av.p,
-                                   p->part1.p, cur);
-                hipLaunchKernelGGL(k_mr_lanczos, dim3(nb), dim3(VB), 0, s, n, nb, p->part1.p, v, vp, p->Av.p,
-                                   p->part2.p, cur);
-                hipLaunchKernelGGL(k_mr_update, dim3(nb), dim3(VB), 0, s, n, nb, p->part1.p, p->part2.p, v, vp,
-                                   p->Av.p, w, wp, p->x.p, p->rtol, cur, nxt);
-                std::swap(v, vp);
-                std::swap(w, wp);
-            }
+        // at once. A batch ends in slot 0 with the buffer pairs back in place.
+        // The batches the previous solve needed, less one, go out before the first poll
+        // (steps of a linear analysis converge in nearly the same number of iterations).
+        for (int b = 1; b < p->last_iters / POLL; ++b) {
+            if (p->batch) SVLG_HIP(hipGraphLaunch(p->batch, s));
+            else launch_batch(p, s);
+        }
+        for (;;) {
+            if (p->batch) SVLG_HIP(hipGraphLaunch(p->batch, s));
+            else launch_batch(p, s);
             SVLG_HIP(hipGetLastError());
-            SVLG_HIP(hipMemcpyAsync(p->hst, p->st.p + (j & 1), sizeof(MrState), hipMemcpyDeviceToHost, s));
+            SVLG_HIP(hipMemcpyAsync(p->hst, p->st.p, sizeof(MrState), hipMemcpyDeviceToHost, s));
             SVLG_HIP(hipStreamSynchronize(s));
             SVLG_CHECK(!p->hst->fail, "MINRES broke down (singular effective matrix)");
             if (p->hst->done) break;
@@ -621,6 +677,7 @@ int svlg_pml_steps(svlg_pml *p, int32_t k0, int32_t nsteps, int32_t nrec, const 
         }
         SVLG_CHECK(p->hst->iters <= p->maxit, "MINRES did not converge within maxit iterations");
         if (iters) iters[st] = p->hst->iters;
+        p->last_iters = p->hst->iters;
         hipLaunchKernelGGL(k_pml_update, dim3(nb), dim3(VB), 0, s, n, p->S.p, p->x.p, p->c0, p->c1, p->c2, p->U.p,
                            p->V.p, p->A.p);
         if (nrec)
