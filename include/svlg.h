@@ -281,31 +281,37 @@ int svlg_gnn_train(svlg_model *m, int32_t nsteps, int32_t every, int32_t sweeps,
  * device sustains. `dst` and `src` are device pointers. */
 int svlg_stream_copy(double *dst, const double *src, int64_t n, void *stream);
 
-/* ---- Perfectly matched layer, 2D (PML2DQuad4 + lin2DQuad4 + EQUAL constraints) -------
- * A mixed model: lin2DQuad4 nodes carry 2 unknowns (ux, uy), PML2DQuad4 nodes 5 (ux, uy and
- * the stress-like xx, yy, xy; 04-Elements/06-Quadrilateral/PML2DQuad4.cpp:11-12, 233-466),
- * the two regions tied by EQUAL constraints (06-Mesh/Mesh.cpp:328-383). Integrated with
- * ExtendedNewmarkBeta + Linear (10-Integrators/ExtendedNewmarkBeta.cpp:123-154, 208-245; the
- * 2D element has no history matrix, so G and Ubar vanish). The effective matrix is symmetric
- * and indefinite (negative stress blocks): the device assembles it once, scales it
+/* ---- Perfectly matched layers (PML2DQuad4 / PML3DHexa8 + regular elements + EQUAL) ------
+ * A mixed model: regular nodes carry ndim unknowns, PML nodes 5 in 2D (ux, uy and the
+ * stress-like xx, yy, xy; 04-Elements/06-Quadrilateral/PML2DQuad4.cpp:11-12, 233-466) or 9 in
+ * 3D (ux, uy, uz, xx, yy, zz, xy, yz, xz; 04-Elements/10-Hexahedron/PML3DHexa8.cpp:214-715),
+ * the regions tied by EQUAL constraints (06-Mesh/Mesh.cpp:328-383). Integrated with
+ * ExtendedNewmarkBeta + Linear (10-Integrators/ExtendedNewmarkBeta.cpp:123-154, 208-245): in 3D
+ * the history matrix G and the integrated displacement Ubar enter Keff (+ dt/3 G), Feff
+ * (- G (Ubar + dt U + dt^2/6 V)) and the update; the 2D element has no G. The effective matrix
+ * is symmetric and indefinite (negative stress blocks): the device assembles it once, scales it
  * symmetrically by |diag|^-1/2 and solves with MINRES where SVL factorises it with Eigen
  * (11-Solvers/EigenSolver.cpp). */
 #define SVLG_CLASS_LIN2DQUAD4 0
 #define SVLG_CLASS_PML2DQUAD4 1
+#define SVLG_CLASS_LIN3DHEXA8 2
+#define SVLG_CLASS_PML3DHEXA8 3
 
 typedef struct svlg_mixed_desc {
+    int32_t ndim;              /* 2 (classes 0, 1) or 3 (classes 2, 3) */
     int32_t nnodes;
-    const double *coords;      /* [nnodes][2] */
-    const int32_t *dof_ptr;    /* [nnodes+1] first total DOF of each node (2 or 5 per node) */
+    const double *coords;      /* [nnodes][ndim] */
+    const int32_t *dof_ptr;    /* [nnodes+1] first total DOF of each node */
     const int32_t *dof_free;   /* [ntotal] free DOF of each total DOF, -1 when restrained */
     const double *dof_factor;  /* [ntotal] its entry of T: 1, or the factor of an EQUAL slave */
     int32_t nelem;
     const int32_t *elem_class; /* [nelem] SVLG_CLASS_* */
-    const int32_t *conn;       /* [nelem][4] node indices, counter-clockwise */
-    const double *E, *nu, *rho, *th; /* [nelem] plane strain material and thickness */
-    const double *pml;         /* [nelem][7] m, L, R, x0, y0, nx, ny (read for PML elements) */
-    int32_t lumped;            /* lin2DQuad4 mass: 1 row-sum lumped, 0 consistent; the PML
-                                  mass is always consistent (PML2DQuad4.cpp:286) */
+    const int32_t *conn;       /* [nelem][4 in 2D, 8 in 3D] node indices in SVL's corner order */
+    const double *E, *nu, *rho, *th; /* [nelem] material (plane strain in 2D), thickness (2D) */
+    const double *pml;         /* [nelem][9] m, L, R, x0, y0, z0, nx, ny, nz (PML elements;
+                                  z0, nz unused in 2D) */
+    int32_t lumped;            /* regular-element mass: 1 row-sum lumped, 0 consistent; the
+                                  PML mass is always consistent (PML2DQuad4.cpp:286) */
     int32_t nload, load_nt;    /* POINTLOAD_CONCENTRATED_DYNAMIC on total DOFs */
     const int32_t *load_dof;   /* [nload] */
     const double *load_factor; /* [nload] */
@@ -314,8 +320,8 @@ typedef struct svlg_mixed_desc {
 
 typedef struct svlg_pml svlg_pml;
 
-/* Computes the element matrices, assembles M, C, K and Keff = K + 4/dt^2 M + 2/dt C over the
- * free DOFs (T^T A T) on the device. Replaces Assembler::ComputeMassMatrix / DampingMatrix /
+/* Computes the element matrices, assembles M, C, K, G and Keff = K + 4/dt^2 M + 2/dt C
+ * + dt/3 G over the free DOFs (T^T A T) on the device. Replaces Assembler::ComputeMassMatrix / DampingMatrix /
  * StiffnessMatrix (Assembler.hpp:50-181) and ExtendedNewmarkBeta::Initialize for this model. */
 int svlg_pml_create(const svlg_mixed_desc *desc, double dt, double rtol, int32_t maxit, svlg_pml **out);
 int svlg_pml_destroy(svlg_pml *p);

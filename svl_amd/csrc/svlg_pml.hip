@@ -1,9 +1,11 @@
-// svlg_pml.hip -- 2D perfectly matched layer on the device: PML2DQuad4 and lin2DQuad4
-// elements on nodes of 5 or 2 unknowns, tied by EQUAL constraints, integrated with
-// ExtendedNewmarkBeta + Linear (the 2D element has no history matrix: G = 0, no Ubar).
+// svlg_pml.hip -- perfectly matched layers on the device: PML2DQuad4 / lin2DQuad4 (nodes of 5
+// or 2 unknowns) or PML3DHexa8 / lin3DHexa8 (9 or 3), tied by EQUAL constraints, integrated
+// with ExtendedNewmarkBeta + Linear, including the 3D history matrix G and the integrated
+// displacement Ubar (the 2D element has none).
 //
-// Set-up: one thread per element-matrix entry computes M, C and K (20 x 20, node-major,
-// quads in the leading 8 x 8); the free-space matrices T^T A T are summed per nonzero in
+// Set-up: one thread per element-matrix entry computes M, C, K and G (node-major, nd x nd
+// with nd = nodes x unknowns, stored per element at its own offset, the four values of an
+// entry side by side); the free-space matrices T^T A T are summed per nonzero in
 // element order from a host-built contribution list (pure index work), and Keff is scaled
 // symmetrically by |diag|^-1/2. A step is one fused right-hand-side pass over the three
 // matrices, a MINRES solve of the scaled system (three kernels per iteration, the scalar
@@ -19,8 +21,6 @@
 namespace svlg {
 namespace {
 
-constexpr int LD = 20;              // element-matrix leading dimension (4 nodes x 5 unknowns)
-constexpr int EMAT = LD * LD;
 constexpr int VB = 256;             // block of the vector and row kernels
 constexpr int MAXB = 1024;          // blocks of a reduction (partials per pass)
 constexpr int POLL = 8;             // MINRES iterations between convergence checks
@@ -32,30 +32,27 @@ struct MrState {
     int32_t done, iters, fail, pad;
 };
 
+__host__ __device__ inline int class_nun(int cls) { return cls == 0 ? 2 : cls == 1 ? 5 : cls == 2 ? 3 : 9; }
+__host__ __device__ inline int class_nen(int cls) { return cls < 2 ? 4 : 8; }
+
 struct EmatArgs {
     int nelem, lumped;
     const int32_t *cls, *conn;
+    const int64_t *eoff;  // [nelem+1] first entry of each element (entries of nd x nd)
     const double *X, *E, *nu, *rho, *th, *pml;
-    double *out;  // [nelem][3][EMAT]: M, C, K
+    double *out;          // [entries][4]: M, C, K, G of each entry
 };
 
-// Entry (i, j) of M, C and K of element e, summed over the 2 x 2 Gauss points in SVL's order
-// (r fastest). PML2DQuad4: PML2DQuad4.cpp:233-466 with the stretching factors of :655-693
-// (ax, ay multiply the mass-like blocks in M and the displacement-stress coupling in C;
-// bx, by the same blocks in K and the coupling in K; ax by + ay bx the mass-like blocks in
-// C). lin2DQuad4: plane-strain B^T D B and the consistent or row-sum mass (lin2DQuad4.cpp:243-318).
-__global__ void __launch_bounds__(VB) k_pml_emat(EmatArgs a) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)a.nelem * EMAT) return;
-    const int e = (int)(t / EMAT), ij = (int)(t % EMAT), i = ij / LD, j = ij % LD;
-    double *Me = a.out + (size_t)e * 3 * EMAT, *Ce = Me + EMAT, *Ke = Ce + EMAT;
-    const bool pml = a.cls[e] == SVLG_CLASS_PML2DQUAD4;
-    const int nun = pml ? 5 : 2;
-    if (i >= 4 * nun || j >= 4 * nun) {
-        Me[ij] = 0.0; Ce[ij] = 0.0; Ke[ij] = 0.0;
-        return;
-    }
-    const int na = i / nun, p = i % nun, nb = j / nun, q = j % nun;
+struct Coef4 {
+    double m, c, k, g;
+};
+
+// PML2DQuad4 / lin2DQuad4 entry (a p, b q) of M, C, K summed over the 2 x 2 Gauss points in
+// SVL's order (r fastest). PML2DQuad4: PML2DQuad4.cpp:233-466 with the stretching factors of
+// :655-693 (ax ay, ax by + ay bx, bx by on the mass-like blocks of M, C, K; ay / ax and by / bx
+// on the displacement-stress coupling of C and K). lin2DQuad4: plane-strain stiffness and the
+// consistent or row-sum mass (lin2DQuad4.cpp:243-318).
+__device__ Coef4 entry2d(const EmatArgs &a, int e, bool pml, int na, int p, int nb, int q) {
     double x[4], y[4];
     for (int c = 0; c < 4; ++c) {
         const int n = a.conn[4 * e + c];
@@ -64,7 +61,7 @@ __global__ void __launch_bounds__(VB) k_pml_emat(EmatArgs a) {
     }
     const double E = a.E[e], nu = a.nu[e], rho = a.rho[e], th = a.th[e];
     const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = E / (2.0 * (1.0 + nu));
-    double mv = 0.0, cv = 0.0, kv = 0.0;
+    Coef4 r4{0.0, 0.0, 0.0, 0.0};
     for (int g = 0; g < 4; ++g) {
         const double r = (g & 1) ? GA : -GA, s = (g & 2) ? GA : -GA;
         const double N[4] = {0.25 * (1 - r) * (1 - s), 0.25 * (1 + r) * (1 - s), 0.25 * (1 + r) * (1 + s),
@@ -78,20 +75,16 @@ __global__ void __launch_bounds__(VB) k_pml_emat(EmatArgs a) {
         }
         const double det = J00 * J11 - J01 * J10;
         const double w = th * fabs(det);
-        const double nxa = (J11 * Nr[na] - J01 * Ns[na]) / det, nya = (-J10 * Nr[na] + J00 * Ns[na]) / det;
-        const double nxb = (J11 * Nr[nb] - J01 * Ns[nb]) / det, nyb = (-J10 * Nr[nb] + J00 * Ns[nb]) / det;
+        const double da[2] = {(J11 * Nr[na] - J01 * Ns[na]) / det, (-J10 * Nr[na] + J00 * Ns[na]) / det};
+        const double db[2] = {(J11 * Nr[nb] - J01 * Ns[nb]) / det, (-J10 * Nr[nb] + J00 * Ns[nb]) / det};
         if (!pml) {
-            const double l2 = lam + 2.0 * mu;
-            double kk;
-            if (p == 0 && q == 0) kk = nxa * l2 * nxb + nya * mu * nyb;
-            else if (p == 0) kk = nxa * lam * nyb + nya * mu * nxb;
-            else if (q == 0) kk = nya * lam * nxb + nxa * mu * nyb;
-            else kk = nya * l2 * nyb + nxa * mu * nxb;
-            kv += w * kk;
+            // K_ab,pq = lambda dNa_p dNb_q + mu dNa_q dNb_p + mu delta_pq grad Na . grad Nb
+            const double gg = da[0] * db[0] + da[1] * db[1];
+            r4.k += w * (lam * da[p] * db[q] + mu * da[q] * db[p] + (p == q ? mu * gg : 0.0));
             if (a.lumped) {
-                if (i == j) mv += rho * N[na] * w;
+                if (na == nb && p == q) r4.m += rho * N[na] * w;
             } else if (p == q) {
-                mv += rho * N[na] * N[nb] * w;
+                r4.m += rho * N[na] * N[nb] * w;
             }
             continue;
         }
@@ -100,63 +93,182 @@ __global__ void __launch_bounds__(VB) k_pml_emat(EmatArgs a) {
             xg += N[c] * x[c];
             yg += N[c] * y[c];
         }
-        const double *pp = a.pml + 7 * (size_t)e;  // m, L, R, x0, y0, nx, ny
+        const double *pp = a.pml + 9 * (size_t)e;  // m, L, R, x0, y0, z0, nx, ny, nz
         const double m = pp[0], L = pp[1], R = pp[2];
         const double vp = sqrt((lam + 2.0 * mu) / rho);
         const double a0 = (m + 1.0) * (L / 10.0) / 2.0 / L * log(1.0 / R);
         const double b0 = (m + 1.0) * vp / 2.0 / L * log(1.0 / R);
-        const double fx = pow((xg - pp[3]) * pp[5] / L, m), fy = pow((yg - pp[4]) * pp[6] / L, m);
+        const double fx = pow((xg - pp[3]) * pp[6] / L, m), fy = pow((yg - pp[4]) * pp[7] / L, m);
         const double ax = 1.0 + a0 * fx, ay = 1.0 + a0 * fy, bx = b0 * fx, by = b0 * fy;
         if ((p < 2) == (q < 2)) {
-            // Mass-like blocks: rho on the displacements, minus the plane-strain compliance on
-            // the stress unknowns (xx, yy coupled through lambda, xy through 1/mu).
             double bv = 0.0;
             if (p == q) bv = p < 2 ? rho : (p < 4 ? -(lam + 2.0 * mu) / (4.0 * mu * (lam + mu)) : -1.0 / mu);
             else if (p >= 2 && q >= 2 && p < 4 && q < 4) bv = lam / (4.0 * mu * (lam + mu));
             const double nn = bv * N[na] * N[nb] * w;
-            mv += ax * ay * nn;
-            cv += (ax * by + ay * bx) * nn;
-            kv += bx * by * nn;
+            r4.m += ax * ay * nn;
+            r4.c += (ax * by + ay * bx) * nn;
+            r4.k += bx * by * nn;
         } else {
-            // Displacement-stress coupling: u_x with xx (d/dx) and xy (d/dy), u_y with yy (d/dy)
-            // and xy (d/dx); the derivative falls on the displacement node's shape function.
+            // u_x with xx (d/dx) and xy (d/dy), u_y with yy (d/dy) and xy (d/dx); the derivative
+            // falls on the displacement node's shape function.
             const int pu = p < 2 ? p : q, ps = p < 2 ? q : p;
-            const double ndu_x = p < 2 ? nxa : nxb, ndu_y = p < 2 ? nya : nyb;
+            const double *du = p < 2 ? da : db;
             const double ns = p < 2 ? N[nb] : N[na];
             if ((pu == 0 && ps == 2) || (pu == 1 && ps == 4)) {
-                const double v = ndu_x * ns * w;
-                cv += ay * v;
-                kv += by * v;
+                const double v = du[0] * ns * w;
+                r4.c += ay * v;
+                r4.k += by * v;
             } else if ((pu == 0 && ps == 4) || (pu == 1 && ps == 3)) {
-                const double v = ndu_y * ns * w;
-                cv += ax * v;
-                kv += bx * v;
+                const double v = du[1] * ns * w;
+                r4.c += ax * v;
+                r4.k += bx * v;
             }
         }
     }
-    Me[ij] = mv;
-    Ce[ij] = cv;
-    Ke[ij] = kv;
+    return r4;
+}
+
+// PML3DHexa8 / lin3DHexa8 entry (a p, b q) of M, C, K, G over the 2 x 2 x 2 Gauss points.
+// PML3DHexa8.cpp:214-715 with the stretching of :952-996: mass-like blocks (rho on the
+// displacements, minus the compliance on xx, yy, zz and 1/mu on xy, yz, xz) times ax ay az (M),
+// ax ay bz + ax by az + bx ay az (C), ax by bz + bx by az + bx ay bz (K), bx by bz (G); the
+// displacement-stress coupling dN/dx_d N with, for d = x: ay az (C), ay bz + az by (K), by bz
+// (G), and likewise for y and z. lin3DHexa8: isotropic stiffness and consistent / row-sum mass.
+__device__ Coef4 entry3d(const EmatArgs &a, int e, bool pml, int na, int p, int nb, int q) {
+    const int sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+              sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    double X[8][3];
+    for (int c = 0; c < 8; ++c) {
+        const int n = a.conn[8 * e + c];
+        for (int d = 0; d < 3; ++d) X[c][d] = a.X[3 * n + d];
+    }
+    const double E = a.E[e], nu = a.nu[e], rho = a.rho[e];
+    const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = E / (2.0 * (1.0 + nu));
+    Coef4 r4{0.0, 0.0, 0.0, 0.0};
+    for (int g = 0; g < 8; ++g) {
+        const double r = (g & 1) ? GA : -GA, s = (g & 2) ? GA : -GA, t = (g & 4) ? GA : -GA;
+        double N[8], D[3][8];
+        for (int c = 0; c < 8; ++c) {
+            const double fr = 1 + sx[c] * r, fs = 1 + sy[c] * s, ft = 1 + sz[c] * t;
+            N[c] = 0.125 * fr * fs * ft;
+            D[0][c] = 0.125 * sx[c] * fs * ft;
+            D[1][c] = 0.125 * sy[c] * fr * ft;
+            D[2][c] = 0.125 * sz[c] * fr * fs;
+        }
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // J[k][j] = d x_j / d xi_k
+        for (int c = 0; c < 8; ++c)
+            for (int k = 0; k < 3; ++k)
+                for (int j = 0; j < 3; ++j) J[k][j] += D[k][c] * X[c][j];
+        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
+                     c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        const double id = 1.0 / det;
+        // inverse: Ji[j][k] = d xi_k / d x_j
+        const double Ji[3][3] = {
+            {c00 * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id, (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id},
+            {c01 * id, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id, (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id},
+            {c02 * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id, (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id}};
+        const double w = fabs(det);
+        double da[3], db[3];
+        for (int j = 0; j < 3; ++j) {
+            da[j] = Ji[j][0] * D[0][na] + Ji[j][1] * D[1][na] + Ji[j][2] * D[2][na];
+            db[j] = Ji[j][0] * D[0][nb] + Ji[j][1] * D[1][nb] + Ji[j][2] * D[2][nb];
+        }
+        if (!pml) {
+            const double gg = da[0] * db[0] + da[1] * db[1] + da[2] * db[2];
+            r4.k += w * (lam * da[p] * db[q] + mu * da[q] * db[p] + (p == q ? mu * gg : 0.0));
+            if (a.lumped) {
+                if (na == nb && p == q) r4.m += rho * N[na] * w;
+            } else if (p == q) {
+                r4.m += rho * N[na] * N[nb] * w;
+            }
+            continue;
+        }
+        double xg[3] = {0, 0, 0};
+        for (int c = 0; c < 8; ++c)
+            for (int d = 0; d < 3; ++d) xg[d] += N[c] * X[c][d];
+        const double *pp = a.pml + 9 * (size_t)e;  // m, L, R, x0, y0, z0, nx, ny, nz
+        const double m = pp[0], L = pp[1], R = pp[2];
+        const double vp = sqrt((lam + 2.0 * mu) / rho);
+        const double a0 = (m + 1.0) * (L / 10.0) / 2.0 / L * log(1.0 / R);
+        const double b0 = (m + 1.0) * vp / 2.0 / L * log(1.0 / R);
+        double sa[3], sb[3];
+        for (int d = 0; d < 3; ++d) {
+            const double f = pow((xg[d] - pp[3 + d]) * pp[6 + d] / L, m);
+            sa[d] = 1.0 + a0 * f;
+            sb[d] = b0 * f;
+        }
+        if ((p < 3) == (q < 3)) {
+            double bv = 0.0;
+            if (p == q) bv = p < 3 ? rho : (p < 6 ? -(lam + mu) / mu / (3.0 * lam + 2.0 * mu) : -1.0 / mu);
+            else if (p >= 3 && q >= 3 && p < 6 && q < 6) bv = lam / 2.0 / mu / (3.0 * lam + 2.0 * mu);
+            const double nn = bv * N[na] * N[nb] * w;
+            r4.m += sa[0] * sa[1] * sa[2] * nn;
+            r4.c += (sa[0] * sa[1] * sb[2] + sa[0] * sb[1] * sa[2] + sb[0] * sa[1] * sa[2]) * nn;
+            r4.k += (sa[0] * sb[1] * sb[2] + sb[0] * sb[1] * sa[2] + sb[0] * sa[1] * sb[2]) * nn;
+            r4.g += sb[0] * sb[1] * sb[2] * nn;
+        } else {
+            const int pu = p < 3 ? p : q, ps = p < 3 ? q : p;
+            // derivative direction of the pair: x (0,3) (1,6) (2,8); y (0,6) (1,4) (2,7);
+            // z (0,8) (1,7) (2,5)
+            int d = -1;
+            if ((pu == 0 && ps == 3) || (pu == 1 && ps == 6) || (pu == 2 && ps == 8)) d = 0;
+            else if ((pu == 0 && ps == 6) || (pu == 1 && ps == 4) || (pu == 2 && ps == 7)) d = 1;
+            else if ((pu == 0 && ps == 8) || (pu == 1 && ps == 7) || (pu == 2 && ps == 5)) d = 2;
+            if (d >= 0) {
+                const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;  // the two other directions
+                const double v = (p < 3 ? da[d] : db[d]) * (p < 3 ? N[nb] : N[na]) * w;
+                r4.c += sa[d1] * sa[d2] * v;
+                r4.k += (sa[d1] * sb[d2] + sa[d2] * sb[d1]) * v;
+                r4.g += sb[d1] * sb[d2] * v;
+            }
+        }
+    }
+    return r4;
+}
+
+__global__ void __launch_bounds__(VB) k_pml_emat(EmatArgs a, int64_t nent) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nent) return;
+    int lo = 0, hi = a.nelem;  // the element whose entry range holds t
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (a.eoff[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    const int e = lo, cls = a.cls[e], nun = class_nun(cls), nd = class_nen(cls) * nun;
+    const int ij = (int)(t - a.eoff[e]), i = ij / nd, j = ij % nd;
+    const bool pml = cls == SVLG_CLASS_PML2DQUAD4 || cls == SVLG_CLASS_PML3DHEXA8;
+    const Coef4 v = cls < 2 ? entry2d(a, e, pml, i / nun, i % nun, j / nun, j % nun)
+                            : entry3d(a, e, pml, i / nun, i % nun, j / nun, j % nun);
+    double *o = a.out + 4 * t;
+    o[0] = v.m;
+    o[1] = v.c;
+    o[2] = v.k;
+    o[3] = v.g;
 }
 
 // One thread per free-space nonzero: its contributions (element-matrix offset, product of
 // the two T entries) in element order, as Assembler sums its triplets.
 __global__ void __launch_bounds__(VB) k_pml_assemble(int64_t nnz, const int64_t *cptr, const int64_t *cidx,
                                                      const double *cfac, const double *emat, double c0, double c1,
-                                                     double *vM, double *vC, double *vK, double *vE) {
+                                                     double c3, double *vM, double *vC, double *vK, double *vG,
+                                                     double *vE) {
     for (int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; z < nnz; z += (int64_t)gridDim.x * blockDim.x) {
-        double m = 0.0, c = 0.0, k = 0.0;
+        double m = 0.0, c = 0.0, k = 0.0, g = 0.0;
         for (int64_t t = cptr[z]; t < cptr[z + 1]; ++t) {
-            const double *b = emat + cidx[t];
+            const double *b = emat + 4 * cidx[t];
             const double f = cfac[t];
             m += f * b[0];
-            c += f * b[EMAT];
-            k += f * b[2 * EMAT];
+            c += f * b[1];
+            k += f * b[2];
+            g += f * b[3];
         }
         vM[z] = m;
         vC[z] = c;
         vK[z] = k;
-        vE[z] = k + c0 * m + c1 * c;
+        if (vG) vG[z] = g;
+        vE[z] = k + c0 * m + c1 * c + c3 * g;
     }
 }
 
@@ -213,13 +325,13 @@ __device__ inline double reduce_parts(const double *part, int nb) {
 struct RhsArgs {
     int n, k, nt;
     const int32_t *rowptr, *col, *lptr, *lrow;
-    const double *vM, *vC, *vK, *lfac, *series, *S, *U, *V, *A;
-    double c2;
+    const double *vM, *vC, *vK, *vG, *lfac, *series, *S, *U, *V, *A, *Ub;  // vG, Ub: NULL without G
+    double c2, dt;
     double *r, *part;
 };
 
-// Scaled effective load S T^T (F_k - K U + C V + M (4/dt V + A)) (ExtendedNewmarkBeta.cpp:208-245
-// with G = 0) and the partial sums of its square.
+// Scaled effective load S T^T (F_k - K U - G (Ubar + dt U + dt^2/6 V) + C V + M (4/dt V + A))
+// (ExtendedNewmarkBeta.cpp:208-245) and the partial sums of its square.
 __global__ void __launch_bounds__(VB) k_pml_rhs(RhsArgs a) {
     double acc = 0.0;
     const int lane = threadIdx.x % LPR;
@@ -231,6 +343,7 @@ __global__ void __launch_bounds__(VB) k_pml_rhs(RhsArgs a) {
         for (int z = a.rowptr[i] + lane; z < a.rowptr[i + 1]; z += LPR) {
             const int c = a.col[z];
             b += -a.vK[z] * a.U[c] + a.vC[z] * a.V[c] + a.vM[z] * (a.c2 * a.V[c] + a.A[c]);
+            if (a.vG) b -= a.vG[z] * (a.Ub[c] + a.dt * a.U[c] + a.dt * a.dt / 6.0 * a.V[c]);
         }
         for (int o = LPR / 2; o > 0; o >>= 1) b += __shfl_xor(b, o, LPR);
         if (lane == 0) {
@@ -348,12 +461,15 @@ __global__ void __launch_bounds__(VB) k_mr_update(int n, int nbs, int nb, const 
     }
 }
 
-// dU = S x, then ExtendedNewmarkBeta's update (ExtendedNewmarkBeta.cpp:208-245):
-// U += dU, A = 4/dt^2 dU - 4/dt V - A, V = 2/dt dU - V.
+// dU = S x, then ExtendedNewmarkBeta's update (ExtendedNewmarkBeta.cpp:123-154):
+// Ubar += dt U + dt/3 dU + dt^2/6 V (old U, V; only with a history matrix), U += dU,
+// A = 4/dt^2 dU - 4/dt V - A, V = 2/dt dU - V.
 __global__ void __launch_bounds__(VB) k_pml_update(int n, const double *S, const double *x, double c0, double c1,
-                                                   double c2, double *U, double *V, double *A) {
+                                                   double c2, double dt, double *U, double *V, double *A,
+                                                   double *Ub) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double du = S[i] * x[i], v = V[i];
+        if (Ub) Ub[i] += dt * U[i] + dt / 3.0 * du + dt * dt / 6.0 * v;
         U[i] += du;
         A[i] = c0 * du - c2 * v - A[i];
         V[i] = c1 * du - v;
@@ -384,10 +500,11 @@ struct svlg_pml {
     hipStream_t stream = nullptr;
     int nfree = 0, ntot = 0, nnodes = 0, nelem = 0, nb = 1, nbs = 1, nt = 0, maxit = 0;
     int64_t nnz = 0, ncontrib = 0;
-    double dt = 0, rtol = 0, c0 = 0, c1 = 0, c2 = 0;
+    double dt = 0, rtol = 0, c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    bool has_g = false;  // a PML3DHexa8 element: history matrix G and Ubar
     DevBuf<int32_t> rowptr, col, lptr, lrow, dfree;
     DevBuf<double> dfac, lfac, series;
-    DevBuf<double> vM, vC, vK, vA, S, U, V, A, r, x, Av, vb0, vb1, wb0, wb1, part0, part1, part2;
+    DevBuf<double> vM, vC, vK, vG, vA, S, U, V, A, Ub, r, x, Av, vb0, vb1, wb0, wb1, part0, part1, part2;
     DevBuf<MrState> st;
     MrState *hst = nullptr;  // pinned copy of the polled state
     int last_iters = 0;              // MINRES iterations of the previous solve
@@ -445,11 +562,13 @@ int svlg_pml_create(const svlg_mixed_desc *d, double dt, double rtol, int32_t ma
                    d->elem_class && d->conn && d->E && d->nu && d->rho && d->th,
                "incomplete mixed-model description");
     SVLG_CHECK(dt > 0.0 && rtol > 0.0 && maxit > 0, "dt, rtol and maxit must be positive");
-    const int nn = d->nnodes, ne = d->nelem;
+    SVLG_CHECK(d->ndim == 2 || d->ndim == 3, "ndim must be 2 or 3");
+    const int nn = d->nnodes, ne = d->nelem, dim = d->ndim;
     const int ntot = d->dof_ptr[nn];
     for (int n = 0; n < nn; ++n) {
         const int k = d->dof_ptr[n + 1] - d->dof_ptr[n];
-        SVLG_CHECK(k == 2 || k == 5, "every node carries 2 or 5 unknowns");
+        SVLG_CHECK(dim == 2 ? (k == 2 || k == 5) : (k == 3 || k == 9),
+                   "every node carries 2 or 5 unknowns in 2D, 3 or 9 in 3D");
     }
     int nfree = 0;
     for (int t = 0; t < ntot; ++t) {
@@ -457,26 +576,34 @@ int svlg_pml_create(const svlg_mixed_desc *d, double dt, double rtol, int32_t ma
         nfree = std::max(nfree, d->dof_free[t] + 1);
     }
     SVLG_CHECK(nfree > 0, "the model has no free DOF");
-    bool any_pml = false;
+    bool any_pml = false, has_g = false;
+    const int nen = dim == 2 ? 4 : 8;
+    std::vector<int64_t> eoff(ne + 1, 0);
     for (int e = 0; e < ne; ++e) {
         const int cls = d->elem_class[e];
-        SVLG_CHECK(cls == SVLG_CLASS_LIN2DQUAD4 || cls == SVLG_CLASS_PML2DQUAD4, "unknown element class");
-        const int nun = cls == SVLG_CLASS_PML2DQUAD4 ? 5 : 2;
-        any_pml |= cls == SVLG_CLASS_PML2DQUAD4;
-        for (int c = 0; c < 4; ++c) {
-            const int n = d->conn[4 * e + c];
+        SVLG_CHECK(dim == 2 ? (cls == SVLG_CLASS_LIN2DQUAD4 || cls == SVLG_CLASS_PML2DQUAD4)
+                            : (cls == SVLG_CLASS_LIN3DHEXA8 || cls == SVLG_CLASS_PML3DHEXA8),
+                   "unknown element class for this dimension");
+        const int nun = class_nun(cls), nd = nen * nun;
+        const bool pml = cls == SVLG_CLASS_PML2DQUAD4 || cls == SVLG_CLASS_PML3DHEXA8;
+        any_pml |= pml;
+        has_g |= cls == SVLG_CLASS_PML3DHEXA8;
+        eoff[e + 1] = eoff[e] + (int64_t)nd * nd;
+        for (int c = 0; c < nen; ++c) {
+            const int n = d->conn[nen * e + c];
             SVLG_CHECK(n >= 0 && n < nn, "element connectivity out of range");
             SVLG_CHECK(d->dof_ptr[n + 1] - d->dof_ptr[n] == nun,
                        "a node's unknown count does not match its element class");
         }
-        SVLG_CHECK(d->E[e] > 0.0 && d->rho[e] > 0.0 && d->nu[e] > -1.0 && d->nu[e] < 0.5 && d->th[e] > 0.0,
+        SVLG_CHECK(d->E[e] > 0.0 && d->rho[e] > 0.0 && d->nu[e] > -1.0 && d->nu[e] < 0.5 &&
+                       (dim == 3 || d->th[e] > 0.0),
                    "element material or thickness out of range");
     }
     if (any_pml) {
         SVLG_CHECK(d->pml != nullptr, "PML elements need their parameters");
         for (int e = 0; e < ne; ++e)
-            if (d->elem_class[e] == SVLG_CLASS_PML2DQUAD4)
-                SVLG_CHECK(d->pml[7 * e + 1] > 0.0 && d->pml[7 * e + 2] > 0.0 && d->pml[7 * e + 2] <= 1.0,
+            if (d->elem_class[e] == SVLG_CLASS_PML2DQUAD4 || d->elem_class[e] == SVLG_CLASS_PML3DHEXA8)
+                SVLG_CHECK(d->pml[9 * e + 1] > 0.0 && d->pml[9 * e + 2] > 0.0 && d->pml[9 * e + 2] <= 1.0,
                            "PML depth L must be positive and 0 < R <= 1");
     }
     SVLG_CHECK(d->nload == 0 || (d->load_dof && d->load_factor && d->load_series && d->load_nt > 0),
@@ -488,20 +615,20 @@ int svlg_pml_create(const svlg_mixed_desc *d, double dt, double rtol, int32_t ma
     std::vector<Contrib> cl;
     cl.reserve((size_t)ne * 100);
     for (int e = 0; e < ne; ++e) {
-        const int nun = d->elem_class[e] == SVLG_CLASS_PML2DQUAD4 ? 5 : 2;
-        int fr[LD];
-        double fa[LD];
-        for (int a = 0; a < 4; ++a)
+        const int nun = class_nun(d->elem_class[e]), nd = nen * nun;
+        int fr[72];
+        double fa[72];
+        for (int a = 0; a < nen; ++a)
             for (int p = 0; p < nun; ++p) {
-                const int t = d->dof_ptr[d->conn[4 * e + a]] + p;
+                const int t = d->dof_ptr[d->conn[nen * e + a]] + p;
                 fr[a * nun + p] = d->dof_free[t];
                 fa[a * nun + p] = d->dof_factor[t];
             }
-        for (int i = 0; i < 4 * nun; ++i) {
+        for (int i = 0; i < nd; ++i) {
             if (fr[i] < 0) continue;
-            for (int j = 0; j < 4 * nun; ++j) {
+            for (int j = 0; j < nd; ++j) {
                 if (fr[j] < 0) continue;
-                cl.push_back({(int64_t)fr[i] * nfree + fr[j], (int64_t)e * 3 * EMAT + i * LD + j, fa[i] * fa[j]});
+                cl.push_back({(int64_t)fr[i] * nfree + fr[j], eoff[e] + (int64_t)i * nd + j, fa[i] * fa[j]});
             }
         }
     }
@@ -540,7 +667,8 @@ int svlg_pml_create(const svlg_mixed_desc *d, double dt, double rtol, int32_t ma
     p->nfree = nfree; p->ntot = ntot; p->nnodes = nn; p->nelem = ne;
     p->nnz = (int64_t)col.size(); p->ncontrib = (int64_t)cl.size();
     p->dt = dt; p->rtol = rtol; p->maxit = maxit;
-    p->c0 = 4.0 / (dt * dt); p->c1 = 2.0 / dt; p->c2 = 4.0 / dt;
+    p->c0 = 4.0 / (dt * dt); p->c1 = 2.0 / dt; p->c2 = 4.0 / dt; p->c3 = dt / 3.0;
+    p->has_g = has_g;
     p->nb = blocks_for(nfree);
     p->nbs = blocks_for((int64_t)nfree * LPR);
     p->nt = d->nload ? d->load_nt : 0;
@@ -549,17 +677,18 @@ int svlg_pml_create(const svlg_mixed_desc *d, double dt, double rtol, int32_t ma
     hipStream_t s = p->stream;
     DevBuf<int32_t> cls, conn, bad;
     DevBuf<double> X, E, nu, rho, th, pml, emat, cf;
-    DevBuf<int64_t> cp, ci;
-    std::vector<double> pml_h((size_t)7 * ne, 0.0);
-    if (any_pml) std::copy(d->pml, d->pml + (size_t)7 * ne, pml_h.begin());
+    DevBuf<int64_t> cp, ci, eo;
+    std::vector<double> pml_h((size_t)9 * ne, 0.0);
+    if (any_pml) std::copy(d->pml, d->pml + (size_t)9 * ne, pml_h.begin());
 #define UP(buf, src, cnt)                                                              \
     if ((buf).upload((src), (cnt), s) != hipSuccess) {                                 \
         set_error("device allocation/upload failed for " #buf);                         \
         return fail(10);                                                               \
     }
     UP(cls, d->elem_class, ne);
-    UP(conn, d->conn, (size_t)4 * ne);
-    UP(X, d->coords, (size_t)2 * nn);
+    UP(conn, d->conn, (size_t)nen * ne);
+    UP(X, d->coords, (size_t)dim * nn);
+    UP(eo, eoff.data(), eoff.size());
     UP(E, d->E, ne);
     UP(nu, d->nu, ne);
     UP(rho, d->rho, ne);
@@ -584,19 +713,23 @@ int svlg_pml_create(const svlg_mixed_desc *d, double dt, double rtol, int32_t ma
     DevBuf<double> *mats[] = {&p->vM, &p->vC, &p->vK, &p->vA};
     for (DevBuf<double> *b : mats)
         if (b->alloc(p->nnz) != hipSuccess) { set_error("device allocation failed"); return fail(10); }
+    if (has_g && (p->vG.alloc(p->nnz) != hipSuccess || p->Ub.alloc(nfree) != hipSuccess)) {
+        set_error("device allocation failed");
+        return fail(10);
+    }
+    const int64_t nent = eoff[ne];
     if (p->part0.alloc(MAXB) != hipSuccess || p->part1.alloc(MAXB) != hipSuccess ||
-        p->part2.alloc(MAXB) != hipSuccess || p->st.alloc(2) != hipSuccess || emat.alloc((size_t)ne * 3 * EMAT) != hipSuccess ||
+        p->part2.alloc(MAXB) != hipSuccess || p->st.alloc(2) != hipSuccess || emat.alloc((size_t)4 * nent) != hipSuccess ||
         bad.upload(&zero, 1, s) != hipSuccess) {
         set_error("device allocation failed");
         return fail(10);
     }
     DevBuf<double> vE;
     if (vE.alloc(p->nnz) != hipSuccess) { set_error("device allocation failed"); return fail(10); }
-    const int64_t nent = (int64_t)ne * EMAT;
-    EmatArgs ea{ne, d->lumped ? 1 : 0, cls.p, conn.p, X.p, E.p, nu.p, rho.p, th.p, pml.p, emat.p};
-    hipLaunchKernelGGL(k_pml_emat, dim3((unsigned)((nent + VB - 1) / VB)), dim3(VB), 0, s, ea);
+    EmatArgs ea{ne, d->lumped ? 1 : 0, cls.p, conn.p, eo.p, X.p, E.p, nu.p, rho.p, th.p, pml.p, emat.p};
+    hipLaunchKernelGGL(k_pml_emat, dim3((unsigned)((nent + VB - 1) / VB)), dim3(VB), 0, s, ea, nent);
     hipLaunchKernelGGL(k_pml_assemble, dim3(blocks_for(p->nnz)), dim3(VB), 0, s, p->nnz, cp.p, ci.p, cf.p, emat.p,
-                       p->c0, p->c1, p->vM.p, p->vC.p, p->vK.p, vE.p);
+                       p->c0, p->c1, p->c3, p->vM.p, p->vC.p, p->vK.p, has_g ? p->vG.p : nullptr, vE.p);
     hipLaunchKernelGGL(k_pml_scale, dim3(p->nb), dim3(VB), 0, s, nfree, p->rowptr.p, p->col.p, vE.p, p->S.p, bad.p);
     hipLaunchKernelGGL(k_pml_scaled, dim3(p->nb), dim3(VB), 0, s, nfree, p->rowptr.p, p->col.p, vE.p, p->S.p, p->vA.p);
     if (hipGetLastError() != hipSuccess) { set_error("PML set-up launch failed"); return fail(10); }
@@ -609,7 +742,8 @@ int svlg_pml_create(const svlg_mixed_desc *d, double dt, double rtol, int32_t ma
     if (hbad) { set_error("Keff has a zero diagonal entry"); return fail(11); }
     const size_t nbytes = sizeof(double) * nfree;
     if (hipMemsetAsync(p->U.p, 0, nbytes, s) != hipSuccess || hipMemsetAsync(p->V.p, 0, nbytes, s) != hipSuccess ||
-        hipMemsetAsync(p->A.p, 0, nbytes, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        hipMemsetAsync(p->A.p, 0, nbytes, s) != hipSuccess ||
+        (has_g && hipMemsetAsync(p->Ub.p, 0, nbytes, s) != hipSuccess) || hipStreamSynchronize(s) != hipSuccess) {
         set_error("PML state initialisation failed");
         return fail(10);
     }
@@ -650,8 +784,9 @@ int svlg_pml_steps(svlg_pml *p, int32_t k0, int32_t nsteps, int32_t nrec, const 
         SVLG_HIP(hist.alloc(3 * hsz));
     }
     if (!p->batch) capture_batch(p);
-    RhsArgs ra{n, 0, p->nt, p->rowptr.p, p->col.p, p->lptr.p, p->lrow.p, p->vM.p, p->vC.p, p->vK.p, p->lfac.p,
-               p->series.p, p->S.p, p->U.p, p->V.p, p->A.p, p->c2, p->r.p, p->part0.p};
+    RhsArgs ra{n, 0, p->nt, p->rowptr.p, p->col.p, p->lptr.p, p->lrow.p, p->vM.p, p->vC.p, p->vK.p,
+               p->has_g ? p->vG.p : nullptr, p->lfac.p, p->series.p, p->S.p, p->U.p, p->V.p, p->A.p,
+               p->has_g ? p->Ub.p : nullptr, p->c2, p->dt, p->r.p, p->part0.p};
     for (int st = 0; st < nsteps; ++st) {
         ra.k = k0 + st;
         hipLaunchKernelGGL(k_pml_rhs, dim3(p->nbs), dim3(VB), 0, s, ra);
@@ -678,8 +813,8 @@ int svlg_pml_steps(svlg_pml *p, int32_t k0, int32_t nsteps, int32_t nrec, const 
         SVLG_CHECK(p->hst->iters <= p->maxit, "MINRES did not converge within maxit iterations");
         if (iters) iters[st] = p->hst->iters;
         p->last_iters = p->hst->iters;
-        hipLaunchKernelGGL(k_pml_update, dim3(nb), dim3(VB), 0, s, n, p->S.p, p->x.p, p->c0, p->c1, p->c2, p->U.p,
-                           p->V.p, p->A.p);
+        hipLaunchKernelGGL(k_pml_update, dim3(nb), dim3(VB), 0, s, n, p->S.p, p->x.p, p->c0, p->c1, p->c2, p->dt,
+                           p->U.p, p->V.p, p->A.p, p->has_g ? p->Ub.p : nullptr);
         if (nrec)
             hipLaunchKernelGGL(k_pml_record, dim3((nrec + VB - 1) / VB), dim3(VB), 0, s, nrec, rec.p, p->dfree.p,
                                p->dfac.p, p->U.p, p->V.p, p->A.p, hist.p + (size_t)st * nrec,

@@ -5,6 +5,9 @@
   interface, top and a PML stress unknown within 1e-10 relative L2 of the oracle's dense solve.
 * A half-space wrapped by PML on three sides (corner elements stretched both ways), lumped and
   consistent regular mass, against the oracle.
+* SVL's validation case J12 (10 PML3DHexa8 under 90 lin3DHexa8, EQUAL ties on uz), all 999
+  steps, and a 3D half-space wrapped by PML on five faces, whose edge and corner elements
+  carry the history matrix G (Ubar in Feff and the update), against the oracle.
 * At a size the dense oracle does not reach, properties every linear time-invariant model
   holds: a load delayed by d steps gives the response delayed by d steps, and doubling the
   load doubles the response (to the MINRES tolerance).
@@ -16,7 +19,7 @@ import numpy as np
 import pytest
 
 from _pml_meshes import rel_l2, to_oracle
-from svl_amd.pml import PmlNewmark, halfspace_box, soil_column
+from svl_amd.pml import PmlNewmark, halfspace_box, halfspace_box3d, soil_column, soil_rod3d
 
 pytestmark = pytest.mark.gpu
 GOLD = Path(__file__).resolve().parent / "golden"
@@ -107,3 +110,47 @@ def test_large_box_time_invariance_and_linearity():
     assert np.abs(ub[:d]).max() <= 1e-9 * np.abs(ua).max()
     assert rel_l2(uc, 2 * ua) < 1e-9
     print("MINRES iterations per step: mean", its.mean(), "max", its.max())
+
+
+def test_j12_rod_matches_oracle():
+    load = np.load(GOLD / "F11.npz")["load"]    # J12 applies the same Ricker file as F11
+    m = soil_rod3d(90, 10, load)
+    nt = load.size
+    r0 = 4 * 11
+    top = len(m.ndof) - 4
+    rec = [(r0, 2), (top, 2), (20, 5), (20, 3), (r0 - 4, 2)]
+    print("J12 rod: 1000 steps, dt 0.01", flush=True)
+    hu, hv, ha, its = PmlNewmark(m, 0.01).advance(1, nt - 1, rec)
+    ou, ov, oa = to_oracle(m).run_newmark(0.01, nt, rec, k0=1, uva=True)
+    for c in range(len(rec)):
+        for h, o in ((hu, ou), (hv, ov), (ha, oa)):
+            assert rel_l2(h[:, c], o[:, c]) < TOL, (rec[c], rel_l2(h[:, c], o[:, c]))
+    np.testing.assert_array_equal(hu[:, 4], hu[:, 0])
+    print("MINRES iterations per step: mean", its.mean(), "max", its.max())
+
+
+@pytest.mark.parametrize("lumped", [False, True])
+def test_halfspace_box3d_with_history_matrix(lumped):
+    nt, dt = 81, 0.005
+    m = halfspace_box3d(4, 4, 3, 1, lumped=lumped, load_series=ricker(nt, dt, t0=0.05, f0=20.0))
+    top = m.loads[0][0]
+    off = reg_node3(m, 2.0, 2.0, 4.0)        # off both symmetry planes
+    corner = int(np.argmin(np.sum((m.coords - [0.0, 1.0, 1.0]) ** 2, axis=1)))
+    assert m.ndof[corner] == 9
+    # the six stress unknowns of a node on the outer face of a corner column (its
+    # displacements are fixed)
+    rec = [(top, 2), (off, 0), (off, 1), (off, 2)] + [(corner, a) for a in range(3, 9)]
+    print(f"3D half-space box 6x6x4 with PML on five faces, lumped={lumped}, {nt - 1} steps", flush=True)
+    hu, hv, ha, its = PmlNewmark(m, dt).advance(1, nt - 1, rec)
+    ou, ov, oa = to_oracle(m).run_newmark(dt, nt, rec, k0=1, uva=True)
+    for c in range(len(rec)):
+        for h, o in ((hu, ou), (hv, ov), (ha, oa)):
+            assert np.linalg.norm(o[:, c]) > 0
+            assert rel_l2(h[:, c], o[:, c]) < TOL, (rec[c], rel_l2(h[:, c], o[:, c]))
+    print("MINRES iterations per step: mean", its.mean(), "max", its.max())
+
+
+def reg_node3(m, x, y, z):
+    hit = np.nonzero((np.abs(m.coords - [x, y, z]).sum(axis=1) < 1e-9) & (m.ndof == 3))[0]
+    assert hit.size == 1
+    return int(hit[0])

@@ -178,3 +178,129 @@ def test_dense_solve_is_accurate():
         xl = xl + solve((b.astype(np.longdouble) - KL @ xl).astype(np.float64)).astype(np.longdouble)
     xt = xl.astype(np.float64)
     assert np.linalg.norm(x - xt) / np.linalg.norm(xt) < 1e-14
+
+
+# ---- 3D (PML3DHexa8, history matrix G, integrated displacement Ubar) -------------------------
+
+CUBE = np.array([[a, b, c] for a, b, c in P.CORNERS3], float) * 0.5 + 0.5   # unit cube, SVL corner order
+
+
+def _m1(s, t):
+    """int_0^1 phi_s phi_t for the 1D hats phi_-(x) = 1 - x, phi_+(x) = x."""
+    return 1 / 3 if s == t else 1 / 6
+
+
+def test_pml3_matrices_symmetric():
+    X = CUBE + np.random.default_rng(4).uniform(-0.15, 0.15, CUBE.shape)
+    mats = P.pml3_matrices(X, 5e7, 0.25, 2000.0, (2.0, 3.0, 1e-4, 0.2, -0.3, 1.5, 1.0, -1.0, -1.0))
+    for A in mats:
+        assert np.abs(A - A.T).max() <= 1e-12 * np.abs(A).max()
+        assert np.abs(A).max() > 0
+
+
+def test_pml3_closed_forms_without_stretching():
+    """R = 1 on the unit cube: K = G = 0; M and C from tensor-product integrals of the hats
+    (int N_j N_k = prod of 1/3 or 1/6; int dN_j/dx N_k = s_j/2 * the other two factors)."""
+    E, nu, rho = 5e7, 0.25, 2000.0
+    lam, mu = P.lame(E, nu)
+    M, C, K, G = P.pml3_matrices(CUBE, E, nu, rho, (2.0, 3.0, 1.0, 0, 0, 0, 1, 1, 1))
+    assert np.abs(K).max() == 0.0 and np.abs(G).max() == 0.0
+    S = P.CORNERS3
+    NN = np.array([[_m1(a[0], b[0]) * _m1(a[1], b[1]) * _m1(a[2], b[2]) for b in S] for a in S])
+    D = [np.array([[a[d] / 2 * np.prod([_m1(a[e], b[e]) for e in range(3) if e != d]) for b in S] for a in S])
+         for d in range(3)]
+    np.testing.assert_allclose(M[0::9, 0::9], rho * NN, rtol=1e-13)
+    np.testing.assert_allclose(M[3::9, 3::9], -(lam + mu) / mu / (3 * lam + 2 * mu) * NN, rtol=1e-13)
+    np.testing.assert_allclose(M[3::9, 5::9], lam / 2 / mu / (3 * lam + 2 * mu) * NN, rtol=1e-13)
+    np.testing.assert_allclose(M[7::9, 7::9], -NN / mu, rtol=1e-13)
+    for d, pairs in P.COUPLE3.items():
+        for pu, ps in pairs:
+            np.testing.assert_allclose(C[pu::9, ps::9], D[d], rtol=1e-12, atol=1e-15)
+            np.testing.assert_allclose(C[ps::9, pu::9], D[d].T, rtol=1e-12, atol=1e-15)
+    assert np.abs(C[0::9, 0::9]).max() == 0.0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_uniform_stretching_laplace_identity(dim):
+    """With the same stretching s(p) = a + b/p along every axis (m = 0), the coordinate
+    stretch multiplies the mass-like operator by s^dim and each derivative by 1/s. In the
+    Laplace domain the element operator p^2 M + p C + K + G/p therefore equals
+    p^2 s^2 M0 + p s C0 in 2D and p^2 s^3 M0 + p s^2 C0 in 3D, M0 and C0 the unstretched (R = 1)
+    mass and coupling. Checked at complex p; this fixes every stretching coefficient of M, C,
+    K and G (each is a binomial term of these products)."""
+    E, nu, rho = 5e7, 0.25, 2000.0
+    lam, mu = P.lame(E, nu)
+    if dim == 2:
+        X = UNIT + np.array([[0.0, 0.0], [0.1, 0.05], [0.05, 0.1], [-0.05, 0.0]])
+        M, C, K = P.pml_matrices(X, E, nu, rho, (1.0, 0.0, 4.0, 1e-3, 0.0, 0.0, 1.0, 1.0))
+        G = np.zeros_like(M)
+        M0, C0, _ = P.pml_matrices(X, E, nu, rho, (1.0, 0.0, 4.0, 1.0, 0.0, 0.0, 1.0, 1.0))
+        a, _, b, _ = P.stretch(0.0, 0.0, rho, mu, lam, (1.0, 0.0, 4.0, 1e-3, 0.0, 0.0, 1.0, 1.0))
+    else:
+        X = CUBE + np.random.default_rng(5).uniform(-0.1, 0.1, CUBE.shape)
+        p9 = (0.0, 4.0, 1e-3, 0.0, 0.0, 0.0, 1.0, -1.0, 1.0)
+        M, C, K, G = P.pml3_matrices(X, E, nu, rho, p9)
+        M0, C0, _, _ = P.pml3_matrices(X, E, nu, rho, p9[:2] + (1.0,) + p9[3:])
+        a, _, _, b, _, _ = P.stretch3(0.0, 0.0, 0.0, rho, mu, lam, p9)
+    assert b > 0 and a > 1
+    for pz in (0.7 + 2.0j, 3.0 - 1.5j, 40.0 + 90.0j):
+        s = a + b / pz
+        Z = pz ** 2 * M + pz * C + K + G / pz
+        ref = (pz ** 2 * s ** 2 * M0 + pz * s * C0) if dim == 2 else (pz ** 2 * s ** 3 * M0 + pz * s ** 2 * C0)
+        assert np.abs(Z - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("lumped", [False, True])
+def test_hexa_matrices_match_oracle(lumped):
+    X = CUBE * [1.2, 0.9, 1.1] + np.random.default_rng(6).uniform(-0.1, 0.1, CUBE.shape)
+    m = Model(ndim=3, coords=X, node_tags=tags_for(8), elem_type=1, conn=np.arange(8)[None, :],
+              E=np.array([3e7]), nu=np.array([0.3]), rho=np.array([1800.0]))
+    m.massform_lumped = lumped
+    K, M, _ = oracle.element_matrices(m, 0)
+    Kp, Mp = P.hexa_matrices(X, 3e7, 0.3, 1800.0, lumped)
+    np.testing.assert_allclose(Kp, K, rtol=1e-12, atol=1e-12 * np.abs(K).max())
+    np.testing.assert_allclose(Mp, M, rtol=1e-12, atol=1e-12 * np.abs(M).max())
+
+
+def test_rod3d_reflection_coefficient():
+    """SVL's J12 layout (PML3DHexa8 under lin3DHexa8, EQUAL ties on uz): the top response of a
+    40-element rod on a 10-element layer differs from an unbounded rod's by about 2R."""
+    from _pml_meshes import to_oracle
+    from svl_amd.pml import soil_rod3d
+    dt, nt = 0.005, 200
+    s = ricker(np.arange(nt) * dt, t0=0.15, f0=10.0)
+    ref_m = to_oracle(soil_rod3d(300, 1, s, R=1.0))     # a 1-element layer with R = 1 is a plain free end far below
+    top = len(ref_m.ndof) - 4
+    ref = ref_m.run_newmark(dt, nt, [(top, 2)])[:, 0]
+    for R in (1e-1, 1e-2):
+        m = to_oracle(soil_rod3d(40, 10, s, R=R))
+        u = m.run_newmark(dt, nt, [(len(m.ndof) - 4, 2)])[:, 0]
+        ratio = np.abs(u - ref).max() / np.abs(ref).max() / (2 * R)
+        print(f"3D rod R={R}: reflected / (2R incident) = {ratio:.3f}")
+        assert 0.9 < ratio < 1.25
+
+
+def test_newmark_with_history_matrix():
+    """A 3D half-space with PML on five faces (corner elements stretched in three directions,
+    so G != 0): every step of the oracle satisfies the equation of motion at t_k,
+    T^T (M A + C V + K U + G Ubar - F) = 0, and Ubar grows by the exact integral of the
+    piecewise-quadratic displacement of constant average acceleration, written here as the
+    Hermite rule dt/2 (U_k-1 + U_k) + dt^2/12 (V_k-1 - V_k)."""
+    from _pml_meshes import to_oracle
+    from svl_amd.pml import halfspace_box3d
+    dt, nt = 0.005, 40
+    mm = to_oracle(halfspace_box3d(2, 2, 2, 1, load_series=ricker(np.arange(nt) * dt, t0=0.05, f0=20.0)))
+    off, T, M, C, K = mm.assemble()
+    G = mm._G
+    assert np.abs(G).max() > 0
+    U, V, A, Ub, F = mm.run_newmark(dt, nt, [], full=True)
+    scale = np.abs(T.T @ F.T).max()
+    for k in range(U.shape[0]):
+        r = T.T @ (M @ A[k] + C @ V[k] + K @ U[k] + G @ Ub[k] - F[k])
+        assert np.abs(r).max() <= 1e-8 * scale
+    U0 = np.vstack([np.zeros(U.shape[1]), U[:-1]])
+    V0 = np.vstack([np.zeros(V.shape[1]), V[:-1]])
+    Ub0 = np.vstack([np.zeros(Ub.shape[1]), Ub[:-1]])
+    herm = Ub0 + dt / 2 * (U0 + U) + dt * dt / 12 * (V0 - V)
+    np.testing.assert_allclose(Ub, herm, rtol=0, atol=1e-12 * np.abs(Ub).max())
+    assert np.abs(Ub).max() > 0
