@@ -22,10 +22,12 @@ dense models:
   + dt/3 G, Feff adds -G (Ubar + dt U + dt^2/6 V), Ubar += dt U + dt/3 dU + dt^2/6 V before
   U moves; G = 0 in 2D), solved densely after a symmetric diagonal scaling (keff_solver).
 
-Pins: tests/test_oracle_pml.py (symmetry, closed forms without stretching, constant-stretching
-scaling, the lin2DQuad4 matrices against the C++ oracle, a soil column whose PML-truncated
-response follows a much longer column while a fixed truncation does not, and the reflection
-coefficient 2R at the free surface that fixes the damping profile b0).
+Pins: tests/test_oracle_pml.py (symmetry, closed forms without stretching, the Laplace-domain
+identity of a uniform stretch in 2D and 3D, the lin2DQuad4 / lin3DHexa8 matrices against the
+C++ oracle, a soil column whose PML-truncated response follows a much longer column while a
+fixed truncation does not, the reflection coefficient 2R of the 2D column and the 3D rod that
+fixes the damping profile b0, the accuracy of the scaled dense solve, and the equation of
+motion with G Ubar plus the Hermite integral of Ubar at every step of a 3D model with G != 0).
 
 Parity unpinned: the evanescent stretch coefficient a0 beyond its R = 1 and m = 0 cases. It
 does not enter the reflection of a propagating wave in the continuous layer, and SVL ships no
