@@ -553,8 +553,13 @@ def main(argv=None):
     ap.add_argument("--cg-rtol", type=float, default=1e-12)
     ap.add_argument("--learned", action="store_true", help="learned preconditioner for the implicit solves")
     a = ap.parse_args(argv)
-    r = read_run(a.file)
-    files = run(r, out_dir=a.out, json_path=a.file, cg_rtol=a.cg_rtol, learned=a.learned)
+    from .pmlfile import is_pml_run, read_pml_run, run_pml
+    if is_pml_run(a.file):
+        r = read_pml_run(a.file)
+        files = run_pml(r, out_dir=a.out, json_path=a.file)
+    else:
+        r = read_run(a.file)
+        files = run(r, out_dir=a.out, json_path=a.file, cg_rtol=a.cg_rtol, learned=a.learned)
     for p in files:
         print(p)
     for s in r.skipped_recorders:

@@ -154,3 +154,30 @@ def reg_node3(m, x, y, z):
     hit = np.nonzero((np.abs(m.coords - [x, y, z]).sum(axis=1) < 1e-9) & (m.ndof == 3))[0]
     assert hit.size == 1
     return int(hit[0])
+
+
+@pytest.mark.parametrize("case", ["F11", "J12"])
+def test_run_file_recorders(tmp_path, case):
+    """SVL's F11 (Newmark) and J12 (ExtendedNewmark) written as run files and run by
+    `python -m svl_amd.svlfile RUN.json`: the NODE recorder files of SVL's recorded nodes
+    (F11: 23 and 203; J12: 45 and 405) against the oracle."""
+    from svl_amd import svlfile
+    load = np.load(GOLD / "F11.npz")["load"]
+    if case == "F11":
+        m, integ, tags = soil_column(90, 10, load), "NEWMARK", [23, 203]
+    else:
+        m, integ, tags = soil_rod3d(90, 10, load), "EXTENDEDNEWMARK", [45, 405]
+    from svl_amd.pmlfile import write_pml_run
+    path = str(tmp_path / "run.json")
+    recs = [(f"{r}.out", r, tags, 15) for r in ("DISP", "VEL", "ACCEL")]
+    write_pml_run(m, path, 0.01, load.size, integrator=integ, recorders=recs)
+    print(f"{case} run file: {load.size - 1} steps", flush=True)
+    svlfile.main([path, "--out", str(tmp_path / "out")])
+    rec = [(t - 1, a) for t in tags for a in range(int(m.ndof[t - 1]))]
+    ou, ov, oa = to_oracle(m).run_newmark(0.01, load.size, rec, k0=1, uva=True)
+    for r, o in (("DISP", ou), ("VEL", ov), ("ACCEL", oa)):
+        ftags, tot, ntotal, nt, vals = svlfile.read_node_recorder(str(tmp_path / "out" / f"{r}.out"))
+        assert ftags == tags and nt == load.size and vals.shape == o.shape
+        for c in range(o.shape[1]):
+            if np.linalg.norm(o[:, c]) > 0:
+                assert rel_l2(vals[:, c], o[:, c]) < TOL, (r, rec[c])
