@@ -489,6 +489,24 @@ __global__ void __launch_bounds__(VB) k_pml_record(int nrec, const int32_t *rec,
     hA[r] = f >= 0 ? g * A[f] : 0.0;
 }
 
+// Whether unknowns p, q of two nodes of an element of class cls can couple. PML elements couple
+// a displacement only with itself (mass-like blocks) and with the stress unknowns of its
+// derivative pairs; stresses couple among themselves only within the normal block or on the
+// diagonal. Pairs outside this pattern are exact zeros in M, C, K and G and stay out of the
+// sparse matrices (their contributions would add 0).
+bool pair_can_couple(int cls, int p, int q) {
+    if (cls == SVLG_CLASS_LIN2DQUAD4 || cls == SVLG_CLASS_LIN3DHEXA8) return true;
+    const int nd = cls == SVLG_CLASS_PML2DQUAD4 ? 2 : 3;  // displacement unknowns
+    const int nn = cls == SVLG_CLASS_PML2DQUAD4 ? 4 : 6;  // end of the normal-stress block
+    const bool up = p < nd, uq = q < nd;
+    if (up && uq) return p == q;
+    if (!up && !uq) return p == q || (p < nn && q < nn);
+    const int pu = up ? p : q, ps = up ? q : p;
+    if (nd == 2) return (pu == 0 && (ps == 2 || ps == 4)) || (pu == 1 && (ps == 3 || ps == 4));
+    return (pu == 0 && (ps == 3 || ps == 6 || ps == 8)) || (pu == 1 && (ps == 4 || ps == 6 || ps == 7)) ||
+           (pu == 2 && (ps == 5 || ps == 7 || ps == 8));
+}
+
 int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(MAXB, (n + VB - 1) / VB)); }
 
 }  // namespace
@@ -624,10 +642,11 @@ int svlg_pml_create(const svlg_mixed_desc *d, double dt, double rtol, int32_t ma
                 fr[a * nun + p] = d->dof_free[t];
                 fa[a * nun + p] = d->dof_factor[t];
             }
+        const int cls = d->elem_class[e];
         for (int i = 0; i < nd; ++i) {
             if (fr[i] < 0) continue;
             for (int j = 0; j < nd; ++j) {
-                if (fr[j] < 0) continue;
+                if (fr[j] < 0 || !pair_can_couple(cls, i % nun, j % nun)) continue;
                 cl.push_back({(int64_t)fr[i] * nfree + fr[j], eoff[e] + (int64_t)i * nd + j, fa[i] * fa[j]});
             }
         }
