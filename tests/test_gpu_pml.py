@@ -181,3 +181,13 @@ def test_run_file_recorders(tmp_path, case):
         for c in range(o.shape[1]):
             if np.linalg.norm(o[:, c]) > 0:
                 assert rel_l2(vals[:, c], o[:, c]) < TOL, (r, rec[c])
+
+
+def test_minres_iteration_limit_stops():
+    """A solve that cannot meet rtol within maxit stops with SVL's stop semantics (the limit is
+    checked on the host between batches; every kernel exits normally)."""
+    from svl_amd._capi import SvlgError
+    m = soil_column(40, 10, ricker(20, 0.01))
+    pm = PmlNewmark(m, 0.01, maxit=1)
+    with pytest.raises(SvlgError, match="did not converge"):
+        pm.advance(1, 5)
