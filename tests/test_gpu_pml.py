@@ -120,7 +120,10 @@ def test_j12_rod_matches_oracle():
     top = len(m.ndof) - 4
     rec = [(r0, 2), (top, 2), (20, 5), (20, 3), (r0 - 4, 2)]
     print("J12 rod: 1000 steps, dt 0.01", flush=True)
-    hu, hv, ha, its = PmlNewmark(m, 0.01).advance(1, nt - 1, rec)
+    pm = PmlNewmark(m, 0.01)
+    t0 = time.perf_counter()
+    hu, hv, ha, its = pm.advance(1, nt - 1, rec)
+    print(f"J12 rod: {(time.perf_counter() - t0) / (nt - 1) * 1e3:.3f} ms per step, {pm.info()}", flush=True)
     ou, ov, oa = to_oracle(m).run_newmark(0.01, nt, rec, k0=1, uva=True)
     for c in range(len(rec)):
         for h, o in ((hu, ou), (hv, ov), (ha, oa)):
