@@ -195,9 +195,11 @@ int svlg_autotune_tiles(const svlg_mesh_desc *desc, const svlg_exec_opts *base, 
 /* ---- ExtendedNewmarkBeta + Linear + LinearSystem ---------------------------------
  * (10-Integrators/03-Newmark/ExtendedNewmarkBeta.cpp, gamma = 1/2, beta = 1/4, G = 0;
  * 09-Algorithms/01-Linear/Linear.cpp:22-56; 11-Solvers/LinearSystem.hpp:44-66.)
- * Needs a lumped mass, elastic materials and a corner-unique mesh (explicit variant 3).
- * Keff = K + 4/dt^2 M + 2/dt C is applied matrix-free; the linear system is solved by
- * preconditioned conjugate gradients to ||b - Keff x|| <= rtol ||b||. */
+ * Needs a corner-unique mesh (explicit variant 3). The mass is the lumped diagonal or SVL's
+ * consistent element mass (massform CONSISTENT, its default: Driver.hpp:1998,
+ * lin3DHexa8.cpp:243-285), applied element by element; models with a domain reduction load
+ * need the lumped mass. Keff = K + 4/dt^2 M + 2/dt C is applied matrix-free; the linear system
+ * is solved by preconditioned conjugate gradients to ||b - Keff x|| <= rtol ||b||. */
 /* Initialize (:36-63 + ComputeEffectiveStiffness :233-245): zero state, Keff terms and
  * the preconditioner (SVLG_PRECOND_JACOBI). */
 int svlg_nm_init(svlg_model *m, double dt, double rtol, int32_t maxit, int32_t precond);

@@ -97,6 +97,9 @@ struct Model {
 
     // Diagonals (node-DOF layout).
     DevBuf<double> mdiag, cdiag;      // lumped mass, damping
+    DevBuf<double> cdash;             // damping of the dashpots alone (consistent-mass operators)
+    std::vector<double> h_rho, h_am;  // [nelem] density, Rayleigh am (host)
+    std::vector<double> h_escale;     // [nelem] |J|/8 (Hexa8) or |J| th/4 (Quad4) on the table paths
     DevBuf<double> scratch;           // [nelem*nen] per-element lumped masses
 
     // Loads.
@@ -163,6 +166,10 @@ struct Model {
     DevBuf<double> dgn;               // (4/dt^2) M + (2/dt) C_diag   (lumped, per DOF)
     DevBuf<double> jac;               // 1 / diag(Keff), 0 on restrained DOFs
     DevBuf<double> coef_keff, coef_rhs;  // per tile element: 1 + 2 ak/dt, -ak
+    // Consistent mass (MassFormulation CONSISTENT), per tile element: the element-mass scale
+    // rs (rho |J|/8, rho th |J|/4 on the table paths, else rho (th)) times 4/dt^2 + 2 am/dt for
+    // the operator, rs alone and am for the right-hand side.
+    DevBuf<double> cmass_keff, cmass_rhs, cmass_am;
     DevBuf<double> cg_r, cg_z, cg_q, cg_b, cg_p[2];
     DevBuf<double> part;              // partial sums of a solve (CgParts)
     DevBuf<double> scal;              // device scalars of the solver

@@ -608,6 +608,124 @@ __device__ __host__ inline double quad_center_geometry(const double (&x)[4][2], 
     return fabs(det);
 }
 
+// ------------------------------------------------------------- consistent mass
+// M_e w with SVL's consistent mass sum_g w rho (th) |J| H^T H (lin2DQuad4.cpp:243-285,
+// lin3DHexa8.cpp:243-285, before the lumping branch), added to f.
+// Affine elements: |J| is constant and the 2x2(x2) Gauss sum of N_a N_b factorises per
+// direction into [[(1+g^2)/2, (1-g^2)/2], [(1-g^2)/2, (1+g^2)/2]], whose eigenvectors are
+// the constant (eigenvalue 1) and the linear (eigenvalue g^2) corner patterns. The element
+// matrix is therefore diagonal in the modal basis of the corner values: the constant mode
+// keeps weight 1, each linear factor of a mode contributes g^2. rs = rho |J| / 8 (Hexa8) or
+// rho th |J| / 4 (Quad4): the same scaling the stiffness table paths use for lambda, mu.
+__device__ __forceinline__ void hexa_mass_modal(double rs, const double (&w)[8][3], double (&f)[8][3]) {
+    const double g2 = kGP * kGP, g4 = g2 * g2, g6 = g4 * g2;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double v[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) v[a] = w[a][c];
+        double mx, my, mz, mxy, myz, mzx, mxyz;
+        hexa_modes(v, mx, my, mz, mxy, myz, mzx, mxyz);
+        const double m0 = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+        double o[8];
+        hexa_unmodes(g2 * mx, g2 * my, g2 * mz, g4 * mxy, g4 * myz, g4 * mzx, g6 * mxyz, o);
+#pragma unroll
+        for (int a = 0; a < 8; ++a) f[a][c] += rs * (o[a] + m0);
+    }
+}
+
+__device__ __forceinline__ void quad_mass_modal(double rs, const double (&w)[4][2], double (&f)[4][2]) {
+    const double g2 = kGP * kGP, g4 = g2 * g2;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        double v[4] = {w[0][c], w[1][c], w[2][c], w[3][c]};
+        double mx, my, mxy;
+        quad_modes(v, mx, my, mxy);
+        const double m0 = (v[0] + v[1]) + (v[2] + v[3]);
+        double o[4];
+        quad_unmodes(g2 * mx, g2 * my, g4 * mxy, o);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) f[a][c] += rs * (o[a] + m0);
+    }
+}
+
+// General quadrilateral / hexahedron: the Gauss-point sum itself, f += sum_g rho (th) |J_g|
+// N(g) (N(g) . w). rt = rho th (Quad4) or rho (Hexa8).
+__device__ __forceinline__ void quad_mass_gauss(const double (&x)[4][2], double rt, const double (&w)[4][2],
+                                                double (&f)[4][2]) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        double r, s, t;
+        gauss_point(g, r, s, t);
+        double dn[4][2];
+        const double sc = rt * quad_dn(x, r, s, dn);
+        const double h[4] = {1.0 / 4.0 * (1.0 - r) * (1.0 - s), 1.0 / 4.0 * (1.0 + r) * (1.0 - s),
+                             1.0 / 4.0 * (1.0 + r) * (1.0 + s), 1.0 / 4.0 * (1.0 - r) * (1.0 + s)};
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const double hw = (h[0] * w[0][c] + h[1] * w[1][c]) + (h[2] * w[2][c] + h[3] * w[3][c]);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) f[a][c] += sc * h[a] * hw;
+        }
+    }
+}
+
+__device__ __forceinline__ void hexa_mass_gauss(const double (&x)[8][3], double rt, const double (&w)[8][3],
+                                                double (&f)[8][3]) {
+#pragma unroll 1
+    for (int g = 0; g < 8; ++g) {
+        double r, s, t;
+        gauss_point(g, r, s, t);
+        double dn[8][3];
+        const double sc = rt * hexa_dn(x, r, s, t, dn);
+        double h[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) h[a] = 1.0 / 8.0 * (1.0 + sgx(a) * r) * (1.0 + sgy(a) * s) * (1.0 + sgz(a) * t);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double hw = 0.0;
+#pragma unroll
+            for (int a = 0; a < 8; ++a) hw += h[a] * w[a][c];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) f[a][c] += sc * h[a] * hw;
+        }
+    }
+}
+
+// Consistent element mass of one tile element added to f: PATH 2, 3 take rs = rho |J| / 8 (Hexa8)
+// or rho th |J| / 4 (Quad4) from the tile arrays; PATH 1 forms |J| from the staged coordinates;
+// PATH 0 integrates at the Gauss points (rs = rho, times th in 2D).
+template <int DIM, int PATH>
+__device__ __forceinline__ void tile_elem_mass(const int (&lc)[Elem<DIM>::nen], const double *sX, double rs,
+                                               const double (&w)[Elem<DIM>::nen][DIM],
+                                               double (&f)[Elem<DIM>::nen][DIM]) {
+    constexpr int NEN = Elem<DIM>::nen;
+    if constexpr (PATH >= 2) {
+        if constexpr (DIM == 3)
+            hexa_mass_modal(rs, w, f);
+        else
+            quad_mass_modal(rs, w, f);
+    } else {
+        double x[NEN][DIM];
+#pragma unroll
+        for (int c = 0; c < NEN; ++c)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) x[c][d] = sX[lc[c] * DIM + d];
+        if constexpr (PATH == 1) {
+            double Lam[DIM][DIM];
+            if constexpr (DIM == 3)
+                hexa_mass_modal(rs * 0.125 * hexa_center_geometry(x, Lam), w, f);
+            else
+                quad_mass_modal(rs * 0.25 * quad_center_geometry(x, Lam), w, f);
+        } else {
+            if constexpr (DIM == 3)
+                hexa_mass_gauss(x, rs, w, f);
+            else
+                quad_mass_gauss(x, rs, w, f);
+        }
+    }
+}
+
 // Element force f = K_e u for the operand u of one tile element; lam and mu carry any
 // per-element factor (and |J|-scaling on the geometry-table paths). PATH: 0 Gauss points on
 // staged coordinates sX, 1 modal form on staged coordinates, 2 modal form with the inverse

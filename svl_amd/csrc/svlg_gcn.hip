@@ -78,6 +78,8 @@ struct GcnArgs {
     int iter, normalize;
     const int *jdev;           // non-null: the iteration is *jdev + iter (graph replays)
     const float *elamf, *emuf, *dgnf;  // FP32 copies of elam, emu, dgn (one-channel kernel)
+    const double *cmass;       // consistent mass: (4/dt^2 + 2 am/dt) times the element-mass scale
+                               // per tile element (null: lumped, inside dgn)
 };
 
 __device__ __forceinline__ float mixw(const float *M, int C, int c, int k, int trans) {
@@ -112,6 +114,7 @@ __device__ __forceinline__ void gcn_elem_valu(const GcnArgs &a, int e0, int ne, 
 #pragma unroll
         for (int d = 0; d < DIM; ++d) u[c][d] = double(sXc[lc[c] * DIM + d]);
     tile_elem_force<DIM, PATH>(a.geo, a.th, lc, sCoord, lam, mu, gid, u, f);
+    if (a.cmass) tile_elem_mass<DIM, PATH>(lc, sCoord, a.cmass[e0 + tid], u, f);
 #pragma unroll
     for (int c = 0; c < NEN; ++c)
         if (lc[c] < nown) {
@@ -665,6 +668,7 @@ static GcnArgs gcn_args(const Model &m) {
     a.max_own = t.max_own;
     a.kmat = m.gcn_kmat.p;
     a.dgn = m.dgn.p;
+    a.cmass = m.lumped ? nullptr : m.cmass_keff.p;
     a.s = m.gcn_s.p;
     a.sf = m.gcn_sf.p;
     a.C = m.gcn_C;
@@ -965,8 +969,9 @@ int svlg_gnn_init(svlg_model *h, int32_t channels, int32_t layers, int32_t mfma,
     SVLG_CHECK(m.nm_ready, "svlg_nm_init has not been called");
     SVLG_CHECK(channels >= 1 && channels <= kMaxC, "channels must be in [1, 4]");
     SVLG_CHECK(layers >= 2 && layers <= kMaxL, "layers must be in [2, 16]");
-    const bool mfma_ok = m.ndim == 3 && m.elem_path >= 2 && m.ngeo == 1;
-    SVLG_CHECK(!(mfma == 1 && !mfma_ok), "the MFMA element step needs a Hexa8 mesh with a single element geometry");
+    const bool mfma_ok = m.ndim == 3 && m.elem_path >= 2 && m.ngeo == 1 && m.lumped;
+    SVLG_CHECK(!(mfma == 1 && !mfma_ok),
+               "the MFMA element step needs a Hexa8 mesh with a single element geometry and a lumped mass");
     m.gcn_ready = false;
     m.drop_cg_graphs();  // the network buffers are reallocated below
     m.gcn_C = channels;

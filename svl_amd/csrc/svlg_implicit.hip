@@ -10,9 +10,11 @@
 //   LinearSystem::SolveSystem / GetSolution         (RP/11-Solvers/LinearSystem.hpp:44-66): SVL
 //     factors Keff (EigenSolver LDL^T, MUMPS) or calls PETSc KSP; here CG to a relative
 //     residual tolerance ||b - Keff x|| <= rtol ||b||.
-// With lumped mass, Rayleigh damping C_e = am M_e + ak K_e and dashpots, and G = 0:
+// With Rayleigh damping C_e = am M_e + ak K_e and dashpots, and G = 0:
 //   Keff = sum_e (1 + 2 ak_e/dt) K_e + (4/dt^2) M + (2/dt) (sum_e am_e M_e + C_dash)
 //   Feff = Fext - sum_e K_e (U_e - ak_e V_e) + C_diag V + M (4/dt V + A)
+// The lumped mass enters through per-DOF diagonals; the consistent mass (SVL's default
+// massform) element by element, beside the stiffness, in the same tile pass.
 // The element operator reuses the node tiles of the explicit corner-slot kernel: a
 // workgroup stages the operand of its owned + halo nodes in LDS, recomputes every element
 // touching its nodes and sums the corner slots of each owned node in a fixed order, so a
@@ -73,6 +75,10 @@ struct OpArgs {
     const int *jdev;       // non-null: the iteration is *jdev + iter (graph replays)
     int nelem;             // elements (the Gauss-point state stride)
     DrmRef drm;            // OP_RHS: domain reduction forces of every step (F null: none)
+    // Consistent mass (kernels with CM): per tile element the mass scale (OP_CG: times
+    // 4/dt^2 + 2 am/dt; OP_RHS: alone) and, for OP_RHS, am. cdiag then holds the dashpots alone
+    // and dgn (2/dt) times them.
+    const double *cmass, *cam;
 };
 
 // One node tile of the effective-stiffness operator (one thread per tile element in the
@@ -87,11 +93,17 @@ struct OpArgs {
 // MOC > 0: every tile owns at most MOC nodes; the slot planes then have the compile-time
 // stride MOC + 1 with column MOC as the trash column (branch-free corner writes, immediate
 // plane offsets).
-template <int DIM, int PATH, int MODE, bool NL, int MOC = 0>
+// CM: consistent element mass (MassFormulation CONSISTENT). Every element adds its mass
+// matrix times the operand: OP_CG (4/dt^2 + 2 am/dt) M_e p; OP_RHS stages the mass operand
+// w = 4/dt V + A (Newton: 4/dt V + A - 4/dt^2 dU) and subtracts M_e (w_e + am V_e) from its
+// slot, so that the node sum gives Feff = Fext - sum_e [K_e (U - ak V) - M_e (w + am V)]_e
+// + C_dash V.
+template <int DIM, int PATH, int MODE, bool NL, int MOC = 0, bool CM = false>
 __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
     constexpr int NEN = Elem<DIM>::nen;
     constexpr bool STAGE_X = PATH < 2 || NL;  // J2 Gauss-point loops need the coordinates
     constexpr bool RHS = MODE == OP_RHS;
+    constexpr int NPRE = RHS ? (CM ? 3 : 2) : 1;  // staged state arrays before the coordinates
     extern __shared__ double smem[];
     __shared__ double red[3 * 17];
     double beta = 0.0;
@@ -138,8 +150,9 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
     const int MS = a.max_stage, MO = MOC > 0 ? MOC + 1 : a.max_own;
     double *sU = smem;
     double *sV = smem + size_t(MS) * DIM;
-    double *sX = smem + size_t((RHS ? 2 : 1) * MS) * DIM;
-    double *sIn = smem + size_t(((RHS ? 2 : 1) + (STAGE_X ? 1 : 0)) * MS) * DIM;
+    double *sW = smem + size_t(2 * MS) * DIM;  // CM && RHS: mass operand
+    double *sX = smem + size_t(NPRE * MS) * DIM;
+    double *sIn = smem + size_t((NPRE + (STAGE_X ? 1 : 0)) * MS) * DIM;
     const int tid = threadIdx.x, B = blockDim.x;
     const size_t g0 = size_t(o0) * DIM;
     const bool has_elem = tid < ne;
@@ -169,6 +182,9 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
         if constexpr (RHS) {
             sU[i] = a.U[g];
             sV[i] = a.V[g];
+            if constexpr (CM)
+                sW[i] = a.Vm ? 4.0 / a.dt * a.Vm[g] + a.A[g] - 4.0 / a.dt / a.dt * a.dUn[g]
+                             : 4.0 / a.dt * a.V[g] + a.A[g];
         } else {
             const double v = a.z[g] + beta * a.p_old[g];
             sU[i] = v;
@@ -184,6 +200,9 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
             if constexpr (RHS) {
                 sU[i] = a.U[n + c];
                 sV[i] = a.V[n + c];
+                if constexpr (CM)
+                    sW[i] = a.Vm ? 4.0 / a.dt * a.Vm[n + c] + a.A[n + c] - 4.0 / a.dt / a.dt * a.dUn[n + c]
+                                 : 4.0 / a.dt * a.V[n + c] + a.A[n + c];
             } else {
                 sU[i] = a.z[n + c] + beta * a.p_old[n + c];
             }
@@ -224,6 +243,24 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
                     if constexpr (RHS) u[c][d] += c2 * sV[lc[c] * DIM + d];
                 }
             tile_elem_force<DIM, PATH>(a.geo, a.th, lc, sX, lam, mu, gid, u, f);
+        }
+        if constexpr (CM) {
+            double w[NEN][DIM];
+            double rs = a.cmass[e0 + tid];
+            if constexpr (RHS) {
+                const double am = a.cam[e0 + tid];
+                rs = -rs;
+#pragma unroll
+                for (int c = 0; c < NEN; ++c)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) w[c][d] = sW[lc[c] * DIM + d] + am * sV[lc[c] * DIM + d];
+            } else {
+#pragma unroll
+                for (int c = 0; c < NEN; ++c)
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) w[c][d] = sU[lc[c] * DIM + d];
+            }
+            tile_elem_mass<DIM, PATH>(lc, sX, rs, w, f);
         }
         if constexpr (MOC > 0) {
 #pragma unroll
@@ -267,7 +304,10 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
                 // the step's V and dU.
                 const double mt = a.Vm ? 4.0 / a.dt * a.Vm[g] + a.A[g] - 4.0 / a.dt / a.dt * a.dUn[g]
                                        : 4.0 / a.dt * v + a.A[g];
-                a.b[g] = freedof ? ((fe - fi) + a.cdiag[g] * v) + a.mdiag[g] * mt : 0.0;
+                if constexpr (CM)
+                    a.b[g] = freedof ? (fe - fi) + a.cdiag[g] * v : 0.0;
+                else
+                    a.b[g] = freedof ? ((fe - fi) + a.cdiag[g] * v) + a.mdiag[g] * mt : 0.0;
             } else {
                 const double x = sU[tid * DIM + d];
                 const double yv = freedof ? fi + a.dgn[g] * x : 0.0;
@@ -427,11 +467,13 @@ __global__ void k_nm_update(int n, double dt, const double *__restrict__ dU, con
 
 // Diagonal of sum_e (1 + 2 ak_e/dt) K_e by unit probes through the Gauss-point routine
 // (valid for every element shape), one colour class per launch so no two threads of a
-// launch touch the same node.
+// launch touch the same node. rho non-null (consistent mass): plus (4/dt^2 + 2 am_e/dt) times
+// the diagonal of the consistent element mass.
 template <int DIM>
 __global__ void k_kdiag_colored(int e0, int e1, const int32_t *__restrict__ conn_c, const int32_t *__restrict__ eid_c,
                                 const double *__restrict__ X, const double *__restrict__ lam,
                                 const double *__restrict__ mu, const double *__restrict__ ak, double inv_dt, double th,
+                                const double *__restrict__ rho, const double *__restrict__ am,
                                 double *__restrict__ kd) {
     const int e = e0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= e1) return;
@@ -459,7 +501,19 @@ __global__ void k_kdiag_colored(int e0, int e1, const int32_t *__restrict__ conn
                 quad_fint(x, u, lam[tag], mu[tag], th, f);
             else
                 hexa_fint(x, u, lam[tag], mu[tag], f);
-            kd[size_t(nodes[a]) * DIM + d] += cf * f[a][d];
+            double v = cf * f[a][d];
+            if (rho) {
+#pragma unroll
+                for (int b = 0; b < nen; ++b)
+#pragma unroll
+                    for (int c = 0; c < DIM; ++c) f[b][c] = 0.0;
+                if constexpr (DIM == 2)
+                    quad_mass_gauss(x, rho[tag] * th, u, f);
+                else
+                    hexa_mass_gauss(x, rho[tag], u, f);
+                v += (4.0 * inv_dt * inv_dt + 2.0 * am[tag] * inv_dt) * f[a][d];
+            }
+            kd[size_t(nodes[a]) * DIM + d] += v;
         }
 }
 
@@ -476,11 +530,12 @@ __global__ void k_jacobi(int n, const int32_t *__restrict__ fdof, const double *
     jac[i] = d > 0.0 ? 1.0 / d : 0.0;
 }
 
+// M null (consistent mass, applied by the elements): the dashpot term alone.
 __global__ void k_dgn(int n, double dt, const double *__restrict__ M, const double *__restrict__ C,
                       const int32_t *__restrict__ fdof, double *__restrict__ dgn) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    dgn[i] = fdof[i] < 0 ? 0.0 : 4.0 / dt / dt * M[i] + 2.0 / dt * C[i];
+    dgn[i] = fdof[i] < 0 ? 0.0 : (M ? 4.0 / dt / dt * M[i] + 2.0 / dt * C[i] : 2.0 / dt * C[i]);
 }
 
 __global__ void k_record_nm(int nrec, const int32_t *__restrict__ rec, const double *__restrict__ U,
@@ -525,32 +580,32 @@ static OpArgs op_args(const Model &m) {
 // Compile-time slot stride of the CG operator on axis-aligned Hexa8 tiles of up to 168 or 336
 // owned nodes (0: the stride read from the tile set).
 static int op_moc(const Model &m, int mode) {
-    if (mode != OP_CG || m.ndim != 3 || m.elem_path != 3) return 0;
+    if (mode != OP_CG || m.ndim != 3 || m.elem_path != 3 || !m.lumped) return 0;
     const int mo = m.tiles.max_own;
     return mo <= 168 ? 168 : mo <= 336 ? 336 : 0;
 }
 
 static size_t op_lds(const Model &m, int mode) {
-    const int staged = (mode == OP_RHS ? 2 : 1) + (m.elem_path < 2 || m.has_j2 ? 1 : 0);
+    const int staged = (mode == OP_RHS ? (m.lumped ? 2 : 3) : 1) + (m.elem_path < 2 || m.has_j2 ? 1 : 0);
     const int moc = op_moc(m, mode);
     const size_t stride = moc ? moc + 1 : m.tiles.max_own;
     return (size_t(staged) * m.tiles.max_stage * m.ndim + size_t(m.nen) * m.ndim * stride) * sizeof(double);
 }
 
-template <int DIM, int MODE>
-static const void *op_fn_t(int path, bool nl) {
-    if (nl) switch (path) {
-            case 3: return reinterpret_cast<const void *>(&k_nm_op<DIM, 3, MODE, true>);
-            case 2: return reinterpret_cast<const void *>(&k_nm_op<DIM, 2, MODE, true>);
-            case 1: return reinterpret_cast<const void *>(&k_nm_op<DIM, 1, MODE, true>);
-            default: return reinterpret_cast<const void *>(&k_nm_op<DIM, 0, MODE, true>);
-        }
+template <int DIM, int MODE, bool NL, bool CM>
+static const void *op_fn_p(int path) {
     switch (path) {
-        case 3: return reinterpret_cast<const void *>(&k_nm_op<DIM, 3, MODE, false>);
-        case 2: return reinterpret_cast<const void *>(&k_nm_op<DIM, 2, MODE, false>);
-        case 1: return reinterpret_cast<const void *>(&k_nm_op<DIM, 1, MODE, false>);
-        default: return reinterpret_cast<const void *>(&k_nm_op<DIM, 0, MODE, false>);
+        case 3: return reinterpret_cast<const void *>(&k_nm_op<DIM, 3, MODE, NL, 0, CM>);
+        case 2: return reinterpret_cast<const void *>(&k_nm_op<DIM, 2, MODE, NL, 0, CM>);
+        case 1: return reinterpret_cast<const void *>(&k_nm_op<DIM, 1, MODE, NL, 0, CM>);
+        default: return reinterpret_cast<const void *>(&k_nm_op<DIM, 0, MODE, NL, 0, CM>);
     }
+}
+
+template <int DIM, int MODE>
+static const void *op_fn_t(int path, bool nl, bool cm) {
+    if (nl) return cm ? op_fn_p<DIM, MODE, true, true>(path) : op_fn_p<DIM, MODE, true, false>(path);
+    return cm ? op_fn_p<DIM, MODE, false, true>(path) : op_fn_p<DIM, MODE, false, false>(path);
 }
 
 static const void *op_fn_sel(const Model &m, int mode, bool nl) {
@@ -563,9 +618,10 @@ static const void *op_fn_sel(const Model &m, int mode, bool nl) {
                       : reinterpret_cast<const void *>(&k_nm_op<3, 3, OP_CG, false, 336>);
         default: break;
     }
+    const bool cm = !m.lumped;
     if (m.ndim == 3)
-        return mode == OP_CG ? op_fn_t<3, OP_CG>(m.elem_path, nl) : op_fn_t<3, OP_RHS>(m.elem_path, nl);
-    return mode == OP_CG ? op_fn_t<2, OP_CG>(m.elem_path, nl) : op_fn_t<2, OP_RHS>(m.elem_path, nl);
+        return mode == OP_CG ? op_fn_t<3, OP_CG>(m.elem_path, nl, cm) : op_fn_t<3, OP_RHS>(m.elem_path, nl, cm);
+    return mode == OP_CG ? op_fn_t<2, OP_CG>(m.elem_path, nl, cm) : op_fn_t<2, OP_RHS>(m.elem_path, nl, cm);
 }
 
 static const void *op_fn(const Model &m, int mode) {
@@ -576,8 +632,16 @@ static const void *op_fn(const Model &m, int mode) {
     return op_fn_sel(m, mode, nl);
 }
 
-static int launch_op(Model &m, int mode, const OpArgs &a) {
-    void *args[] = {const_cast<OpArgs *>(&a)};
+static int launch_op(Model &m, int mode, const OpArgs &a0) {
+    OpArgs a = a0;
+    if (!m.lumped) {
+        // Consistent mass: the element mass coefficients of this mode; the node terms keep the
+        // dashpots alone.
+        a.cmass = mode == OP_CG ? m.cmass_keff.p : m.cmass_rhs.p;
+        a.cam = m.cmass_am.p;
+        if (mode == OP_RHS) a.cdiag = m.cdash.p;
+    }
+    void *args[] = {&a};
     SVLG_HIP(hipLaunchKernel(op_fn(m, mode), dim3(m.tiles.ntiles), dim3(m.tiles.block), args, op_lds(m, mode),
                              m.stream));
     return 0;
@@ -780,7 +844,9 @@ int svlg_nm_init(svlg_model *h, double dt, double rtol, int32_t maxit, int32_t p
     SVLG_CHECK(h, "null model");
     Model &m = h->m;
     SVLG_CHECK(dt > 0.0 && rtol > 0.0 && maxit > 0, "dt, rtol and maxit must be positive");
-    SVLG_CHECK(m.lumped, "the device Newmark path needs MassFormulation = LUMPED");
+    SVLG_CHECK(m.lumped || m.ndrm == 0,
+               "domain reduction loads on the device need MassFormulation = LUMPED (their element forces are "
+               "evaluated with a diagonal mass)");
     SVLG_CHECK(m.variant == 3, "the device Newmark path needs corner-slot node tiles (a corner-unique mesh)");
     SVLG_CHECK(precond == SVLG_PRECOND_JACOBI,
                "svlg_nm_init takes SVLG_PRECOND_JACOBI; switch with svlg_nm_set_precond after svlg_gnn_init");
@@ -815,9 +881,25 @@ int svlg_nm_init(svlg_model *h, double dt, double rtol, int32_t maxit, int32_t p
     }
     SVLG_HIP(m.coef_keff.upload(ck.data(), ck.size(), s));
     SVLG_HIP(m.coef_rhs.upload(cr.data(), cr.size(), s));
-    // dgn = (4/dt^2) M + (2/dt) C_diag; Jacobi = 1 / diag(Keff).
+    if (!m.lumped) {
+        // Consistent mass: rs = rho |J|/8 (rho th |J|/4) on the table paths, rho (rho th) where the
+        // element routine forms |J| itself.
+        std::vector<double> mk(ck.size()), mr(ck.size()), ma(ck.size());
+        for (size_t i = 0; i < ck.size(); ++i) {
+            const int e = m.tile_elem_id[i];
+            const double rs = m.elem_path >= 2 ? m.h_rho[e] * m.h_escale[e] : m.h_rho[e] * (m.ndim == 2 ? m.th : 1.0);
+            mk[i] = rs * (4.0 / dt / dt + 2.0 * m.h_am[e] / dt);
+            mr[i] = rs;
+            ma[i] = m.h_am[e];
+        }
+        SVLG_HIP(m.cmass_keff.upload(mk.data(), mk.size(), s));
+        SVLG_HIP(m.cmass_rhs.upload(mr.data(), mr.size(), s));
+        SVLG_HIP(m.cmass_am.upload(ma.data(), ma.size(), s));
+    }
+    // dgn = (4/dt^2) M + (2/dt) C_diag (lumped), (2/dt) C_dash (consistent); Jacobi = 1 / diag(Keff).
     const int nb = (n + 255) / 256;
-    hipLaunchKernelGGL(k_dgn, dim3(nb), dim3(256), 0, s, n, dt, m.mdiag.p, m.cdiag.p, m.fdof.p, m.dgn.p);
+    hipLaunchKernelGGL(k_dgn, dim3(nb), dim3(256), 0, s, n, dt, m.lumped ? m.mdiag.p : nullptr,
+                       m.lumped ? m.cdiag.p : m.cdash.p, m.fdof.p, m.dgn.p);
     DevBuf<double> kd;
     SVLG_HIP(kd.alloc(n));
     SVLG_HIP(hipMemsetAsync(kd.p, 0, sizeof(double) * n, s));
@@ -826,10 +908,12 @@ int svlg_nm_init(svlg_model *h, double dt, double rtol, int32_t maxit, int32_t p
         if (e1 <= e0) continue;
         if (m.ndim == 3)
             hipLaunchKernelGGL(k_kdiag_colored<3>, dim3((e1 - e0 + 127) / 128), dim3(128), 0, s, e0, e1, m.conn_c.p,
-                               m.eid_c.p, m.X.p, m.lam.p, m.mu.p, m.ak.p, 1.0 / dt, m.th, kd.p);
+                               m.eid_c.p, m.X.p, m.lam.p, m.mu.p, m.ak.p, 1.0 / dt, m.th,
+                               m.lumped ? nullptr : m.rho.p, m.am.p, kd.p);
         else
             hipLaunchKernelGGL(k_kdiag_colored<2>, dim3((e1 - e0 + 127) / 128), dim3(128), 0, s, e0, e1, m.conn_c.p,
-                               m.eid_c.p, m.X.p, m.lam.p, m.mu.p, m.ak.p, 1.0 / dt, m.th, kd.p);
+                               m.eid_c.p, m.X.p, m.lam.p, m.mu.p, m.ak.p, 1.0 / dt, m.th,
+                               m.lumped ? nullptr : m.rho.p, m.am.p, kd.p);
     }
     DevBuf<int> bad;
     SVLG_HIP(bad.alloc(1));

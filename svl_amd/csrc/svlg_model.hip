@@ -414,6 +414,8 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
     UP(m.lam, lam.data(), lam.size());
     UP(m.mu, mu.data(), mu.size());
     UP(m.rho, d->rho, size_t(ne));
+    m.h_rho.assign(d->rho, d->rho + ne);
+    m.h_am = am;
     UP(m.am, am.data(), am.size());
     UP(m.ak, ak.data(), ak.size());
     UP(m.fdof, m.h_free.data(), m.h_free.size());
@@ -477,6 +479,7 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
     if (m.elem_path >= 2) {
         UP(m.geo, gtab.data(), gtab.size());
     }
+    m.h_escale = escale;
     // Tiles.
     {
         TileSet &t = m.tiles;
@@ -585,7 +588,8 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
     }
     // Mass and damping diagonals.
     if (m.mdiag.alloc(m.ndofn) != hipSuccess || m.cdiag.alloc(m.ndofn) != hipSuccess ||
-        m.scratch.alloc(size_t(ne) * nen) != hipSuccess) {
+        m.cdash.alloc(m.ndofn) != hipSuccess || m.scratch.alloc(size_t(ne) * nen) != hipSuccess ||
+        hipMemsetAsync(m.cdash.p, 0, sizeof(double) * m.ndofn, s) != hipSuccess) {
         set_error("device allocation failed (diagonals)");
         return fail(10);
     }
@@ -625,6 +629,8 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         const int nd = int(dofs.size());
         hipLaunchKernelGGL(k_dashpots, dim3((nd + 255) / 256), dim3(256), 0, s, nd, dd.p, dp.p, de.p, 1e-12,
                            m.cdiag.p);
+        hipLaunchKernelGGL(k_dashpots, dim3((nd + 255) / 256), dim3(256), 0, s, nd, dd.p, dp.p, de.p, 1e-12,
+                           m.cdash.p);
         if (hipStreamSynchronize(s) != hipSuccess) { set_error("dashpot assembly failed"); return fail(10); }
     }
 #undef UP

@@ -7,8 +7,10 @@
 // lin*::ComputeInternalDynamicForces, lin3DHexa8.cpp:416-437). With the lumped mass and
 // C = am M + ak K + dashpots this is, per node,
 //   R = sum_e [K_e (U_e + ak_e V_e)]_node + M_ii A_i + C_diag,ii V_i - Fext_i(k)
-// (J2 elements: B^T sigma from the Gauss-point states + ak B^T D0 B V). Only the requested
-// nodes are computed; the restrained partner node of a dashpot carries -eta V of its node.
+// (J2 elements: B^T sigma from the Gauss-point states + ak B^T D0 B V). With the consistent
+// mass the inertia and the Rayleigh am term come from the element: [M_e (A_e + am_e V_e)]_node,
+// and the node term keeps the dashpots alone. Only the requested nodes are computed; the
+// restrained partner node of a dashpot carries -eta V of its node.
 #include <vector>
 
 #include "svlg_common.hpp"
@@ -37,6 +39,7 @@ struct ReacArgs {
     const double *j2p, *state;  // null state: J2 elements as their initial elastic material
     int nelem;
     DrmRef drm;                 // domain reduction forces of every step (F null: none)
+    const double *rho, *am;     // consistent mass: element density and Rayleigh am (null: lumped)
     double *R;                  // [nq * DIM]
 };
 
@@ -70,8 +73,9 @@ __global__ void k_reactions(ReacArgs a) {
     const double *Ucur = a.mode == 2 ? a.U : a.Uk;
     for (int t = a.n2e_ptr[node]; t < a.n2e_ptr[node + 1]; ++t) {
         const int ea = a.n2e[t], e = ea / NEN, corner = ea % NEN;
-        double x[NEN][DIM], u[NEN][DIM], v[NEN][DIM], f[NEN][DIM];
+        double x[NEN][DIM], u[NEN][DIM], v[NEN][DIM], f[NEN][DIM], w[NEN][DIM];
         const double ak = a.ak[e];
+        const double am = a.rho ? a.am[e] : 0.0;
 #pragma unroll
         for (int c = 0; c < NEN; ++c) {
             const size_t nd = size_t(a.conn[size_t(e) * NEN + c]) * DIM;
@@ -82,6 +86,7 @@ __global__ void k_reactions(ReacArgs a) {
                 x[c][d] = a.X[nd + d];
                 v[c][d] = vv;
                 u[c][d] = Ucur[nd + d] + ak * vv;
+                w[c][d] = aa + am * vv;
             }
         }
         if (a.state && a.mat[e] == SVLG_MAT_J2) {
@@ -92,6 +97,12 @@ __global__ void k_reactions(ReacArgs a) {
             hexa_fint(x, u, a.lam[e], a.mu[e], f);
         } else {
             quad_fint(x, u, a.lam[e], a.mu[e], a.th, f);
+        }
+        if (a.rho) {
+            if constexpr (DIM == 3)
+                hexa_mass_gauss(x, a.rho[e], w, f);
+            else
+                quad_mass_gauss(x, a.rho[e] * a.th, w, f);
         }
 #pragma unroll
         for (int d = 0; d < DIM; ++d) r[d] += f[corner][d];
@@ -109,7 +120,8 @@ __global__ void k_reactions(ReacArgs a) {
         }
         const int ds = drm_slot(a.drm, node);
         if (ds >= 0) fe += drm_force(a.drm, ds, a.k, DIM, d);
-        a.R[size_t(q) * DIM + d] = ((r[d] + a.cdiag[g] * vv) + a.mdiag[g] * aa) - fe;
+        a.R[size_t(q) * DIM + d] =
+            a.rho ? (r[d] + a.cdiag[g] * vv) - fe : ((r[d] + a.cdiag[g] * vv) + a.mdiag[g] * aa) - fe;
     }
 }
 
@@ -160,7 +172,9 @@ int svlg_reactions(svlg_model *h, int32_t n, const int32_t *nodes, int32_t k, do
         a.have_velocity = m.steps_done > 0;
     }
     a.mdiag = m.mdiag.p;
-    a.cdiag = m.cdiag.p;
+    a.cdiag = m.lumped ? m.cdiag.p : m.cdash.p;
+    a.rho = m.lumped ? nullptr : m.rho.p;
+    a.am = m.am.p;
     a.nl_ptr = m.nl_ptr.p;
     a.nl_idx = m.nl_idx.p;
     a.load_dir = m.load_dir.p;

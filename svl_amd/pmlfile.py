@@ -99,6 +99,9 @@ def read_pml_run(path: str) -> PmlRun:
         if (ndim == 2) != (c in (LIN2DQUAD4, PML2DQUAD4)):
             raise ValueError(f"element {e['name']} does not match ndim {ndim}")
         at = e.get("attributes", {})
+        nq = 4 if ndim == 2 else 8
+        if _lc(at.get("rule", "GAUSS")) != "gauss" or int(at.get("np", nq)) != nq:
+            raise ValueError(f"element {e['name']}: only the {nq}-point Gauss rule is on the PML device path")
         mt = mats[int(at["material"])]
         if _lc(mt["name"]) not in ("elastic2dplanestrain", "elastic3dlinear"):
             raise ValueError(f"material {mt['name']} is not on the PML device path")
@@ -115,6 +118,16 @@ def read_pml_run(path: str) -> PmlRun:
             pml.append([float(at["n"]), float(at["L"]), float(at["R"])] + x0 + npml)
         else:
             pml.append([0.0] * 9)
+    # The device PML path has no Rayleigh term (SVL adds am M_e + ak K_e to lin2DQuad4 /
+    # lin3DHexa8 under Rayleigh damping, lin2DQuad4.cpp:352-364): refuse any that is not zero.
+    for _, d in sorted(js.get("Dampings", {}).items(), key=lambda x: int(x[0])):
+        dn, da = _lc(d["name"]), d.get("attributes", {})
+        if dn == "free":
+            continue
+        if dn != "rayleigh":
+            raise ValueError(f"damping {d['name']} is not on the PML device path")
+        if (float(da.get("am", 0.0)) != 0.0 or float(da.get("ak", 0.0)) != 0.0) and da.get("list"):
+            raise ValueError("Rayleigh damping is not on the PML device path")
     sim = js["Simulations"]
     sa = sim["attributes"]
     integ = _lc(sa["integrator"]["name"]).upper()

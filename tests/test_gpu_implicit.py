@@ -160,12 +160,6 @@ def test_newmark_deterministic():
 
 def test_newmark_rejects_unsupported_models():
     m, dt, nt = hexa_case("axis")
-    m.massform_lumped = False
-    dm = DeviceModel(m)
-    with pytest.raises(SvlgError):
-        newmark(dm, 1e-3)
-    dm.close()
-    m, dt, nt = hexa_case("axis")
     dm = DeviceModel(m, variant=1)
     with pytest.raises(SvlgError):
         newmark(dm, 1e-3)

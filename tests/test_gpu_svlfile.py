@@ -28,18 +28,21 @@ def case(kind):
     if kind == "cd":
         m, dt, nt = W.layer2d(16, 8, nt=200)
         return m, dt, 200, "CENTRALDIFFERENCE", "LINEAR", oracle.CENTRAL_DIFFERENCE, {}
-    if kind == "newmark":
+    if kind in ("newmark", "newmark_default_mass"):
         m, dt, nt, src = block_model(4, 4, 3)
         m.ray_am = np.full(m.nelem, 0.5)
         m.ray_ak = np.full(m.nelem, 1e-3)
+        m.massform_lumped = kind == "newmark"
         return m, dt, 120, "EXTENDEDNEWMARK", "LINEAR", oracle.EXTENDED_NEWMARK, {}
     m, dt, nt = W.soil_structure_j2(8, 8, 4, nt=41, block=(2, 2, 2), Ap=2e7, t0=0.08, f0=8.0)
     return m, dt, 41, "NEWMARK", "NEWTON", oracle.NEWMARK, dict(algorithm=oracle.NEWTON, nmax=20, cnvgtest=5,
                                                                    cnvgtol=1e-6)
 
 
-@pytest.mark.parametrize("kind", ["cd", "newmark", "newton"])
+@pytest.mark.parametrize("kind", ["cd", "newmark", "newmark_default_mass", "newton"])
 def test_run_file_through_device(tmp_path, kind):
+    """newmark_default_mass: the run file names no massform, so SVL's default (CONSISTENT,
+    Driver.hpp:1998) applies."""
     m, dt, nt, integ, alg, ointeg, okw = case(kind)
     nodes = [0, m.nnodes // 3, m.nnodes - 1, int(m.load_node[0])]
     ntags = [n + 1 for n in nodes]
@@ -58,6 +61,13 @@ def test_run_file_through_device(tmp_path, kind):
     import os
     os.makedirs(os.path.dirname(p))
     S.write_run(m, p, integ, dt, nt, alg, cnvgtol=1e-6, nmax=20, cnvgtest=5, recorders=recs)
+    if kind == "newmark_default_mass":
+        import json
+        with open(p) as f:
+            js = json.load(f)
+        assert js["Global"].pop("massform") == "CONSISTENT"
+        with open(p, "w") as f:
+            json.dump(js, f)
     r = S.read_run(p)
     out_dir = str(tmp_path / "Solution")
     S.run(r, out_dir=out_dir, cg_rtol=1e-13)

@@ -54,3 +54,33 @@ def test_refuses_multi_master_constraint(tmp_path):
     json.dump(js, open(path, "w"))
     with pytest.raises(ValueError, match="one master"):
         read_pml_run(path)
+
+
+def test_refuses_rayleigh_damping(tmp_path):
+    """The device PML path has no Rayleigh term: a run file whose Dampings give the elements a
+    nonzero am or ak is refused; Free damping (or Rayleigh with zero coefficients) passes."""
+    m = soil_column(4, 2, np.ones(5))
+    path = str(tmp_path / "run.json")
+    write_pml_run(m, path, 0.01, 5)
+    js = json.load(open(path))
+    tags = [int(t) for t in js["Elements"]]
+    js["Dampings"] = {"1": {"name": "Rayleigh", "attributes": {"am": 0.0, "ak": 0.0, "list": tags}}}
+    json.dump(js, open(path, "w"))
+    read_pml_run(path)
+    js["Dampings"]["1"]["attributes"]["ak"] = 1e-3
+    json.dump(js, open(path, "w"))
+    with pytest.raises(ValueError, match="Rayleigh"):
+        read_pml_run(path)
+
+
+@pytest.mark.parametrize("attr", [{"rule": "LOBATTO"}, {"np": 9}])
+def test_refuses_other_quadrature(tmp_path, attr):
+    m = soil_column(4, 2, np.ones(5))
+    path = str(tmp_path / "run.json")
+    write_pml_run(m, path, 0.01, 5)
+    js = json.load(open(path))
+    e = next(iter(js["Elements"].values()))
+    e.setdefault("attributes", {}).update(attr)
+    json.dump(js, open(path, "w"))
+    with pytest.raises(ValueError, match="Gauss rule"):
+        read_pml_run(path)
