@@ -216,6 +216,13 @@ int svlg_nm_get_state(svlg_model *m, double *U, double *V, double *A);
 int svlg_nm_set_precond(svlg_model *m, int32_t precond);
 /* y = Keff x over the free DOFs (the matrix of LinearSystem::SetMatrix, applied). */
 int svlg_keff_apply(svlg_model *m, const double *x /*[nfree]*/, double *y /*[nfree]*/);
+/* y = Keff x as svlg_keff_apply, allocating nothing: no counterpart in SVL. For callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). The work vectors live in a fixed device array of 3 x 2^20 doubles (24 MiB,
+ * held by every GPU the library runs on until the process ends); models with more than 2^20
+ * node DOFs (nodes x ndim) are refused. Calls that overlap on one GPU share that array and race.
+ * Same kernels, so the same bits as svlg_keff_apply. */
+int svlg_keff_apply_noalloc(svlg_model *m, const double *x /*[nfree]*/, double *y /*[nfree]*/);
 /* Solves Keff x = b from x = 0 (LinearSystem::SetVector + SolveSystem + GetSolution). */
 int svlg_solve(svlg_model *m, const double *b /*[nfree]*/, double *x /*[nfree]*/, double rtol,
                int32_t maxit, int32_t *iters, double *relres);

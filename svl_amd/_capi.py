@@ -152,6 +152,7 @@ SIGNATURES = {
     "svlg_nm_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, i32p, f64p, i32p, f64p]),
     "svlg_nm_get_state": (C.c_int, [C.c_void_p, f64p, f64p, f64p]),
     "svlg_keff_apply": (C.c_int, [C.c_void_p, f64p, f64p]),
+    "svlg_keff_apply_noalloc": (C.c_int, [C.c_void_p, f64p, f64p]),
     "svlg_solve": (C.c_int, [C.c_void_p, f64p, f64p, C.c_double, C.c_int32, i32p, f64p]),
     "svlg_nm_set_precond": (C.c_int, [C.c_void_p, C.c_int32]),
     "svlg_nl_init": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.c_double, C.c_int32, C.c_int32]),

@@ -73,6 +73,7 @@ struct TileSet {
 
 struct Model {
     int ndim = 0, nen = 0, nnodes = 0, nelem = 0, nfree = 0, ndofn = 0;  // ndofn = nnodes*ndim
+    int device = 0;                   // HIP device current at creation (the model's memory lives there)
     double th = 1.0;
     bool lumped = true;
     bool affine = false;

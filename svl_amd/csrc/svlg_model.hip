@@ -288,6 +288,10 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         if (own) (void)hipStreamDestroy(own);
         return code;
     };
+    if (hipGetDevice(&m.device) != hipSuccess) {
+        set_error("hipGetDevice failed");
+        return fail(10);
+    }
     m.ndim = ndim;
     m.nen = nen;
     m.nnodes = nn;
