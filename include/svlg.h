@@ -160,6 +160,22 @@ int svlg_cd_steps_async(svlg_model *m, int32_t k0, int32_t nsteps);
 /* Current U, V, A in free DOFs (CentralDifference::GetDisplacements/Velocities/
  * Accelerations). Any pointer may be NULL. */
 int svlg_cd_get_state(svlg_model *m, double *U, double *V, double *A);
+/* The pieces of the step for callers that keep SVL's Algorithm and LinearSystem (Linear::
+ * ComputeNewIncrement, Linear.cpp:22-56, calls them in this order); svlg_cd_steps fuses them.
+ * CentralDifference::ComputeEffectiveForce (:206-221) for step k at the current state:
+ * Feff = T^T [Fext(k) - Fint(U) + (M/dt^2 - C/2dt)(U - Up)] over the free DOFs (Fbar, the force
+ * left by a previous analysis stage, is 0: one stage). `factor` is unused, as in SVL. */
+int svlg_cd_effective_force(svlg_model *m, int32_t k, double factor, double *Feff /*[nfree]*/);
+/* CentralDifference::ComputeEffectiveStiffness (:225-231): Keff = T^T (M/dt^2 + C/2dt) T, diagonal
+ * for the lumped mass and dashpots; its diagonal over the free DOFs. */
+int svlg_cd_effective_stiffness(svlg_model *m, double *Kdiag /*[nfree]*/);
+/* The state update of CentralDifference::ComputeNewStep (:137-148) with the algorithm's
+ * increment dU (free DOFs): Up = U, U += dU; V and A follow from the kept U history. */
+int svlg_cd_apply_increment(svlg_model *m, const double *dU /*[nfree]*/);
+/* Integrator::ComputeProgressiveForce (CentralDifference.cpp:174-186; Fbar = 0): Fext(k) at
+ * the mesh nodes, [nnodes*ndim] in description order (Assembler::ComputeExternalForceVector,
+ * Assembler.cpp:291-489: point loads and domain reduction forces). */
+int svlg_external_force(svlg_model *m, int32_t k, double *F);
 /* Blocks until all work queued on the model's stream is done. */
 int svlg_sync(svlg_model *m);
 
@@ -208,6 +224,14 @@ int svlg_nm_init(svlg_model *m, double dt, double rtol, int32_t maxit, int32_t p
  * NULL) receive the CG iteration count and the final relative residual of step s. */
 int svlg_nm_steps(svlg_model *m, int32_t k0, int32_t nsteps, int32_t nrec, const int32_t *rec_dofs,
                   double *hist, int32_t *iters, double *relres);
+/* ExtendedNewmarkBeta::ComputeEffectiveForce (:208-229) for step k at the current state, the
+ * Linear algorithm's dU = 0 (Fbar = 0, G = 0): Feff = T^T [Fext(k) - Fint(U) + C V + M (4/dt V + A)]
+ * over the free DOFs. `factor` is unused, as in SVL. Models with J2 elements form their Newton
+ * residual in svlg_nl_steps. */
+int svlg_nm_effective_force(svlg_model *m, int32_t k, double factor, double *Feff /*[nfree]*/);
+/* The state update of ExtendedNewmarkBeta::ComputeNewStep (:141-150) with the algorithm's
+ * increment dU (free DOFs): U += dU, A = 4/dt^2 dU - 4/dt V - A, V = 2/dt dU - V. */
+int svlg_nm_apply_increment(svlg_model *m, const double *dU /*[nfree]*/);
 /* Current U, V, A in free DOFs (ExtendedNewmarkBeta::GetDisplacements/...). */
 int svlg_nm_get_state(svlg_model *m, double *U, double *V, double *A);
 /* Switches the preconditioner of later solves (SVLG_PRECOND_LEARNED needs svlg_gnn_init;
@@ -343,6 +367,9 @@ int svlg_pml_info(svlg_pml *p, int64_t *info4);
 int svlg_pml_steps(svlg_pml *p, int32_t k0, int32_t nsteps, int32_t nrec, const int32_t *rec_dofs,
                    double *histU, double *histV, double *histA, int32_t *iters);
 int svlg_pml_get_state(svlg_pml *p, double *U, double *V, double *A /* [nfree] each */);
+/* ExtendedNewmarkBeta::GetPMLHistoryVector (Integrator.hpp:90): Ubar over the free DOFs (0
+ * without PML3DHexa8 elements, which alone carry the history matrix G). */
+int svlg_pml_get_history(svlg_pml *p, double *Ubar /* [nfree] */);
 
 #ifdef __cplusplus
 }

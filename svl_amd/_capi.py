@@ -152,6 +152,12 @@ SIGNATURES = {
     "svlg_nm_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, i32p, f64p, i32p, f64p]),
     "svlg_nm_get_state": (C.c_int, [C.c_void_p, f64p, f64p, f64p]),
     "svlg_keff_apply": (C.c_int, [C.c_void_p, f64p, f64p]),
+    "svlg_cd_effective_force": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, f64p]),
+    "svlg_cd_effective_stiffness": (C.c_int, [C.c_void_p, f64p]),
+    "svlg_cd_apply_increment": (C.c_int, [C.c_void_p, f64p]),
+    "svlg_external_force": (C.c_int, [C.c_void_p, C.c_int32, f64p]),
+    "svlg_nm_effective_force": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, f64p]),
+    "svlg_nm_apply_increment": (C.c_int, [C.c_void_p, f64p]),
     "svlg_keff_apply_noalloc": (C.c_int, [C.c_void_p, f64p, f64p]),
     "svlg_solve": (C.c_int, [C.c_void_p, f64p, f64p, C.c_double, C.c_int32, i32p, f64p]),
     "svlg_nm_set_precond": (C.c_int, [C.c_void_p, C.c_int32]),
@@ -171,6 +177,7 @@ SIGNATURES = {
     "svlg_pml_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "svlg_pml_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, i32p, f64p, f64p, f64p, i32p]),
     "svlg_pml_get_state": (C.c_int, [C.c_void_p, f64p, f64p, f64p]),
+    "svlg_pml_get_history": (C.c_int, [C.c_void_p, f64p]),
 }
 
 _LIB = None

@@ -24,23 +24,27 @@ namespace svlg {
 
 int launch_fint_colored(Model &m, const double *U, double ftol, double *F);
 int gather_free(Model &m, const double *dnode, double *dfree);
+int scatter_free(Model &m, const double *dfree, double *dnode);
 
 // keff = M/dt^2 + C/2dt, kforce = M/dt^2 - C/2dt per DOF (CentralDifference.cpp:64-70); dtv
 // (per DOF, may be null) gives every DOF its own time step (independent batched meshes).
 __global__ void k_cd_coeffs(int n, double dt0, const double *__restrict__ dtv, const double *__restrict__ M,
                             const double *__restrict__ C, const int32_t *__restrict__ fdof,
-                            double *__restrict__ keff, double *__restrict__ kforce, int *__restrict__ singular) {
+                            double *__restrict__ keff, double *__restrict__ kforce, double *__restrict__ kinv,
+                            int *__restrict__ singular) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (fdof[i] < 0) {
         keff[i] = 0.0;
         kforce[i] = 0.0;
+        kinv[i] = 0.0;
         return;
     }
     const double dt = dtv ? dtv[i] : dt0;
     double k = 1.0 / dt / dt * M[i] + 1.0 / 2.0 / dt * C[i];
     keff[i] = k;
     kforce[i] = 1.0 / dt / dt * M[i] - 1.0 / 2.0 / dt * C[i];
+    kinv[i] = k != 0.0 ? 1.0 / k : 0.0;
     if (!(k != 0.0)) atomicOr(singular, 1);
 }
 
@@ -76,6 +80,27 @@ __global__ void k_cd_update(int n, const double *__restrict__ U, const double *_
     Unext[i] = u + feff / k;
 }
 
+// Feff = (Fext - F) + kforce (U - Up) on every node DOF (0 where keff = 0, the restrained
+// DOFs); resets F for the next step (CentralDifference::ComputeEffectiveForce, :206-221).
+__global__ void k_cd_feff(int n, const double *__restrict__ U, const double *__restrict__ Up, double *__restrict__ F,
+                          const double *__restrict__ Fext, const double *__restrict__ keff,
+                          const double *__restrict__ kforce, double *__restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double fi = F[i];
+    F[i] = 0.0;
+    out[i] = keff[i] == 0.0 ? 0.0 : (Fext[i] - fi) + kforce[i] * (U[i] - Up[i]);
+}
+
+// U_next = U + dU on the free DOFs, 0 on the restrained ones (CentralDifference::ComputeNewStep,
+// :137-148, with the algorithm's increment).
+__global__ void k_cd_increment(int n, const double *__restrict__ U, const double *__restrict__ dU,
+                               const double *__restrict__ keff, double *__restrict__ Unext) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Unext[i] = keff[i] == 0.0 ? 0.0 : U[i] + dU[i];
+}
+
 __global__ void k_record(int nrec, const int32_t *__restrict__ rec, const double *__restrict__ U,
                          double *__restrict__ out) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -91,7 +116,7 @@ struct FusedArgs {
     const double *geo;
     const double *X, *Ucur, *Uprev;
     double *Unext;
-    const double *keff, *kforce;
+    const double *keff, *kforce, *kinv;
     const int32_t *nl_ptr, *nl_idx;
     const double *load_dir, *load_factor, *load_series;
     int load_nt, k;
@@ -320,6 +345,8 @@ __global__ __launch_bounds__(512) void k_cd_fused(FusedArgs a) {
 // MOC > 0: every tile owns at most MOC nodes; the slot planes then have the compile-time
 // stride MOC + 1 with column MOC as the trash column, so that every slot address is one
 // per-corner base plus an immediate offset.
+// The update multiplies by the stored 1/keff (kinv) instead of dividing by keff: the
+// diagonal solve as Eigen's LDL^T applies it (the inverse of D times the vector).
 template <int DIM, int PATH, int MOC = 0>
 __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
     constexpr int NEN = Elem<DIM>::nen;
@@ -349,7 +376,7 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
         up[c] = has_node ? a.Uprev[g0 + size_t(tid) * DIM + c] : 0.0;
-        kd[c] = has_node ? a.keff[g0 + size_t(tid) * DIM + c] : 0.0;
+        kd[c] = has_node ? a.kinv[g0 + size_t(tid) * DIM + c] : 0.0;
     }
     const bool has_elem = PATH != 4 && tid < ne;
     int lc[NEN];
@@ -529,7 +556,8 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
         const double u = sU[tid * DIM + d];
         const size_t g = g0 + size_t(tid) * DIM + d;
         if (tid < npl) {
-            a.Unext[g] = u + ((0.0 - fi[d]) + kd[d] * (u - up[d])) / kd[d];
+            // Plain nodes: kforce = keff (no damping), so Feff / keff = (u - up) - fi / keff.
+            a.Unext[g] = u + ((u - up[d]) - fi[d] * kd[d]);
         } else if (kd[d] == 0.0) {
             a.Unext[g] = 0.0;
         } else {
@@ -542,7 +570,7 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
                 fe += a.load_factor[l] * (amp * a.load_dir[size_t(l) * DIM + d]);
             }
             if (ds >= 0) fe += drm_force(a.drm, ds, kstep, DIM, d);
-            a.Unext[g] = u + ((fe - fi[d]) + a.kforce[g] * (u - up[d])) / kd[d];
+            a.Unext[g] = u + ((fe - fi[d]) + a.kforce[g] * (u - up[d])) * kd[d];
         }
     }
 }
@@ -632,6 +660,7 @@ static int launch_fused(Model &m, int k, const double *Ucur, const double *Uprev
     a.Unext = Unext;
     a.keff = m.keff.p;
     a.kforce = m.kforce.p;
+    a.kinv = m.kinv.p;
     a.nl_ptr = m.nl_ptr.p;
     a.nl_idx = m.nl_idx.p;
     a.load_dir = m.load_dir.p;
@@ -727,6 +756,7 @@ static int cd_init_impl(Model &m, double dt, const std::vector<double> *dtd, dou
     SVLG_HIP(hipMemsetAsync(m.Fext.p, 0, sizeof(double) * m.ndofn, m.stream));
     SVLG_HIP(m.keff.alloc(m.ndofn));
     SVLG_HIP(m.kforce.alloc(m.ndofn));
+    SVLG_HIP(m.kinv.alloc(m.ndofn));
     if (dtd) {
         m.h_dt_dof = *dtd;
         SVLG_HIP(m.dt_dof.upload(dtd->data(), dtd->size(), m.stream));
@@ -738,7 +768,7 @@ static int cd_init_impl(Model &m, double dt, const std::vector<double> *dtd, dou
     SVLG_HIP(hipMemsetAsync(sing.p, 0, sizeof(int), m.stream));
     int nb = (m.ndofn + 255) / 256;
     hipLaunchKernelGGL(k_cd_coeffs, dim3(nb), dim3(256), 0, m.stream, m.ndofn, dt, dtd ? m.dt_dof.p : nullptr,
-                       m.mdiag.p, m.cdiag.p, m.fdof.p, m.keff.p, m.kforce.p, sing.p);
+                       m.mdiag.p, m.cdiag.p, m.fdof.p, m.keff.p, m.kforce.p, m.kinv.p, sing.p);
     SVLG_HIP(hipGetLastError());
     int hs = 0;
     SVLG_HIP(hipMemcpyAsync(&hs, sing.p, sizeof(int), hipMemcpyDeviceToHost, m.stream));
@@ -928,6 +958,81 @@ int svlg_cd_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
     if (nrec > 0 && nsteps > 0)
         SVLG_HIP(hipMemcpyAsync(hist, m.hist.p, sizeof(double) * size_t(nrec) * nsteps, hipMemcpyDeviceToHost,
                                 m.stream));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+int svlg_cd_effective_force(svlg_model *h, int32_t k, double factor, double *Feff) {
+    (void)factor;  // unused by CentralDifference::ComputeEffectiveForce as well
+    SVLG_CHECK(h && Feff, "null argument");
+    Model &m = h->m;
+    SVLG_CHECK(m.cd_ready, "svlg_cd_init has not been called");
+    const double *Ucur = m.U[m.ring].p, *Up = m.U[(m.ring + 2) % 3].p;
+    if (m.nload > 0)
+        hipLaunchKernelGGL(k_loads, dim3(1), dim3(64), 0, m.stream, k, m.nload, m.load_nt, m.ndim,
+                           (int)m.h_load_dofs.size(), m.load_dofs.p, m.load_node.p, m.load_dir.p, m.load_factor.p,
+                           m.load_series.p, m.Fext.p);
+    if (int rc = drm_add_fext(m, k, m.Fext.p)) return rc;
+    if (int rc = launch_fint_colored(m, Ucur, m.ftol, m.F.p)) return rc;
+    DevBuf<double> fn, ff;
+    SVLG_HIP(fn.alloc(m.ndofn));
+    SVLG_HIP(ff.alloc(std::max(m.nfree, 1)));
+    const int nb = (m.ndofn + 255) / 256;
+    hipLaunchKernelGGL(k_cd_feff, dim3(nb), dim3(256), 0, m.stream, m.ndofn, Ucur, Up, m.F.p, m.Fext.p, m.keff.p,
+                       m.kforce.p, fn.p);
+    if (int rc = gather_free(m, fn.p, ff.p)) return rc;
+    SVLG_HIP(hipMemcpyAsync(Feff, ff.p, sizeof(double) * m.nfree, hipMemcpyDeviceToHost, m.stream));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+int svlg_external_force(svlg_model *h, int32_t k, double *F) {
+    SVLG_CHECK(h && F, "null argument");
+    Model &m = h->m;
+    DevBuf<double> fe;
+    SVLG_HIP(fe.alloc(m.ndofn));
+    SVLG_HIP(hipMemsetAsync(fe.p, 0, sizeof(double) * m.ndofn, m.stream));
+    if (m.nload > 0)
+        hipLaunchKernelGGL(k_loads, dim3(1), dim3(64), 0, m.stream, k, m.nload, m.load_nt, m.ndim,
+                           (int)m.h_load_dofs.size(), m.load_dofs.p, m.load_node.p, m.load_dir.p, m.load_factor.p,
+                           m.load_series.p, fe.p);
+    if (int rc = drm_add_fext(m, k, fe.p)) return rc;
+    SVLG_HIP(hipGetLastError());
+    std::vector<double> h_fe(m.ndofn);
+    SVLG_HIP(hipMemcpyAsync(h_fe.data(), fe.p, sizeof(double) * m.ndofn, hipMemcpyDeviceToHost, m.stream));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    // Device node order -> description order.
+    for (int n = 0; n < m.nnodes; ++n)
+        for (int c = 0; c < m.ndim; ++c) F[size_t(n) * m.ndim + c] = h_fe[size_t(m.inv[n]) * m.ndim + c];
+    return 0;
+}
+
+int svlg_cd_effective_stiffness(svlg_model *h, double *Kdiag) {
+    SVLG_CHECK(h && Kdiag, "null argument");
+    Model &m = h->m;
+    SVLG_CHECK(m.cd_ready, "svlg_cd_init has not been called");
+    DevBuf<double> ff;
+    SVLG_HIP(ff.alloc(std::max(m.nfree, 1)));
+    if (int rc = gather_free(m, m.keff.p, ff.p)) return rc;
+    SVLG_HIP(hipMemcpyAsync(Kdiag, ff.p, sizeof(double) * m.nfree, hipMemcpyDeviceToHost, m.stream));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+int svlg_cd_apply_increment(svlg_model *h, const double *dU) {
+    SVLG_CHECK(h && dU, "null argument");
+    Model &m = h->m;
+    SVLG_CHECK(m.cd_ready, "svlg_cd_init has not been called");
+    DevBuf<double> ff, dn;
+    SVLG_HIP(ff.upload(dU, std::max(m.nfree, 1), m.stream));
+    SVLG_HIP(dn.alloc(m.ndofn));
+    if (int rc = scatter_free(m, ff.p, dn.p)) return rc;
+    const int nb = (m.ndofn + 255) / 256;
+    hipLaunchKernelGGL(k_cd_increment, dim3(nb), dim3(256), 0, m.stream, m.ndofn, m.U[m.ring].p, dn.p, m.keff.p,
+                       m.U[(m.ring + 1) % 3].p);
+    SVLG_HIP(hipGetLastError());
+    m.ring = (m.ring + 1) % 3;
+    m.steps_done++;
     SVLG_HIP(hipStreamSynchronize(m.stream));
     return 0;
 }

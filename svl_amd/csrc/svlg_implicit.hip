@@ -1004,6 +1004,54 @@ int svlg_nm_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
     return 0;
 }
 
+int svlg_nm_effective_force(svlg_model *h, int32_t k, double factor, double *Feff) {
+    (void)factor;  // unused by ExtendedNewmarkBeta::ComputeEffectiveForce as well
+    SVLG_CHECK(h && Feff, "null argument");
+    Model &m = h->m;
+    SVLG_CHECK(m.nm_ready, "svlg_nm_init has not been called");
+    SVLG_CHECK(!m.has_j2 || m.nm_initial, "models with J2 materials form their residual in svlg_nl_steps");
+    OpArgs a = op_args(m);
+    a.coef = m.coef_rhs.p;
+    a.U = m.nU.p;
+    a.V = m.nV.p;
+    a.A = m.nA.p;
+    a.mdiag = m.mdiag.p;
+    a.cdiag = m.cdiag.p;
+    a.dt = m.nm_dt;
+    a.b = m.cg_b.p;
+    a.nl_ptr = m.nl_ptr.p;
+    a.nl_idx = m.nl_idx.p;
+    a.load_dir = m.load_dir.p;
+    a.load_factor = m.load_factor.p;
+    a.load_series = m.load_series.p;
+    a.load_nt = m.load_nt;
+    a.k = k;
+    a.drm = drm_ref(m);
+    if (int rc = launch_op(m, OP_RHS, a)) return rc;
+    DevBuf<double> ff;
+    SVLG_HIP(ff.alloc(std::max(m.nfree, 1)));
+    if (int rc = gather_free(m, m.cg_b.p, ff.p)) return rc;
+    SVLG_HIP(hipMemcpyAsync(Feff, ff.p, sizeof(double) * m.nfree, hipMemcpyDeviceToHost, m.stream));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+int svlg_nm_apply_increment(svlg_model *h, const double *dU) {
+    SVLG_CHECK(h && dU, "null argument");
+    Model &m = h->m;
+    SVLG_CHECK(m.nm_ready, "svlg_nm_init has not been called");
+    DevBuf<double> ff;
+    SVLG_HIP(ff.upload(dU, std::max(m.nfree, 1), m.stream));
+    if (int rc = scatter_free(m, ff.p, m.ndU.p)) return rc;
+    const int nb = (m.ndofn + 255) / 256;
+    hipLaunchKernelGGL(k_nm_update, dim3(nb), dim3(256), 0, m.stream, m.ndofn, m.nm_dt, m.ndU.p, m.jac.p, m.nU.p,
+                       m.nV.p, m.nA.p);
+    SVLG_HIP(hipGetLastError());
+    m.nm_steps_done++;
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
 int svlg_nm_set_precond(svlg_model *h, int32_t precond) {
     SVLG_CHECK(h, "null model");
     Model &m = h->m;

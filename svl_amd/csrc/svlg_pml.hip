@@ -849,6 +849,18 @@ int svlg_pml_steps(svlg_pml *p, int32_t k0, int32_t nsteps, int32_t nrec, const 
     return 0;
 }
 
+int svlg_pml_get_history(svlg_pml *p, double *Ubar) {
+    SVLG_CHECK(p && Ubar, "null argument");
+    const size_t nbytes = sizeof(double) * p->nfree;
+    if (!p->has_g) {
+        std::fill(Ubar, Ubar + p->nfree, 0.0);
+        return 0;
+    }
+    SVLG_HIP(hipMemcpyAsync(Ubar, p->Ub.p, nbytes, hipMemcpyDeviceToHost, p->stream));
+    SVLG_HIP(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
 int svlg_pml_get_state(svlg_pml *p, double *U, double *V, double *A) {
     SVLG_CHECK(p, "null handle");
     const size_t nbytes = sizeof(double) * p->nfree;

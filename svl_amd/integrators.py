@@ -113,6 +113,14 @@ def plan_tiles(model: Model, tile_nodes: int = 0) -> dict:
     return out
 
 
+def external_force(mesh: DeviceModel, k: int) -> np.ndarray:
+    """Fext(k) at the mesh nodes, [nnodes, ndim] (Integrator::ComputeProgressiveForce with Fbar = 0)."""
+    info = mesh.info()
+    out = np.zeros(info["nnodes"] * mesh.ndim)
+    capi.check(capi.lib().svlg_external_force(mesh.h, int(k), capi.fptr(out)), "svlg_external_force")
+    return out.reshape(info["nnodes"], mesh.ndim)
+
+
 class CentralDifference:
     """CentralDifference(mesh, TimeStep, mtol, ktol, ftol) on the device."""
 
@@ -143,6 +151,28 @@ class CentralDifference:
 
     def advance_async(self, k0: int, nsteps: int):
         capi.check(capi.lib().svlg_cd_steps_async(self.mesh.h, int(k0), int(nsteps)), "svlg_cd_steps_async")
+
+    # The pieces of one step, for a caller that drives SVL's Linear algorithm itself
+    # (Linear::ComputeNewIncrement, Linear.cpp:22-56): Feff, Keff, solve, then the update.
+    def ComputeEffectiveForce(self, k: int, factor: float = 1.0) -> np.ndarray:
+        out = np.zeros(self.mesh.nfree)
+        capi.check(capi.lib().svlg_cd_effective_force(self.mesh.h, int(k), float(factor), capi.fptr(out)),
+                   "svlg_cd_effective_force")
+        return out
+
+    def ComputeEffectiveStiffness(self) -> np.ndarray:
+        """The diagonal of Keff = T^T (M/dt^2 + C/2dt) T over the free DOFs."""
+        out = np.zeros(self.mesh.nfree)
+        capi.check(capi.lib().svlg_cd_effective_stiffness(self.mesh.h, capi.fptr(out)), "svlg_cd_effective_stiffness")
+        return out
+
+    def ApplyIncrement(self, dU: np.ndarray):
+        """The update of ComputeNewStep with the algorithm's increment (free DOFs)."""
+        d = np.ascontiguousarray(dU, dtype=np.float64)
+        capi.check(capi.lib().svlg_cd_apply_increment(self.mesh.h, capi.fptr(d)), "svlg_cd_apply_increment")
+
+    def ComputeProgressiveForce(self, k: int) -> np.ndarray:
+        return external_force(self.mesh, k)
 
     def _state(self):
         n = self.mesh.nfree
@@ -209,6 +239,20 @@ class ExtendedNewmarkBeta:
         self.iters, self.relres = it[:nsteps], rr[:nsteps]
         capi.check(rc, "svlg_nm_steps")
         return hist
+
+    def ComputeEffectiveForce(self, k: int, factor: float = 1.0) -> np.ndarray:
+        out = np.zeros(self.mesh.nfree)
+        capi.check(capi.lib().svlg_nm_effective_force(self.mesh.h, int(k), float(factor), capi.fptr(out)),
+                   "svlg_nm_effective_force")
+        return out
+
+    def ApplyIncrement(self, dU: np.ndarray):
+        """The update of ComputeNewStep with the algorithm's increment (free DOFs)."""
+        d = np.ascontiguousarray(dU, dtype=np.float64)
+        capi.check(capi.lib().svlg_nm_apply_increment(self.mesh.h, capi.fptr(d)), "svlg_nm_apply_increment")
+
+    def ComputeProgressiveForce(self, k: int) -> np.ndarray:
+        return external_force(self.mesh, k)
 
     def _state(self):
         n = self.mesh.nfree

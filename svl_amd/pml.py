@@ -175,6 +175,12 @@ class PmlNewmark:
     def GetDisplacements(self):
         return self.state()[0]
 
+    def GetPMLHistoryVector(self):
+        """Ubar over the free DOFs (ExtendedNewmarkBeta's integrated displacement; 0 in 2D)."""
+        out = np.zeros(self.nfree)
+        capi.check(capi.lib().svlg_pml_get_history(self.h, capi.fptr(out)), "svlg_pml_get_history")
+        return out
+
     def GetVelocities(self):
         return self.state()[1]
 

@@ -144,13 +144,27 @@ def test_halfspace_box3d_with_history_matrix(lumped):
     # displacements are fixed)
     rec = [(top, 2), (off, 0), (off, 1), (off, 2)] + [(corner, a) for a in range(3, 9)]
     print(f"3D half-space box 6x6x4 with PML on five faces, lumped={lumped}, {nt - 1} steps", flush=True)
-    hu, hv, ha, its = PmlNewmark(m, dt).advance(1, nt - 1, rec)
-    ou, ov, oa = to_oracle(m).run_newmark(dt, nt, rec, k0=1, uva=True)
+    pn = PmlNewmark(m, dt)
+    hu, hv, ha, its = pn.advance(1, nt - 1, rec)
+    orc = to_oracle(m)
+    ou, ov, oa = orc.run_newmark(dt, nt, rec, k0=1, uva=True)
     for c in range(len(rec)):
         for h, o in ((hu, ou), (hv, ov), (ha, oa)):
             assert np.linalg.norm(o[:, c]) > 0
             assert rel_l2(h[:, c], o[:, c]) < TOL, (rec[c], rel_l2(h[:, c], o[:, c]))
     print("MINRES iterations per step: mean", its.mean(), "max", its.max())
+    # GetPMLHistoryVector: the device's Ubar (free DOFs) against the oracle's after the last step,
+    # at the recorded unknowns (the bar above); the whole-vector figures of Ubar and U are printed.
+    off_t, T = orc.assemble()[:2]
+    full = orc.run_newmark(dt, nt, rec, k0=1, full=True)
+    ub_ref, u_ref = full[3][-1], full[0][-1]
+    ub = T @ pn.GetPMLHistoryVector()
+    u = T @ pn.GetDisplacements()
+    idx = [off_t[n] + a for n, a in rec]
+    print(f"Ubar at the recorded unknowns {rel_l2(ub[idx], ub_ref[idx]):.2e}; whole vectors: "
+          f"Ubar {rel_l2(ub, ub_ref):.2e}, U {rel_l2(u, u_ref):.2e}")
+    assert np.linalg.norm(ub_ref[idx]) > 0
+    assert rel_l2(ub[idx], ub_ref[idx]) < TOL, rel_l2(ub[idx], ub_ref[idx])
 
 
 def reg_node3(m, x, y, z):
