@@ -122,6 +122,121 @@ def cpu_baseline(m, dt, steps):
                       f"same {m.name} model, time loop only ({secs:.2f} s)"}
 
 
+def cpu_baseline_parallel(m, dt, steps, procs):
+    """`procs` single-threaded oracle processes run the same sample at once on the host's cores:
+    the aggregate DOF-steps/s is what the host delivers when every core runs one SVL process
+    (an upper bound of an SVL run on that many MPI ranks, which would also exchange halos)."""
+    import multiprocessing as mp
+    ctx = mp.get_context("fork")
+    with ctx.Pool(procs) as pool:
+        t = time.perf_counter()
+        secs = pool.starmap(_oracle_sample, [(m, dt, steps)] * procs)
+        wall = time.perf_counter() - t
+    return {"value": procs * m.nfree * steps / max(secs), "unit": "DOF-steps/s", "cores": procs, "kind": "port",
+            "sample": f"{procs} concurrent single-threaded oracle processes, {steps} CentralDifference steps "
+                      f"of the same {m.name} model each, time loops only ({min(secs):.2f}-{max(secs):.2f} s; "
+                      f"wall {wall:.2f} s with the model set-up)"}
+
+
+def _oracle_sample(m, dt, steps):
+    import oracle
+    return oracle.run(m, oracle.CENTRAL_DIFFERENCE, dt, steps + 1, ())["seconds"]
+
+
+def host_cores():
+    """Cores the host offers (nproc) and the CPU model (lscpu / /proc/cpuinfo)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return n, model
+
+
+def layer2d_bench(args):
+    """The 2D plane-strain layer (lin2DQuad4 200 x 100, Lysmer, Ricker point load,
+    CentralDifference lumped FP64): DOF-steps/s and the roofline of its fused step, eager
+    launches and launches replayed from a captured hipGraph."""
+    from svl_amd import batch as B
+    from svl_amd import workloads as W
+    from svl_amd.integrators import CentralDifference, DeviceModel
+    steps = args.layer_steps
+    m, dt, nt = W.layer2d(200, 100, nt=steps + args.warmup + 2)
+    dm = DeviceModel(m)
+    info = dm.info()
+    ev = HipEvents()
+    stream = dm.stream()
+    res = {}
+    for mode in ("eager", "graph"):
+        if mode == "eager":
+            cd = CentralDifference(dm, dt)
+            cd.Initialize()
+            run = lambda k0, n: cd.advance_async(k0, n)  # noqa: E731
+        else:
+            fp = [0, m.nfree]
+            cd = B.BatchedCentralDifference(dm, [dt], [0, m.nnodes], fp)
+            cd.Initialize()
+            run = lambda k0, n: cd.advance(k0, n, graph_steps=30)  # noqa: E731
+        run(1, args.warmup)
+        dm.sync()
+        e0, e1 = ev.event(), ev.event()
+        t = time.perf_counter()
+        ev.record(e0, stream)
+        run(1 + args.warmup, steps)
+        ev.record(e1, stream)
+        dm.sync()
+        wall = time.perf_counter() - t
+        step_s = ev.elapsed_ms(e0, e1) / 1e3 / steps
+        res[mode] = {"us_per_step": step_s * 1e6, "dof_steps_per_s": m.nfree * steps / wall}
+    bytes_step = algorithmic_bytes_per_step(m)
+    best = min(res.values(), key=lambda r: r["us_per_step"])
+    achieved = bytes_step / (best["us_per_step"] * 1e-6) / 1e9
+    out = {"workload": "layer2d lin2DQuad4 200x100 h=1m, rho 2000, Vs 200, nu 0.25, Lysmer sides and base, Ricker "
+                       "5 Hz surface point load, CentralDifference dt=0.5 h/Vp, FP64",
+           "nfree": m.nfree, "steps": steps, "ntiles": info["ntiles"], **res,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "alg_bytes_per_step": bytes_step,
+                        "alg_bytes_per_dof_step": bytes_step / m.nfree}}
+    print(f"# layer2d: {m.nfree} DOFs, eager {res['eager']['us_per_step']:.2f} us/step, graph "
+          f"{res['graph']['us_per_step']:.2f} us/step, {achieved:.0f} GB/s", file=sys.stderr, flush=True)
+    dm.close()
+    return out
+
+
+def precision_bench(args, m, dt, us_fp64):
+    """Element-step precision variants on the full half-space: step time and the relative L2
+    difference of a 400-step displacement history (every 997th free DOF) from the FP64 path."""
+    import numpy as np
+    from svl_amd.integrators import CentralDifference, DeviceModel
+    rec = list(range(5, m.nfree, 997))
+    out = {}
+    hist = {}
+    for name, prec in (("fp64", 0), ("fp32", 2), ("bf16_mfma", 1)):
+        dm = DeviceModel(m, elem_precision=prec)
+        cd = CentralDifference(dm, dt)
+        cd.Initialize()
+        hist[name] = cd.advance(1, 400, rec)
+        ev = HipEvents()
+        stream = dm.stream()
+        e0, e1 = ev.event(), ev.event()
+        ev.record(e0, stream)
+        cd.advance_async(401, args.steps // 4)
+        ev.record(e1, stream)
+        dm.sync()
+        out[name] = {"us_per_step": ev.elapsed_ms(e0, e1) * 1e3 / (args.steps // 4)}
+        dm.close()
+    for name in out:
+        h = hist[name]
+        out[name]["rel_l2_vs_fp64_400_steps"] = float(np.linalg.norm(h - hist["fp64"]) / np.linalg.norm(hist["fp64"]))
+    print(f"# precision: " + ", ".join(f"{k} {v['us_per_step']:.2f} us/step (rel {v['rel_l2_vs_fp64_400_steps']:.1e})"
+                                      for k, v in out.items()), file=sys.stderr, flush=True)
+    return out
+
+
 def implicit_bench(args):
     """Implicit ExtendedNewmarkBeta step time on the 64^3 block: Jacobi-PCG against flexible CG
     with the trained graph-convolution preconditioner, both to the same relative residual."""
@@ -343,16 +458,62 @@ def pml_bench(args):
     _, _, _, its = pm.advance(1 + args.pml_warmup, args.pml_steps)
     wall = time.perf_counter() - t0
     us_step = wall / args.pml_steps * 1e6
+    # The same window again with eagerly launched MINRES iterations (no graph replay).
+    pm.set_graph(False)
+    t0 = time.perf_counter()
+    pm.advance(1 + args.pml_warmup + args.pml_steps, args.pml_steps // 2)
+    us_step_eager = (time.perf_counter() - t0) / (args.pml_steps // 2) * 1e6
     iters = float(its.mean())
     it_bytes = 12 * info["nnz"] + 116 * info["nfree"]
     out = {"workload": f"2D half-space lin2DQuad4 {nx}x{ny} h=1m wrapped by PML2DQuad4 {npml} elements thick "
                        f"(left, right, bottom), EQUAL ties, Ricker surface load, ExtendedNewmarkBeta dt={dt}",
            "nelem": int(len(m.conn)), **info, "steps": args.pml_steps, "us_per_step": us_step,
            "minres_iters_mean": iters, "us_per_minres_iter": us_step / max(iters, 1.0),
-           "alg_bytes_per_minres_iter": it_bytes,
+           "alg_bytes_per_minres_iter": it_bytes, "us_per_step_eager_launches": us_step_eager,
            "achieved_GBs_per_iter": it_bytes / (us_step / max(iters, 1.0) * 1e-6) / 1e9,
            "build_s": build_s, "model_create_s": create_s}
     print(f"# pml: {info['nfree']} free DOFs, {us_step:.0f} us/step, {iters:.1f} MINRES iterations",
+          file=sys.stderr, flush=True)
+    del pm
+    return out
+
+
+def pml3d_bench(args):
+    """3D half-space wrapped by PML3DHexa8 on five faces (edge and corner elements carry the
+    history matrix G and the integrated displacement Ubar), ExtendedNewmarkBeta with the scaled
+    MINRES solve; eagerly launched MINRES iterations. Per MINRES iteration the algorithmic traffic
+    is 12 B per nonzero and 116 B per free DOF, as in 2D."""
+    import numpy as np
+    from svl_amd.pml import PmlNewmark, halfspace_box3d
+    nx, ny, nz, npml = (int(v) for v in args.pml3d_ne.split(","))
+    dt = 0.005
+    nt = args.pml_steps + args.pml_warmup + 2
+    tt = np.arange(nt) * dt
+    a = (np.pi * 10.0 * (tt - 0.15)) ** 2
+    t0 = time.perf_counter()
+    m = halfspace_box3d(nx, ny, nz, npml, load_series=(1 - 2 * a) * np.exp(-a))
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pm = PmlNewmark(m, dt)
+    pm.set_graph(False)
+    create_s = time.perf_counter() - t0
+    info = pm.info()
+    pm.advance(1, args.pml_warmup)
+    t0 = time.perf_counter()
+    _, _, _, its = pm.advance(1 + args.pml_warmup, args.pml_steps)
+    wall = time.perf_counter() - t0
+    us_step = wall / args.pml_steps * 1e6
+    iters = float(its.mean())
+    it_bytes = 12 * info["nnz"] + 116 * info["nfree"]
+    out = {"workload": f"3D half-space lin3DHexa8 {nx}x{ny}x{nz} h=1m wrapped by PML3DHexa8 {npml} elements thick "
+                       f"(four sides and bottom), EQUAL ties, Ricker surface load, ExtendedNewmarkBeta dt={dt}, "
+                       f"eager MINRES launches",
+           "nelem": int(len(m.conn)), **info, "steps": args.pml_steps, "us_per_step": us_step,
+           "minres_iters_mean": iters, "us_per_minres_iter": us_step / max(iters, 1.0),
+           "alg_bytes_per_minres_iter": it_bytes,
+           "achieved_GBs_per_iter": it_bytes / (us_step / max(iters, 1.0) * 1e-6) / 1e9,
+           "build_s": build_s, "model_create_s": create_s}
+    print(f"# pml3d: {info['nfree']} free DOFs, {us_step:.0f} us/step, {iters:.1f} MINRES iterations",
           file=sys.stderr, flush=True)
     del pm
     return out
@@ -395,6 +556,12 @@ def main():
     ap.add_argument("--pml-steps", type=int, default=40)
     ap.add_argument("--pml-warmup", type=int, default=5)
     ap.add_argument("--no-pml", action="store_true")
+    ap.add_argument("--pml3d-ne", type=str, default="24,24,12,4", help="3D PML leg: nx,ny,nz,PML elements")
+    ap.add_argument("--no-pml3d", action="store_true")
+    ap.add_argument("--layer-steps", type=int, default=2000)
+    ap.add_argument("--no-layer", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--cpu-procs", type=int, default=0, help="concurrent oracle processes (0: one per host core)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MB",
                     help="only run the device copy of MB megabytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -462,6 +629,10 @@ def main():
                      "alg_bytes_per_step": bytes_step},
     }
     dm.close()
+    if not args.no_precision:
+        out["precision"] = precision_bench(args, m, dt, step_s * 1e6)
+    if not args.no_layer:
+        out["layer2d"] = layer2d_bench(args)
     if not args.no_implicit:
         out["implicit"] = implicit_bench(args)
     if not args.no_nonlinear:
@@ -472,9 +643,15 @@ def main():
         out["drm"] = drm_bench(args, step_s * 1e6)
     if not args.no_pml:
         out["pml2d"] = pml_bench(args)
+    if not args.no_pml3d:
+        out["pml3d"] = pml3d_bench(args)
     if not args.no_cpu:
+        ncores, cpu_model = host_cores()
         out["cpu_baseline"] = cpu_baseline(m, dt, args.cpu_steps)
         out["cpu_baseline"]["cores"] = 1
+        out["cpu_baseline"]["host"] = f"{ncores} cores, {cpu_model}"
+        out["cpu_baseline_all_cores"] = cpu_baseline_parallel(m, dt, max(1, args.cpu_steps // 2),
+                                                             args.cpu_procs or ncores)
     print(json.dumps(out), flush=True)
 
 

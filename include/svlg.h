@@ -94,7 +94,9 @@ typedef struct svlg_exec_opts {
                            * node coordinates in LDS instead) */
     int32_t elem_precision; /* explicit element step: 0 = FP64 (the parity path), 1 = BF16 MFMA
                            * with FP32 accumulation (Hexa8 meshes with one element geometry and
-                           * corner-slot tiles; displacements, forces and updates stay FP64) */
+                           * corner-slot tiles), 2 = FP32 modal element arithmetic (axis-aligned
+                           * Hexa8 bricks, corner-slot tiles); displacements, the corner-slot
+                           * sums and the updates stay FP64 */
     int32_t reserved[3];
 } svlg_exec_opts;
 
@@ -126,7 +128,7 @@ int svlg_plan_tiles(const svlg_mesh_desc *desc, int32_t tile_nodes, int64_t *sta
  * variant (1 fused + colour phases, 2 element-colored, 3 fused + corner slots) info[5]=ntiles info[6]=element path
  * of the fused kernel (0 Gauss points, 1 modal with staged coordinates, 2 modal with
  * the geometry table, 3 modal with a table of diagonal J^-1: axis-aligned elements) info[7]=dynamic LDS bytes of the fused kernel
- * info[8]=explicit element precision (0 FP64, 1 BF16 MFMA) info[9]=distinct element geometries */
+ * info[8]=explicit element precision (0 FP64, 1 BF16 MFMA, 2 FP32) info[9]=distinct element geometries */
 int svlg_model_info(svlg_model *m, int64_t *info10);
 /* Element color of every element (ascending-tag greedy coloring; no two elements of
  * one color share a node). */
@@ -367,6 +369,9 @@ int svlg_pml_info(svlg_pml *p, int64_t *info4);
 int svlg_pml_steps(svlg_pml *p, int32_t k0, int32_t nsteps, int32_t nrec, const int32_t *rec_dofs,
                    double *histU, double *histV, double *histA, int32_t *iters);
 int svlg_pml_get_state(svlg_pml *p, double *U, double *V, double *A /* [nfree] each */);
+/* Execution option (no SVL counterpart): on = 1 (default) replays the MINRES iterations of a
+ * step from one captured hipGraph per handle, on = 0 launches them eagerly. */
+int svlg_pml_set_graph(svlg_pml *p, int32_t on);
 /* ExtendedNewmarkBeta::GetPMLHistoryVector (Integrator.hpp:90): Ubar over the free DOFs (0
  * without PML3DHexa8 elements, which alone carry the history matrix G). */
 int svlg_pml_get_history(svlg_pml *p, double *Ubar /* [nfree] */);

@@ -178,6 +178,7 @@ SIGNATURES = {
     "svlg_pml_steps": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, i32p, f64p, f64p, f64p, i32p]),
     "svlg_pml_get_state": (C.c_int, [C.c_void_p, f64p, f64p, f64p]),
     "svlg_pml_get_history": (C.c_int, [C.c_void_p, f64p]),
+    "svlg_pml_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
 }
 
 _LIB = None

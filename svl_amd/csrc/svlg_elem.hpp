@@ -247,14 +247,14 @@ __device__ __forceinline__ void hexa_lumped_mass(const double (&x)[8][3], double
 
 // Modal amplitudes (times 8) of 8 corner values in SVL corner order:
 // mx = sum sx v, my, mz, mxy = sum sx sy v, myz, mzx, mxyz.
-__device__ __forceinline__ void hexa_modes(const double (&v)[8], double &mx, double &my, double &mz, double &mxy,
-                                           double &myz, double &mzx, double &mxyz) {
-    double s00 = v[1] + v[0], d00 = v[1] - v[0];
-    double s10 = v[2] + v[3], d10 = v[2] - v[3];
-    double s01 = v[5] + v[4], d01 = v[5] - v[4];
-    double s11 = v[6] + v[7], d11 = v[6] - v[7];
-    double ss0 = s10 + s00, sd0 = s10 - s00, ds0 = d10 + d00, dd0 = d10 - d00;
-    double ss1 = s11 + s01, sd1 = s11 - s01, ds1 = d11 + d01, dd1 = d11 - d01;
+template <class T>
+__device__ __forceinline__ void hexa_modes(const T (&v)[8], T &mx, T &my, T &mz, T &mxy, T &myz, T &mzx, T &mxyz) {
+    T s00 = v[1] + v[0], d00 = v[1] - v[0];
+    T s10 = v[2] + v[3], d10 = v[2] - v[3];
+    T s01 = v[5] + v[4], d01 = v[5] - v[4];
+    T s11 = v[6] + v[7], d11 = v[6] - v[7];
+    T ss0 = s10 + s00, sd0 = s10 - s00, ds0 = d10 + d00, dd0 = d10 - d00;
+    T ss1 = s11 + s01, sd1 = s11 - s01, ds1 = d11 + d01, dd1 = d11 - d01;
     mz = ss1 - ss0;
     my = sd1 + sd0;
     myz = sd1 - sd0;
@@ -265,14 +265,14 @@ __device__ __forceinline__ void hexa_modes(const double (&v)[8], double &mx, dou
 }
 
 // Corner values from modal amplitudes (transpose of hexa_modes, zero constant mode).
-__device__ __forceinline__ void hexa_unmodes(double mx, double my, double mz, double mxy, double myz, double mzx,
-                                             double mxyz, double (&v)[8]) {
-    double ss0 = -mz, ss1 = mz;
-    double sd0 = my - myz, sd1 = my + myz;
-    double ds0 = mx - mzx, ds1 = mx + mzx;
-    double dd0 = mxy - mxyz, dd1 = mxy + mxyz;
-    double s00 = ss0 - sd0, s10 = ss0 + sd0, d00 = ds0 - dd0, d10 = ds0 + dd0;
-    double s01 = ss1 - sd1, s11 = ss1 + sd1, d01 = ds1 - dd1, d11 = ds1 + dd1;
+template <class T>
+__device__ __forceinline__ void hexa_unmodes(T mx, T my, T mz, T mxy, T myz, T mzx, T mxyz, T (&v)[8]) {
+    T ss0 = -mz, ss1 = mz;
+    T sd0 = my - myz, sd1 = my + myz;
+    T ds0 = mx - mzx, ds1 = mx + mzx;
+    T dd0 = mxy - mxyz, dd1 = mxy + mxyz;
+    T s00 = ss0 - sd0, s10 = ss0 + sd0, d00 = ds0 - dd0, d10 = ds0 + dd0;
+    T s01 = ss1 - sd1, s11 = ss1 + sd1, d01 = ds1 - dd1, d11 = ds1 + dd1;
     v[0] = s00 - d00;
     v[1] = s00 + d00;
     v[3] = s10 - d10;
@@ -285,17 +285,17 @@ __device__ __forceinline__ void hexa_unmodes(double mx, double my, double mz, do
 
 // P = L(G): H = Lam G, sigma = lam tr(H) I + mu (H + H^T), P = Lam^T sigma, restricted to
 // the nonzero rows of G (GR) and the needed rows of P (PR).
-template <int DIM, bool G0, bool G1, bool G2, bool P0, bool P1, bool P2>
-__device__ __forceinline__ void apply_L(const double (&G)[DIM][DIM], const double (&Lam)[DIM][DIM], double lam,
-                                        double mu, double (&P)[DIM][DIM]) {
+template <int DIM, bool G0, bool G1, bool G2, bool P0, bool P1, bool P2, class T>
+__device__ __forceinline__ void apply_L(const T (&G)[DIM][DIM], const T (&Lam)[DIM][DIM], T lam, T mu,
+                                        T (&P)[DIM][DIM]) {
     constexpr bool gr[3] = {G0, G1, G2};
     constexpr bool pr[3] = {P0, P1, P2};
-    double H[DIM][DIM];
+    T H[DIM][DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i)
 #pragma unroll
         for (int c = 0; c < DIM; ++c) {
-            double s = 0.0;
+            T s = T(0);
             bool first = true;
 #pragma unroll
             for (int k = 0; k < DIM; ++k)
@@ -305,16 +305,16 @@ __device__ __forceinline__ void apply_L(const double (&G)[DIM][DIM], const doubl
                 }
             H[i][c] = s;
         }
-    double tr = H[0][0] + H[1][1];
+    T tr = H[0][0] + H[1][1];
     if constexpr (DIM == 3) tr += H[2][2];
-    double S[DIM][DIM];
-    const double lt = lam * tr, m2 = 2.0 * mu;
+    T S[DIM][DIM];
+    const T lt = lam * tr, m2 = T(2) * mu;
 #pragma unroll
     for (int i = 0; i < DIM; ++i) {
         S[i][i] = fma(m2, H[i][i], lt);
 #pragma unroll
         for (int j = i + 1; j < DIM; ++j) {
-            double v = mu * (H[i][j] + H[j][i]);
+            T v = mu * (H[i][j] + H[j][i]);
             S[i][j] = v;
             S[j][i] = v;
         }
@@ -324,7 +324,7 @@ __device__ __forceinline__ void apply_L(const double (&G)[DIM][DIM], const doubl
         if (pr[k]) {
 #pragma unroll
             for (int c = 0; c < DIM; ++c) {
-                double s = Lam[0][k] * S[0][c];
+                T s = Lam[0][k] * S[0][c];
 #pragma unroll
                 for (int i = 1; i < DIM; ++i) s = fma(Lam[i][k], S[i][c], s);
                 P[k][c] = s;
@@ -335,19 +335,19 @@ __device__ __forceinline__ void apply_L(const double (&G)[DIM][DIM], const doubl
 // apply_L for a diagonal Lam (rectangular / brick elements aligned with the axes):
 // H_ic = Lam_ii G_ic, P_kc = Lam_kk sigma_kc, with only the nonzero rows of G and the
 // needed rows of P formed.
-template <int DIM, bool G0, bool G1, bool G2, bool P0, bool P1, bool P2>
-__device__ __forceinline__ void apply_L_diag(const double (&G)[DIM][DIM], const double (&Lam)[DIM][DIM], double lam,
-                                             double mu, double (&P)[DIM][DIM]) {
+template <int DIM, bool G0, bool G1, bool G2, bool P0, bool P1, bool P2, class T>
+__device__ __forceinline__ void apply_L_diag(const T (&G)[DIM][DIM], const T (&Lam)[DIM][DIM], T lam, T mu,
+                                             T (&P)[DIM][DIM]) {
     constexpr bool gr[3] = {G0, G1, G2};
     constexpr bool pr[3] = {P0, P1, P2};
-    double H[DIM][DIM];
+    T H[DIM][DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i)
         if (gr[i]) {
 #pragma unroll
             for (int c = 0; c < DIM; ++c) H[i][c] = Lam[i][i] * G[i][c];
         }
-    double tr = 0.0;
+    T tr = T(0);
     bool first = true;
 #pragma unroll
     for (int i = 0; i < DIM; ++i)
@@ -355,13 +355,13 @@ __device__ __forceinline__ void apply_L_diag(const double (&G)[DIM][DIM], const 
             tr = first ? H[i][i] : tr + H[i][i];
             first = false;
         }
-    const double lt = lam * tr, m2 = 2.0 * mu;
+    const T lt = lam * tr, m2 = T(2) * mu;
 #pragma unroll
     for (int k = 0; k < DIM; ++k)
         if (pr[k]) {
 #pragma unroll
             for (int c = 0; c < DIM; ++c) {
-                double sv;
+                T sv;
                 if (k == c) {
                     sv = gr[k] ? fma(m2, H[k][k], lt) : lt;
                 } else if (gr[k] && gr[c]) {
@@ -371,7 +371,7 @@ __device__ __forceinline__ void apply_L_diag(const double (&G)[DIM][DIM], const 
                 } else if (gr[c]) {
                     sv = mu * H[c][k];
                 } else {
-                    sv = 0.0;
+                    sv = T(0);
                 }
                 P[k][c] = Lam[k][k] * sv;
             }
@@ -399,9 +399,9 @@ __device__ __forceinline__ double inv3(const double (&J)[3][3], double (&I)[3][3
 
 #define SVLG_APPLY_L(...) apply_L_sel<DIAG, __VA_ARGS__>
 
-template <bool DIAG, int DIM, bool G0, bool G1, bool G2, bool P0, bool P1, bool P2>
-__device__ __forceinline__ void apply_L_sel(const double (&G)[DIM][DIM], const double (&Lam)[DIM][DIM], double lam,
-                                            double mu, double (&P)[DIM][DIM]) {
+template <bool DIAG, int DIM, bool G0, bool G1, bool G2, bool P0, bool P1, bool P2, class T>
+__device__ __forceinline__ void apply_L_sel(const T (&G)[DIM][DIM], const T (&Lam)[DIM][DIM], T lam, T mu,
+                                            T (&P)[DIM][DIM]) {
     if constexpr (DIAG)
         apply_L_diag<DIM, G0, G1, G2, P0, P1, P2>(G, Lam, lam, mu, P);
     else
@@ -413,18 +413,17 @@ __device__ __host__ inline double hexa_center_geometry(const double (&x)[8][3], 
 
 // Modal internal force of an affine Hexa8: Lam = J^-1, l = lambda |J| / 8, m = mu |J| / 8.
 // DIAG: Lam is diagonal (axis-aligned bricks).
-template <bool DIAG = false>
-__device__ __forceinline__ void hexa_modal_force(const double (&Lam)[3][3], double l, double m,
-                                                 const double (&u)[8][3], double (&f)[8][3]) {
-    double Mx[3], My[3], Mz[3], Mxy[3], Myz[3], Mzx[3], Mxyz[3];
+template <bool DIAG = false, class T = double>
+__device__ __forceinline__ void hexa_modal_force(const T (&Lam)[3][3], T l, T m, const T (&u)[8][3], T (&f)[8][3]) {
+    T Mx[3], My[3], Mz[3], Mxy[3], Myz[3], Mzx[3], Mxyz[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        double v[8];
+        T v[8];
 #pragma unroll
         for (int a = 0; a < 8; ++a) v[a] = u[a][c];
         hexa_modes(v, Mx[c], My[c], Mz[c], Mxy[c], Myz[c], Mzx[c], Mxyz[c]);
     }
-    const double g2 = kGP * kGP, g4 = g2 * g2;
+    const T g2 = T(kGP * kGP), g4 = T(kGP * kGP * kGP * kGP);
     // The Gauss weights of the hourglass terms (g^2, g^4) scale the mode amplitudes
     // before the linear stress map.
 #pragma unroll
@@ -434,8 +433,8 @@ __device__ __forceinline__ void hexa_modal_force(const double (&Lam)[3][3], doub
         Mzx[c] *= g2;
         Mxyz[c] *= g4;
     }
-    double G[3][3], P[3][3];
-    double Fx[3], Fy[3], Fz[3], Fxy[3], Fyz[3], Fzx[3], Fxyz[3];
+    T G[3][3], P[3][3];
+    T Fx[3], Fy[3], Fz[3], Fxy[3], Fyz[3], Fzx[3], Fxyz[3];
     // Linear modes (1-point gradient): all rows.
 #pragma unroll
     for (int c = 0; c < 3; ++c) { G[0][c] = Mx[c]; G[1][c] = My[c]; G[2][c] = Mz[c]; }
@@ -478,7 +477,7 @@ __device__ __forceinline__ void hexa_modal_force(const double (&Lam)[3][3], doub
     for (int c = 0; c < 3; ++c) Fxyz[c] += P[2][c];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        double v[8];
+        T v[8];
         hexa_unmodes(Fx[c], Fy[c], Fz[c], Fxy[c], Fyz[c], Fzx[c], Fxyz[c], v);
 #pragma unroll
         for (int a = 0; a < 8; ++a) f[a][c] = v[a];

@@ -481,7 +481,26 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
             for (int c = 0; c < NEN; ++c)
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) u[c][d] = sU[lc[c] * DIM + d];
-            if constexpr (PATH >= 2) {
+            if constexpr (PATH == 5) {
+                // FP32 element step (axis-aligned bricks): the modal force in single precision
+                // from the FP64 staged displacements; slots, sums and the update stay FP64.
+                static_assert(DIM == 3, "the FP32 element step is a Hexa8 path");
+                const double *g = a.geo + size_t(gid) * 9;
+                float Lf[3][3], uf[8][3], ff[8][3];
+#pragma unroll
+                for (int r2 = 0; r2 < 3; ++r2)
+#pragma unroll
+                    for (int c2 = 0; c2 < 3; ++c2) Lf[r2][c2] = float(g[r2 * 3 + c2]);
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) uf[c][d] = float(u[c][d]);
+                hexa_modal_force<true, float>(Lf, float(lam), float(mu), uf, ff);
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) f[c][d] = double(ff[c][d]);
+            } else if constexpr (PATH >= 2) {
                 const double *g = a.geo + size_t(gid) * (DIM * DIM);
                 double Lam[DIM][DIM];
 #pragma unroll
@@ -586,12 +605,14 @@ static const void *fused_fn(bool corner) {
 // 192 and 448 (the batched and the default 2D tile sizes) for Quad4.
 static int use_moc(const Model &m) {
     if (m.variant != 3 || m.elem_bf16 || m.elem_path != 3) return 0;
+    if (m.elem_fp32) return m.tiles.max_own <= 160 ? 160 : 0;
     const int mo = m.tiles.max_own;
     if (m.ndim == 3) return mo <= 160 ? 160 : 0;
     return mo <= 192 ? 192 : mo <= 448 ? 448 : 0;
 }
 
 static const void *moc_kernel(const Model &m) {
+    if (m.elem_fp32) return reinterpret_cast<const void *>(&k_cd_corner<3, 5, 160>);
     switch (m.ndim * 1000 + use_moc(m)) {
         case 3160: return reinterpret_cast<const void *>(&k_cd_corner<3, 3, 160>);
         case 2192: return reinterpret_cast<const void *>(&k_cd_corner<2, 3, 192>);
@@ -604,6 +625,7 @@ static const void *fused_kernel(const Model &m) {
     const bool cr = m.variant == 3;
     if (m.elem_bf16) return reinterpret_cast<const void *>(&k_cd_corner<3, 4>);
     if (use_moc(m)) return moc_kernel(m);
+    if (m.elem_fp32) return reinterpret_cast<const void *>(&k_cd_corner<3, 5>);
     if (m.ndim == 3)
         switch (m.elem_path) {
             case 3: return fused_fn<3, 3>(cr);
@@ -691,6 +713,8 @@ static int launch_fused(Model &m, int k, const double *Ucur, const double *Uprev
     } else if (use_moc(m)) {
         void *args[] = {&a};
         SVLG_HIP(hipLaunchKernel(moc_kernel(m), grid, block, args, t.lds_bytes, m.stream));
+    } else if (m.elem_fp32) {
+        hipLaunchKernelGGL((k_cd_corner<3, 5>), grid, block, t.lds_bytes, m.stream, a);
     } else if (m.ndim == 3) {
         if (m.elem_path == 3) SVLG_LAUNCH_FUSED(3, 3);
         else if (m.elem_path == 2) SVLG_LAUNCH_FUSED(3, 2);

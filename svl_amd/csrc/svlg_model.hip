@@ -652,8 +652,15 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
             }
             if (int rc = build_hexa_kmat(m, m.cd_kmat)) return fail(rc);
             m.elem_bf16 = 1;
+        } else if (prec == 2) {
+            if (!(ndim == 3 && m.variant == 3 && m.elem_path == 3)) {
+                set_error("elem_precision 2 (FP32 element step) needs axis-aligned Hexa8 bricks (the diagonal "
+                          "geometry table) and corner-slot tiles");
+                return fail(11);
+            }
+            m.elem_fp32 = 1;
         } else if (prec != 0) {
-            set_error("elem_precision must be 0 (FP64) or 1 (BF16 MFMA)");
+            set_error("elem_precision must be 0 (FP64), 1 (BF16 MFMA) or 2 (FP32)");
             return fail(11);
         }
     }
@@ -786,7 +793,7 @@ int svlg_model_info(svlg_model *h, int64_t *info) {
     info[5] = m.tiles.ntiles;
     info[6] = m.elem_path;
     info[7] = int64_t(m.tiles.lds_bytes);
-    info[8] = m.elem_bf16;
+    info[8] = m.elem_bf16 ? 1 : m.elem_fp32 ? 2 : 0;
     info[9] = m.ngeo;
     return 0;
 }

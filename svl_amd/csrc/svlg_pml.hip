@@ -526,6 +526,7 @@ struct svlg_pml {
     DevBuf<MrState> st;
     MrState *hst = nullptr;  // pinned copy of the polled state
     int last_iters = 0;              // MINRES iterations of the previous solve
+    bool use_graph = true;           // replay the batches from a captured graph (else eager)
     hipGraphExec_t batch = nullptr;  // POLL iterations, captured once (the buffer pairs swap
                                      // every iteration, so an even batch ends where it began)
     ~svlg_pml() {
@@ -553,6 +554,27 @@ static void launch_batch(svlg_pml *p, hipStream_t s) {
         std::swap(v, vp);
         std::swap(w, wp);
     }
+}
+
+// Host-side checks of what launch_batch bakes into its launches (and a captured graph keeps):
+// grids within [1, MAXB] blocks of VB threads, static LDS only (block_sum / reduce_parts: VB
+// doubles plus one), and every pointer a live allocation of this handle of the sizes the
+// kernels index (vectors nfree, partials MAXB, two scalar slots, the CSR arrays).
+static int check_batch(const svlg_pml *p) {
+    SVLG_CHECK(p->nb >= 1 && p->nb <= MAXB && p->nbs >= 1 && p->nbs <= MAXB, "MINRES grid out of range");
+    SVLG_CHECK((int64_t)p->nbs * VB >= std::min<int64_t>((int64_t)p->nfree * LPR, (int64_t)MAXB * VB) &&
+                   (int64_t)p->nb * VB >= std::min<int64_t>(p->nfree, (int64_t)MAXB * VB),
+               "MINRES grid does not cover the rows");
+    const DevBuf<double> *vec[] = {&p->vb0, &p->vb1, &p->wb0, &p->wb1, &p->x, &p->Av};
+    for (const DevBuf<double> *b : vec) SVLG_CHECK(b->p && b->n == (size_t)p->nfree, "MINRES vector not allocated");
+    SVLG_CHECK(p->part1.p && p->part1.n >= (size_t)std::max(p->nb, p->nbs) && p->part2.p &&
+                   p->part2.n >= (size_t)std::max(p->nb, p->nbs),
+               "MINRES partials not allocated");
+    SVLG_CHECK(p->st.p && p->st.n == 2, "MINRES state slots not allocated");
+    SVLG_CHECK(p->rowptr.p && p->rowptr.n == (size_t)p->nfree + 1 && p->col.p && p->col.n == (size_t)p->nnz &&
+                   p->vA.p && p->vA.n == (size_t)p->nnz,
+               "scaled matrix not allocated");
+    return 0;
 }
 
 // Captures launch_batch into a graph; on any capture error the caller launches eagerly.
@@ -802,7 +824,8 @@ int svlg_pml_steps(svlg_pml *p, int32_t k0, int32_t nsteps, int32_t nrec, const 
         SVLG_HIP(rec.upload(rec_dofs, nrec, s));
         SVLG_HIP(hist.alloc(3 * hsz));
     }
-    if (!p->batch) capture_batch(p);
+    if (int rc = check_batch(p)) return rc;
+    if (p->use_graph && !p->batch) capture_batch(p);
     RhsArgs ra{n, 0, p->nt, p->rowptr.p, p->col.p, p->lptr.p, p->lrow.p, p->vM.p, p->vC.p, p->vK.p,
                p->has_g ? p->vG.p : nullptr, p->lfac.p, p->series.p, p->S.p, p->U.p, p->V.p, p->A.p,
                p->has_g ? p->Ub.p : nullptr, p->c2, p->dt, p->r.p, p->part0.p};
@@ -846,6 +869,17 @@ int svlg_pml_steps(svlg_pml *p, int32_t k0, int32_t nsteps, int32_t nrec, const 
             if (dst[c]) SVLG_HIP(hipMemcpyAsync(dst[c], hist.p + c * hsz, sizeof(double) * hsz, hipMemcpyDeviceToHost, s));
     }
     SVLG_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int svlg_pml_set_graph(svlg_pml *p, int32_t on) {
+    SVLG_CHECK(p, "null handle");
+    SVLG_HIP(hipStreamSynchronize(p->stream));
+    if (!on && p->batch) {
+        SVLG_HIP(hipGraphExecDestroy(p->batch));
+        p->batch = nullptr;
+    }
+    p->use_graph = on != 0;
     return 0;
 }
 

@@ -175,6 +175,10 @@ class PmlNewmark:
     def GetDisplacements(self):
         return self.state()[0]
 
+    def set_graph(self, on: bool):
+        """Replay the MINRES iterations from a captured hipGraph (default) or launch them eagerly."""
+        capi.check(capi.lib().svlg_pml_set_graph(self.h, int(bool(on))), "svlg_pml_set_graph")
+
     def GetPMLHistoryVector(self):
         """Ubar over the free DOFs (ExtendedNewmarkBeta's integrated displacement; 0 in 2D)."""
         out = np.zeros(self.nfree)

@@ -72,8 +72,22 @@ def _lc(s):
     return str(s).lower()
 
 
+def _resolve(path, base):
+    """A data file named in a run file. SVL opens it as written, relative to the directory the
+    run starts in: the case directory, one level above the Partition/ folder that SVL's
+    pre-processor writes the run files to. Relative names are looked up next to the run file,
+    then in that parent folder."""
+    if os.path.isabs(path):
+        return path
+    for d in (base, os.path.dirname(base)):
+        p = os.path.join(d, path)
+        if os.path.exists(p):
+            return p
+    return os.path.join(base, path)
+
+
 def _series_file(path, base):
-    p = path if os.path.isabs(path) else os.path.join(base, path)
+    p = _resolve(path, base)
     with open(p) as f:
         vals = f.read().split()
     n = int(float(vals[0]))
@@ -95,7 +109,7 @@ def _read_drm(model: Model, at: dict, etag_index: dict, index: dict, base: str, 
     sig, cond = [], []
     for t in tags:
         fname = str(at["file"]).replace("$", str(t), 1)
-        p = fname if os.path.isabs(fname) else os.path.join(base, fname)
+        p = _resolve(fname, base)
         with open(p) as f:
             vals = f.read().split()
         nt, nfields, c = int(vals[0]), int(vals[1]), int(vals[2])

@@ -4,9 +4,10 @@ Bars: the FP64-element forward pass against the numpy restatement (oracle/gcn.py
 relative (hidden states are FP32 on the device); the BF16 MFMA element step against the FP64
 one to 3e-2; loss to 1e-5 and gradient to 1e-4 relative against the restatement's analytic
 gradient (itself pinned by finite differences on the CPU). Training must lower the loss,
-flexible CG with the network must need at most half the iterations of Jacobi-PCG, training
+flexible CG with the network must need at most a third of the iterations of Jacobi-PCG, training
 must never leave it worse than its (Chebyshev) starting point, and the Newmark history it
-produces must match the oracle's direct solves to 1e-10.
+produces must match the oracle's direct solves to 1e-10. A strict gain of training over the
+Chebyshev start is expected to fail (xfail, with the reason).
 """
 import numpy as np
 import pytest
@@ -124,24 +125,40 @@ def newmark_iters(m, dt, nsteps, learned=None, rtol=1e-8):
     return it, U, out
 
 
-def test_training_gain_over_jacobi_and_chebyshev():
-    m, dt, nt = W.block3d_implicit(12, 12, 10, nt=61)
+def trained_vs_start(m, dt):
     init = {"channels": 1, "layers": 6, "mfma": -1}
     it_j, U_j, _ = newmark_iters(m, dt, 30)
     it_c, U_c, _ = newmark_iters(m, dt, 30, {"init": init})
     it_l, U_l, pc = newmark_iters(m, dt, 30, {"init": init, "train": {"nsteps": 24, "every": 3, "sweeps": 2,
                                                                       "epochs": 60, "lr": 2e-3}})
-    info = pc
     print(f"FCG iterations per step: Jacobi {it_j.mean():.2f}, Chebyshev-initialised network "
-          f"{it_c.mean():.2f}, trained {it_l.mean():.2f} (accepted {info['accepted']}, validation "
-          f"{info['val_iters_before']} -> {info['val_iters_after']}, loss {info['loss_hist'][0]:.4f} -> "
-          f"{info['loss_hist'][-1]:.4f})")
-    assert info["loss_hist"][-1] < info["loss_hist"][0]
-    # The network never ends up worse than where training started, and both beat Jacobi.
-    assert it_l.mean() <= it_c.mean() + 0.5
-    assert it_c.mean() <= 0.5 * it_j.mean()
-    # Same solution to the solver tolerance.
+          f"{it_c.mean():.2f}, trained {it_l.mean():.2f} (accepted {pc['accepted']}, validation "
+          f"{pc['val_iters_before']} -> {pc['val_iters_after']}, loss {pc['loss_hist'][0]:.4f} -> "
+          f"{pc['loss_hist'][-1]:.4f})")
+    return it_j, U_j, it_c, U_c, it_l, U_l, pc
+
+
+def test_network_gain_over_jacobi_pcg():
+    """What the network gains over the method without it (Jacobi-PCG, the fallback and judge):
+    at most a third of the iterations per step, the same solution to the solver tolerance, and
+    training lowers its loss and never leaves it worse than its start (the validation guard)."""
+    m, dt, nt = W.block3d_implicit(12, 12, 10, nt=61)
+    it_j, U_j, it_c, U_c, it_l, U_l, pc = trained_vs_start(m, dt)
+    assert it_l.mean() <= it_j.mean() / 3.0 and it_c.mean() <= it_j.mean() / 3.0
+    assert pc["loss_hist"][-1] < pc["loss_hist"][0]
+    assert pc["val_iters_after"] <= pc["val_iters_before"]
     assert rel(U_l, U_j) < 1e-6 and rel(U_c, U_j) < 1e-6
+
+
+@pytest.mark.xfail(reason="the network is a polynomial in Ehat of fixed degree, and its Chebyshev start is "
+                          "already near the best such polynomial for this spectrum (least-squares polynomials "
+                          "of the same degree need as many or more FCG iterations), so training keeps the start",
+                   strict=False)
+def test_training_gain_over_chebyshev_start():
+    m, dt, nt = W.block3d_implicit(12, 12, 10, nt=61)
+    it_j, U_j, it_c, U_c, it_l, U_l, pc = trained_vs_start(m, dt)
+    assert pc["accepted"]
+    assert it_l.mean() < it_c.mean()
 
 
 def test_learned_fcg_history_matches_oracle():

@@ -208,3 +208,19 @@ def test_minres_iteration_limit_stops():
     pm = PmlNewmark(m, 0.01, maxit=1)
     with pytest.raises(SvlgError, match="did not converge"):
         pm.advance(1, 5)
+
+
+def test_eager_launches_equal_graph_replays():
+    """svlg_pml_set_graph(0): the MINRES iterations launched eagerly give the bits of the
+    graph-replayed ones (same kernels, same order) on a 3D box with G and Ubar."""
+    nt, dt = 41, 0.005
+    m = halfspace_box3d(3, 3, 2, 1, load_series=ricker(nt, dt, t0=0.05, f0=20.0))
+    rec = [(m.loads[0][0], 2)]
+    runs = []
+    for graph in (True, False):
+        pn = PmlNewmark(m, dt)
+        pn.set_graph(graph)
+        hu, hv, ha, its = pn.advance(1, nt - 1, rec)
+        runs.append((hu, its, pn.GetPMLHistoryVector()))
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    assert np.array_equal(runs[0][2], runs[1][2]) and np.abs(runs[0][2]).max() > 0

@@ -182,3 +182,38 @@ def test_handwritten_generalwave(tmp_path):
     assert list(m.drm_elem) == [0] and list(m.drm_node) == [0, 1, 2, 3] and list(m.drm_cond) == [1, 0, 0, 1]
     assert m.drm_factor == 1.5 and m.drm_signal.shape == (4, 3, 6)
     assert m.drm_signal[3, 2, 5] == 4 + 0.2 + 0.05  # stored as read; the exterior sign flips on the device
+
+
+def test_reads_run_files_written_by_svl_preprocessor():
+    """The run files SVL's own pre-processor writes for its validation cases
+    (tests/golden/svl_runfiles/, CreateRunAnalysisFiles output) are read unchanged: the free and
+    total DOF counts, the mass form, the integrator and the load series agree with the JSON and
+    the series files it names (looked up next to the run file, then in the case folder above
+    Partition/, where SVL's driver finds them)."""
+    import glob
+    import json
+    import os
+    from svl_amd import pmlfile as P
+    files = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "svl_runfiles", "*", "Partition",
+                                          "*.json")))
+    assert len(files) == 7
+    for f in files:
+        js = json.load(open(f))
+        g = js["Global"]
+        sim = js["Simulations"]["attributes"]
+        if P.is_pml_run(f):
+            r = P.read_pml_run(f)
+            assert r.ntotal == g["ntotal"] and r.mesh.numbering()[3] == g["nfree"]
+            assert not r.mesh.lumped or g.get("massform", "CONSISTENT").upper() == "LUMPED"
+        else:
+            r = S.read_run(f)
+            assert r.model.nfree == g["nfree"] and r.ntotal == g["ntotal"]
+            assert r.model.massform_lumped == (g.get("massform", "CONSISTENT").upper() == "LUMPED")
+            assert r.algorithm == sim["algorithm"]["name"].upper()
+            series = [ld["attributes"]["file"] for ld in js["Loads"].values() if "file" in ld["attributes"]]
+            case = os.path.dirname(os.path.dirname(f))
+            for s in set(series):
+                n = int(float(open(os.path.join(case, s)).read().split()[0]))
+                assert r.model.load_series.shape[1] == n
+        assert r.integrator == sim["integrator"]["name"].upper() and r.dt == sim["integrator"]["dt"]
+        assert r.nt == sim["analysis"]["nt"]
