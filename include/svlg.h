@@ -77,6 +77,24 @@ typedef struct svlg_mesh_desc {
     int32_t drm_nt;           /* rows of every node signal (row k at step k) */
     const double *drm_signal; /* [ndrm_node*drm_nt*3*ndim] rows (U, V, A components) */
     double drm_factor;        /* combination factor of the DRM load */
+    /* Support motion: Supports (TIMESERIES / CONSTANT per DOF) activated by the combination's
+     * POINTLOAD_SUPPORT_MOTION loads (Driver.hpp:505-563, 1725-1734; Assembler::
+     * ComputeSupportMotionIncrement, Assembler.cpp:493-532). Entry j prescribes the displacement
+     * g_j of a restrained DOF (node, dir) with the load's factor; with i = k - sm_offset, step k
+     * moves the DOF by the sum over its entries of factor (g_j(i) - g_j(i-1)) (factor g_j(0) at
+     * i = 0), g_j(i) = values[i] for i below its length, values[0] past it (Node::
+     * GetSupportMotion, Node.cpp:228-249). sm_offset 0: CentralDifference and
+     * ExtendedNewmarkBeta (CentralDifference.cpp:195, ExtendedNewmarkBeta.cpp:197); 1:
+     * NewmarkBeta (NewmarkBeta.cpp:187). The internal force of step k sees the support DOFs
+     * one increment late (SVL updates the strains before adding the increment, Algorithm.cpp:
+     * 18-56). Elastic models with the Linear algorithm only. nsm = 0: none. */
+    int32_t nsm;
+    const int32_t *sm_node;   /* [nsm] mesh node (restrained in sm_dir) */
+    const int32_t *sm_dir;    /* [nsm] 0-based direction */
+    const double *sm_factor;  /* [nsm] combination factor */
+    const int32_t *sm_ptr;    /* [nsm+1] offsets of each entry's values */
+    const double *sm_values;  /* [sm_ptr[nsm]] displacement series */
+    int32_t sm_offset;        /* 0 or 1, see above */
 } svlg_mesh_desc;
 
 typedef struct svlg_model svlg_model;
@@ -152,8 +170,9 @@ int svlg_internal_force(svlg_model *m, const double *U /*[nfree]*/, double ftol,
 int svlg_cd_init(svlg_model *m, double dt, double ftol);
 /* Advances nsteps steps starting at step index k0 (ComputeNewStep :124-152 via
  * Linear::ComputeNewIncrement, with DynamicAnalysis' index convention: step k uses
- * load sample k). If nrec > 0, U at the free DOFs rec_dofs[] after every step is
- * written to hist[s*nrec + r] (the NODE recorder's row k-1). */
+ * load sample k). If nrec > 0, U at the DOFs rec_dofs[] after every step is written to
+ * hist[s*nrec + r] (the NODE recorder's row k-1); rec_dofs[r] >= 0 is a free DOF index,
+ * -1 - t the total DOF t = node*ndim + c of a mesh node (restrained: 0 or its support motion). */
 int svlg_cd_steps(svlg_model *m, int32_t k0, int32_t nsteps, int32_t nrec,
                   const int32_t *rec_dofs, double *hist);
 /* Same as svlg_cd_steps but leaves results on the device and returns immediately
@@ -171,9 +190,10 @@ int svlg_cd_effective_force(svlg_model *m, int32_t k, double factor, double *Fef
 /* CentralDifference::ComputeEffectiveStiffness (:225-231): Keff = T^T (M/dt^2 + C/2dt) T, diagonal
  * for the lumped mass and dashpots; its diagonal over the free DOFs. */
 int svlg_cd_effective_stiffness(svlg_model *m, double *Kdiag /*[nfree]*/);
-/* The state update of CentralDifference::ComputeNewStep (:137-148) with the algorithm's
- * increment dU (free DOFs): Up = U, U += dU; V and A follow from the kept U history. */
-int svlg_cd_apply_increment(svlg_model *m, const double *dU /*[nfree]*/);
+/* The state update of CentralDifference::ComputeNewStep (:137-148) for step k with the
+ * algorithm's increment dU (free DOFs): Up = U, U += T dU + SupportMotion (a_k at the support-
+ * motion DOFs); V and A follow from the kept U history. */
+int svlg_cd_apply_increment(svlg_model *m, int32_t k, const double *dU /*[nfree]*/);
 /* Integrator::ComputeProgressiveForce (CentralDifference.cpp:174-186; Fbar = 0): Fext(k) at
  * the mesh nodes, [nnodes*ndim] in description order (Assembler::ComputeExternalForceVector,
  * Assembler.cpp:291-489: point loads and domain reduction forces). */
@@ -231,9 +251,20 @@ int svlg_nm_steps(svlg_model *m, int32_t k0, int32_t nsteps, int32_t nrec, const
  * over the free DOFs. `factor` is unused, as in SVL. Models with J2 elements form their Newton
  * residual in svlg_nl_steps. */
 int svlg_nm_effective_force(svlg_model *m, int32_t k, double factor, double *Feff /*[nfree]*/);
-/* The state update of ExtendedNewmarkBeta::ComputeNewStep (:141-150) with the algorithm's
- * increment dU (free DOFs): U += dU, A = 4/dt^2 dU - 4/dt V - A, V = 2/dt dU - V. */
-int svlg_nm_apply_increment(svlg_model *m, const double *dU /*[nfree]*/);
+/* The state update of ExtendedNewmarkBeta::ComputeNewStep (:123-154) for step k with the
+ * algorithm's increment dU (free DOFs): U += T dU + SupportMotion, A = 4/dt^2 dU - 4/dt V - A,
+ * V = 2/dt dU - V (V and A from the free increment alone, as in SVL). */
+int svlg_nm_apply_increment(svlg_model *m, int32_t k, const double *dU /*[nfree]*/);
+/* Integrator::ComputeSupportMotionVector for step k of the initialised integrator: Feff -=
+ * T^T (K a_k) with a_k the step's support increments and K the total effective stiffness
+ * (ExtendedNewmarkBeta.cpp:191-204: K + 4/dt^2 M + 2/dt C; CentralDifference.cpp:190-203:
+ * M/dt^2 + C/2dt, diagonal on the device, so Feff is unchanged). `factor` is unused, as in
+ * SVL. Feff [nfree] is read and written. */
+int svlg_support_motion_vector(svlg_model *m, int32_t k, double factor, double *Feff);
+/* Integrator::GetDisplacements: the total displacement vector of the initialised integrator
+ * over the mesh nodes, [nnodes*ndim] in description order (restrained DOFs: 0, or their
+ * support motion). */
+int svlg_get_displacements(svlg_model *m, double *U);
 /* Current U, V, A in free DOFs (ExtendedNewmarkBeta::GetDisplacements/...). */
 int svlg_nm_get_state(svlg_model *m, double *U, double *V, double *A);
 /* Switches the preconditioner of later solves (SVLG_PRECOND_LEARNED needs svlg_gnn_init;

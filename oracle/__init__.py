@@ -106,7 +106,8 @@ def _f(a):
 
 def run(model, integrator: int, dt: float, nt: int, rec_dofs=(), ftol: float = 1e-12, want_va: bool = False,
         algorithm: int = LINEAR, nmax: int = 20, cnvgtest: int = 5, cnvgtol: float = 1e-6, reac_tot=()):
-    """Runs k = 1..nt-1; returns dict with U (and V, A) histories [nt-1, nrec],
+    """Runs k = 1..nt-1; returns dict with U (and V, A) histories [nt-1, nrec] (rec_dofs: free
+    DOF indices, or -1 - t for total DOF t, e.g. a restrained DOF with support motion),
     the final free-DOF displacement, the time-loop wall time in seconds and, for Newton,
     the iterations of every step and the convergence-test value of every iteration
     ("residuals", [nt-1, nmax], NaN past the last iteration). reac_tot: total DOF indices

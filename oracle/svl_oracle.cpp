@@ -34,6 +34,12 @@
 //     :460-478, RP/12-Utilities/Driver.hpp:1669-1720); rows past the signal read as zero;
 //   * DynamicAnalysis loop and index convention (RP/08-Analysis/02-Dynamic/
 //     DynamicAnalysis.cpp:25-64): step k uses load sample k, rows written at k-1.
+//   * Support motion (POINTLOAD_SUPPORT_MOTION): Node::GetSupportMotion (RP/01-Node/Node.cpp
+//     :228-249), Assembler::ComputeSupportMotionIncrement (RP/07-Assembler/Assembler.cpp
+//     :493-532), the integrators' ComputeSupportMotionVector (Feff -= T^T K SM) and the SM
+//     term of ComputeNewStep (CentralDifference.cpp:137, 190-203; ExtendedNewmarkBeta.cpp
+//     :134-135, 191-204; NewmarkBeta.cpp:132, 181-195 with sample k-1), with the strains
+//     updated from U + T dU before SM is added (Linear.cpp:22-56, Algorithm.cpp:18-56).
 //   * EigenSolver with update OFF (RP/11-Solvers/01-Direct/EigenSolver.cpp:19-59):
 //     restated as a symmetric skyline LDL^T factorised once; with a diagonal matrix
 //     the solve is x_i = b_i / D_ii.
@@ -387,6 +393,10 @@ struct Model {
     int drm_nt = 0;
     std::vector<double> drm_sig;
     double drm_factor = 1.0;
+    // Support motion entries (load order): total DOF, load factor, displacement series.
+    std::vector<int> sm_dof;
+    std::vector<double> sm_fac;
+    std::vector<std::vector<double>> sm_vals;
     GaussCache gc;
 
     void element_X(const Continuum &e, double *X) const {
@@ -789,6 +799,14 @@ Model build(const svlg_mesh_desc *d) {
                 if (m.drm_slot[m.elems[e].nodes[a]] < 0) throw std::runtime_error("DRM element node without a signal");
         }
     }
+    for (int j = 0; j < std::max(d->nsm, 0); ++j) {
+        const int n = d->sm_node[j], c = d->sm_dir[j];
+        if (n < 0 || n >= d->nnodes || c < 0 || c >= m.dim) throw std::runtime_error("bad support motion entry");
+        if (d->sm_ptr[j + 1] <= d->sm_ptr[j]) throw std::runtime_error("support motion without values");
+        m.sm_dof.push_back(n * m.dim + c);
+        m.sm_fac.push_back(d->sm_factor[j]);
+        m.sm_vals.emplace_back(d->sm_values + d->sm_ptr[j], d->sm_values + d->sm_ptr[j + 1]);
+    }
     return m;
 }
 
@@ -908,6 +926,19 @@ struct Assembler {
             const double w = -V[di] + V[dj];  // a^T V with a = [-e_dir, +e_dir]
             F[di] += dp.eta * (-1.0) * w;
             F[dj] += dp.eta * w;
+        }
+    }
+    // Assembler::ComputeSupportMotionIncrement (Assembler.cpp:493-532): per entry (load order)
+    // factor * (g(j) - g(j-1)), g(0) alone at j = 0; g(j) = Node::GetSupportMotion (Node.cpp
+    // :228-249): values[j] below the series length, values[0] past it.
+    void support_increment(int j, std::vector<double> &SM) const {
+        SM.assign(m->ntotal, 0.0);
+        for (size_t e = 0; e < m->sm_dof.size(); ++e) {
+            const std::vector<double> &g = m->sm_vals[e];
+            auto val = [&](int i) { return i < int(g.size()) ? g[i] : g[0]; };
+            double dg = val(j);
+            if (j > 0) dg -= val(j - 1);
+            SM[m->sm_dof[e]] += m->sm_fac[e] * dg;
         }
     }
     // Assembler::ComputeExternalForceVector (Assembler.cpp:291-350, 485).
@@ -1090,10 +1121,21 @@ int oracle_run(const svlg_mesh_desc *d, const oracle_opts *o, int32_t nrec, cons
         std::vector<double> Fint, Fext, Ft(m.ntotal), Feff, x, dUt, tmp, tmp2;
         std::vector<int> rec_tot(nrec, -1);
         for (int r = 0; r < nrec; ++r) {
+            if (rec_dofs[r] < 0) {  // -1 - t: total DOF t (a restrained DOF, e.g. one with support motion)
+                rec_tot[r] = -1 - rec_dofs[r];
+                if (rec_tot[r] >= m.ntotal) { g_err = "recorded total dof out of range"; return 2; }
+                continue;
+            }
             for (int i = 0; i < m.ntotal; ++i)
                 if (m.freedof[i] == rec_dofs[r]) { rec_tot[r] = i; break; }
             if (rec_tot[r] < 0) { g_err = "recorded dof is not free"; return 2; }
         }
+        const bool has_sm = !m.sm_dof.empty();
+        if (has_sm && newton) {
+            g_err = "support motion with the Newton-Raphson algorithm is not restated";
+            return 2;
+        }
+        std::vector<double> SM(m.ntotal, 0.0);
         auto t0 = std::chrono::steady_clock::now();
         for (int k = 1; k < o->nt; ++k) {
             if (newton) {
@@ -1166,26 +1208,36 @@ int oracle_run(const svlg_mesh_desc *d, const oracle_opts *o, int32_t nrec, cons
                 for (int i = 0; i < m.ntotal; ++i) Ft[i] = (((Fext[i] + 0.0) - Fint[i]) + tmp[i]) + tmp2[i];
             }
             to_free(m, Ft, Feff);
+            if (has_sm) {
+                // ComputeSupportMotionVector: SM of sample k (NewmarkBeta: k - 1), Feff -= T^T K SM.
+                as.support_increment(o->integrator == 2 ? k - 1 : k, SM);
+                std::vector<double> lg, lgf;
+                Kt.mul(SM, lg);
+                to_free(m, lg, lgf);
+                for (int i = 0; i < m.nfree; ++i) Feff[i] -= lgf[i];
+            }
             solver.solve(Feff, x);
             to_total(m, x, dUt);
-            // UpdateStatesIncrements: strains from U + dU.
+            // UpdateStatesIncrements: strains from U + dU (before SM enters U).
             std::vector<double> Un(m.ntotal);
             for (int i = 0; i < m.ntotal; ++i) Un[i] = U[i] + dUt[i];
             update_states(m, Un);
             }
             if (o->integrator == 0) {
-                // CentralDifference::ComputeNewStep (:141-148).
+                // CentralDifference::ComputeNewStep (:137-148): dU = T x + SupportMotion.
                 for (int i = 0; i < m.ntotal; ++i) {
-                    double v = 1.0 / 2.0 / dt * ((U[i] + dUt[i]) - Up[i]);
-                    double a = 1.0 / dt / dt * ((dUt[i] - U[i]) + Up[i]);
+                    const double du = has_sm ? dUt[i] + SM[i] : dUt[i];
+                    double v = 1.0 / 2.0 / dt * ((U[i] + du) - Up[i]);
+                    double a = 1.0 / dt / dt * ((du - U[i]) + Up[i]);
                     V[i] = v; A[i] = a;
                     Up[i] = U[i];
-                    U[i] += dUt[i];
+                    U[i] += du;
                 }
             } else {
-                // ExtendedNewmarkBeta::ComputeNewStep (:141-150) with G = 0, no support motion.
+                // ExtendedNewmarkBeta::ComputeNewStep (:123-154) with G = 0 (NewmarkBeta.cpp
+                // :115-144 alike): U += T x + SupportMotion, V and A from T x alone.
                 for (int i = 0; i < m.ntotal; ++i) {
-                    U[i] += dUt[i];
+                    U[i] += has_sm ? dUt[i] + SM[i] : dUt[i];
                     double a = 4.0 / dt / dt * dUt[i] - 4.0 / dt * V[i] - A[i];
                     double v = 2.0 / dt * dUt[i] - V[i];
                     A[i] = a; V[i] = v;

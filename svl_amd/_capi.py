@@ -56,6 +56,13 @@ class MeshDesc(C.Structure):
         ("drm_nt", C.c_int32),
         ("drm_signal", f64p),
         ("drm_factor", C.c_double),
+        ("nsm", C.c_int32),
+        ("sm_node", i32p),
+        ("sm_dir", i32p),
+        ("sm_factor", f64p),
+        ("sm_ptr", i32p),
+        ("sm_values", f64p),
+        ("sm_offset", C.c_int32),
     ]
 
 
@@ -118,6 +125,15 @@ def make_desc(m: Model):
         d.drm_nt = int(sig.shape[1])
         d.drm_signal = _p(sig, f64p)
         d.drm_factor = float(m.drm_factor)
+    if len(m.sm_node):
+        ptr = np.concatenate([[0], np.cumsum([len(g) for g in m.sm_series])]).astype(np.int32)
+        d.nsm = int(len(m.sm_node))
+        d.sm_node = _p(arr(m.sm_node, np.int32), i32p)
+        d.sm_dir = _p(arr(m.sm_dir, np.int32), i32p)
+        d.sm_factor = _p(arr(m.sm_factor, np.float64), f64p)
+        d.sm_ptr = _p(arr(ptr, np.int32), i32p)
+        d.sm_values = _p(arr(np.concatenate(m.sm_series), np.float64), f64p)
+        d.sm_offset = int(m.sm_offset)
     return d, keep
 
 
@@ -154,10 +170,12 @@ SIGNATURES = {
     "svlg_keff_apply": (C.c_int, [C.c_void_p, f64p, f64p]),
     "svlg_cd_effective_force": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, f64p]),
     "svlg_cd_effective_stiffness": (C.c_int, [C.c_void_p, f64p]),
-    "svlg_cd_apply_increment": (C.c_int, [C.c_void_p, f64p]),
+    "svlg_cd_apply_increment": (C.c_int, [C.c_void_p, C.c_int32, f64p]),
     "svlg_external_force": (C.c_int, [C.c_void_p, C.c_int32, f64p]),
     "svlg_nm_effective_force": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, f64p]),
-    "svlg_nm_apply_increment": (C.c_int, [C.c_void_p, f64p]),
+    "svlg_nm_apply_increment": (C.c_int, [C.c_void_p, C.c_int32, f64p]),
+    "svlg_support_motion_vector": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, f64p]),
+    "svlg_get_displacements": (C.c_int, [C.c_void_p, f64p]),
     "svlg_keff_apply_noalloc": (C.c_int, [C.c_void_p, f64p, f64p]),
     "svlg_solve": (C.c_int, [C.c_void_p, f64p, f64p, C.c_double, C.c_int32, i32p, f64p]),
     "svlg_nm_set_precond": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -118,6 +118,13 @@ struct Model {
     DevBuf<double> drm_F;
     DevBuf<int32_t> drm_slot, drm_dnode;
 
+    // Support motion (svlg_support.hpp): the restrained node-DOFs with a prescribed motion.
+    int sm_ndofs = 0, sm_offset = 0;
+    DevBuf<int32_t> sm_slot, sm_dof, sm_dptr, sm_vptr;
+    DevBuf<double> sm_fac, sm_vals;
+    DevBuf<double> sm_vec, sm_y;      // Newmark: a_k as a node-DOF vector, Keff times it
+    DevBuf<double> sm_save, sm_lag;   // [sm_ndofs] U while lagged; Newmark: U_(k-1) there
+
     // Device node numbering (tile order): perm[device] = original, inv[original] = device.
     std::vector<int32_t> perm, inv;
 

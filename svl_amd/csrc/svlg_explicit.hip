@@ -17,6 +17,7 @@
 
 #include "svlg_common.hpp"
 #include "svlg_drm.hpp"
+#include "svlg_support.hpp"
 #include "svlg_elem.hpp"
 #include "svlg_mfma.hpp"
 
@@ -24,6 +25,7 @@ namespace svlg {
 
 int launch_fint_colored(Model &m, const double *U, double ftol, double *F);
 int gather_free(Model &m, const double *dnode, double *dfree);
+int record_positions(const Model &m, int nrec, const int32_t *rec_dofs, std::vector<int32_t> &pos);
 int scatter_free(Model &m, const double *dfree, double *dnode);
 
 // keff = M/dt^2 + C/2dt, kforce = M/dt^2 - C/2dt per DOF (CentralDifference.cpp:64-70); dtv
@@ -93,7 +95,7 @@ __global__ void k_cd_feff(int n, const double *__restrict__ U, const double *__r
 }
 
 // U_next = U + dU on the free DOFs, 0 on the restrained ones (CentralDifference::ComputeNewStep,
-// :137-148, with the algorithm's increment).
+// :137-148, with the algorithm's increment; support motion is added by its own kernel).
 __global__ void k_cd_increment(int n, const double *__restrict__ U, const double *__restrict__ dU,
                                const double *__restrict__ keff, double *__restrict__ Unext) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -734,8 +736,12 @@ static int cd_one_step(Model &m, int k) {
     double *Ucur = m.U[m.ring].p;
     double *Uprev = m.U[(m.ring + 2) % 3].p;
     double *Unext = m.U[(m.ring + 1) % 3].p;
+    // Support motion: the internal force sees U_(k-2) at its DOFs (the strains lag one
+    // increment); afterwards U_(k-1) returns and U_k = U_(k-1) + a_k there.
+    if (int rc = sm_lag_in(m, Ucur, Uprev, nullptr)) return rc;
     if (m.variant == 1 || m.variant == 3) {
         if (int rc = launch_fused(m, k, Ucur, Uprev, Unext)) return rc;
+        if (int rc = sm_finish(m, 1, k, Ucur, Unext)) return rc;
         m.ring = (m.ring + 1) % 3;
         m.steps_done++;
         return 0;
@@ -749,6 +755,7 @@ static int cd_one_step(Model &m, int k) {
     int nb = (m.ndofn + 255) / 256;
     hipLaunchKernelGGL(k_cd_update, dim3(nb), dim3(256), 0, m.stream, m.ndofn, Ucur, Uprev, Unext, m.F.p, m.Fext.p,
                        m.keff.p, m.kforce.p);
+    if (int rc = sm_finish(m, 1, k, Ucur, Unext)) return rc;
     m.ring = (m.ring + 1) % 3;
     m.steps_done++;
     return 0;
@@ -827,7 +834,10 @@ static int cd_capture(Model &m, int steps) {
     SVLG_HIP(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal));
     int ring = m.ring, rc = 0;
     for (int s = 0; s < steps && rc == 0; ++s) {
-        rc = launch_fused(m, s, m.U[ring].p, m.U[(ring + 2) % 3].p, m.U[(ring + 1) % 3].p, m.kdev.p);
+        double *Ucur = m.U[ring].p, *Uprev = m.U[(ring + 2) % 3].p, *Unext = m.U[(ring + 1) % 3].p;
+        rc = sm_lag_in(m, Ucur, Uprev, nullptr);
+        if (rc == 0) rc = launch_fused(m, s, Ucur, Uprev, Unext, m.kdev.p);
+        if (rc == 0) rc = sm_finish(m, 1, s, Ucur, Unext, m.kdev.p);
         ring = (ring + 1) % 3;
     }
     if (rc == 0) hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, m.stream, m.kdev.p, steps);
@@ -960,15 +970,8 @@ int svlg_cd_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
     SVLG_CHECK(nsteps >= 0 && nrec >= 0, "negative count");
     if (nrec > 0) {
         SVLG_CHECK(rec_dofs && hist, "recorder arrays missing");
-        // Free DOF -> node-DOF position.
-        std::vector<int32_t> f2n(m.nfree, -1);
-        for (int i = 0; i < m.ndofn; ++i)
-            if (m.h_free[i] >= 0) f2n[m.h_free[i]] = i;
-        std::vector<int32_t> rn(nrec);
-        for (int r = 0; r < nrec; ++r) {
-            SVLG_CHECK(rec_dofs[r] >= 0 && rec_dofs[r] < m.nfree, "recorded DOF is not a free DOF");
-            rn[r] = f2n[rec_dofs[r]];
-        }
+        std::vector<int32_t> rn;
+        if (int rc = record_positions(m, nrec, rec_dofs, rn)) return rc;
         SVLG_HIP(m.rec.upload(rn.data(), rn.size(), m.stream));
         SVLG_HIP(m.hist.alloc(size_t(nrec) * std::max(nsteps, 1)));
     }
@@ -991,7 +994,9 @@ int svlg_cd_effective_force(svlg_model *h, int32_t k, double factor, double *Fef
     SVLG_CHECK(h && Feff, "null argument");
     Model &m = h->m;
     SVLG_CHECK(m.cd_ready, "svlg_cd_init has not been called");
-    const double *Ucur = m.U[m.ring].p, *Up = m.U[(m.ring + 2) % 3].p;
+    double *Ucur = m.U[m.ring].p;
+    const double *Up = m.U[(m.ring + 2) % 3].p;
+    if (int rc = sm_lag_in(m, Ucur, Up, nullptr)) return rc;
     if (m.nload > 0)
         hipLaunchKernelGGL(k_loads, dim3(1), dim3(64), 0, m.stream, k, m.nload, m.load_nt, m.ndim,
                            (int)m.h_load_dofs.size(), m.load_dofs.p, m.load_node.p, m.load_dir.p, m.load_factor.p,
@@ -1004,6 +1009,7 @@ int svlg_cd_effective_force(svlg_model *h, int32_t k, double factor, double *Fef
     const int nb = (m.ndofn + 255) / 256;
     hipLaunchKernelGGL(k_cd_feff, dim3(nb), dim3(256), 0, m.stream, m.ndofn, Ucur, Up, m.F.p, m.Fext.p, m.keff.p,
                        m.kforce.p, fn.p);
+    if (int rc = sm_finish(m, 0, k, Ucur, nullptr)) return rc;
     if (int rc = gather_free(m, fn.p, ff.p)) return rc;
     SVLG_HIP(hipMemcpyAsync(Feff, ff.p, sizeof(double) * m.nfree, hipMemcpyDeviceToHost, m.stream));
     SVLG_HIP(hipStreamSynchronize(m.stream));
@@ -1043,7 +1049,7 @@ int svlg_cd_effective_stiffness(svlg_model *h, double *Kdiag) {
     return 0;
 }
 
-int svlg_cd_apply_increment(svlg_model *h, const double *dU) {
+int svlg_cd_apply_increment(svlg_model *h, int32_t k, const double *dU) {
     SVLG_CHECK(h && dU, "null argument");
     Model &m = h->m;
     SVLG_CHECK(m.cd_ready, "svlg_cd_init has not been called");
@@ -1055,6 +1061,7 @@ int svlg_cd_apply_increment(svlg_model *h, const double *dU) {
     hipLaunchKernelGGL(k_cd_increment, dim3(nb), dim3(256), 0, m.stream, m.ndofn, m.U[m.ring].p, dn.p, m.keff.p,
                        m.U[(m.ring + 1) % 3].p);
     SVLG_HIP(hipGetLastError());
+    if (int rc = sm_add_increment(m, k, m.U[m.ring].p, m.U[(m.ring + 1) % 3].p)) return rc;
     m.ring = (m.ring + 1) % 3;
     m.steps_done++;
     SVLG_HIP(hipStreamSynchronize(m.stream));

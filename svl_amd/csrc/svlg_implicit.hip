@@ -26,6 +26,7 @@
 
 #include "svlg_common.hpp"
 #include "svlg_drm.hpp"
+#include "svlg_support.hpp"
 #include "svlg_elem.hpp"
 #include "svlg_j2.hpp"
 #include "svlg_solver.hpp"
@@ -33,6 +34,7 @@
 namespace svlg {
 
 int gather_free(Model &m, const double *dnode, double *dfree);
+int record_positions(const Model &m, int nrec, const int32_t *rec_dofs, std::vector<int32_t> &pos);
 int scatter_free(Model &m, const double *dfree, double *dnode);
 
 enum { OP_CG = 0, OP_RHS = 1 };
@@ -465,6 +467,12 @@ __global__ void k_nm_update(int n, double dt, const double *__restrict__ dU, con
     V[i] = 2.0 / dt * du - v;
 }
 
+// b -= y (the support-motion term of the effective force; y is 0 on the restrained DOFs).
+__global__ void k_sub(int n, const double *__restrict__ y, double *__restrict__ b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] -= y[i];
+}
+
 // Diagonal of sum_e (1 + 2 ak_e/dt) K_e by unit probes through the Gauss-point routine
 // (valid for every element shape), one colour class per launch so no two threads of a
 // launch touch the same node. rho non-null (consistent mass): plus (4/dt^2 + 2 am_e/dt) times
@@ -658,6 +666,17 @@ int keff_apply_node(Model &m, const double *x, double *y) {
     a.y = y;
     a.reduce = 0;
     return launch_op(m, OP_CG, a);
+}
+
+// b -= Keff a_k over the free DOFs (ExtendedNewmarkBeta::ComputeSupportMotionVector, :191-204:
+// Feff -= T^T (K SupportMotion) with K the total effective stiffness).
+int sm_keff_subtract(Model &m, int k, double *b) {
+    if (m.sm_ndofs == 0) return 0;
+    if (int rc = sm_vector(m, k, m.sm_vec.p)) return rc;
+    if (int rc = keff_apply_node(m, m.sm_vec.p, m.sm_y.p)) return rc;
+    hipLaunchKernelGGL(k_sub, dim3((m.ndofn + 255) / 256), dim3(256), 0, m.stream, m.ndofn, m.sm_y.p, b);
+    SVLG_HIP(hipGetLastError());
+    return 0;
 }
 
 // Partial-sum buffers of a solve inside Model::part.
@@ -866,6 +885,13 @@ int svlg_nm_init(svlg_model *h, double dt, double rtol, int32_t maxit, int32_t p
         SVLG_HIP(b->alloc(n));
         SVLG_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * n, s));
     }
+    if (m.sm_ndofs > 0) {
+        for (DevBuf<double> *b : {&m.sm_vec, &m.sm_y}) {
+            SVLG_HIP(b->alloc(n));
+            SVLG_HIP(hipMemsetAsync(b->p, 0, sizeof(double) * n, s));
+        }
+        SVLG_HIP(hipMemsetAsync(m.sm_lag.p, 0, sizeof(double) * m.sm_ndofs, s));
+    }
     SVLG_HIP(m.part.alloc(cg_parts_size(m)));
     SVLG_HIP(m.scal.alloc(SC_COUNT));
     SVLG_HIP(hipMemsetAsync(m.scal.p, 0, sizeof(double) * SC_COUNT, s));
@@ -951,14 +977,8 @@ int svlg_nm_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
     DevBuf<double> hbuf;
     if (nrec > 0 && nsteps > 0) {
         SVLG_CHECK(rec_dofs && hist, "recorder arrays missing");
-        std::vector<int32_t> f2n(m.nfree, -1);
-        for (int i = 0; i < m.ndofn; ++i)
-            if (m.h_free[i] >= 0) f2n[m.h_free[i]] = i;
-        std::vector<int32_t> rn(nrec);
-        for (int r = 0; r < nrec; ++r) {
-            SVLG_CHECK(rec_dofs[r] >= 0 && rec_dofs[r] < m.nfree, "recorded DOF is not a free DOF");
-            rn[r] = f2n[rec_dofs[r]];
-        }
+        std::vector<int32_t> rn;
+        if (int rc = record_positions(m, nrec, rec_dofs, rn)) return rc;
         SVLG_HIP(rec.upload(rn.data(), rn.size(), s));
         SVLG_HIP(hbuf.alloc(size_t(nrec) * nsteps));
     }
@@ -982,16 +1002,25 @@ int svlg_nm_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
         // Feff (ExtendedNewmarkBeta::ComputeEffectiveForce with the Linear algorithm's dU = 0).
         a.k = k0 + st;
         a.drm = drm_ref(m);
+        // Support motion: the internal force sees U before the last support increment (the
+        // strains lag one increment), and Feff -= Keff a_k.
+        if (int rc = sm_lag_in(m, m.nU.p, nullptr, m.sm_lag.p)) return rc;
         if (int rc = launch_op(m, OP_RHS, a)) return rc;
+        if (int rc = sm_keff_subtract(m, a.k, m.cg_b.p)) return rc;
         // Keff dU = Feff; the previous increment is the initial guess.
         int it = 0;
         double rel = 0.0;
         const int rc = cg_solve(m, m.cg_b.p, m.ndU.p, m.nm_rtol, m.nm_maxit, m.nm_precond, &it, &rel, true);
         if (iters) iters[st] = it;
         if (relres) relres[st] = rel;
-        if (rc) return rc;
+        if (rc) {
+            (void)sm_finish(m, 0, a.k, m.nU.p, nullptr);
+            return rc;
+        }
         hipLaunchKernelGGL(k_nm_update, dim3(nb), dim3(256), 0, s, m.ndofn, m.nm_dt, m.ndU.p, m.jac.p, m.nU.p, m.nV.p,
                            m.nA.p);
+        // U_k = U_(k-1) + a_k at the support-motion DOFs; U_(k-1) becomes the next lag.
+        if (int rc = sm_finish(m, 2, a.k, m.nU.p, nullptr)) return rc;
         if (nrec > 0)
             hipLaunchKernelGGL(k_record_nm, dim3((nrec + 63) / 64), dim3(64), 0, s, nrec, rec.p, m.nU.p,
                                hbuf.p + size_t(st) * nrec);
@@ -1027,7 +1056,9 @@ int svlg_nm_effective_force(svlg_model *h, int32_t k, double factor, double *Fef
     a.load_nt = m.load_nt;
     a.k = k;
     a.drm = drm_ref(m);
+    if (int rc = sm_lag_in(m, m.nU.p, nullptr, m.sm_lag.p)) return rc;
     if (int rc = launch_op(m, OP_RHS, a)) return rc;
+    if (int rc = sm_finish(m, 0, k, m.nU.p, nullptr)) return rc;
     DevBuf<double> ff;
     SVLG_HIP(ff.alloc(std::max(m.nfree, 1)));
     if (int rc = gather_free(m, m.cg_b.p, ff.p)) return rc;
@@ -1036,7 +1067,7 @@ int svlg_nm_effective_force(svlg_model *h, int32_t k, double factor, double *Fef
     return 0;
 }
 
-int svlg_nm_apply_increment(svlg_model *h, const double *dU) {
+int svlg_nm_apply_increment(svlg_model *h, int32_t k, const double *dU) {
     SVLG_CHECK(h && dU, "null argument");
     Model &m = h->m;
     SVLG_CHECK(m.nm_ready, "svlg_nm_init has not been called");
@@ -1047,8 +1078,33 @@ int svlg_nm_apply_increment(svlg_model *h, const double *dU) {
     hipLaunchKernelGGL(k_nm_update, dim3(nb), dim3(256), 0, m.stream, m.ndofn, m.nm_dt, m.ndU.p, m.jac.p, m.nU.p,
                        m.nV.p, m.nA.p);
     SVLG_HIP(hipGetLastError());
+    // DU = dU + SupportMotion (:134-135): the lag takes U_(k-1), U gains a_k.
+    if (int rc = sm_lag_in(m, m.nU.p, m.nU.p, nullptr)) return rc;
+    if (int rc = sm_finish(m, 2, k, m.nU.p, nullptr)) return rc;
     m.nm_steps_done++;
     SVLG_HIP(hipStreamSynchronize(m.stream));
+    return 0;
+}
+
+int svlg_support_motion_vector(svlg_model *h, int32_t k, double factor, double *Feff) {
+    (void)factor;  // unused by the integrators as well
+    SVLG_CHECK(h && Feff, "null argument");
+    Model &m = h->m;
+    SVLG_CHECK(m.active == 1 || m.active == 2, "no integrator has been initialised");
+    // CentralDifference: K = M/dt^2 + C/2dt is diagonal on the device (lumped M, Rayleigh ak = 0,
+    // dashpots), so T^T (K a_k) has no free entries and Feff is unchanged.
+    if (m.sm_ndofs == 0 || m.active == 1) return 0;
+    SVLG_CHECK(m.nm_ready, "svlg_nm_init has not been called");
+    DevBuf<double> bn, ff;
+    SVLG_HIP(bn.alloc(m.ndofn));
+    SVLG_HIP(ff.alloc(std::max(m.nfree, 1)));
+    SVLG_HIP(hipMemsetAsync(bn.p, 0, sizeof(double) * m.ndofn, m.stream));
+    if (int rc = sm_keff_subtract(m, k, bn.p)) return rc;
+    if (int rc = gather_free(m, bn.p, ff.p)) return rc;
+    std::vector<double> hb(m.nfree);
+    SVLG_HIP(hipMemcpyAsync(hb.data(), ff.p, sizeof(double) * m.nfree, hipMemcpyDeviceToHost, m.stream));
+    SVLG_HIP(hipStreamSynchronize(m.stream));
+    for (int i = 0; i < m.nfree; ++i) Feff[i] += hb[i];  // hb = -(Keff a_k) at the free DOFs
     return 0;
 }
 

@@ -16,6 +16,7 @@
 
 #include "svlg_common.hpp"
 #include "svlg_drm.hpp"
+#include "svlg_support.hpp"
 #include "svlg_elem.hpp"
 #include "svlg_tiles.hpp"
 
@@ -162,6 +163,28 @@ int scatter_free(Model &m, const double *dfree, double *dnode) {
     SVLG_HIP(hipGetLastError());
     return 0;
 }
+// Recorder positions: rec_dofs[r] >= 0 is a free DOF index, -1 - t the total DOF t = node *
+// ndim + c of a mesh node (a restrained DOF, e.g. one with support motion); pos[r] is its
+// device node-DOF position.
+int record_positions(const Model &m, int nrec, const int32_t *rec_dofs, std::vector<int32_t> &pos) {
+    std::vector<int32_t> f2n(m.nfree, -1);
+    for (int i = 0; i < m.ndofn; ++i)
+        if (m.h_free[i] >= 0) f2n[m.h_free[i]] = i;
+    pos.assign(nrec, 0);
+    for (int r = 0; r < nrec; ++r) {
+        const int32_t v = rec_dofs[r];
+        if (v >= 0) {
+            SVLG_CHECK(v < m.nfree, "recorded DOF is not a free DOF");
+            pos[r] = f2n[v];
+        } else {
+            const int64_t t = -1 - int64_t(v);
+            SVLG_CHECK(t < int64_t(m.nnodes) * m.ndim, "recorded total DOF out of range");
+            pos[r] = m.inv[t / m.ndim] * m.ndim + int32_t(t % m.ndim);
+        }
+    }
+    return 0;
+}
+
 int gather_free(Model &m, const double *dnode, double *dfree) {
     int nb = (m.ndofn + 255) / 256;
     hipLaunchKernelGGL(k_gather_free, dim3(nb), dim3(256), 0, m.stream, m.ndofn, m.fdof.p, dnode, dfree);
@@ -665,6 +688,7 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         }
     }
     if (int rc = drm_setup(m, d)) return fail(rc);
+    if (int rc = sm_setup(m, d)) return fail(rc);
     if (int rc = prepare_fused(m)) return fail(rc);
     *out = h;
     return 0;

@@ -25,6 +25,7 @@
 namespace svlg {
 
 int gather_free(Model &m, const double *dnode, double *dfree);
+int record_positions(const Model &m, int nrec, const int32_t *rec_dofs, std::vector<int32_t> &pos);
 int launch_nl_rhs(Model &m, int k, const double *un, const double *w, const double *dU, double *b);
 
 static constexpr int kNlGrid = 256, kNlBlock = 512;
@@ -200,6 +201,8 @@ int svlg_nl_init(svlg_model *h, double dt, double cg_rtol, int32_t cg_maxit, dou
     Model &m = h->m;
     SVLG_CHECK(cnvgtol > 0.0 && nmax > 0, "cnvgtol and nmax must be positive");
     SVLG_CHECK(cnvgtest >= 1 && cnvgtest <= 8, "cnvgtest must be 1..8 (Algorithm::ComputeConvergence)");
+    // NewtonRaphson subtracts K SupportMotion again in every iteration (NewtonRaphson.cpp:40-48).
+    SVLG_CHECK(m.sm_ndofs == 0, "support motion with the Newton-Raphson algorithm is not on the device path");
     // Mass, damping, Jacobi preconditioner of the initial effective stiffness, work vectors.
     if (int rc = svlg_nm_init(h, dt, cg_rtol, cg_maxit, SVLG_PRECOND_JACOBI)) return rc;
     hipStream_t s = m.stream;
@@ -244,14 +247,8 @@ int svlg_nl_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
     DevBuf<double> hbuf;
     if (nrec > 0 && nsteps > 0) {
         SVLG_CHECK(rec_dofs && hist, "recorder arrays missing");
-        std::vector<int32_t> f2n(m.nfree, -1);
-        for (int i = 0; i < m.ndofn; ++i)
-            if (m.h_free[i] >= 0) f2n[m.h_free[i]] = i;
-        std::vector<int32_t> rn(nrec);
-        for (int r = 0; r < nrec; ++r) {
-            SVLG_CHECK(rec_dofs[r] >= 0 && rec_dofs[r] < m.nfree, "recorded DOF is not a free DOF");
-            rn[r] = f2n[rec_dofs[r]];
-        }
+        std::vector<int32_t> rn;
+        if (int rc = record_positions(m, nrec, rec_dofs, rn)) return rc;
         SVLG_HIP(rec.upload(rn.data(), rn.size(), s));
         SVLG_HIP(hbuf.alloc(size_t(nrec) * nsteps));
     }

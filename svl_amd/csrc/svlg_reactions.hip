@@ -10,11 +10,14 @@
 // (J2 elements: B^T sigma from the Gauss-point states + ak B^T D0 B V). With the consistent
 // mass the inertia and the Rayleigh am term come from the element: [M_e (A_e + am_e V_e)]_node,
 // and the node term keeps the dashpots alone. Only the requested nodes are computed; the
-// restrained partner node of a dashpot carries -eta V of its node.
+// restrained partner node of a dashpot carries -eta V of its node. With support motion the
+// element forces use the displacement the strains hold (U_(k-1) at those DOFs: the strains lag
+// one support increment, Algorithm.cpp:18-56 before CentralDifference.cpp:137).
 #include <vector>
 
 #include "svlg_common.hpp"
 #include "svlg_drm.hpp"
+#include "svlg_support.hpp"
 #include "svlg_elem.hpp"
 #include "svlg_j2.hpp"
 
@@ -40,6 +43,8 @@ struct ReacArgs {
     int nelem;
     DrmRef drm;                 // domain reduction forces of every step (F null: none)
     const double *rho, *am;     // consistent mass: element density and Rayleigh am (null: lumped)
+    SmRef sm;                   // support motion (slot null: none)
+    const double *smlag;        // Newmark: U before the last support increment, per slot
     double *R;                  // [nq * DIM]
 };
 
@@ -85,7 +90,12 @@ __global__ void k_reactions(ReacArgs a) {
                 state_va(a, nd + d, vv, aa);
                 x[c][d] = a.X[nd + d];
                 v[c][d] = vv;
-                u[c][d] = Ucur[nd + d] + ak * vv;
+                double uc = Ucur[nd + d];
+                if (a.sm.slot) {
+                    const int sl = a.sm.slot[nd + d];
+                    if (sl >= 0) uc = a.mode == 2 ? a.smlag[sl] : a.Uk1[nd + d];
+                }
+                u[c][d] = uc + ak * vv;
                 w[c][d] = aa + am * vv;
             }
         }
@@ -188,6 +198,8 @@ int svlg_reactions(svlg_model *h, int32_t n, const int32_t *nodes, int32_t k, do
     a.state = (m.active == 2 && m.has_j2 && m.nl_ready) ? m.gp_state.p : nullptr;
     a.nelem = m.nelem;
     a.drm = drm_ref(m);
+    a.sm = sm_ref(m);
+    a.smlag = m.sm_lag.p;
     a.R = rb.p;
     const int nb = (n + 63) / 64;
     if (m.ndim == 3)

@@ -121,6 +121,23 @@ def external_force(mesh: DeviceModel, k: int) -> np.ndarray:
     return out.reshape(info["nnodes"], mesh.ndim)
 
 
+def support_motion_vector(mesh: DeviceModel, Feff: np.ndarray, k: int, factor: float = 1.0) -> np.ndarray:
+    """Integrator::ComputeSupportMotionVector of the integrator initialised last: a copy of Feff
+    (free DOFs) minus T^T K a_k."""
+    out = np.array(Feff, dtype=np.float64, copy=True)
+    capi.check(capi.lib().svlg_support_motion_vector(mesh.h, int(k), float(factor), capi.fptr(out)),
+               "svlg_support_motion_vector")
+    return out
+
+
+def total_displacements(mesh: DeviceModel) -> np.ndarray:
+    """The total displacement vector [nnodes, ndim] of the integrator initialised last
+    (restrained DOFs: 0, or their support motion)."""
+    out = np.zeros((mesh.model.nnodes, mesh.ndim))
+    capi.check(capi.lib().svlg_get_displacements(mesh.h, capi.fptr(out)), "svlg_get_displacements")
+    return out
+
+
 class CentralDifference:
     """CentralDifference(mesh, TimeStep, mtol, ktol, ftol) on the device."""
 
@@ -166,10 +183,16 @@ class CentralDifference:
         capi.check(capi.lib().svlg_cd_effective_stiffness(self.mesh.h, capi.fptr(out)), "svlg_cd_effective_stiffness")
         return out
 
-    def ApplyIncrement(self, dU: np.ndarray):
-        """The update of ComputeNewStep with the algorithm's increment (free DOFs)."""
+    def ComputeSupportMotionVector(self, Feff: np.ndarray, k: int, factor: float = 1.0) -> np.ndarray:
+        """Feff - T^T K a_k (a_k: the step's support increments); K = M/dt^2 + C/2dt is diagonal
+        here, so Feff comes back unchanged."""
+        return support_motion_vector(self.mesh, Feff, k, factor)
+
+    def ApplyIncrement(self, dU: np.ndarray, k: int):
+        """The update of ComputeNewStep for step k with the algorithm's increment (free DOFs);
+        the support-motion DOFs move by their increment of step k."""
         d = np.ascontiguousarray(dU, dtype=np.float64)
-        capi.check(capi.lib().svlg_cd_apply_increment(self.mesh.h, capi.fptr(d)), "svlg_cd_apply_increment")
+        capi.check(capi.lib().svlg_cd_apply_increment(self.mesh.h, int(k), capi.fptr(d)), "svlg_cd_apply_increment")
 
     def ComputeProgressiveForce(self, k: int) -> np.ndarray:
         return external_force(self.mesh, k)
@@ -246,10 +269,15 @@ class ExtendedNewmarkBeta:
                    "svlg_nm_effective_force")
         return out
 
-    def ApplyIncrement(self, dU: np.ndarray):
-        """The update of ComputeNewStep with the algorithm's increment (free DOFs)."""
+    def ComputeSupportMotionVector(self, Feff: np.ndarray, k: int, factor: float = 1.0) -> np.ndarray:
+        """Feff - T^T Keff a_k (ExtendedNewmarkBeta::ComputeSupportMotionVector, :191-204)."""
+        return support_motion_vector(self.mesh, Feff, k, factor)
+
+    def ApplyIncrement(self, dU: np.ndarray, k: int):
+        """The update of ComputeNewStep for step k with the algorithm's increment (free DOFs):
+        U += T dU + SupportMotion; V and A from dU alone."""
         d = np.ascontiguousarray(dU, dtype=np.float64)
-        capi.check(capi.lib().svlg_nm_apply_increment(self.mesh.h, capi.fptr(d)), "svlg_nm_apply_increment")
+        capi.check(capi.lib().svlg_nm_apply_increment(self.mesh.h, int(k), capi.fptr(d)), "svlg_nm_apply_increment")
 
     def ComputeProgressiveForce(self, k: int) -> np.ndarray:
         return external_force(self.mesh, k)

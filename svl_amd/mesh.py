@@ -74,6 +74,15 @@ class Model:
     drm_cond: np.ndarray | None = None
     drm_signal: np.ndarray | None = None     # [n_drm_node, nt, 3*ndim]
     drm_factor: float = 1.0
+    # Support motion (Supports + POINTLOAD_SUPPORT_MOTION of the combination): per entry a
+    # restrained node, its direction, the load's combination factor and the prescribed
+    # displacement series g (read at step k as g[k], or g[0] past its end, as Node::
+    # GetSupportMotion does).
+    sm_node: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    sm_dir: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    sm_factor: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float64))
+    sm_series: list = field(default_factory=list)
+    sm_offset: int = 0      # 1: step k applies the increment of sample k-1 (NewmarkBeta)
 
     def __post_init__(self):
         self.coords = np.ascontiguousarray(self.coords, dtype=np.float64)
@@ -111,6 +120,16 @@ class Model:
             self.load_series = np.zeros((0, 1))
         self.load_dir = np.ascontiguousarray(self.load_dir)
         self.load_series = np.ascontiguousarray(self.load_series)
+
+    def add_support_motion(self, node: int, direction: int, series, factor: float = 1.0):
+        """Prescribed displacement history of a restrained DOF (Supports TIMESERIES / CONSTANT
+        activated by a SupportMotion load with the given combination factor)."""
+        if not self.restrained[node, direction]:
+            raise ValueError("support motion needs a restrained DOF")
+        self.sm_node = np.append(self.sm_node, np.int32(node)).astype(np.int32)
+        self.sm_dir = np.append(self.sm_dir, np.int32(direction)).astype(np.int32)
+        self.sm_factor = np.append(self.sm_factor, float(factor)).astype(np.float64)
+        self.sm_series = list(self.sm_series) + [np.ascontiguousarray(series, dtype=np.float64).reshape(-1)]
 
     def set_j2(self, K, G, sigma_y, H, beta, elems=None):
         """Makes elements J2 (Plastic3DJ2 / PlasticPlaneStrainJ2) with bulk modulus K, shear

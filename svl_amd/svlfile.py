@@ -12,9 +12,11 @@ algorithm, integrator, analysis, combo :1747-1850) and ``Recorders`` (:1858-1927
 Continuum nodes become mesh nodes in ascending tag order; a ZeroLength1D element with a
 Viscous1DLinear material between a mesh node and a fully restrained node is a Lysmer dashpot;
 the free-DOF numbering is taken from the file. Anything else (other elements or materials,
-support motion, constraints, point masses, body loads) is refused with ValueError: it is
-not on the device path. ``ElementLoad`` GENERALWAVE (domain reduction, :1669-1720) is read with
-its per-node signal files.
+constraints, point masses, body loads) is refused with ValueError: it is not on the device
+path. ``ElementLoad`` GENERALWAVE (domain reduction, :1669-1720) is read with its per-node
+signal files; ``Supports`` (CONSTANT value or TIMESERIES file per DOF, :505-563) with the
+combination's ``SupportMotion`` loads (:1725-1734) become the model's support motion (restrained
+DOFs of mesh nodes, Linear algorithm).
 
 Writer (``write_run``) of the same schema from a ``Model`` (the inverse, with a restrained
 copy node per dashpot as setLysmerDomain makes them, PP/Method/Builder.py:1086-1131), so a
@@ -125,6 +127,35 @@ def _read_drm(model: Model, at: dict, etag_index: dict, index: dict, base: str, 
     attach(model, elems, np.asarray([index[t] for t in tags], np.int32), np.asarray(cond, np.int32), signal, factor)
 
 
+def _read_support_motion(model: Model, at: dict, supports: dict, index: dict, base: str, factor: float):
+    """A SupportMotion load (Driver.hpp:1725-1734): every listed node's Supports entry
+    (Driver.hpp:505-563: per listed DOF a CONSTANT value or a TIMESERIES file holding the number
+    of samples and the samples) with the load's combination factor. A node without a Supports
+    entry moves nothing (Node::GetSupportMotion is zero there)."""
+    for t in at.get("list", []):
+        t = int(t)
+        if t not in supports:
+            continue
+        if t not in index:
+            raise ValueError(f"support motion at node {t}: not a node of the continuum mesh")
+        sp = supports[t]
+        kind = _lc(sp.get("type", ""))
+        per_dof = {}
+        for i, dof in enumerate(sp.get("dof", [])):
+            if kind == "constant":
+                per_dof[int(dof)] = np.asarray([float(sp["value"][i])])
+            elif kind == "timeseries":
+                per_dof[int(dof)] = _series_file(sp["file"][i], base)
+            else:
+                raise ValueError(f"support motion at node {t}: type {sp.get('type')} is not CONSTANT or TIMESERIES")
+        for dof, g in sorted(per_dof.items()):  # Node::SupportMotion is a map ordered by DOF
+            if dof < 0 or dof >= model.ndim:
+                raise ValueError(f"support motion at node {t}: DOF {dof} out of range")
+            if not model.restrained[index[t], dof]:
+                raise ValueError(f"support motion at node {t}, DOF {dof}: a free DOF is not on the device path")
+            model.add_support_motion(index[t], dof, g, factor)
+
+
 def read_run(path: str) -> SvlRun:
     """Reads an SVL run file into a device-path model (see the module docstring)."""
     base = os.path.dirname(os.path.abspath(path))
@@ -133,7 +164,7 @@ def read_run(path: str) -> SvlRun:
     g = js.get("Global", {})
     ndim = int(g.get("ndim", 0))
     lumped = _lc(g.get("massform", "CONSISTENT")) == "lumped"
-    for key in ("Supports", "Constraints", "Masses", "Sections"):
+    for key in ("Constraints", "Masses", "Sections"):
         if js.get(key):
             raise ValueError(f"run file has {key}: not on the device path")
     mats = {int(t): v for t, v in js.get("Materials", {}).items()}
@@ -250,9 +281,13 @@ def read_run(path: str) -> SvlRun:
     ca = combo.get("attributes", {})
     loads = js.get("Loads", {})
     series, rows = [], []
+    supports = {int(t): v for t, v in js.get("Supports", {}).items()}
     for lt, fac in zip(ca.get("load", []), ca.get("factor", [])):
         ld = loads[str(lt)]
         at = ld["attributes"]
+        if _lc(ld["name"]) == "supportmotion":
+            _read_support_motion(model, at, supports, index, base, float(fac))
+            continue
         if _lc(ld["name"]) == "elementload" and _lc(at.get("type")) == "generalwave":
             if model.drm_elem is not None:
                 raise ValueError(f"load {lt}: one domain reduction load per combination is on the device path")
@@ -285,6 +320,11 @@ def read_run(path: str) -> SvlRun:
         model.load_series = np.zeros((len(rows), nts))
         for i, s in enumerate(series):
             model.load_series[i, :len(s)] = s
+    if len(model.sm_node):
+        if run.algorithm != "LINEAR":
+            raise ValueError("support motion with the Newton-Raphson algorithm is not on the device path")
+        # NewmarkBeta applies the increment of sample k - 1 at step k (NewmarkBeta.cpp:187).
+        model.sm_offset = 1 if integ == "NEWMARK" else 0
     # NODE recorders (others are listed as skipped).
     for _, r in sorted(js.get("Recorders", {}).items(), key=lambda x: int(x[0])):
         if _lc(r["name"]) == "node" and _lc(r.get("resp", "")) in ("disp", "vel", "accel", "reaction"):
@@ -394,6 +434,40 @@ def write_run(model: Model, path: str, integrator: str, dt: float, nt: int, algo
             "list": [int(e) + 1 for e in model.drm_elem]}}
         lids.append(lid)
         facs.append(float(model.drm_factor))
+    if len(model.sm_node):
+        # Supports (one TIMESERIES file per node DOF) and one SupportMotion load per distinct
+        # factor listing its nodes; a load moves every DOF of its nodes, so each node's DOFs must
+        # carry the same factors and each (node, DOF) one series.
+        per_node = {}
+        for j in range(len(model.sm_node)):
+            n, c, f = int(model.sm_node[j]), int(model.sm_dir[j]), float(model.sm_factor[j])
+            dofs = per_node.setdefault(n, {})
+            g = np.asarray(model.sm_series[j], np.float64)
+            if c in dofs and not np.array_equal(dofs[c][0], g):
+                raise ValueError(f"node {n}, DOF {c}: SVL holds one support series per DOF")
+            dofs.setdefault(c, (g, []))[1].append(f)
+        sup, by_factor = {}, {}
+        for n, dofs in sorted(per_node.items()):
+            fsets = {tuple(v[1]) for v in dofs.values()}
+            if len(fsets) != 1:
+                raise ValueError(f"node {n}: its DOFs carry different support-motion factors")
+            files = []
+            for c, (g, _) in sorted(dofs.items()):
+                fname = os.path.join(base, f"{stem}_support{n + 1}_{c}.txt")
+                with open(fname, "w") as f:
+                    f.write(f"{g.size}\n")
+                    f.write("\n".join(repr(float(v)) for v in g))
+                    f.write("\n")
+                files.append(fname)
+            sup[str(n + 1)] = {"type": "TIMESERIES", "file": files, "dof": sorted(dofs)}
+            for f in fsets.pop():
+                by_factor.setdefault(f, []).append(n + 1)
+        out["Supports"] = sup
+        for f, lst in by_factor.items():
+            lid = len(loads) + 1
+            loads[str(lid)] = {"name": "SupportMotion", "attributes": {"list": lst}}
+            lids.append(lid)
+            facs.append(f)
     out["Loads"] = loads
     out["Combinations"] = {"1": {"name": "Combo", "attributes": {"load": lids, "factor": facs}}}
     out["Simulations"] = {"combo": 1, "attributes": {
@@ -470,12 +544,16 @@ def run(svl: SvlRun, out_dir: str | None = None, json_path: str | None = None, c
         raise ValueError(f"recorder nodes {unknown[:5]} are not in the model")
     needed = sorted(needed)
     rec_dofs, where = [], {}
+    moving = {(int(n), int(c)) for n, c in zip(m.sm_node, m.sm_dir)}
     for t in needed:
         for c in range(nd):
             f = int(fd[tags.index(t)][c])
             if f >= 0:
                 where[(t, c)] = len(rec_dofs)
                 rec_dofs.append(f)
+            elif (tags.index(t), c) in moving:  # a restrained DOF with support motion: total DOF code
+                where[(t, c)] = len(rec_dofs)
+                rec_dofs.append(-1 - (tags.index(t) * nd + c))
     # Reactions are set only at fixed nodes (DynamicAnalysis.cpp:140-150); elsewhere 0.
     fixed_mesh = sorted({t for r in svl.recorders if r.response == "REACTION" for t in r.tags
                          if t in mesh and bool(m.restrained[tags.index(t)].any())})
@@ -519,8 +597,11 @@ def run(svl: SvlRun, out_dir: str | None = None, json_path: str | None = None, c
     else:
         V, A = np.zeros_like(U), np.zeros_like(U)
         v, a = np.zeros(U.shape[1]), np.zeros(U.shape[1])
+        # V and A follow the free increment alone (ExtendedNewmarkBeta.cpp:134-150): 0 at the
+        # support-motion DOFs.
+        free_col = np.asarray([r >= 0 for r in rec_dofs], dtype=bool)
         for k in range(steps):
-            du = U[k] - Up[k]
+            du = np.where(free_col, U[k] - Up[k], 0.0)
             a = 4.0 / dt / dt * du - 4.0 / dt * v - a
             v = 2.0 / dt * du - v
             V[k], A[k] = v, a

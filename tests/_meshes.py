@@ -80,3 +80,50 @@ def drm_hexa_model(nt, dt, n=(8, 8, 6)):
     m.add_point_load(src, [0.3, 0.5, -1.0], ricker_velocity(np.arange(nt) * dt, 0.02, 40.0, 1e6))
     c = np.array(n, float) / 2
     return m, ((c[0], c[1] + 0.5, c[2]), (n[0] / 5.3, n[1] / 5.3, n[2] / 4))
+
+
+def base_motion_series(nt, dt, length=None, g0=5e-4, amp=2e-3, t1=0.04):
+    """A base displacement history that starts at g0 (nonzero, so the sample-0 rule matters),
+    rises by amp with a (1 - cos) pulse over t1 and is cut at `length` samples (past the end
+    SVL reads the first value again)."""
+    n = nt if length is None else length
+    t = np.arange(n) * dt
+    return g0 + amp * np.where(t < t1, 0.5 * (1.0 - np.cos(2.0 * np.pi * t / t1)), 0.0) + 1e-3 * np.minimum(t / t1, 1.0)
+
+
+def support_model(ndim, nt, dt, am=0.0, ak=0.0, lumped=True, dashpots=False, load=True, length=None, factor=1.0,
+                  dirs=(0,)):
+    """Quad4 (10 x 6) or Hexa8 (5 x 4 x 4) block on a base whose nodes are restrained in every
+    direction and move in `dirs` with base_motion_series (one SupportMotion load with the given
+    combination factor); optional Rayleigh damping, consistent mass, Lysmer dashpots on the sides
+    and a point load on top."""
+    if ndim == 2:
+        coords, conn, bnd = make_domain_area(10, 6, (0, 0), (10, 0), (0, 6))
+        etype, side_names = ELEM_QUAD4, ("left", "right")
+    else:
+        coords, conn, bnd = make_domain_volume(5, 4, 4, (0, 0, 0), (5, 0, 0), (0, 4, 0), (0, 0, 4))
+        etype, side_names = ELEM_HEXA8, ("left", "right", "front", "back")
+    restrained = np.zeros(coords.shape, bool)
+    restrained[bnd["bottom"]] = True
+    rng = np.random.default_rng(11)
+    m = Model(ndim=ndim, coords=coords.astype(np.float64), node_tags=tags_for(coords.shape[0]), elem_type=etype,
+              conn=conn, E=rng.uniform(6e7, 1.4e8, conn.shape[0]), nu=0.3, rho=2000.0, restrained=restrained,
+              massform_lumped=lumped)
+    m.ray_am = np.full(conn.shape[0], am)
+    m.ray_ak = np.full(conn.shape[0], ak)
+    if dashpots:
+        side = np.unique(np.concatenate([bnd[k] for k in side_names]))
+        side = side[~np.isin(side, bnd["bottom"])]
+        m.dash_node = np.repeat(side, ndim).astype(np.int32)
+        m.dash_dir = np.tile(np.arange(ndim), side.size).astype(np.int32)
+        m.dash_eta = np.full(side.size * ndim, 2.0e5)
+    if load:
+        top = bnd["top"]
+        src = int(top[len(top) // 2])
+        m.add_point_load(src, [0.4, -1.0] if ndim == 2 else [0.4, 0.2, -1.0],
+                         ricker_velocity(np.arange(nt) * dt, 0.02, 30.0, 2e5))
+    g = base_motion_series(nt, dt, length)
+    for n in bnd["bottom"]:
+        for d in dirs:
+            m.add_support_motion(int(n), int(d), g, factor)
+    return m, g, bnd

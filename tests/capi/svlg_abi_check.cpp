@@ -11,6 +11,9 @@
 //      ComputeEffectiveStiffness, the linear solve (CentralDifference: the diagonal division
 //      of the LDL^T solve; ExtendedNewmarkBeta: svlg_solve), then ComputeNewStep's update;
 // and requires B and C to match A to 1e-10 relative L2 (CG at 1e-13 on the implicit path).
+// Each model runs twice: as described, and with a vertical support motion of the base nodes
+// (ComputeSupportMotionVector between the stiffness and the solve; the recorded DOFs then
+// include base nodes, read through svlg_get_displacements).
 // Exit status 0 = every check passed.
 #include <cmath>
 #include <cstdint>
@@ -40,12 +43,16 @@ struct Layer {
     int nx, ny, nn, ne, nfree = 0;
     std::vector<double> coords, E, nu, rho, am, ak, dash_eta, load_dir, load_factor, load_series;
     std::vector<int32_t> free_dof, conn, dash_node, dash_dir, load_node;
+    std::vector<int32_t> sm_node, sm_dir, sm_ptr;
+    std::vector<double> sm_factor, sm_values;
     svlg_mesh_desc d;
 };
 
 // nx x ny Quad4 of 1 m, rho 2000, Vs 200, nu 0.25; base nodes restrained vertically; Lysmer
 // dashpots (eta = rho V h) on the sides (both directions) and the base (horizontal).
-void make_layer(Layer &L, int nx, int ny, int nt, double dt, bool rayleigh) {
+// support: every base node moves vertically by 2 mm (1 - cos) over the first 0.1 s, the
+// left half with factor 1, the right half with factor 0.5.
+void make_layer(Layer &L, int nx, int ny, int nt, double dt, bool rayleigh, bool support) {
     L.nx = nx;
     L.ny = ny;
     L.nn = (nx + 1) * (ny + 1);
@@ -120,6 +127,53 @@ void make_layer(Layer &L, int nx, int ny, int nt, double dt, bool rayleigh) {
     L.d.load_dir = L.load_dir.data();
     L.d.load_factor = L.load_factor.data();
     L.d.load_series = L.load_series.data();
+    if (support) {
+        std::vector<double> g;
+        for (int k = 0; k < nt; ++k) {
+            const double t = k * dt;
+            g.push_back(t < 0.1 ? 1e-3 * (1.0 - std::cos(2.0 * M_PI * t / 0.1)) : 0.0);
+        }
+        L.sm_ptr.push_back(0);
+        for (int i = 0; i <= nx; ++i) {
+            L.sm_node.push_back(i);
+            L.sm_dir.push_back(1);
+            L.sm_factor.push_back(i <= nx / 2 ? 1.0 : 0.5);
+            L.sm_values.insert(L.sm_values.end(), g.begin(), g.end());
+            L.sm_ptr.push_back((int32_t)L.sm_values.size());
+        }
+        L.d.nsm = (int32_t)L.sm_node.size();
+        L.d.sm_node = L.sm_node.data();
+        L.d.sm_dir = L.sm_dir.data();
+        L.d.sm_factor = L.sm_factor.data();
+        L.d.sm_ptr = L.sm_ptr.data();
+        L.d.sm_values = L.sm_values.data();
+        L.d.sm_offset = 0;
+    }
+}
+
+// Recorded DOFs: every 7th free DOF, and with support motion the vertical DOF of base nodes
+// 0, 3, 6, ... (oracle: -1 - total DOF).
+void recorded(const Layer &L, int stride, std::vector<int32_t> &rec_free, std::vector<int32_t> &rec_base,
+              std::vector<int32_t> &rec_oracle) {
+    for (int f = stride / 2; f < L.nfree; f += stride) rec_free.push_back(f);
+    if (L.d.nsm > 0)
+        for (int i = 0; i <= L.nx; i += 3) rec_base.push_back(i);
+    rec_oracle = rec_free;
+    for (int n : rec_base) rec_oracle.push_back(-1 - (n * 2 + 1));
+}
+
+// The recorded values of the current state: free DOFs from get_state, base DOFs from the
+// total displacement vector.
+int gather(svlg_model *m, const Layer &L, const std::vector<int32_t> &rec_free, const std::vector<int32_t> &rec_base,
+           bool nm, double *row) {
+    std::vector<double> U(L.nfree), Ut((size_t)L.nn * 2);
+    if ((nm ? svlg_nm_get_state(m, U.data(), nullptr, nullptr) : svlg_cd_get_state(m, U.data(), nullptr, nullptr)) != 0)
+        return 2;
+    if (svlg_get_displacements(m, Ut.data()) != 0) return 2;
+    size_t r = 0;
+    for (int f : rec_free) row[r++] = U[f];
+    for (int n : rec_base) row[r++] = Ut[(size_t)n * 2 + 1];
+    return 0;
 }
 
 double rel_l2(const std::vector<double> &a, const std::vector<double> &b) {
@@ -146,100 +200,103 @@ void expect(bool ok, const char *what, double err) {
         }                                                                                \
     } while (0)
 
-int run_central_difference() {
+int run_central_difference(bool support) {
     const int nt = 301;
     Layer L;
     const double vp = 200.0 * std::sqrt(3.0);
     const double dt = 0.5 / vp;
-    make_layer(L, 24, 12, nt, dt, false);
-    std::vector<int32_t> rec;
-    for (int f = 0; f < L.nfree; f += 7) rec.push_back(f);
-    const int nrec = (int)rec.size(), ns = nt - 1;
-    std::vector<double> ref((size_t)ns * nrec);
+    make_layer(L, 24, 12, nt, dt, false, support);
+    std::vector<int32_t> rec, base, rec_or;
+    recorded(L, 7, rec, base, rec_or);
+    const int nrec = (int)rec.size(), nall = (int)rec_or.size(), ns = nt - 1;
+    std::vector<double> ref((size_t)ns * nall), ref_free((size_t)ns * nrec);
     oracle_opts o{0, nt, dt, 1e-12, 1e-12, 1e-12, 0, 20, 5, 0, 1e-6};
-    if (oracle_run(&L.d, &o, nrec, rec.data(), ref.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+    if (oracle_run(&L.d, &o, nall, rec_or.data(), ref.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
                    nullptr, nullptr) != 0) {
         std::printf("oracle_run failed: %s\n", oracle_last_error());
         return 2;
     }
+    for (int s = 0; s < ns; ++s)
+        for (int r = 0; r < nrec; ++r) ref_free[(size_t)s * nrec + r] = ref[(size_t)s * nall + r];
     svlg_model *m = nullptr;
     CHECK_RC(svlg_model_create(&L.d, &m));
     // B: fused steps.
     std::vector<double> hb((size_t)ns * nrec);
     CHECK_RC(svlg_cd_init(m, dt, 1e-12));
     CHECK_RC(svlg_cd_steps(m, 1, ns, nrec, rec.data(), hb.data()));
-    expect(rel_l2(hb, ref) < 1e-10, "CentralDifference svlg_cd_steps vs oracle", rel_l2(hb, ref));
+    expect(rel_l2(hb, ref_free) < 1e-10,
+           support ? "CentralDifference svlg_cd_steps vs oracle (support motion)"
+                   : "CentralDifference svlg_cd_steps vs oracle",
+           rel_l2(hb, ref_free));
     // C: the Integrator methods one by one.
-    std::vector<double> hc((size_t)ns * nrec), feff(L.nfree), kd(L.nfree), du(L.nfree), U(L.nfree);
+    std::vector<double> hc((size_t)ns * nall), feff(L.nfree), kd(L.nfree), du(L.nfree);
     CHECK_RC(svlg_cd_init(m, dt, 1e-12));
     CHECK_RC(svlg_cd_effective_stiffness(m, kd.data()));
     for (int k = 1; k < nt; ++k) {
         CHECK_RC(svlg_cd_effective_force(m, k, 1.0, feff.data()));
+        CHECK_RC(svlg_support_motion_vector(m, k, 1.0, feff.data()));
         for (int f = 0; f < L.nfree; ++f) du[f] = feff[f] / kd[f];
-        CHECK_RC(svlg_cd_apply_increment(m, du.data()));
-        CHECK_RC(svlg_cd_get_state(m, U.data(), nullptr, nullptr));
-        for (int r = 0; r < nrec; ++r) hc[(size_t)(k - 1) * nrec + r] = U[rec[r]];
+        CHECK_RC(svlg_cd_apply_increment(m, k, du.data()));
+        CHECK_RC(gather(m, L, rec, base, false, &hc[(size_t)(k - 1) * nall]));
     }
-    expect(rel_l2(hc, ref) < 1e-10, "CentralDifference Integrator methods vs oracle", rel_l2(hc, ref));
-    // Progressive force: the point load of step k at the loaded node.
-    std::vector<double> F((size_t)L.nn * 2);
-    CHECK_RC(svlg_external_force(m, 20, F.data()));
-    const double want = -L.load_series[20];
-    const double got = F[(size_t)L.load_node[0] * 2 + 1];
-    expect(got == want, "svlg_external_force at the loaded node (exact)", std::fabs(got - want));
+    expect(rel_l2(hc, ref) < 1e-10,
+           support ? "CentralDifference Integrator methods vs oracle (support)"
+                   : "CentralDifference Integrator methods vs oracle",
+           rel_l2(hc, ref));
+    if (!support) {
+        // Progressive force: the point load of step k at the loaded node.
+        std::vector<double> F((size_t)L.nn * 2);
+        CHECK_RC(svlg_external_force(m, 20, F.data()));
+        const double want = -L.load_series[20];
+        const double got = F[(size_t)L.load_node[0] * 2 + 1];
+        expect(got == want, "svlg_external_force at the loaded node (exact)", std::fabs(got - want));
+    }
     CHECK_RC(svlg_model_destroy(m));
     return 0;
 }
 
-int run_newmark() {
+int run_newmark(bool support) {
     const int nt = 121;
     const double dt = 2e-3;
     Layer L;
-    make_layer(L, 20, 10, nt, dt, true);
-    std::vector<int32_t> rec;
-    for (int f = 1; f < L.nfree; f += 5) rec.push_back(f);
-    const int nrec = (int)rec.size(), ns = nt - 1;
-    std::vector<double> ref((size_t)ns * nrec);
+    make_layer(L, 20, 10, nt, dt, true, support);
+    std::vector<int32_t> rec, base, rec_or;
+    recorded(L, 5, rec, base, rec_or);
+    const int nrec = (int)rec.size(), nall = (int)rec_or.size(), ns = nt - 1;
+    std::vector<double> ref((size_t)ns * nall), ref_free((size_t)ns * nrec);
     oracle_opts o{1, nt, dt, 1e-12, 1e-12, 1e-12, 0, 20, 5, 0, 1e-6};
-    if (oracle_run(&L.d, &o, nrec, rec.data(), ref.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
-                   nullptr, nullptr) != 0) {
-        std::printf("oracle_run failed: %s\n", oracle_last_error());
-        return 2;
-    }
     for (int lumped : {1, 0}) {
         L.d.lumped = lumped;
-        if (!lumped) {
-            o.integrator = 1;
-            if (oracle_run(&L.d, &o, nrec, rec.data(), ref.data(), nullptr, nullptr, nullptr, nullptr, nullptr,
-                           nullptr, 0, nullptr, nullptr) != 0) {
-                std::printf("oracle_run failed: %s\n", oracle_last_error());
-                return 2;
-            }
+        if (oracle_run(&L.d, &o, nall, rec_or.data(), ref.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       0, nullptr, nullptr) != 0) {
+            std::printf("oracle_run failed: %s\n", oracle_last_error());
+            return 2;
         }
+        for (int s = 0; s < ns; ++s)
+            for (int r = 0; r < nrec; ++r) ref_free[(size_t)s * nrec + r] = ref[(size_t)s * nall + r];
+        char what[96];
         svlg_model *m = nullptr;
         CHECK_RC(svlg_model_create(&L.d, &m));
         std::vector<double> hb((size_t)ns * nrec);
         CHECK_RC(svlg_nm_init(m, dt, 1e-13, 5000, SVLG_PRECOND_JACOBI));
         CHECK_RC(svlg_nm_steps(m, 1, ns, nrec, rec.data(), hb.data(), nullptr, nullptr));
-        expect(rel_l2(hb, ref) < 1e-10,
-               lumped ? "ExtendedNewmarkBeta svlg_nm_steps vs oracle (lumped)"
-                      : "ExtendedNewmarkBeta svlg_nm_steps vs oracle (consistent)",
-               rel_l2(hb, ref));
-        std::vector<double> hc((size_t)ns * nrec), feff(L.nfree), du(L.nfree), U(L.nfree);
+        std::snprintf(what, sizeof(what), "ExtendedNewmarkBeta svlg_nm_steps vs oracle (%s%s)",
+                      lumped ? "lumped" : "consistent", support ? ", support" : "");
+        expect(rel_l2(hb, ref_free) < 1e-10, what, rel_l2(hb, ref_free));
+        std::vector<double> hc((size_t)ns * nall), feff(L.nfree), du(L.nfree);
         CHECK_RC(svlg_nm_init(m, dt, 1e-13, 5000, SVLG_PRECOND_JACOBI));
         for (int k = 1; k < nt; ++k) {
             int32_t it = 0;
             double rr = 0.0;
             CHECK_RC(svlg_nm_effective_force(m, k, 1.0, feff.data()));
+            CHECK_RC(svlg_support_motion_vector(m, k, 1.0, feff.data()));
             CHECK_RC(svlg_solve(m, feff.data(), du.data(), 1e-13, 5000, &it, &rr));
-            CHECK_RC(svlg_nm_apply_increment(m, du.data()));
-            CHECK_RC(svlg_nm_get_state(m, U.data(), nullptr, nullptr));
-            for (int r = 0; r < nrec; ++r) hc[(size_t)(k - 1) * nrec + r] = U[rec[r]];
+            CHECK_RC(svlg_nm_apply_increment(m, k, du.data()));
+            CHECK_RC(gather(m, L, rec, base, true, &hc[(size_t)(k - 1) * nall]));
         }
-        expect(rel_l2(hc, ref) < 1e-10,
-               lumped ? "ExtendedNewmarkBeta Integrator methods vs oracle (lumped)"
-                      : "ExtendedNewmarkBeta Integrator methods vs oracle (consistent)",
-               rel_l2(hc, ref));
+        std::snprintf(what, sizeof(what), "ExtendedNewmarkBeta Integrator methods vs oracle (%s%s)",
+                      lumped ? "lumped" : "consistent", support ? ", support" : "");
+        expect(rel_l2(hc, ref) < 1e-10, what, rel_l2(hc, ref));
         CHECK_RC(svlg_model_destroy(m));
     }
     return 0;
@@ -253,8 +310,10 @@ int main() {
         std::printf("no HIP device\n");
         return 3;
     }
-    if (int rc = run_central_difference()) return rc;
-    if (int rc = run_newmark()) return rc;
+    for (bool support : {false, true}) {
+        if (int rc = run_central_difference(support)) return rc;
+        if (int rc = run_newmark(support)) return rc;
+    }
     std::printf("%s\n", failures ? "FAILED" : "all checks passed");
     return failures ? 1 : 0;
 }

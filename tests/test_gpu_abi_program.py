@@ -2,8 +2,9 @@
 include/svlg.h with __graft_entry__.build()) runs a Quad4 layer through the fused device steps
 and through the Integrator methods one by one (ComputeEffectiveForce, ComputeEffectiveStiffness,
 the solve, ComputeNewStep's update) for CentralDifference and ExtendedNewmarkBeta (lumped and
-consistent mass), each against the oracle to 1e-10 relative L2; it exits 0 when every check
-passes."""
+consistent mass), with and without a vertical support motion of the base nodes
+(ComputeSupportMotionVector, the base DOFs read through svlg_get_displacements), each against
+the oracle to 1e-10 relative L2; it exits 0 when every check passes."""
 import subprocess
 from pathlib import Path
 

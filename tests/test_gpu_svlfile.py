@@ -123,3 +123,40 @@ def test_drm_run_file_through_device(tmp_path):
     free = [i for i, f in enumerate(cols) if f >= 0]
     assert np.abs(ref["U"]).max() > 0
     assert rel(vals[:, free], ref["U"]) < 1e-10
+
+
+@pytest.mark.parametrize("integ", ["CENTRALDIFFERENCE", "EXTENDEDNEWMARK", "NEWMARK"])
+def test_support_motion_run_file_through_device(tmp_path, integ):
+    """A run file with Supports (TIMESERIES per DOF) and two SupportMotion loads: the DISP
+    recorder of a base node holds its prescribed motion and a free node matches the oracle; the
+    VEL recorder follows the integrator (Newmark: 0 at the support DOFs)."""
+    from _meshes import support_model
+    dt, nt = (2e-4, 240) if integ == "CENTRALDIFFERENCE" else (1e-3, 80)
+    m, g, bnd = support_model(3, nt, dt, am=0.4, dashpots=True, length=nt // 2, factor=0.8, dirs=(0, 2))
+    for n in bnd["bottom"][:4]:
+        m.add_support_motion(int(n), 0, g, 0.3)
+        m.add_support_motion(int(n), 2, g, 0.3)
+    b0 = int(bnd["bottom"][1])
+    top = int(bnd["top"][3])
+    p = str(tmp_path / "sm.json")
+    tags = S.write_run(m, p, integ, dt, nt, recorders=[("U.out", "DISP", [b0 + 1, top + 1], 10),
+                                                       ("V.out", "VEL", [b0 + 1], 10)])
+    r = S.read_run(p)
+    out = S.run(r, out_dir=str(tmp_path), json_path=p, cg_rtol=1e-13)
+    U = out[str(tmp_path / "U.out")]
+    Vb = out[str(tmp_path / "V.out")]
+    fd = m.free_dof()
+    rec = [-1 - (b0 * 3 + c) for c in range(3)] + [int(fd[top, c]) for c in range(3)]
+    io = {"CENTRALDIFFERENCE": oracle.CENTRAL_DIFFERENCE, "EXTENDEDNEWMARK": oracle.EXTENDED_NEWMARK,
+          "NEWMARK": oracle.NEWMARK}[integ]
+    ref = oracle.run(m, io, dt, nt, rec, want_va=True)
+    dev = np.concatenate([U[:, 0, :], U[:, 1, :]], axis=1)
+    err = rel(dev, ref["U"])
+    print(f"{integ} run file with support motion: base + free node DISP vs oracle {err:.2e}, "
+          f"base x at the end {U[-1, 0, 0]:.4e}")
+    assert err < 1e-10
+    assert np.array_equal(U[:, 0, :], ref["U"][:, :3])
+    if integ == "CENTRALDIFFERENCE":
+        assert rel(Vb[:, 0, :], ref["V"][:, :3]) < 1e-10
+    else:
+        assert np.all(Vb[:, 0, :] == 0.0)

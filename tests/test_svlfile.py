@@ -115,7 +115,15 @@ def test_handwritten_run_file(tmp_path):
 
 
 @pytest.mark.parametrize("edit", [
-    lambda js: js.update({"Supports": {"1": {"type": "Constant", "dof": [1], "value": [0.1]}}}),
+    # Support motion on a free DOF (node 2, x), and with the Newton algorithm.
+    lambda js: (js.update({"Supports": {"2": {"type": "Constant", "dof": [0], "value": [0.1]}}}),
+                js["Loads"].update({"3": {"name": "SupportMotion", "attributes": {"list": [2]}}}),
+                js["Combinations"]["1"]["attributes"].update({"load": [1, 2, 3], "factor": [2.0, 0.5, 1.0]})),
+    lambda js: (js.update({"Supports": {"1": {"type": "Constant", "dof": [1], "value": [0.1]}}}),
+                js["Loads"].update({"3": {"name": "SupportMotion", "attributes": {"list": [1]}}}),
+                js["Combinations"]["1"]["attributes"].update({"load": [1, 2, 3], "factor": [2.0, 0.5, 1.0]}),
+                js["Simulations"]["attributes"].update({"algorithm": {"name": "NEWTON"},
+                                                        "integrator": {"name": "NEWMARK", "dt": 1e-3}})),
     lambda js: js["Elements"].update({"3": {"name": "lin2DQuad8", "conn": [1, 2, 3, 4], "attributes": {}}}),
     lambda js: js["Materials"]["1"].update({"name": "PlasticPlaneStrainBA"}),
     lambda js: js["Loads"]["1"]["attributes"].update({"type": "BODY"}),
@@ -124,6 +132,52 @@ def test_handwritten_run_file(tmp_path):
 def test_refuses_what_is_not_on_the_path(tmp_path, edit):
     with pytest.raises(ValueError):
         S.read_run(handwritten(tmp_path, edit))
+
+
+def test_handwritten_support_motion(tmp_path):
+    """Supports as SVL's pre-processor writes them (dof 0-based; CONSTANT value / TIMESERIES file
+    with the sample count first) activated by a SupportMotion load of the combination."""
+    ser = tmp_path / "base.txt"
+    ser.write_text("4\n0.1 0.2 0.3 0.4\n")
+
+    def edit(js, integ="CentralDifference"):
+        js["Supports"] = {"1": {"type": "TIMESERIES", "file": [str(ser), str(ser)], "dof": [1, 0]},
+                          "2": {"type": "CONSTANT", "value": [0.05], "dof": [1]}}
+        js["Loads"]["3"] = {"name": "SupportMotion", "attributes": {"list": [1, 2]}}
+        js["Combinations"]["1"]["attributes"].update({"load": [1, 2, 3], "factor": [2.0, 0.5, 1.5]})
+        js["Simulations"]["attributes"]["integrator"]["name"] = integ
+
+    m = S.read_run(handwritten(tmp_path, edit)).model
+    assert list(m.sm_node) == [0, 0, 1] and list(m.sm_dir) == [0, 1, 1]
+    assert list(m.sm_factor) == [1.5] * 3 and m.sm_offset == 0
+    assert np.array_equal(m.sm_series[0], [0.1, 0.2, 0.3, 0.4]) and np.array_equal(m.sm_series[2], [0.05])
+    m2 = S.read_run(handwritten(tmp_path, lambda js: edit(js, "Newmark"))).model
+    assert m2.sm_offset == 1
+    m3 = S.read_run(handwritten(tmp_path, lambda js: edit(js, "ExtendedNewmark"))).model
+    assert m3.sm_offset == 0
+
+
+@pytest.mark.parametrize("integ", ["CENTRALDIFFERENCE", "EXTENDEDNEWMARK", "NEWMARK"])
+def test_support_motion_roundtrip(tmp_path, integ):
+    from _meshes import support_model
+    dt = 2e-4 if integ == "CENTRALDIFFERENCE" else 1e-3
+    m, g, bnd = support_model(2, 40, dt, am=0.3, dashpots=True, length=25, factor=0.9, dirs=(0, 1))
+    for n in bnd["bottom"][:3]:
+        m.add_support_motion(int(n), 0, g, 0.4)
+        m.add_support_motion(int(n), 1, g, 0.4)
+    p = str(tmp_path / "sm.json")
+    S.write_run(m, p, integ, dt, 40)
+    q = S.read_run(p).model
+    key = lambda mm: sorted(zip(mm.sm_node.tolist(), mm.sm_dir.tolist(), mm.sm_factor.tolist(),  # noqa: E731
+                                [tuple(x) for x in mm.sm_series]))
+    assert key(q) == key(m)
+    assert q.sm_offset == (1 if integ == "NEWMARK" else 0)
+    io = {"CENTRALDIFFERENCE": oracle.CENTRAL_DIFFERENCE, "EXTENDEDNEWMARK": oracle.EXTENDED_NEWMARK,
+          "NEWMARK": oracle.NEWMARK}[integ]
+    rec = list(range(0, m.nfree, 3)) + [-1 - int(bnd["bottom"][1]) * 2]
+    a = oracle.run(m, io, dt, 30, rec)["U"]
+    b = oracle.run(q, io, dt, 30, rec)["U"]
+    assert np.array_equal(a, b) and np.abs(a[:, -1]).max() > 0
 
 
 def test_node_recorder_format(tmp_path):
