@@ -341,20 +341,23 @@ __global__ __launch_bounds__(512) void k_cd_fused(FusedArgs a) {
 // to at most one element (conforming structured meshes): one thread per tile element
 // and one thread per owned node, all tile elements in a single round. Each element
 // writes its corner forces to the unique LDS slot (corner a, component c, node); each
-// owned node then adds its NEN slots in corner order. Two barriers per tile, no color
-// phases, and every global load that does not depend on staged data is issued before
-// the first barrier.
+// owned node then adds its NEN slots in corner order. No color phases; the corners go
+// through the slot planes in NP passes of NEN / NP corners (same summation order as one
+// pass, 1/NP of the slot LDS, one barrier per pass).
 // MOC > 0: every tile owns at most MOC nodes; the slot planes then have the compile-time
 // stride MOC + 1 with column MOC as the trash column, so that every slot address is one
 // per-corner base plus an immediate offset.
+// LATE: the update operands of a node (U_{k-2}, 1/keff) are loaded after the element
+// phase instead of before the first barrier: 14 fewer registers live through the element
+// step, more resident waves (on the Hexa8 table path, 86 -> 72 VGPRs).
 // The update multiplies by the stored 1/keff (kinv) instead of dividing by keff: the
 // diagonal solve as Eigen's LDL^T applies it (the inverse of D times the vector).
-template <int DIM, int PATH, int MOC = 0>
-__global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
+template <int DIM, int PATH, int MOC, bool LATE, int NP>
+__device__ __forceinline__ void cd_corner_body(const FusedArgs &a) {
     constexpr int NEN = Elem<DIM>::nen;
     constexpr int NB = 4;
     // Corner-slot planes per pass: the MFMA element step writes all corners at once.
-    constexpr int NPASS = PATH == 4 ? 1 : 2, NH = NEN / NPASS;
+    constexpr int NPASS = PATH == 4 ? 1 : NP, NH = NEN / NPASS;
     extern __shared__ double smem[];
     const int nt = a.ntiles;
     const int kstep = a.kdev ? *a.kdev + a.k : a.k;
@@ -375,11 +378,12 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
     // Prefetch: update operands of this thread's node, element data of this thread's element.
     const bool has_node = tid < nown;
     double up[DIM], kd[DIM];
+    if constexpr (!LATE)
 #pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-        up[c] = has_node ? a.Uprev[g0 + size_t(tid) * DIM + c] : 0.0;
-        kd[c] = has_node ? a.kinv[g0 + size_t(tid) * DIM + c] : 0.0;
-    }
+        for (int c = 0; c < DIM; ++c) {
+            up[c] = has_node ? a.Uprev[g0 + size_t(tid) * DIM + c] : 0.0;
+            kd[c] = has_node ? a.kinv[g0 + size_t(tid) * DIM + c] : 0.0;
+        }
     const bool has_elem = PATH != 4 && tid < ne;
     int lc[NEN];
     double lam = 0.0, mu = 0.0;
@@ -570,6 +574,12 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
         }
     }
     if (!has_node) return;
+    if constexpr (LATE)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) {
+            up[c] = a.Uprev[g0 + size_t(tid) * DIM + c];
+            kd[c] = a.kinv[g0 + size_t(tid) * DIM + c];
+        }
     const int npl = a.nplain[tile];
     const int ds = tid >= npl ? drm_slot(a.drm, o0 + tid) : -1;
 #pragma unroll
@@ -596,10 +606,23 @@ __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
     }
 }
 
+template <int DIM, int PATH, int MOC = 0, bool LATE = false, int NP = 2>
+__global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
+    cd_corner_body<DIM, PATH, MOC, LATE, NP>(a);
+}
+
 template <int DIM, int PATH>
 static const void *fused_fn(bool corner) {
     return corner ? reinterpret_cast<const void *>(&k_cd_corner<DIM, PATH>)
                   : reinterpret_cast<const void *>(&k_cd_fused<DIM, PATH>);
+}
+
+// Corner passes of the corner-slot step (the slot planes hold NEN / passes corners): four on
+// the compile-time stride Hexa8 paths, two elsewhere (one for the MFMA element step).
+static int use_moc(const Model &m);
+static int cd_passes(const Model &m) {
+    if (m.elem_bf16) return 1;
+    return m.ndim == 3 && use_moc(m) ? 4 : 2;
 }
 
 // Owned-node bound of the compile-time slot stride of the corner-slot step on axis-aligned
@@ -614,9 +637,9 @@ static int use_moc(const Model &m) {
 }
 
 static const void *moc_kernel(const Model &m) {
-    if (m.elem_fp32) return reinterpret_cast<const void *>(&k_cd_corner<3, 5, 160>);
+    if (m.elem_fp32) return reinterpret_cast<const void *>(&k_cd_corner<3, 5, 160, true, 4>);
     switch (m.ndim * 1000 + use_moc(m)) {
-        case 3160: return reinterpret_cast<const void *>(&k_cd_corner<3, 3, 160>);
+        case 3160: return reinterpret_cast<const void *>(&k_cd_corner<3, 3, 160, true, 4>);
         case 2192: return reinterpret_cast<const void *>(&k_cd_corner<2, 3, 192>);
         case 2448: return reinterpret_cast<const void *>(&k_cd_corner<2, 3, 448>);
         default: return nullptr;
@@ -649,7 +672,7 @@ int prepare_fused(Model &m) {
         // Staged U (and X unless the geometry table is used), the corner-slot planes of one
         // pass (half the corners; all of them for the MFMA element step) and the trash cell.
         const int staged = m.elem_path >= 2 ? 1 : 2;
-        const int planes = m.elem_bf16 ? m.nen : m.nen / 2;
+        const int planes = m.nen / cd_passes(m);
         m.tiles.lds_bytes =
             use_moc(m) ? (size_t(staged) * m.tiles.max_stage * m.ndim + size_t(planes) * m.ndim * (use_moc(m) + 1)) *
                              sizeof(double)
