@@ -536,11 +536,11 @@ __device__ __forceinline__ void cd_corner_body(const FusedArgs &a) {
                 }
             }
         }
-        // Two passes of NH corners through NH slot planes: elements write corners
+        // NPASS passes of NH corners through NH slot planes: elements write corners
         // pass*NH .. pass*NH+NH-1 (branch-free: corners of nodes this tile does not own go to
         // the trash cell after the slots), then every owned node adds its planes in corner
         // order and clears them for the next pass. Same summation order as one pass over
-        // all corners, half the LDS.
+        // all corners, 1/NPASS of the LDS.
         const int trash = NH * DIM * MO;
 #pragma unroll
         for (int pass = 0; pass < NPASS; ++pass) {
