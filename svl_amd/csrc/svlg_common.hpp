@@ -139,6 +139,7 @@ struct Model {
     DevBuf<double> Fext;              // dense external force (loaded DOFs only written)
     DevBuf<double> keff, kforce;      // diag(M/dt^2 + C/2dt), diag(M/dt^2 - C/2dt)
     DevBuf<double> kinv;              // 1 / keff (0 where keff = 0: the restrained DOFs)
+    DevBuf<uint16_t> egid;            // [nelem] geometry-table entry per element (table paths)
     DevBuf<double> hist;              // recorder buffer
     DevBuf<int32_t> rec;              // recorded node-DOFs
     // Independent meshes advanced together (one model whose parts share no node): each part

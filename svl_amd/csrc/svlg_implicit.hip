@@ -103,7 +103,10 @@ struct OpArgs {
 template <int DIM, int PATH, int MODE, bool NL, int MOC = 0, bool CM = false>
 __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
     constexpr int NEN = Elem<DIM>::nen;
-    constexpr bool STAGE_X = PATH < 2 || NL;  // J2 Gauss-point loops need the coordinates
+    // J2 Gauss-point loops need the coordinates, except on axis-aligned Hexa8 bricks (the
+    // diagonal geometry table), where they run on the corner sums (j2_brick_force).
+    constexpr bool BRICK = PATH == 3 && DIM == 3;
+    constexpr bool STAGE_X = PATH < 2 || (NL && !BRICK);
     constexpr bool RHS = MODE == OP_RHS;
     constexpr int NPRE = RHS ? (CM ? 3 : 2) : 1;  // staged state arrays before the coordinates
     extern __shared__ double smem[];
@@ -224,16 +227,23 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
                 j2 = true;
                 const double *q5 = a.j2p + size_t(tag) * 5;
                 const J2Par jp{q5[0], q5[1], q5[2], q5[3], q5[4]};
-                double x[NEN][DIM];
 #pragma unroll
                 for (int c = 0; c < NEN; ++c)
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) {
-                        x[c][d] = sX[lc[c] * DIM + d];
-                        u[c][d] = RHS ? sV[lc[c] * DIM + d] : sU[lc[c] * DIM + d];
-                    }
+                    for (int d = 0; d < DIM; ++d) u[c][d] = RHS ? sV[lc[c] * DIM + d] : sU[lc[c] * DIM + d];
                 const double cR = RHS ? c2 : a.coef[e0 + tid] - 1.0;
-                j2_elem_force<DIM>(x, u, jp, a.state, size_t(a.nelem), tag, cR, !RHS, a.th, f);
+                if constexpr (BRICK) {
+                    const double *gl = a.geo + size_t(gid) * 9;
+                    const double l[3] = {gl[0], gl[4], gl[8]};
+                    j2_brick_force(l, u, jp, a.state, size_t(a.nelem), tag, cR, !RHS, f);
+                } else {
+                    double x[NEN][DIM];
+#pragma unroll
+                    for (int c = 0; c < NEN; ++c)
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) x[c][d] = sX[lc[c] * DIM + d];
+                    j2_elem_force<DIM>(x, u, jp, a.state, size_t(a.nelem), tag, cR, !RHS, a.th, f);
+                }
             }
         }
         if (!j2) {
@@ -594,7 +604,8 @@ static int op_moc(const Model &m, int mode) {
 }
 
 static size_t op_lds(const Model &m, int mode) {
-    const int staged = (mode == OP_RHS ? (m.lumped ? 2 : 3) : 1) + (m.elem_path < 2 || m.has_j2 ? 1 : 0);
+    const bool brick = m.elem_path == 3 && m.ndim == 3;  // J2 points on the corner sums: no coordinates
+    const int staged = (mode == OP_RHS ? (m.lumped ? 2 : 3) : 1) + (m.elem_path < 2 || (m.has_j2 && !brick) ? 1 : 0);
     const int moc = op_moc(m, mode);
     const size_t stride = moc ? moc + 1 : m.tiles.max_own;
     return (size_t(staged) * m.tiles.max_stage * m.ndim + size_t(m.nen) * m.ndim * stride) * sizeof(double);

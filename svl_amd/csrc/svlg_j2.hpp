@@ -255,4 +255,121 @@ __device__ inline void j2_elem_force(const double (&x)[Elem<DIM>::nen][DIM], con
     }
 }
 
+// ---------------------------------------------------------------- axis-aligned Hexa8 bricks
+// For a brick aligned with the axes (J diagonal, J^-1 = diag(lx, ly, lz), |J| = 1/(lx ly lz))
+// the shape-function derivatives at a Gauss point follow from the corner sums
+// S_m = sum_a m_a u_a over the sign patterns m = xi, eta, zeta, xi eta, eta zeta, zeta xi,
+// xi eta zeta of SVL's corner order: du/dxi = (S_x + eta S_xy + zeta S_zx + eta zeta S_xyz) / 8
+// and cyclically; the same patterns carry B^T sigma back to the corners. This is the Gauss
+// loop of lin3DHexa8 (B from J^-1 at each point) evaluated without forming J or B.
+
+// Corner sums S[m][c], m = x, y, z, xy, yz, zx, xyz (Walsh-Hadamard butterfly, 23 adds per c).
+__device__ __forceinline__ void brick_sums(const double (&u)[8][3], double (&S)[7][3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        // v[i][j][k]: corner with xi bit i, eta bit j, zeta bit k (SVL order 0..7).
+        const double v000 = u[0][c], v100 = u[1][c], v110 = u[2][c], v010 = u[3][c];
+        const double v001 = u[4][c], v101 = u[5][c], v111 = u[6][c], v011 = u[7][c];
+        const double a00 = v100 + v000, b00 = v100 - v000, a10 = v110 + v010, b10 = v110 - v010;
+        const double a01 = v101 + v001, b01 = v101 - v001, a11 = v111 + v011, b11 = v111 - v011;
+        const double aa0 = a10 + a00, ab0 = a10 - a00, ba0 = b10 + b00, bb0 = b10 - b00;
+        const double aa1 = a11 + a01, ab1 = a11 - a01, ba1 = b11 + b01, bb1 = b11 - b01;
+        S[0][c] = ba1 + ba0;  // x
+        S[1][c] = ab1 + ab0;  // y
+        S[2][c] = aa1 - aa0;  // z
+        S[3][c] = bb1 + bb0;  // xy
+        S[4][c] = ab1 - ab0;  // yz
+        S[5][c] = ba1 - ba0;  // zx
+        S[6][c] = bb1 - bb0;  // xyz
+    }
+}
+
+// Engineering strain [xx, yy, zz, xy, yz, zx] at the Gauss point (r, s, t) from the corner sums;
+// l8 = (lx, ly, lz) / 8.
+__device__ __forceinline__ void brick_strain(const double (&S)[7][3], const double (&l8)[3], double r, double s,
+                                             double t, double (&ev)[6]) {
+    double gx[3], gy[3], gz[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        gx[c] = l8[0] * (((S[0][c] + s * S[3][c]) + t * S[5][c]) + (s * t) * S[6][c]);
+        gy[c] = l8[1] * (((S[1][c] + r * S[3][c]) + t * S[4][c]) + (r * t) * S[6][c]);
+        gz[c] = l8[2] * (((S[2][c] + s * S[4][c]) + r * S[5][c]) + (r * s) * S[6][c]);
+    }
+    ev[0] = gx[0];
+    ev[1] = gy[1];
+    ev[2] = gz[2];
+    ev[3] = gy[0] + gx[1];
+    ev[4] = gz[1] + gy[2];
+    ev[5] = gz[0] + gx[2];
+}
+
+// f[a][c] = (|J| / 8) sum_m m_a F[m][c] (the transpose of brick_sums).
+__device__ __forceinline__ void brick_corners(const double (&F)[7][3], double scale, double (&f)[8][3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double aa1 = F[2][c], aa0 = -F[2][c];
+        const double ab1 = F[1][c] + F[4][c], ab0 = F[1][c] - F[4][c];
+        const double ba1 = F[0][c] + F[5][c], ba0 = F[0][c] - F[5][c];
+        const double bb1 = F[3][c] + F[6][c], bb0 = F[3][c] - F[6][c];
+        const double a00 = aa0 - ab0, a10 = aa0 + ab0, b00 = ba0 - bb0, b10 = ba0 + bb0;
+        const double a01 = aa1 - ab1, a11 = aa1 + ab1, b01 = ba1 - bb1, b11 = ba1 + bb1;
+        f[0][c] = scale * (a00 - b00);
+        f[1][c] = scale * (a00 + b00);
+        f[2][c] = scale * (a10 + b10);
+        f[3][c] = scale * (a10 - b10);
+        f[4][c] = scale * (a01 - b01);
+        f[5][c] = scale * (a01 + b01);
+        f[6][c] = scale * (a11 + b11);
+        f[7][c] = scale * (a11 - b11);
+    }
+}
+
+// j2_elem_force for an axis-aligned Hexa8 brick with J^-1 = diag(l): the same sum over the
+// eight Gauss points (weights 1, |J| = 1 / (lx ly lz)).
+__device__ inline void j2_brick_force(const double (&l)[3], const double (&u)[8][3], const J2Par &p,
+                                      const double *state, size_t nelem, int e, double c, bool tangent,
+                                      double (&f)[8][3]) {
+    double S[7][3], F[7][3];
+    brick_sums(u, S);
+#pragma unroll
+    for (int m = 0; m < 7; ++m)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) F[m][k] = 0.0;
+    const double l8[3] = {0.125 * l[0], 0.125 * l[1], 0.125 * l[2]};
+#pragma unroll 1
+    for (int g = 0; g < 8; ++g) {
+        double r, s, t;
+        gauss_point(g, r, s, t);
+        double ev[6], et[6], e6[6], s6[6], i6[6];
+        brick_strain(S, l8, r, s, t, ev);
+        j2_embed<3>(ev, et, e6);
+        j2_initial_apply(p, e6, i6);
+        const J2CRef sg = j2_at(state, nelem, 8, e, g);
+        if (tangent) {
+            j2_tangent_apply(p, sg, e6, s6);
+#pragma unroll
+            for (int i = 0; i < 6; ++i) s6[i] += c * i6[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) s6[i] = sg[13 + i] + c * i6[i];
+        }
+        // Directional stresses of component k: (sigma_xk, sigma_yk, sigma_zk).
+        const double tx[3] = {s6[0], s6[3], s6[5]};
+        const double ty[3] = {s6[3], s6[1], s6[4]};
+        const double tz[3] = {s6[5], s6[4], s6[2]};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double X = l[0] * tx[k], Y = l[1] * ty[k], Z = l[2] * tz[k];
+            F[0][k] += X;
+            F[1][k] += Y;
+            F[2][k] += Z;
+            F[3][k] += s * X + r * Y;
+            F[4][k] += t * Y + s * Z;
+            F[5][k] += t * X + r * Z;
+            F[6][k] += ((s * t) * X + (r * t) * Y) + (r * s) * Z;
+        }
+    }
+    brick_corners(F, 0.125 / ((l[0] * l[1]) * l[2]), f);
+}
+
 }  // namespace svlg

@@ -505,6 +505,7 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
     }
     if (m.elem_path >= 2) {
         UP(m.geo, gtab.data(), gtab.size());
+        UP(m.egid, gid.data(), gid.size());
     }
     m.h_escale = escale;
     // Tiles.

@@ -48,7 +48,9 @@ __global__ void k_j2_init(int ne, const int32_t *__restrict__ mat, const double 
 // kJ2Block / ngp elements, thread t works on Gauss point t / epb of element t % epb, so the
 // field-major states are read and written contiguously. plastic[e] = 1 when the update left
 // some Gauss point of e with the elastoplastic tangent, 0 when every point holds the elastic
-// one (the tangent operator then takes the element's elastic path).
+// one (the tangent operator then takes the element's elastic path). geo non-null (Hexa8 on
+// the diagonal geometry table): the strain comes from the corner sums with J^-1 of the
+// element's table entry egid[e] (brick_strain), without the coordinates.
 constexpr int kJ2Block = 256;
 
 template <int DIM>
@@ -57,7 +59,9 @@ __global__ __launch_bounds__(kJ2Block) void k_j2_update(int ne, const int32_t *_
                                                         const double *__restrict__ dU,
                                                         const int32_t *__restrict__ mat,
                                                         const double *__restrict__ j2p, double th,
-                                                        double *__restrict__ state, int32_t *__restrict__ plastic) {
+                                                        double *__restrict__ state, int32_t *__restrict__ plastic,
+                                                        const double *__restrict__ geo,
+                                                        const uint16_t *__restrict__ egid) {
     constexpr int NEN = Elem<DIM>::nen, NV = DIM == 2 ? 3 : 6, NGP = Elem<DIM>::ngp;
     constexpr int EPB = kJ2Block / NGP;
     __shared__ int flag[EPB];
@@ -69,20 +73,38 @@ __global__ __launch_bounds__(kJ2Block) void k_j2_update(int ne, const int32_t *_
     if (j2) {
         const double *q = j2p + size_t(e) * 5;
         const J2Par p{q[0], q[1], q[2], q[3], q[4]};
-        double x[NEN][DIM], u[NEN][DIM];
+        double u[NEN][DIM];
 #pragma unroll
         for (int a = 0; a < NEN; ++a) {
             const size_t nd = size_t(conn[size_t(e) * NEN + a]) * DIM;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                x[a][d] = X[nd + d];
-                u[a][d] = U[nd + d] + dU[nd + d];
+            for (int d = 0; d < DIM; ++d) u[a][d] = U[nd + d] + dU[nd + d];
+        }
+        double ev[NV], et[6], e6[6];
+        bool brick = false;
+        if constexpr (DIM == 3) {
+            if (geo) {
+                brick = true;
+                const double *gl = geo + size_t(egid[e]) * 9;
+                const double l8[3] = {0.125 * gl[0], 0.125 * gl[4], 0.125 * gl[8]};
+                double S[7][3], r, s, t;
+                brick_sums(u, S);
+                gauss_point(g, r, s, t);
+                brick_strain(S, l8, r, s, t, ev);
             }
         }
-        double dn[NEN][DIM];
-        (void)gp_dn<DIM>(x, g, th, dn);
-        double ev[NV], et[6], e6[6];
-        gp_strain<DIM>(dn, u, ev);
+        if (!brick) {
+            double x[NEN][DIM];
+#pragma unroll
+            for (int a = 0; a < NEN; ++a) {
+                const size_t nd = size_t(conn[size_t(e) * NEN + a]) * DIM;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) x[a][d] = X[nd + d];
+            }
+            double dn[NEN][DIM];
+            (void)gp_dn<DIM>(x, g, th, dn);
+            gp_strain<DIM>(dn, u, ev);
+        }
         j2_embed<DIM>(ev, et, e6);
         const J2Ref st = j2_at(state, size_t(ne), NGP, e, g);
         j2_update_gp(p, et, st);
@@ -283,10 +305,12 @@ int svlg_nl_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
             if (m.has_j2) {
                 if (m.ndim == 3)
                     hipLaunchKernelGGL(k_j2_update<3>, dim3(neb), dim3(kJ2Block), 0, s, m.nelem, m.conn.p, m.X.p,
-                                       m.nU.p, m.nl_dUa.p, m.mat.p, m.j2p.p, m.th, m.gp_state.p, m.j2_plastic.p);
+                                       m.nU.p, m.nl_dUa.p, m.mat.p, m.j2p.p, m.th, m.gp_state.p, m.j2_plastic.p,
+                                       m.elem_path == 3 ? m.geo.p : nullptr, m.egid.p);
                 else
                     hipLaunchKernelGGL(k_j2_update<2>, dim3(neb), dim3(kJ2Block), 0, s, m.nelem, m.conn.p, m.X.p,
-                                       m.nU.p, m.nl_dUa.p, m.mat.p, m.j2p.p, m.th, m.gp_state.p, m.j2_plastic.p);
+                                       m.nU.p, m.nl_dUa.p, m.mat.p, m.j2p.p, m.th, m.gp_state.p, m.j2_plastic.p,
+                                       nullptr, nullptr);
             }
             ++it;
         } while (res > m.nl_tol && it < m.nl_nmax);

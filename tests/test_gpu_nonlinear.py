@@ -28,7 +28,7 @@ from svl_amd import workloads as W
 from svl_amd.integrators import DeviceModel, ExtendedNewmarkBeta, LearnedPreconditioner, NewmarkBeta
 from svl_amd.mesh import ELEM_QUAD4, Model, tags_for
 
-from _meshes import block_model
+from _meshes import block_model, rotation
 
 pytestmark = pytest.mark.gpu
 
@@ -149,8 +149,8 @@ def test_soil_column_F07(beta):
     dm.close()
 
 
-def j2_block(test=5):
-    m, dt, nt, src = block_model(4, 4, 3)
+def j2_block(test=5, transform=None):
+    m, dt, nt, src = block_model(4, 4, 3, transform=transform)
     m.ray_am = np.full(m.nelem, 0.5)
     m.ray_ak = np.full(m.nelem, 1e-3)
     sel = np.nonzero(np.arange(m.nelem) % 2 == 0)[0]
@@ -167,6 +167,17 @@ def test_hexa8_block_mixed_j2_elastic(cnvgtest, tol):
     rec = W.dofs_of(m, src) + list(range(0, m.nfree, 17))
     dm, nm = check(m, dt, nt, rec, tol, 30, cnvgtest)
     yield_check(nm, m)
+    dm.close()
+
+
+def test_hexa8_block_mixed_j2_rotated():
+    """The same mixed block rotated and sheared (parallelepipeds off the axes): the J2 points
+    take the Gauss loop on the element coordinates instead of the axis-aligned corner sums."""
+    m, dt, src = j2_block(transform=rotation())
+    nt = 60
+    rec = W.dofs_of(m, src) + list(range(0, m.nfree, 17))
+    dm, nm = check(m, dt, nt, rec, 1e-8, 30, 5)
+    assert dm.info()["elem_path"] < 3
     dm.close()
 
 
