@@ -802,12 +802,15 @@ int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int p
     const int expect = m.cg_last_iters[flexible];
     while (launched < maxit && !done) {
         const int before = launched;
-        if (pair && launched + 2 <= maxit) {
+        // A pair that starts at the predicted count would queue one iteration past the one
+        // whose operator kernel is expected to see DONE: that iteration goes alone.
+        const bool tail = expect > 0 && launched == expect;
+        if (pair && launched + 2 <= maxit && !tail) {
             // The pair graph starts at an even iteration: *jdev tracks `launched`.
             SVLG_HIP(hipGraphLaunch(pair, s));
             launched += 2;
         } else {
-            if (pair) pair = nullptr;  // odd remainder: eager from here on
+            if (pair) pair = nullptr;  // odd remainder or predicted tail: eager from here on
             if (int rc = cg_iteration(m, c, a, x, launched, nullptr)) return rc;
             launched += 1;
         }

@@ -141,12 +141,18 @@ def trained_vs_start(m, dt):
 def test_network_gain_over_jacobi_pcg():
     """What the network gains over the method without it (Jacobi-PCG, the fallback and judge):
     at most a third of the iterations per step, the same solution to the solver tolerance, and
-    training lowers its loss and never leaves it worse than its start (the validation guard)."""
+    training lowers its loss and never leaves the solver worse than its start: the validation
+    guard keeps trained parameters only when they cut the validation count, and otherwise
+    restores the start, whose iteration counts then repeat exactly."""
     m, dt, nt = W.block3d_implicit(12, 12, 10, nt=61)
     it_j, U_j, it_c, U_c, it_l, U_l, pc = trained_vs_start(m, dt)
     assert it_l.mean() <= it_j.mean() / 3.0 and it_c.mean() <= it_j.mean() / 3.0
     assert pc["loss_hist"][-1] < pc["loss_hist"][0]
-    assert pc["val_iters_after"] <= pc["val_iters_before"]
+    if pc["accepted"]:
+        assert pc["val_iters_after"] < pc["val_iters_before"]
+    else:
+        assert np.array_equal(it_l, it_c)
+    assert it_l.mean() <= it_c.mean()
     assert rel(U_l, U_j) < 1e-6 and rel(U_c, U_j) < 1e-6
 
 
