@@ -151,13 +151,15 @@ struct Model {
     hipGraphExec_t cd_graph = nullptr;
     int cd_graph_steps = 0, cd_graph_ring = -1;
     DevBuf<int> kdev;
-    // Captured pairs of CG iterations (even, odd) for the solution buffers the integrators
-    // reuse, keyed by (x, flexible, operator kernel); the iteration base lives in jdev.
+    // Captured runs of CG iterations (2 or 8, starting at an even iteration) for the solution
+    // buffers the integrators reuse, keyed by (x, flexible, operator kernel, count); the
+    // iteration base lives in jdev.
     struct CgGraph {
         hipGraphExec_t exec;
         const double *x;
         int flexible;
         const void *op;
+        int count;
     };
     std::vector<CgGraph> cg_graphs;
     DevBuf<int> jdev;

@@ -736,18 +736,18 @@ static int cg_iteration(Model &m, const CgParts &c, OpArgs &a, double *x, int j,
     return 0;
 }
 
-// The executable graph of one pair of iterations (*jdev, *jdev + 1) followed by *jdev += 2,
-// captured on first use for (x, flexible, operator kernel).
-static hipGraphExec_t cg_pair_graph(Model &m, const CgParts &c, const OpArgs &a0, double *x) {
+// The executable graph of `count` (even) iterations *jdev .. *jdev + count - 1 followed by
+// *jdev += count, captured on first use for (x, flexible, operator kernel, count).
+static hipGraphExec_t cg_run_graph(Model &m, const CgParts &c, const OpArgs &a0, double *x, int count) {
     const void *op = op_fn(m, OP_CG);
     for (auto &g : m.cg_graphs)
-        if (g.x == x && g.flexible == c.flexible && g.op == op) return g.exec;
+        if (g.x == x && g.flexible == c.flexible && g.op == op && g.count == count) return g.exec;
     if (!m.jdev.p && m.jdev.alloc(1) != hipSuccess) return nullptr;
     if (hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
     OpArgs a = a0;
     int rc = 0;
-    for (int j = 0; j < 2 && rc == 0; ++j) rc = cg_iteration(m, c, a, x, j, m.jdev.p);
-    hipLaunchKernelGGL(k_iter_advance, dim3(1), dim3(64), 0, m.stream, m.jdev.p, 2);
+    for (int j = 0; j < count && rc == 0; ++j) rc = cg_iteration(m, c, a, x, j, m.jdev.p);
+    hipLaunchKernelGGL(k_iter_advance, dim3(1), dim3(64), 0, m.stream, m.jdev.p, count);
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(m.stream, &g);
     if (ec != hipSuccess || rc != 0 || !g) {
@@ -762,7 +762,7 @@ static hipGraphExec_t cg_pair_graph(Model &m, const CgParts &c, const OpArgs &a0
         (void)hipGetLastError();
         return nullptr;
     }
-    m.cg_graphs.push_back({exec, x, c.flexible, op});
+    m.cg_graphs.push_back({exec, x, c.flexible, op, count});
     return exec;
 }
 
@@ -789,7 +789,8 @@ int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int p
         if (int rc = gcn_precond(m, c, 0, m.cg_r.p, m.cg_q.p, m.cg_z.p)) return rc;
     a.z = m.cg_z.p;
     a.reduce = 1;
-    hipGraphExec_t pair = graph && maxit >= 2 ? cg_pair_graph(m, c, a, x) : nullptr;
+    hipGraphExec_t pair = graph && maxit >= 2 ? cg_run_graph(m, c, a, x, 2) : nullptr;
+    hipGraphExec_t run8 = pair && maxit >= 8 ? cg_run_graph(m, c, a, x, 8) : nullptr;
     if (pair) hipLaunchKernelGGL(k_iter_set, dim3(1), dim3(64), 0, s, m.jdev.p, 0);
     double hs[SC_COUNT];
     int launched = 0;  // iterations 0 .. launched-1 are queued
@@ -806,9 +807,17 @@ int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int p
         // whose operator kernel is expected to see DONE: that iteration goes alone.
         const bool tail = expect > 0 && launched == expect;
         if (pair && launched + 2 <= maxit && !tail) {
-            // The pair graph starts at an even iteration: *jdev tracks `launched`.
-            SVLG_HIP(hipGraphLaunch(pair, s));
-            launched += 2;
+            // The graphs start at even iterations: *jdev tracks `launched`. Runs of eight while
+            // they stay within the prediction (every graph launch costs a few microseconds of
+            // dispatch), then pairs.
+            const int lim = expect > 0 ? std::min(maxit, expect + 1) : maxit;
+            if (run8 && launched + 8 <= lim) {
+                SVLG_HIP(hipGraphLaunch(run8, s));
+                launched += 8;
+            } else {
+                SVLG_HIP(hipGraphLaunch(pair, s));
+                launched += 2;
+            }
         } else {
             if (pair) pair = nullptr;  // odd remainder or predicted tail: eager from here on
             if (int rc = cg_iteration(m, c, a, x, launched, nullptr)) return rc;
