@@ -212,6 +212,7 @@ struct Model {
     DevBuf<float> gcn_w;              // theta in FP32
     DevBuf<double> gcn_s;             // D^-1/2 (0 on restrained DOFs)
     DevBuf<float> gcn_sf;             // the same in FP32
+    DevBuf<float> gcn_jacf;           // D^-1 in FP32 (one-channel kernel: states scaled by D^-1/2)
     DevBuf<double> gcn_elam, gcn_emu; // per tile element (lambda, mu) * (1 + 2 ak/dt)
     DevBuf<float> gcn_elamf, gcn_emuf, gcn_dgnf;  // FP32 copies read by the one-channel layer kernel
     DevBuf<float> gcn_kmat;           // [2][24][24] K_lambda, K_mu of the single geometry class (MFMA path)

@@ -78,6 +78,7 @@ struct GcnArgs {
     int iter, normalize;
     const int *jdev;           // non-null: the iteration is *jdev + iter (graph replays)
     const float *elamf, *emuf, *dgnf;  // FP32 copies of elam, emu, dgn (one-channel kernel)
+    const float *jacf;                 // D^-1 in FP32 (one-channel kernel)
     const double *cmass;       // consistent mass: (4/dt^2 + 2 am/dt) times the element-mass scale
                                // per tile element (null: lumped, inside dgn)
 };
@@ -264,10 +265,12 @@ __global__ __launch_bounds__(512) void k_gcn_layer(GcnArgs a) {
 }
 
 // Inference layer for one channel with the BF16 MFMA element step (Hexa8, single element
-// geometry): out = a_l in + b_l (Ehat in) + c_l x0, and on the output layer z = sigma s (w out)
-// with the tile partials of z.r and z.q. The element data of the workgroup's element blocks
-// are prefetched before staging, the corner slots and the node-phase arithmetic are FP32
-// (preconditioner precision); z and the partials are FP64.
+// geometry) on the scaled states G = s H (s = D^-1/2): the layer H' = a_l H + b_l (Ehat H) +
+// c_l x0 with Ehat = s Keff s reads G' = a_l G + b_l D^-1 (Keff G) + c_l s x0, so the staged
+// operand is the state itself (no scaling loads) and the output layer gives z = sigma s (w H)
+// = sigma w G, with the tile partials of z.r and z.q. The element data of the workgroup's
+// element blocks are prefetched before staging, the corner slots and the node-phase arithmetic
+// are FP32 (preconditioner precision); z and the partials are FP64.
 // 16-bit staged slot of corner 2w + h from the packed connectivity (four 32-bit words).
 __device__ __forceinline__ int packed_half(uint32_t w, int h) { return int((w >> (16 * h)) & 0xffffu); }
 
@@ -315,15 +318,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MOC > 0 ? 8
         pk[r] = *reinterpret_cast<const uint4 *>(a.lconn + size_t(e0 + el) * 8);
     }
     const size_t g0 = size_t(o0) * 3;
-    for (int i = tid; i < nown * 3; i += B) sX[i] = a.sf[g0 + i] * a.in[g0 + i];
+    for (int i = tid; i < nown * 3; i += B) sX[i] = a.in[g0 + i];
     for (int sl = tid; sl < nh; sl += B) {
         const size_t nd = size_t(a.halo_node[h0 + sl]) * 3;
-        const float s0 = a.sf[nd], s1 = a.sf[nd + 1], s2 = a.sf[nd + 2];
         const float v0 = a.in[nd], v1 = a.in[nd + 1], v2 = a.in[nd + 2];
         float *d = sX + (nown + sl) * 3;
-        d[0] = s0 * v0;
-        d[1] = s1 * v1;
-        d[2] = s2 * v2;
+        d[0] = v0;
+        d[1] = v1;
+        d[2] = v2;
     }
     for (int i = tid; i < 24 * MO; i += B) sIn[i] = 0.f;
     if (tid == 0) sX[zcell] = 0.f;
@@ -384,12 +386,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MOC > 0 ? 8
 #pragma unroll
             for (int cc = 0; cc < 8; ++cc) fi += sIn[(cc * 3 + d) * MO + tid];
             const size_t gd = g0 + size_t(tid) * 3 + d;
-            const float sg = a.sf[gd];
             const float xh = sX[tid * 3 + d];
-            const float y = sg * (fi + a.dgnf[gd] * xh);
-            const float o = am * a.in[gd] + bm * y + cm * a.x0[gd];
+            const float y = a.jacf[gd] * (fi + a.dgnf[gd] * xh);
+            const float o = am * xh + bm * y + cm * a.x0[gd];
             if constexpr (OUT) {
-                const double zv = sig * a.s[gd] * double(o * a.w[0]);
+                const double zv = sig * double(o * a.w[0]);
                 a.z[gd] = zv;
                 acc0 += zv * a.r[gd];
                 acc1 += zv * a.q[gd];
@@ -431,7 +432,7 @@ __global__ void k_gcn_kimg(const float *__restrict__ kmat, uint4 *__restrict__ i
     }
 }
 
-// x0 = s r / sigma, H^1 = x0 c_1^T.
+// x0 = s r / sigma, H^1 = x0 c_1^T (s = D^-1/2, or D^-1 for the one-channel kernel's scaled states).
 __global__ __launch_bounds__(512) void k_gcn_first(int n, int C, int normalize, CgParts cg, int j,
                                                    const double *__restrict__ r, const double *__restrict__ s,
                                                    const float *__restrict__ c1, const double *scal,
@@ -539,12 +540,14 @@ __global__ void k_gcn_kmat(const double *__restrict__ geo, int diag, float *__re
         for (int d = 0; d < 3; ++d) kmat[which * 576 + (c * 3 + d) * 24 + k] = float(f[c][d]);
 }
 
-__global__ void k_gcn_scale(int n, const double *__restrict__ jac, double *__restrict__ s, float *__restrict__ sf) {
+__global__ void k_gcn_scale(int n, const double *__restrict__ jac, double *__restrict__ s, float *__restrict__ sf,
+                            float *__restrict__ jacf) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double v = jac[i] > 0.0 ? sqrt(jac[i]) : 0.0;
     s[i] = v;
     sf[i] = float(v);
+    jacf[i] = float(jac[i]);
 }
 
 // e = s * w with w uniform in [-1, 1] from a counter-based hash of (seed, i).
@@ -663,6 +666,7 @@ static GcnArgs gcn_args(const Model &m) {
     a.elamf = m.gcn_elamf.p;
     a.emuf = m.gcn_emuf.p;
     a.dgnf = m.gcn_dgnf.p;
+    a.jacf = m.gcn_jacf.p;
     a.geo = m.geo.p;
     a.X = m.X.p;
     a.th = m.th;
@@ -702,12 +706,17 @@ static int upload_theta(Model &m) {
 // Inference: z = P(r) (node layout). In the solver (normalize = 1) r is r_j of iteration j
 // with its partials in `cg`; the output layer writes the tp[j&1] partials.
 // first = false: x0 and H^1 are already in gcn_x0 / gcn_h[0] (formed by the CG vector kernel).
+// The one-channel MFMA inference kernel runs on scaled states G = D^-1/2 H (its input layer
+// is x0 = D^-1 r); every other layer kernel on H (x0 = D^-1/2 r).
+static bool gcn_scaled_states(const Model &m) { return m.gcn_mfma && m.gcn_C == 1; }
+static const double *gcn_input_scale(const Model &m) { return gcn_scaled_states(m) ? m.jac.p : m.gcn_s.p; }
+
 static int gcn_run(Model &m, const CgParts &cg, int j, const double *r, const double *q, double *z, int normalize,
                    const int *jdev = nullptr, bool first = true) {
     const int n = m.ndofn, C = m.gcn_C, L = m.gcn_L;
     if (first)
         hipLaunchKernelGGL(k_gcn_first, dim3(kVecGrid), dim3(kVecBlock), 0, m.stream, n, C, normalize, cg, j, r,
-                           m.gcn_s.p, theta_c(m, 1), m.scal.p, m.gcn_x0.p, m.gcn_h[0].p, jdev);
+                           gcn_input_scale(m), theta_c(m, 1), m.scal.p, m.gcn_x0.p, m.gcn_h[0].p, jdev);
     GcnArgs a = gcn_args(m);
     int cur = 0;
     for (int l = 2; l <= L; ++l) {
@@ -729,7 +738,7 @@ static int gcn_run(Model &m, const CgParts &cg, int j, const double *r, const do
             a.jdev = jdev;
             a.normalize = normalize;
         }
-        if (m.gcn_mfma && C == 1) {
+        if (gcn_scaled_states(m)) {
             a.kmat = reinterpret_cast<const float *>(m.gcn_kimg.p);
             void *args[] = {&a};
             const void *fn = gcn_fast_fn(m, mode == G_OUT);
@@ -749,7 +758,7 @@ int gcn_precond(Model &m, const CgParts &c, int j, const double *r, const double
 }
 
 GcnInput gcn_input_args(Model &m) {
-    return GcnInput{m.gcn_s.p, theta_c(m, 1), m.gcn_x0.p, m.gcn_h[0].p, m.gcn_C};
+    return GcnInput{gcn_input_scale(m), theta_c(m, 1), m.gcn_x0.p, m.gcn_h[0].p, m.gcn_C};
 }
 
 // Loss and gradient of a batch of pairs (node layout, device); grad is FP64 on the device,
@@ -985,7 +994,9 @@ int svlg_gnn_init(svlg_model *h, int32_t channels, int32_t layers, int32_t mfma,
     const int n = m.ndofn;
     SVLG_HIP(m.gcn_s.alloc(n));
     SVLG_HIP(m.gcn_sf.alloc(n));
-    hipLaunchKernelGGL(k_gcn_scale, dim3((n + 255) / 256), dim3(256), 0, s, n, m.jac.p, m.gcn_s.p, m.gcn_sf.p);
+    SVLG_HIP(m.gcn_jacf.alloc(n));
+    hipLaunchKernelGGL(k_gcn_scale, dim3((n + 255) / 256), dim3(256), 0, s, n, m.jac.p, m.gcn_s.p, m.gcn_sf.p,
+                       m.gcn_jacf.p);
     // (lambda, mu) * (1 + 2 ak/dt) per tile element.
     {
         const size_t ne = m.tile_elem_id.size();

@@ -101,7 +101,8 @@ struct CgParts {
 };
 
 // Input layer of the learned preconditioner, formed by the CG vector kernels of the flexible
-// solve from the residual they produce: x0 = s r and H^1 = x0 c_1^T (C channels, FP32).
+// solve from the residual they produce: x0 = s r and H^1 = x0 c_1^T (C channels, FP32), with s
+// = D^-1/2, or D^-1 for the one-channel kernel's scaled states.
 struct GcnInput {
     const double *s;
     const float *c1;
