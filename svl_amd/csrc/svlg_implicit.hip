@@ -598,7 +598,8 @@ static OpArgs op_args(const Model &m) {
 // Compile-time slot stride of the CG operator on axis-aligned Hexa8 tiles of up to 168 or 336
 // owned nodes (0: the stride read from the tile set).
 static int op_moc(const Model &m, int mode) {
-    if (mode != OP_CG || m.ndim != 3 || m.elem_path != 3 || !m.lumped) return 0;
+    (void)mode;  // both the CG operator and the right-hand side / residual
+    if (m.ndim != 3 || m.elem_path != 3 || !m.lumped) return 0;
     const int mo = m.tiles.max_own;
     return mo <= 168 ? 168 : mo <= 336 ? 336 : 0;
 }
@@ -628,13 +629,19 @@ static const void *op_fn_t(int path, bool nl, bool cm) {
 }
 
 static const void *op_fn_sel(const Model &m, int mode, bool nl) {
-    switch (op_moc(m, mode)) {
+    switch (op_moc(m, mode) * (mode == OP_CG ? 1 : -1)) {
         case 168:
             return nl ? reinterpret_cast<const void *>(&k_nm_op<3, 3, OP_CG, true, 168>)
                       : reinterpret_cast<const void *>(&k_nm_op<3, 3, OP_CG, false, 168>);
         case 336:
             return nl ? reinterpret_cast<const void *>(&k_nm_op<3, 3, OP_CG, true, 336>)
                       : reinterpret_cast<const void *>(&k_nm_op<3, 3, OP_CG, false, 336>);
+        case -168:
+            return nl ? reinterpret_cast<const void *>(&k_nm_op<3, 3, OP_RHS, true, 168>)
+                      : reinterpret_cast<const void *>(&k_nm_op<3, 3, OP_RHS, false, 168>);
+        case -336:
+            return nl ? reinterpret_cast<const void *>(&k_nm_op<3, 3, OP_RHS, true, 336>)
+                      : reinterpret_cast<const void *>(&k_nm_op<3, 3, OP_RHS, false, 336>);
         default: break;
     }
     const bool cm = !m.lumped;
