@@ -743,8 +743,9 @@ static int cg_iteration(Model &m, const CgParts &c, OpArgs &a, double *x, int j,
     return 0;
 }
 
-// The executable graph of `count` (even) iterations *jdev .. *jdev + count - 1 followed by
-// *jdev += count, captured on first use for (x, flexible, operator kernel, count).
+// The executable graph of `count` iterations *jdev .. *jdev + count - 1 (starting at an even
+// iteration) followed by *jdev += count, captured on first use for (x, flexible, operator
+// kernel, count).
 static hipGraphExec_t cg_run_graph(Model &m, const CgParts &c, const OpArgs &a0, double *x, int count) {
     const void *op = op_fn(m, OP_CG);
     for (auto &g : m.cg_graphs)
@@ -798,6 +799,7 @@ int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int p
     a.reduce = 1;
     hipGraphExec_t pair = graph && maxit >= 2 ? cg_run_graph(m, c, a, x, 2) : nullptr;
     hipGraphExec_t run8 = pair && maxit >= 8 ? cg_run_graph(m, c, a, x, 8) : nullptr;
+    hipGraphExec_t one = pair ? cg_run_graph(m, c, a, x, 1) : nullptr;
     if (pair) hipLaunchKernelGGL(k_iter_set, dim3(1), dim3(64), 0, s, m.jdev.p, 0);
     double hs[SC_COUNT];
     int launched = 0;  // iterations 0 .. launched-1 are queued
@@ -825,8 +827,14 @@ int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int p
                 SVLG_HIP(hipGraphLaunch(pair, s));
                 launched += 2;
             }
+        } else if (pair && tail && one) {
+            // The predicted last iteration alone; the count is odd after it, so any further
+            // iteration is launched eagerly.
+            SVLG_HIP(hipGraphLaunch(one, s));
+            launched += 1;
+            pair = nullptr;
         } else {
-            if (pair) pair = nullptr;  // odd remainder or predicted tail: eager from here on
+            if (pair) pair = nullptr;  // odd remainder: eager from here on
             if (int rc = cg_iteration(m, c, a, x, launched, nullptr)) return rc;
             launched += 1;
         }
