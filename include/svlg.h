@@ -115,7 +115,10 @@ typedef struct svlg_exec_opts {
                            * corner-slot tiles), 2 = FP32 modal element arithmetic (axis-aligned
                            * Hexa8 bricks, corner-slot tiles); displacements, the corner-slot
                            * sums and the updates stay FP64 */
-    int32_t reserved[3];
+    int32_t cg_launch;    /* CG solves of the Newmark / Newton steps: 0 = iterations replayed from
+                           * captured hipGraphs (runs of 8, pairs, one), 1 = eager launches
+                           * (the same kernels and iteration order: bitwise the same results) */
+    int32_t reserved[2];
 } svlg_exec_opts;
 
 /* Error string of the last failing call on this thread. */

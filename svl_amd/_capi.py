@@ -68,7 +68,8 @@ class MeshDesc(C.Structure):
 
 class ExecOpts(C.Structure):
     _fields_ = [("tile_nodes", C.c_int32), ("block", C.c_int32), ("variant", C.c_int32),
-                ("geo_table", C.c_int32), ("elem_precision", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("geo_table", C.c_int32), ("elem_precision", C.c_int32), ("cg_launch", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 def _p(a, t):

@@ -230,6 +230,7 @@ struct Model {
     int ngeo = 0;
     DevBuf<double> geo;               // [ngeo * ndim * ndim] inverse Jacobians of distinct geometries
     int elem_bf16 = 0;                // explicit element step on BF16 MFMA (single geometry, Hexa8)
+    int cg_eager = 0;                 // integrator CG solves launched eagerly instead of from graphs
     int elem_fp32 = 0;                // explicit element step in FP32 (axis-aligned Hexa8 bricks)
     DevBuf<float> cd_kmat;            // its K_lambda, K_mu
 

@@ -1048,7 +1048,7 @@ int svlg_nm_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
         // Keff dU = Feff; the previous increment is the initial guess.
         int it = 0;
         double rel = 0.0;
-        const int rc = cg_solve(m, m.cg_b.p, m.ndU.p, m.nm_rtol, m.nm_maxit, m.nm_precond, &it, &rel, true);
+        const int rc = cg_solve(m, m.cg_b.p, m.ndU.p, m.nm_rtol, m.nm_maxit, m.nm_precond, &it, &rel, !m.cg_eager);
         if (iters) iters[st] = it;
         if (relres) relres[st] = rel;
         if (rc) {

@@ -688,6 +688,11 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
             return fail(11);
         }
     }
+    if (o && (o->cg_launch < 0 || o->cg_launch > 1)) {
+        set_error("cg_launch must be 0 (graph replays) or 1 (eager launches)");
+        return fail(11);
+    }
+    m.cg_eager = o ? o->cg_launch : 0;
     if (int rc = drm_setup(m, d)) return fail(rc);
     if (int rc = sm_setup(m, d)) return fail(rc);
     if (int rc = prepare_fused(m)) return fail(rc);

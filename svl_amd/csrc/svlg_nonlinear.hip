@@ -289,7 +289,7 @@ int svlg_nl_steps(svlg_model *h, int32_t k0, int32_t nsteps, int32_t nrec, const
             SVLG_HIP(hipMemsetAsync(m.ndU.p, 0, sizeof(double) * n, s));
             int cit = 0;
             double rel = 0.0;
-            if (int rc = cg_solve(m, m.cg_b.p, m.ndU.p, m.nm_rtol, m.nm_maxit, m.nm_precond, &cit, &rel, true))
+            if (int rc = cg_solve(m, m.cg_b.p, m.ndU.p, m.nm_rtol, m.nm_maxit, m.nm_precond, &cit, &rel, !m.cg_eager))
                 return rc;
             cg_total += cit;
             hipLaunchKernelGGL(k_nl_accum, dim3(kNlGrid), dim3(kNlBlock), 0, s, n, m.ndU.p, m.nl_dUa.p, m.cg_b.p,

@@ -25,11 +25,11 @@ class DeviceModel:
     """A model resident in HBM (Mesh::Initialize + Element::SetDomain)."""
 
     def __init__(self, model: Model, tile_nodes: int = 0, block: int = 0, variant: int = 0, geo_table: int = 0,
-                 elem_precision: int = 0):
+                 elem_precision: int = 0, cg_launch: int = 0):
         self.model = model
         self._desc, self._keep = capi.make_desc(model)
         h = C.c_void_p()
-        opts = capi.ExecOpts(tile_nodes, block, variant, geo_table, elem_precision)
+        opts = capi.ExecOpts(tile_nodes, block, variant, geo_table, elem_precision, cg_launch)
         capi.check(capi.lib().svlg_model_create_ex(C.byref(self._desc), C.byref(opts), C.byref(h)),
                    "svlg_model_create")
         self.h = h
