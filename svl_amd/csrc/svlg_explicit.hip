@@ -355,7 +355,6 @@ __global__ __launch_bounds__(512) void k_cd_fused(FusedArgs a) {
 template <int DIM, int PATH, int MOC, bool LATE, int NP>
 __device__ __forceinline__ void cd_corner_body(const FusedArgs &a) {
     constexpr int NEN = Elem<DIM>::nen;
-    constexpr int NB = 4;
     // Corner-slot planes per pass: the MFMA element step writes all corners at once.
     constexpr int NPASS = PATH == 4 ? 1 : NP, NH = NEN / NPASS;
     extern __shared__ double smem[];
@@ -414,50 +413,89 @@ __device__ __forceinline__ void cd_corner_body(const FusedArgs &a) {
     // Stage U (and X), zero the corner slots.
     {
         const double *Ug = a.Ucur + g0;
-        const double *Xg = a.X + g0;
-        for (int base = 0; base < nU; base += NB * B) {
-            double v[NB], w[NB];
+        if constexpr (!STAGE_X) {
+            // One loop covers the owned DOFs and the halo nodes: every pass issues the halo
+            // index loads and the owned loads together, then the halo loads, then the stores —
+            // two dependent global round trips per pass instead of three.
+            constexpr int NS = 2;  // strides per pass (a 160-node tile needs one)
+            const int nrange = nU > nh ? nU : nh;
+            for (int base = 0; base < nrange; base += NS * B) {
+                size_t n[NS];
 #pragma unroll
-            for (int k = 0; k < NB; ++k) {
-                const int i = base + k * B + tid;
-                v[k] = i < nU ? Ug[i] : 0.0;
-                if constexpr (STAGE_X) w[k] = i < nU ? Xg[i] : 0.0;
-            }
+                for (int k = 0; k < NS; ++k) {
+                    const int sl = base + k * B + tid;
+                    n[k] = sl < nh ? size_t(a.halo_node[h0 + sl]) * DIM : 0;
+                }
+                double v[NS];
 #pragma unroll
-            for (int k = 0; k < NB; ++k) {
-                const int i = base + k * B + tid;
-                if (i < nU) {
-                    sU[i] = v[k];
-                    if constexpr (STAGE_X) sX[i] = w[k];
+                for (int k = 0; k < NS; ++k) {
+                    const int i = base + k * B + tid;
+                    v[k] = i < nU ? Ug[i] : 0.0;
+                }
+                double hv[NS][DIM];
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    const int sl = base + k * B + tid;
+#pragma unroll
+                    for (int c = 0; c < DIM; ++c) hv[k][c] = sl < nh ? a.Ucur[n[k] + c] : 0.0;
+                }
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    const int i = base + k * B + tid;
+                    if (i < nU) sU[i] = v[k];
+                    if (i < nh)
+#pragma unroll
+                        for (int c = 0; c < DIM; ++c) sU[nU + i * DIM + c] = hv[k][c];
                 }
             }
-        }
-        // Halo: one index load per node, then its DIM components.
-        for (int base = 0; base < nh; base += NB * B) {
-            size_t n[NB];
+        } else {
+            // Coordinates staged too (Gauss-point and affine paths): owned values, then the
+            // halo (one index load per node, then its DIM components).
+            constexpr int NB = 4;  // loads kept in flight per thread
+            const double *Xg = a.X + g0;
+            for (int base = 0; base < nU; base += NB * B) {
+                double v[NB], w[NB];
 #pragma unroll
-            for (int k = 0; k < NB; ++k) {
-                const int sl = base + k * B + tid;
-                n[k] = sl < nh ? size_t(a.halo_node[h0 + sl]) * DIM : 0;
-            }
-            double v[NB][DIM], w[NB][DIM];
+                for (int k = 0; k < NB; ++k) {
+                    const int i = base + k * B + tid;
+                    v[k] = i < nU ? Ug[i] : 0.0;
+                    w[k] = i < nU ? Xg[i] : 0.0;
+                }
 #pragma unroll
-            for (int k = 0; k < NB; ++k) {
-                const int sl = base + k * B + tid;
-#pragma unroll
-                for (int c = 0; c < DIM; ++c) {
-                    v[k][c] = sl < nh ? a.Ucur[n[k] + c] : 0.0;
-                    if constexpr (STAGE_X) w[k][c] = sl < nh ? a.X[n[k] + c] : 0.0;
+                for (int k = 0; k < NB; ++k) {
+                    const int i = base + k * B + tid;
+                    if (i < nU) {
+                        sU[i] = v[k];
+                        sX[i] = w[k];
+                    }
                 }
             }
+            for (int base = 0; base < nh; base += NB * B) {
+                size_t n[NB];
 #pragma unroll
-            for (int k = 0; k < NB; ++k) {
-                const int sl = base + k * B + tid;
-                if (sl < nh) {
+                for (int k = 0; k < NB; ++k) {
+                    const int sl = base + k * B + tid;
+                    n[k] = sl < nh ? size_t(a.halo_node[h0 + sl]) * DIM : 0;
+                }
+                double v[NB][DIM], w[NB][DIM];
+#pragma unroll
+                for (int k = 0; k < NB; ++k) {
+                    const int sl = base + k * B + tid;
 #pragma unroll
                     for (int c = 0; c < DIM; ++c) {
-                        sU[nU + sl * DIM + c] = v[k][c];
-                        if constexpr (STAGE_X) sX[nU + sl * DIM + c] = w[k][c];
+                        v[k][c] = sl < nh ? a.Ucur[n[k] + c] : 0.0;
+                        w[k][c] = sl < nh ? a.X[n[k] + c] : 0.0;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < NB; ++k) {
+                    const int sl = base + k * B + tid;
+                    if (sl < nh) {
+#pragma unroll
+                        for (int c = 0; c < DIM; ++c) {
+                            sU[nU + sl * DIM + c] = v[k][c];
+                            sX[nU + sl * DIM + c] = w[k][c];
+                        }
                     }
                 }
             }
