@@ -366,6 +366,7 @@ __device__ __forceinline__ void cd_corner_body(const FusedArgs &a) {
     const int o0 = a.own_ptr[tile], nown = a.own_ptr[tile + 1] - o0;
     const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
     const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
+    const int npl = a.nplain[tile];  // read with the tile bounds (one scalar round trip)
     constexpr bool STAGE_X = PATH < 2;
     const int MO = MOC > 0 ? MOC + 1 : a.max_own;
     double *sU = smem;
@@ -580,9 +581,19 @@ __device__ __forceinline__ void cd_corner_body(const FusedArgs &a) {
         // order and clears them for the next pass. Same summation order as one pass over
         // all corners, 1/NPASS of the LDS.
         const int trash = NH * DIM * MO;
+        // LATE: the owned node's update operands are loaded at the start of the middle pass,
+        // once half of the element forces are written out (their registers are free), so
+        // that they arrive during the remaining passes.
 #pragma unroll
         for (int pass = 0; pass < NPASS; ++pass) {
             if (pass > 0) __syncthreads();
+            if constexpr (LATE)
+                if (pass == NPASS / 2 && has_node)
+#pragma unroll
+                    for (int c = 0; c < DIM; ++c) {
+                        up[c] = a.Uprev[g0 + size_t(tid) * DIM + c];
+                        kd[c] = a.kinv[g0 + size_t(tid) * DIM + c];
+                    }
             if (has_elem)
 #pragma unroll
                 for (int c = 0; c < NH; ++c) {
@@ -612,13 +623,12 @@ __device__ __forceinline__ void cd_corner_body(const FusedArgs &a) {
         }
     }
     if (!has_node) return;
-    if constexpr (LATE)
+    if constexpr (LATE && PATH == 4)
 #pragma unroll
         for (int c = 0; c < DIM; ++c) {
             up[c] = a.Uprev[g0 + size_t(tid) * DIM + c];
             kd[c] = a.kinv[g0 + size_t(tid) * DIM + c];
         }
-    const int npl = a.nplain[tile];
     const int ds = tid >= npl ? drm_slot(a.drm, o0 + tid) : -1;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
