@@ -69,6 +69,15 @@ struct TileSet {
                                              // (times |J|/8, or |J| th/4 in 2D, on the table path)
     DevBuf<uint16_t> elem_geo;               // [sum elems] geometry-table entry (table path)
     DevBuf<int32_t> color_ptr;               // [ntiles * (ncolors+1)]
+    // Corner-slot step: fixed-stride copies of the halo list and of the tile-element data
+    // (tile t at t * hstride / t * estride; padding: node 0, zero elements), so that a
+    // workgroup can load them with its tile index alone, before its tile bounds arrive.
+    // Built when the element path needs no staged coordinates and one staging pass covers
+    // every tile; hstride = 0 otherwise.
+    int hstride = 0, estride = 0;
+    DevBuf<int32_t> hpad;
+    DevBuf<uint16_t> lcpad, geopad;
+    DevBuf<double> lampad, mupad;
 };
 
 struct Model {

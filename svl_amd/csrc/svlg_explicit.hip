@@ -127,6 +127,11 @@ struct FusedArgs {
     const float *kmat;  // PATH 4: K_lambda, K_mu of the single element geometry
     const int *kdev;    // non-null: load sample = *kdev + k (graph replays advance *kdev)
     DrmRef drm;         // domain reduction forces of every step (F null: none)
+    // Corner-slot step: fixed-stride tile data (TileSet::hpad ...; hstride 0: not built).
+    const int32_t *hpad;
+    const uint16_t *lcpad, *geopad;
+    const double *lampad, *mupad;
+    int hstride, estride;
 };
 
 __device__ __forceinline__ int color_of(const int32_t *__restrict__ cp, int i) {
@@ -352,7 +357,7 @@ __global__ __launch_bounds__(512) void k_cd_fused(FusedArgs a) {
 // step, more resident waves (on the Hexa8 table path, 86 -> 72 VGPRs).
 // The update multiplies by the stored 1/keff (kinv) instead of dividing by keff: the
 // diagonal solve as Eigen's LDL^T applies it (the inverse of D times the vector).
-template <int DIM, int PATH, int MOC, bool LATE, int NP>
+template <int DIM, int PATH, int MOC, bool LATE, int NP, bool PAD>
 __device__ __forceinline__ void cd_corner_body(const FusedArgs &a) {
     constexpr int NEN = Elem<DIM>::nen;
     // Corner-slot planes per pass: the MFMA element step writes all corners at once.
@@ -363,16 +368,48 @@ __device__ __forceinline__ void cd_corner_body(const FusedArgs &a) {
     const int b = blockIdx.x, xcd = b & 7, j = b >> 3;
     const int q = nt >> 3, r = nt & 7;
     const int tile = xcd * q + min(xcd, r) + j;
+    constexpr bool STAGE_X = PATH < 2;
+    const int tid = threadIdx.x, B = blockDim.x;
+    // Fixed-stride tile data: the halo node indices and the element data are loaded with
+    // the tile index alone, in the same round trip as the tile bounds below.
+    constexpr int NS = 2;  // strides of the one-pass staging
+    // PAD: the fixed-stride copies exist (the host selects this instantiation only then).
+    constexpr bool pad = PAD && !STAGE_X && PATH != 4;
+    int hn[NS];  // halo node indices (raw: the address is formed where it is used)
+    int lc[NEN];
+    double lam = 0.0, mu = 0.0;
+    int gid = 0;
+    // Unconditional, in-bounds loads (indices clamped to the last entry; what lies past the
+    // tile's counts is not used), unpacked only after the staging loads are issued.
+    uint32_t pkw[NEN / 2];
+    if constexpr (pad) {
+        const int32_t *hp = a.hpad + size_t(tile) * a.hstride;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) hn[k] = hp[min(k * B + tid, a.hstride - 1)];
+        const size_t ei = size_t(tile) * a.estride + min(tid, a.estride - 1);
+        if constexpr (NEN == 8) {
+            const uint4 pk = *reinterpret_cast<const uint4 *>(a.lcpad + ei * 8);
+            pkw[0] = pk.x;
+            pkw[1] = pk.y;
+            pkw[2] = pk.z;
+            pkw[3] = pk.w;
+        } else {
+            const uint2 pk = *reinterpret_cast<const uint2 *>(a.lcpad + ei * 4);
+            pkw[0] = pk.x;
+            pkw[1] = pk.y;
+        }
+        lam = a.lampad[ei];
+        mu = a.mupad[ei];
+        if constexpr (PATH >= 2) gid = a.geopad[ei];
+    }
     const int o0 = a.own_ptr[tile], nown = a.own_ptr[tile + 1] - o0;
     const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
     const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
     const int npl = a.nplain[tile];  // read with the tile bounds (one scalar round trip)
-    constexpr bool STAGE_X = PATH < 2;
     const int MO = MOC > 0 ? MOC + 1 : a.max_own;
     double *sU = smem;
     double *sX = smem + size_t(a.max_stage) * DIM;
     double *sIn = smem + size_t((STAGE_X ? 2 : 1) * a.max_stage) * DIM;  // [NH*DIM][MO]
-    const int tid = threadIdx.x, B = blockDim.x;
     const int nU = nown * DIM, nH = nh * DIM;
     const size_t g0 = size_t(o0) * DIM;
     // Prefetch: update operands of this thread's node, element data of this thread's element.
@@ -385,40 +422,63 @@ __device__ __forceinline__ void cd_corner_body(const FusedArgs &a) {
             kd[c] = has_node ? a.kinv[g0 + size_t(tid) * DIM + c] : 0.0;
         }
     const bool has_elem = PATH != 4 && tid < ne;
-    int lc[NEN];
-    double lam = 0.0, mu = 0.0;
-    int gid = 0;
-    if (has_elem) {
-        if constexpr (NEN == 8) {
-            const uint4 pk = *reinterpret_cast<const uint4 *>(a.lconn + size_t(e0 + tid) * 8);
-            const uint32_t w4[4] = {pk.x, pk.y, pk.z, pk.w};
+    if constexpr (!pad) {
+        // (with PAD the element data are in flight and unpacked after the staging loads)
+        if (has_elem) {
+            if constexpr (NEN == 8) {
+                const uint4 pk = *reinterpret_cast<const uint4 *>(a.lconn + size_t(e0 + tid) * 8);
+                const uint32_t w4[4] = {pk.x, pk.y, pk.z, pk.w};
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                lc[2 * c] = int(w4[c] & 0xffffu);
-                lc[2 * c + 1] = int(w4[c] >> 16);
+                for (int c = 0; c < 4; ++c) {
+                    lc[2 * c] = int(w4[c] & 0xffffu);
+                    lc[2 * c + 1] = int(w4[c] >> 16);
+                }
+            } else {
+                const uint2 pk = *reinterpret_cast<const uint2 *>(a.lconn + size_t(e0 + tid) * 4);
+                lc[0] = int(pk.x & 0xffffu);
+                lc[1] = int(pk.x >> 16);
+                lc[2] = int(pk.y & 0xffffu);
+                lc[3] = int(pk.y >> 16);
             }
+            lam = a.elam[e0 + tid];
+            mu = a.emu[e0 + tid];
+            if constexpr (PATH >= 2) gid = a.egeo[e0 + tid];
         } else {
-            const uint2 pk = *reinterpret_cast<const uint2 *>(a.lconn + size_t(e0 + tid) * 4);
-            lc[0] = int(pk.x & 0xffffu);
-            lc[1] = int(pk.x >> 16);
-            lc[2] = int(pk.y & 0xffffu);
-            lc[3] = int(pk.y >> 16);
-        }
-        lam = a.elam[e0 + tid];
-        mu = a.emu[e0 + tid];
-        if constexpr (PATH >= 2) gid = a.egeo[e0 + tid];
-    } else {
 #pragma unroll
-        for (int c = 0; c < NEN; ++c) lc[c] = 0;
+            for (int c = 0; c < NEN; ++c) lc[c] = 0;
+        }
     }
     // Stage U (and X), zero the corner slots.
     {
         const double *Ug = a.Ucur + g0;
-        if constexpr (!STAGE_X) {
+        if constexpr (pad) {
+            // One pass (the host checked that two strides cover the owned DOFs and the
+            // halo): owned loads with the halo loads through the prefetched indices.
+            double v[NS], hv[NS][DIM];
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                const int i = k * B + tid;
+                v[k] = i < nU ? Ug[i] : 0.0;
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) hv[k][c] = i < nh ? a.Ucur[size_t(hn[k]) * DIM + c] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                const int i = k * B + tid;
+                if (i < nU) sU[i] = v[k];
+                if (i < nh)
+#pragma unroll
+                    for (int c = 0; c < DIM; ++c) sU[nU + i * DIM + c] = hv[k][c];
+            }
+#pragma unroll
+            for (int c = 0; c < NEN / 2; ++c) {
+                lc[2 * c] = int(pkw[c] & 0xffffu);
+                lc[2 * c + 1] = int(pkw[c] >> 16);
+            }
+        } else if constexpr (!STAGE_X) {
             // One loop covers the owned DOFs and the halo nodes: every pass issues the halo
             // index loads and the owned loads together, then the halo loads, then the stores —
             // two dependent global round trips per pass instead of three.
-            constexpr int NS = 2;  // strides per pass (a 160-node tile needs one)
             const int nrange = nU > nh ? nU : nh;
             for (int base = 0; base < nrange; base += NS * B) {
                 size_t n[NS];
@@ -654,9 +714,9 @@ __device__ __forceinline__ void cd_corner_body(const FusedArgs &a) {
     }
 }
 
-template <int DIM, int PATH, int MOC = 0, bool LATE = false, int NP = 2>
+template <int DIM, int PATH, int MOC = 0, bool LATE = false, int NP = 2, bool PAD = false>
 __global__ __launch_bounds__(512) void k_cd_corner(FusedArgs a) {
-    cd_corner_body<DIM, PATH, MOC, LATE, NP>(a);
+    cd_corner_body<DIM, PATH, MOC, LATE, NP, PAD>(a);
 }
 
 template <int DIM, int PATH>
@@ -685,11 +745,20 @@ static int use_moc(const Model &m) {
 }
 
 static const void *moc_kernel(const Model &m) {
-    if (m.elem_fp32) return reinterpret_cast<const void *>(&k_cd_corner<3, 5, 160, true, 4>);
+    const bool pad = m.tiles.hstride > 0;  // fixed-stride tile data built
+    if (m.elem_fp32)
+        return pad ? reinterpret_cast<const void *>(&k_cd_corner<3, 5, 160, true, 4, true>)
+                   : reinterpret_cast<const void *>(&k_cd_corner<3, 5, 160, true, 4>);
     switch (m.ndim * 1000 + use_moc(m)) {
-        case 3160: return reinterpret_cast<const void *>(&k_cd_corner<3, 3, 160, true, 4>);
-        case 2192: return reinterpret_cast<const void *>(&k_cd_corner<2, 3, 192>);
-        case 2448: return reinterpret_cast<const void *>(&k_cd_corner<2, 3, 448>);
+        case 3160:
+            return pad ? reinterpret_cast<const void *>(&k_cd_corner<3, 3, 160, true, 4, true>)
+                       : reinterpret_cast<const void *>(&k_cd_corner<3, 3, 160, true, 4>);
+        case 2192:
+            return pad ? reinterpret_cast<const void *>(&k_cd_corner<2, 3, 192, false, 2, true>)
+                       : reinterpret_cast<const void *>(&k_cd_corner<2, 3, 192>);
+        case 2448:
+            return pad ? reinterpret_cast<const void *>(&k_cd_corner<2, 3, 448, false, 2, true>)
+                       : reinterpret_cast<const void *>(&k_cd_corner<2, 3, 448>);
         default: return nullptr;
     }
 }
@@ -772,6 +841,13 @@ static int launch_fused(Model &m, int k, const double *Ucur, const double *Uprev
     a.egeo = t.elem_geo.p;
     a.geo = m.geo.p;
     a.drm = drm_ref(m);
+    a.hpad = t.hpad.p;
+    a.lcpad = t.lcpad.p;
+    a.geopad = t.geopad.p;
+    a.lampad = t.lampad.p;
+    a.mupad = t.mupad.p;
+    a.hstride = t.hstride;
+    a.estride = t.estride;
     dim3 grid(t.ntiles), block(t.block);
 #define SVLG_LAUNCH_FUSED(D, P)                                                                        \
     do {                                                                                              \

@@ -585,6 +585,36 @@ static int create_impl(const svlg_mesh_desc *d, const svlg_exec_opts *o, svlg_mo
         UP(t.elem_lam, el.data(), el.size());
         UP(t.elem_mu, em.data(), em.size());
         UP(t.color_ptr, ht.color_ptr.data(), ht.color_ptr.size());
+        if (m.variant == 3 && m.elem_path >= 2) {
+            int hmax = 0;
+            for (int ti = 0; ti < ht.ntiles; ++ti) hmax = std::max(hmax, ht.halo_ptr[ti + 1] - ht.halo_ptr[ti]);
+            const int emax = ht.max_elem;
+            // One staging pass of two block strides must cover the owned DOFs and the halo.
+            if (std::max(ht.max_own * ndim, hmax) <= 2 * t.block && emax <= t.block && hmax > 0) {
+                const size_t nt = size_t(ht.ntiles);
+                std::vector<int32_t> hp(nt * hmax, 0);
+                std::vector<uint16_t> lp(nt * emax * nen, 0), gp(nt * emax, 0);
+                std::vector<double> lap(nt * emax, 0.0), mup(nt * emax, 0.0);
+                for (int ti = 0; ti < ht.ntiles; ++ti) {
+                    for (int i = ht.halo_ptr[ti]; i < ht.halo_ptr[ti + 1]; ++i)
+                        hp[size_t(ti) * hmax + (i - ht.halo_ptr[ti])] = ht.halo_node[i];
+                    for (int i = ht.elem_ptr[ti]; i < ht.elem_ptr[ti + 1]; ++i) {
+                        const size_t o = size_t(ti) * emax + (i - ht.elem_ptr[ti]);
+                        for (int a = 0; a < nen; ++a) lp[o * nen + a] = ht.elem_lconn[size_t(i) * nen + a];
+                        lap[o] = el[i];
+                        mup[o] = em[i];
+                        gp[o] = eg[i];
+                    }
+                }
+                UP(t.hpad, hp.data(), hp.size());
+                UP(t.lcpad, lp.data(), lp.size());
+                UP(t.geopad, gp.data(), gp.size());
+                UP(t.lampad, lap.data(), lap.size());
+                UP(t.mupad, mup.data(), mup.size());
+                t.hstride = hmax;
+                t.estride = emax;
+            }
+        }
     }
     // Loads (device node numbering).
     m.nload = d->nload;
