@@ -224,6 +224,7 @@ struct Model {
     DevBuf<float> gcn_jacf;           // D^-1 in FP32 (one-channel kernel: states scaled by D^-1/2)
     DevBuf<double> gcn_elam, gcn_emu; // per tile element (lambda, mu) * (1 + 2 ak/dt)
     DevBuf<float> gcn_elamf, gcn_emuf, gcn_dgnf;  // FP32 copies read by the one-channel layer kernel
+    DevBuf<float> gcn_elamf_pad, gcn_emuf_pad;    // the same at the tiles' fixed element stride
     DevBuf<float> gcn_kmat;           // [2][24][24] K_lambda, K_mu of the single geometry class (MFMA path)
     DevBuf<uint4> gcn_kimg;           // [3][64] per-lane BF16 A operands of the MFMA element step
     DevBuf<float> gcn_x0;             // s * r

@@ -79,6 +79,11 @@ struct GcnArgs {
     const int *jdev;           // non-null: the iteration is *jdev + iter (graph replays)
     const float *elamf, *emuf, *dgnf;  // FP32 copies of elam, emu, dgn (one-channel kernel)
     const float *jacf;                 // D^-1 in FP32 (one-channel kernel)
+    // One-channel kernel, fixed-stride tile data (TileSet::hpad, lcpad; null: not built).
+    const int32_t *hpad;
+    const uint16_t *lcpad;
+    const float *elamf_pad, *emuf_pad;
+    int hstride, estride;
     const double *cmass;       // consistent mass: (4/dt^2 + 2 am/dt) times the element-mass scale
                                // per tile element (null: lumped, inside dgn)
 };
@@ -279,18 +284,36 @@ __device__ __forceinline__ int packed_half(uint32_t w, int h) { return int((w >>
 // plane offsets).
 // With MOC the kernel is held to 64 VGPRs (eight waves per SIMD): four 512-thread workgroups
 // then share a CU, so the default tiles of the 64^3 block all run in one round.
-template <bool OUT, int MOC = 0>
+// PAD: the tiles' fixed-stride halo lists and element data exist (TileSet::hpad, lcpad and the
+// padded FP32 coefficients): the halo indices and the connectivity are loaded with the tile
+// index alone, in the same round trip as the tile bounds, and the solver's done flag is read
+// while they are in flight.
+template <bool OUT, int MOC = 0, bool PAD = false>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MOC > 0 ? 8 : 1))) void k_gcn_fast(GcnArgs a) {
     constexpr int MAXR = 4;  // element rounds per wave: tiles hold at most 512 elements
     extern __shared__ double smem[];
     __shared__ double red[3 * 17];
-    if (a.scal && a.scal[SC_DONE] != 0.0) return;
-    double sig = 1.0;
-    if constexpr (OUT) sig = gcn_sigma(a.cg, iter_of(a.jdev, a.iter), a.normalize, a.ndofn, red);
     const int nt = a.ntiles;
     const int bI = blockIdx.x, xcd = bI & 7, jb = bI >> 3;
     const int qq = nt >> 3, rr = nt & 7;
     const int tile = xcd * qq + min(xcd, rr) + jb;
+    const int tid = threadIdx.x, B = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nw = B >> 6, n16 = lane & 15, g = lane >> 4;
+    int hn[2];
+    uint4 pk[MAXR];
+    if constexpr (PAD) {
+        const int32_t *hp = a.hpad + size_t(tile) * a.hstride;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) hn[k] = hp[min(k * B + tid, a.hstride - 1)];
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) {
+            const int el = min(wave * 16 + r * nw * 16 + n16, a.estride - 1);
+            pk[r] = *reinterpret_cast<const uint4 *>(a.lcpad + (size_t(tile) * a.estride + el) * 8);
+        }
+    }
+    if (a.scal && a.scal[SC_DONE] != 0.0) return;
+    double sig = 1.0;
+    if constexpr (OUT) sig = gcn_sigma(a.cg, iter_of(a.jdev, a.iter), a.normalize, a.ndofn, red);
     const int o0 = a.own_ptr[tile], nown = a.own_ptr[tile + 1] - o0;
     const int h0 = a.halo_ptr[tile], nh = a.halo_ptr[tile + 1] - h0;
     const int e0 = a.elem_ptr[tile], ne = a.elem_ptr[tile + 1] - e0;
@@ -301,8 +324,6 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MOC > 0 ? 8
     const int trash = 24 * MO;
     float *sX = sIn + size_t(24) * MO + 1;
     const int zcell = 3 * MS;
-    const int tid = threadIdx.x, B = blockDim.x;
-    const int lane = tid & 63, wave = tid >> 6, nw = B >> 6, n16 = lane & 15, g = lane >> 4;
     // Prefetch the A-operand image (four 16 x 32 BF16 blocks) and this lane's element of every
     // round. k-slot order: lane group g carries corners 2g and 2g+1 (k = 8g + jj -> element DOF
     // 6g + jj, jj < 6; jj = 6, 7 are zero). Row order: blocks 0, 1 (K_lambda) and 2, 3 (K_mu)
@@ -311,21 +332,41 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MOC > 0 ? 8
     uint4 ak[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) ak[b] = kimg[b * 64 + lane];
-    uint4 pk[MAXR];
+    if constexpr (!PAD)
 #pragma unroll
-    for (int r = 0; r < MAXR; ++r) {
-        const int el = min(wave * 16 + r * nw * 16 + n16, max(ne - 1, 0));
-        pk[r] = *reinterpret_cast<const uint4 *>(a.lconn + size_t(e0 + el) * 8);
-    }
+        for (int r = 0; r < MAXR; ++r) {
+            const int el = min(wave * 16 + r * nw * 16 + n16, max(ne - 1, 0));
+            pk[r] = *reinterpret_cast<const uint4 *>(a.lconn + size_t(e0 + el) * 8);
+        }
     const size_t g0 = size_t(o0) * 3;
-    for (int i = tid; i < nown * 3; i += B) sX[i] = a.in[g0 + i];
-    for (int sl = tid; sl < nh; sl += B) {
-        const size_t nd = size_t(a.halo_node[h0 + sl]) * 3;
-        const float v0 = a.in[nd], v1 = a.in[nd + 1], v2 = a.in[nd + 2];
-        float *d = sX + (nown + sl) * 3;
-        d[0] = v0;
-        d[1] = v1;
-        d[2] = v2;
+    if constexpr (PAD) {
+        // One pass: the host checked that two block strides cover the owned DOFs and the halo.
+        float v[2], hv[2][3];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int i = k * B + tid;
+            v[k] = i < nown * 3 ? a.in[g0 + i] : 0.f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) hv[k][c] = i < nh ? a.in[size_t(hn[k]) * 3 + c] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int i = k * B + tid;
+            if (i < nown * 3) sX[i] = v[k];
+            if (i < nh)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) sX[(nown + i) * 3 + c] = hv[k][c];
+        }
+    } else {
+        for (int i = tid; i < nown * 3; i += B) sX[i] = a.in[g0 + i];
+        for (int sl = tid; sl < nh; sl += B) {
+            const size_t nd = size_t(a.halo_node[h0 + sl]) * 3;
+            const float v0 = a.in[nd], v1 = a.in[nd + 1], v2 = a.in[nd + 2];
+            float *d = sX + (nown + sl) * 3;
+            d[0] = v0;
+            d[1] = v1;
+            d[2] = v2;
+        }
     }
     for (int i = tid; i < 24 * MO; i += B) sIn[i] = 0.f;
     if (tid == 0) sX[zcell] = 0.f;
@@ -341,7 +382,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MOC > 0 ? 8
         // The element coefficients are read per round (issued before the gathers) rather than
         // prefetched with the connectivity: eight fewer live registers.
         const int el = min(eb + n16, ne - 1);
-        const float lamv = a.elamf[e0 + el], muv = a.emuf[e0 + el];
+        const float lamv = PAD ? a.elamf_pad[size_t(tile) * a.estride + el] : a.elamf[e0 + el];
+        const float muv = PAD ? a.emuf_pad[size_t(tile) * a.estride + el] : a.emuf[e0 + el];
         // Gather: word g of the packed connectivity holds corners 2g, 2g+1.
         const uint32_t wg = g == 0 ? pk[r].x : g == 1 ? pk[r].y : g == 2 ? pk[r].z : pk[r].w;
         const int sa = valid ? packed_half(wg, 0) * 3 : zcell;
@@ -611,11 +653,16 @@ static size_t gcn_fast_lds(const Model &m) {
 }
 
 static const void *gcn_fast_fn(const Model &m, bool out) {
+    // The output layer keeps the tile-bound loads first: with the fixed-stride prefetch its
+    // registers (z, the partials) would spill.
+    const bool pad = !out && m.tiles.hstride > 0 && m.gcn_elamf_pad.p != nullptr;
     switch (gcn_moc(m)) {
         case 168:
+            if (pad) return reinterpret_cast<const void *>(&k_gcn_fast<false, 168, true>);
             return out ? reinterpret_cast<const void *>(&k_gcn_fast<true, 168>)
                        : reinterpret_cast<const void *>(&k_gcn_fast<false, 168>);
         case 336:
+            if (pad) return reinterpret_cast<const void *>(&k_gcn_fast<false, 336, true>);
             return out ? reinterpret_cast<const void *>(&k_gcn_fast<true, 336>)
                        : reinterpret_cast<const void *>(&k_gcn_fast<false, 336>);
         default:
@@ -667,6 +714,12 @@ static GcnArgs gcn_args(const Model &m) {
     a.emuf = m.gcn_emuf.p;
     a.dgnf = m.gcn_dgnf.p;
     a.jacf = m.gcn_jacf.p;
+    a.hpad = m.tiles.hpad.p;
+    a.lcpad = m.tiles.lcpad.p;
+    a.elamf_pad = m.gcn_elamf_pad.p;
+    a.emuf_pad = m.gcn_emuf_pad.p;
+    a.hstride = m.tiles.hstride;
+    a.estride = m.tiles.estride;
     a.geo = m.geo.p;
     a.X = m.X.p;
     a.th = m.th;
@@ -1014,6 +1067,23 @@ int svlg_gnn_init(svlg_model *h, int32_t channels, int32_t layers, int32_t mfma,
         std::vector<float> elf(el.begin(), el.end()), emf(em.begin(), em.end());
         SVLG_HIP(m.gcn_elamf.upload(elf.data(), ne, s));
         SVLG_HIP(m.gcn_emuf.upload(emf.data(), ne, s));
+        if (m.tiles.hstride > 0) {
+            const int nt = m.tiles.ntiles;
+            const size_t es = size_t(m.tiles.estride);
+            std::vector<int32_t> ep(nt + 1);
+            SVLG_HIP(hipMemcpyAsync(ep.data(), m.tiles.elem_ptr.p, sizeof(int32_t) * (nt + 1), hipMemcpyDeviceToHost,
+                                    s));
+            SVLG_HIP(hipStreamSynchronize(s));
+            std::vector<float> lp(size_t(nt) * es, 0.f), mp(size_t(nt) * es, 0.f);
+            for (int t = 0; t < nt; ++t)
+                for (int i = ep[t]; i < ep[t + 1]; ++i) {
+                    lp[size_t(t) * es + (i - ep[t])] = elf[i];
+                    mp[size_t(t) * es + (i - ep[t])] = emf[i];
+                }
+            SVLG_HIP(m.gcn_elamf_pad.upload(lp.data(), lp.size(), s));
+            SVLG_HIP(m.gcn_emuf_pad.upload(mp.data(), mp.size(), s));
+            SVLG_HIP(hipStreamSynchronize(s));
+        }
         std::vector<double> dg(n);
         SVLG_HIP(hipMemcpyAsync(dg.data(), m.dgn.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
         SVLG_HIP(hipStreamSynchronize(s));
