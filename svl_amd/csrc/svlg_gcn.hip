@@ -77,6 +77,7 @@ struct GcnArgs {
     CgParts cg;
     int iter, normalize;
     const int *jdev;           // non-null: the iteration is *jdev + iter (graph replays)
+    int test;                  // first layer in the solver: CG's stopping test on r_iter first
     const float *elamf, *emuf, *dgnf;  // FP32 copies of elam, emu, dgn (one-channel kernel)
     const float *jacf;                 // D^-1 in FP32 (one-channel kernel)
     // One-channel kernel, fixed-stride tile data (TileSet::hpad, lcpad; null: not built).
@@ -145,6 +146,7 @@ __global__ __launch_bounds__(512) void k_gcn_layer(GcnArgs a) {
     __shared__ double red[3 * 17];
     if constexpr (MODE != G_BWD) {
         if (a.scal && a.scal[SC_DONE] != 0.0) return;
+        if (a.test && cg_stop_test(a.cg, iter_of(a.jdev, a.iter), a.scal, red)) return;
     }
     double sig = 1.0;
     if constexpr (MODE == G_OUT) sig = gcn_sigma(a.cg, iter_of(a.jdev, a.iter), a.normalize, a.ndofn, red);
@@ -288,7 +290,7 @@ __device__ __forceinline__ int packed_half(uint32_t w, int h) { return int((w >>
 // padded FP32 coefficients): the halo indices and the connectivity are loaded with the tile
 // index alone, in the same round trip as the tile bounds, and the solver's done flag is read
 // while they are in flight.
-template <bool OUT, int MOC = 0, bool PAD = false>
+template <bool OUT, int MOC = 0, bool PAD = false, bool TEST = false>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MOC > 0 ? 8 : 1))) void k_gcn_fast(GcnArgs a) {
     constexpr int MAXR = 4;  // element rounds per wave: tiles hold at most 512 elements
     extern __shared__ double smem[];
@@ -299,6 +301,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MOC > 0 ? 8
     const int tile = xcd * qq + min(xcd, rr) + jb;
     const int tid = threadIdx.x, B = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6, nw = B >> 6, n16 = lane & 15, g = lane >> 4;
+    if constexpr (TEST) {
+        // First layer in the solver: the stopping test on r_iter before anything is in flight
+        // (the prefetched registers would be held across its barriers).
+        if (a.scal[SC_DONE] != 0.0) return;
+        if (cg_stop_test(a.cg, iter_of(a.jdev, a.iter), a.scal, red)) return;
+    }
     int hn[2];
     uint4 pk[MAXR];
     if constexpr (PAD) {
@@ -652,22 +660,23 @@ static size_t gcn_fast_lds(const Model &m) {
     return (size_t(24) * stride + 1 + size_t(m.tiles.max_stage) * 3 + 1) * sizeof(float);
 }
 
-static const void *gcn_fast_fn(const Model &m, bool out) {
+template <int MOC, bool TEST>
+static const void *gcn_fast_fn_t(bool out, bool pad) {
+    if (out) return reinterpret_cast<const void *>(&k_gcn_fast<true, MOC, false, TEST>);
+    if constexpr (MOC > 0)
+        if (pad) return reinterpret_cast<const void *>(&k_gcn_fast<false, MOC, true, TEST>);
+    return reinterpret_cast<const void *>(&k_gcn_fast<false, MOC, false, TEST>);
+}
+
+// test: the solver's first layer (CG's stopping test first).
+static const void *gcn_fast_fn(const Model &m, bool out, bool test) {
     // The output layer keeps the tile-bound loads first: with the fixed-stride prefetch its
     // registers (z, the partials) would spill.
     const bool pad = !out && m.tiles.hstride > 0 && m.gcn_elamf_pad.p != nullptr;
     switch (gcn_moc(m)) {
-        case 168:
-            if (pad) return reinterpret_cast<const void *>(&k_gcn_fast<false, 168, true>);
-            return out ? reinterpret_cast<const void *>(&k_gcn_fast<true, 168>)
-                       : reinterpret_cast<const void *>(&k_gcn_fast<false, 168>);
-        case 336:
-            if (pad) return reinterpret_cast<const void *>(&k_gcn_fast<false, 336, true>);
-            return out ? reinterpret_cast<const void *>(&k_gcn_fast<true, 336>)
-                       : reinterpret_cast<const void *>(&k_gcn_fast<false, 336>);
-        default:
-            return out ? reinterpret_cast<const void *>(&k_gcn_fast<true>)
-                       : reinterpret_cast<const void *>(&k_gcn_fast<false>);
+        case 168: return test ? gcn_fast_fn_t<168, true>(out, pad) : gcn_fast_fn_t<168, false>(out, pad);
+        case 336: return test ? gcn_fast_fn_t<336, true>(out, pad) : gcn_fast_fn_t<336, false>(out, pad);
+        default: return test ? gcn_fast_fn_t<0, true>(out, false) : gcn_fast_fn_t<0, false>(out, false);
     }
 }
 
@@ -765,7 +774,7 @@ static bool gcn_scaled_states(const Model &m) { return m.gcn_mfma && m.gcn_C == 
 static const double *gcn_input_scale(const Model &m) { return gcn_scaled_states(m) ? m.jac.p : m.gcn_s.p; }
 
 static int gcn_run(Model &m, const CgParts &cg, int j, const double *r, const double *q, double *z, int normalize,
-                   const int *jdev = nullptr, bool first = true) {
+                   const int *jdev = nullptr, bool first = true, bool test = false) {
     const int n = m.ndofn, C = m.gcn_C, L = m.gcn_L;
     if (first)
         hipLaunchKernelGGL(k_gcn_first, dim3(kVecGrid), dim3(kVecBlock), 0, m.stream, n, C, normalize, cg, j, r,
@@ -781,6 +790,12 @@ static int gcn_run(Model &m, const CgParts &cg, int j, const double *r, const do
         a.trans = 0;
         a.scal = m.scal.p;
         const int mode = l == L ? G_OUT : G_FWD;
+        a.test = test && l == 2 ? 1 : 0;
+        if (a.test || mode == G_OUT) {
+            a.cg = cg;
+            a.iter = j;
+            a.jdev = jdev;
+        }
         if (mode == G_OUT) {
             a.w = theta_w(m);
             a.r = r;
@@ -794,7 +809,7 @@ static int gcn_run(Model &m, const CgParts &cg, int j, const double *r, const do
         if (gcn_scaled_states(m)) {
             a.kmat = reinterpret_cast<const float *>(m.gcn_kimg.p);
             void *args[] = {&a};
-            const void *fn = gcn_fast_fn(m, mode == G_OUT);
+            const void *fn = gcn_fast_fn(m, mode == G_OUT, a.test != 0);
             SVLG_HIP(hipLaunchKernel(fn, dim3(m.tiles.ntiles), dim3(m.tiles.block), args, gcn_fast_lds(m), m.stream));
         } else if (int rc = launch_gcn(m, mode, m.gcn_mfma != 0, a)) {
             return rc;
@@ -807,7 +822,7 @@ static int gcn_run(Model &m, const CgParts &cg, int j, const double *r, const do
 // In the solver the network runs unnormalised (it is linear in r, and FP32 / BF16 cover the
 // residual's range), so its input layer comes from the CG vector kernel with no global norm.
 int gcn_precond(Model &m, const CgParts &c, int j, const double *r, const double *q, double *z, const int *jdev) {
-    return gcn_run(m, c, j, r, q, z, 0, jdev, false);
+    return gcn_run(m, c, j, r, q, z, 0, jdev, false, true);
 }
 
 GcnInput gcn_input_args(Model &m) {
@@ -1115,7 +1130,8 @@ int svlg_gnn_init(svlg_model *h, int32_t channels, int32_t layers, int32_t mfma,
     }
     if (int rc = upload_theta(m)) return rc;
     if (gcn_fast_lds(m) > 64 * 1024)
-        for (const void *fn : {gcn_fast_fn(m, true), gcn_fast_fn(m, false)})
+        for (const void *fn : {gcn_fast_fn(m, true, false), gcn_fast_fn(m, false, false), gcn_fast_fn(m, true, true),
+                               gcn_fast_fn(m, false, true)})
             SVLG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(gcn_fast_lds(m))));
     const size_t lds = gcn_lds(m);
     if (lds > 64 * 1024)

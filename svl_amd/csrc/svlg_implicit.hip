@@ -805,21 +805,24 @@ int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int p
     int launched = 0;  // iterations 0 .. launched-1 are queued
     bool done = false;
     // The solve that converges at iteration j sets DONE in the operator kernel of iteration
-    // j, so with the previous solve's count as the prediction (successive time steps need
-    // nearly the same number) the flag is first read back once that kernel is queued, then
-    // after every launch; every 16 iterations while there is no prediction. Iterations
-    // queued past convergence exit at once on the device.
+    // j (Jacobi) or in the first preconditioner layer of iteration j - 1 (learned: the
+    // converged residual is not preconditioned), i.e. once `det` iterations are queued. With
+    // the previous solve's count as the prediction (successive time steps need nearly the same
+    // number) the flag is first read back then, and after every launch from there on; every
+    // 16 iterations while there is no prediction. Iterations queued past convergence exit at
+    // once on the device.
     const int expect = m.cg_last_iters[flexible];
+    const int det = expect > 0 ? expect + 1 - flexible : 0;
     while (launched < maxit && !done) {
         const int before = launched;
-        // A pair that starts at the predicted count would queue one iteration past the one
-        // whose operator kernel is expected to see DONE: that iteration goes alone.
-        const bool tail = expect > 0 && launched == expect;
+        // A pair that starts one short of `det` would queue one iteration past the one
+        // expected to see DONE: that iteration goes alone.
+        const bool tail = det > 0 && launched == det - 1;
         if (pair && launched + 2 <= maxit && !tail) {
             // The graphs start at even iterations: *jdev tracks `launched`. Runs of eight while
             // they stay within the prediction (every graph launch costs a few microseconds of
             // dispatch), then pairs.
-            const int lim = expect > 0 ? std::min(maxit, expect + 1) : maxit;
+            const int lim = det > 0 ? std::min(maxit, det) : maxit;
             if (run8 && launched + 8 <= lim) {
                 SVLG_HIP(hipGraphLaunch(run8, s));
                 launched += 8;
@@ -838,7 +841,7 @@ int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int p
             if (int rc = cg_iteration(m, c, a, x, launched, nullptr)) return rc;
             launched += 1;
         }
-        if ((expect > 0 && launched >= expect + 1) || (before >> 4) != (launched >> 4) || launched >= maxit) {
+        if ((det > 0 && launched >= det) || (before >> 4) != (launched >> 4) || launched >= maxit) {
             SVLG_HIP(hipMemcpyAsync(hs, m.scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
             SVLG_HIP(hipStreamSynchronize(s));
             done = hs[SC_DONE] != 0.0;

@@ -120,6 +120,21 @@ __device__ __forceinline__ void gcn_input(const GcnInput &g, int n, int i, doubl
 __device__ __forceinline__ void cg_vec(const CgParts &c, int j, double *buf, double (&o)[3]) {
     sum_records<3>(c.vp + size_t(j & 1) * 3 * c.nvb, c.nvb, 3, buf, o);
 }
+// CG's stopping test on r_j (r.r <= rtol^2 b.b, or b = 0) from the vector partials, as the
+// operator kernel of iteration j applies it (same partials, same order: bitwise the same
+// answer in every workgroup). On success the first workgroup records the scalars and DONE.
+__device__ __forceinline__ bool cg_stop_test(const CgParts &c, int j, double *scal, double *buf) {
+    double v[3];
+    cg_vec(c, j, buf, v);
+    const bool done = v[1] <= scal[SC_TOL2] * v[2] || v[2] == 0.0;
+    if (done && blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[SC_RR] = v[1];
+        scal[SC_BB] = v[2];
+        scal[SC_ITERS] = double(j);
+        scal[SC_DONE] = 1.0;
+    }
+    return done;
+}
 __device__ __forceinline__ double cg_pq(const CgParts &c, int j, double *buf) {
     return sum_partials(c.pq + size_t(j & 1) * c.ntiles, c.ntiles, 1, buf);
 }
@@ -131,7 +146,9 @@ __device__ __forceinline__ void cg_pre(const CgParts &c, int j, double *buf, dou
 // Preconditioner hook of the flexible CG loop (svlg_gcn.hip): z = P(r) on the node-DOF
 // layout for iteration j (z = z_j; r = r_j; q = q_(j-1)) from the input layer the vector
 // kernel already formed (GcnInput); writes the tp[j&1] partials (z.r, z.q per tile). With
-// jdev the iteration is *jdev + j (graph replays).
+// jdev the iteration is *jdev + j (graph replays). Its first layer applies the stopping test
+// on r_j (cg_stop_test), so that a converged residual is not preconditioned: DONE is then set
+// one launch after the vector kernel of iteration j - 1.
 int gcn_precond(Model &m, const CgParts &c, int j, const double *r, const double *q, double *z,
                 const int *jdev = nullptr);
 // The input layer of the solver's preconditioner (x0, H^1 buffers and c_1).
