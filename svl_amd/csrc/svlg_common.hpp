@@ -195,6 +195,7 @@ struct Model {
     DevBuf<double> part;              // partial sums of a solve (CgParts)
     DevBuf<double> scal;              // device scalars of the solver
     int cg_last_iters[2] = {0, 0};    // iterations of the last solve (Jacobi, flexible)
+    double *scal_host = nullptr;      // pinned host copy of the solver scalars (convergence polls)
     std::vector<int32_t> tile_elem_id;   // tag index of every tile element (host)
     DevBuf<int32_t> tile_eid;         // the same on the device
 
@@ -250,6 +251,7 @@ struct Model {
     ~Model() {
         if (cd_graph) (void)hipGraphExecDestroy(cd_graph);
         drop_cg_graphs();
+        if (scal_host) (void)hipHostFree(scal_host);
     }
 };
 

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(512) void k_gcn_layer(GcnArgs a) {
     __shared__ double red[3 * 17];
     if constexpr (MODE != G_BWD) {
         if (a.scal && a.scal[SC_DONE] != 0.0) return;
-        if (a.test && cg_stop_test(a.cg, iter_of(a.jdev, a.iter), a.scal, red)) return;
+        if (a.test && cg_stop_test(a.cg, a.jdev, a.iter, a.scal, red)) return;
     }
     double sig = 1.0;
     if constexpr (MODE == G_OUT) sig = gcn_sigma(a.cg, iter_of(a.jdev, a.iter), a.normalize, a.ndofn, red);
@@ -301,12 +301,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MOC > 0 ? 8
     const int tile = xcd * qq + min(xcd, rr) + jb;
     const int tid = threadIdx.x, B = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6, nw = B >> 6, n16 = lane & 15, g = lane >> 4;
-    if constexpr (TEST) {
-        // First layer in the solver: the stopping test on r_iter before anything is in flight
-        // (the prefetched registers would be held across its barriers).
-        if (a.scal[SC_DONE] != 0.0) return;
-        if (cg_stop_test(a.cg, iter_of(a.jdev, a.iter), a.scal, red)) return;
-    }
+    // First layer in the solver: the stopping test on r_iter (with the DONE flag) before the
+    // prefetch, whose registers would be held across the test's barriers.
+    if constexpr (TEST)
+        if (cg_stop_test(a.cg, a.jdev, a.iter, a.scal, red)) return;
     int hn[2];
     uint4 pk[MAXR];
     if constexpr (PAD) {
@@ -319,7 +317,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MOC > 0 ? 8
             pk[r] = *reinterpret_cast<const uint4 *>(a.lcpad + (size_t(tile) * a.estride + el) * 8);
         }
     }
-    if (a.scal && a.scal[SC_DONE] != 0.0) return;
+    if constexpr (!TEST)
+        if (a.scal && a.scal[SC_DONE] != 0.0) return;
     double sig = 1.0;
     if constexpr (OUT) sig = gcn_sigma(a.cg, iter_of(a.jdev, a.iter), a.normalize, a.ndofn, red);
     const int o0 = a.own_ptr[tile], nown = a.own_ptr[tile + 1] - o0;

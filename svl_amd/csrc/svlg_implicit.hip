@@ -114,10 +114,12 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
     double beta = 0.0;
     if constexpr (!RHS) {
         if (a.reduce) {
-            if (a.scal[SC_DONE] != 0.0) return;
+            // The DONE flag and *jdev are loaded together.
+            const double flag = a.scal[SC_DONE];
+            const int j = iter_of(a.jdev, a.iter);
+            if (flag != 0.0) return;
             // Stopping test and beta of iteration j from the partials of the previous kernels;
             // scal holds r.z and p.q of iteration j-1 (left by the vector kernel).
-            const int j = iter_of(a.jdev, a.iter);
             double v[3];
             cg_vec(a.cg, j, red, v);
             const double rr = v[1], bb = v[2];
@@ -382,8 +384,9 @@ __global__ __launch_bounds__(512) void k_cg_update(int n, int j0, CgParts c, dou
                                                    double *__restrict__ r, double *__restrict__ z,
                                                    const double *__restrict__ jac, const int *jdev, GcnInput g) {
     __shared__ double red[3 * 17];
-    if (scal[SC_DONE] != 0.0) return;
+    const double flag = scal[SC_DONE];
     const int j = iter_of(jdev, j0);
+    if (flag != 0.0) return;
     const double pq = cg_pq(c, j, red);
     double rzj;
     if (c.flexible) {
@@ -801,7 +804,13 @@ int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int p
     hipGraphExec_t run8 = pair && maxit >= 8 ? cg_run_graph(m, c, a, x, 8) : nullptr;
     hipGraphExec_t one = pair ? cg_run_graph(m, c, a, x, 1) : nullptr;
     if (pair) hipLaunchKernelGGL(k_iter_set, dim3(1), dim3(64), 0, s, m.jdev.p, 0);
-    double hs[SC_COUNT];
+    // The polls copy the scalars into pinned host memory (a direct copy, no staging).
+    if (!m.scal_host && hipHostMalloc((void **)&m.scal_host, sizeof(double) * SC_COUNT) != hipSuccess) {
+        m.scal_host = nullptr;
+        set_error("hipHostMalloc failed");
+        return 10;
+    }
+    double *hs = m.scal_host;
     int launched = 0;  // iterations 0 .. launched-1 are queued
     bool done = false;
     // The solve that converges at iteration j sets DONE in the operator kernel of iteration
@@ -842,7 +851,7 @@ int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int p
             launched += 1;
         }
         if ((det > 0 && launched >= det) || (before >> 4) != (launched >> 4) || launched >= maxit) {
-            SVLG_HIP(hipMemcpyAsync(hs, m.scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
+            SVLG_HIP(hipMemcpyAsync(hs, m.scal.p, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s));
             SVLG_HIP(hipStreamSynchronize(s));
             done = hs[SC_DONE] != 0.0;
         }
@@ -851,7 +860,7 @@ int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int p
         // The stopping test on r_maxit, then the scalars.
         hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(256), 0, s, launched, c, m.scal.p);
         SVLG_HIP(hipGetLastError());
-        SVLG_HIP(hipMemcpyAsync(hs, m.scal.p, sizeof(hs), hipMemcpyDeviceToHost, s));
+        SVLG_HIP(hipMemcpyAsync(hs, m.scal.p, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s));
         SVLG_HIP(hipStreamSynchronize(s));
     }
     // The poll that saw DONE holds the final scalars (written with the flag); the work

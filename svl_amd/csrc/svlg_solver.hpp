@@ -49,16 +49,10 @@ __device__ __forceinline__ double sum_partials(const double *part, int count, in
     return block_sum(s, buf);
 }
 
-// Fixed-order sums of the first K components of count records (stride doubles apart), one
-// pass and one workgroup round. buf holds 17 * K doubles.
+// The workgroup part of sum_records: per-thread sums v (fixed order) to the totals, by a
+// fixed shuffle tree inside each wave and the wave sums in wave order. buf holds 17 * K doubles.
 template <int K>
-__device__ __forceinline__ void sum_records(const double *part, int count, int stride, double *buf, double (&out)[K]) {
-    double v[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = 0.0;
-    for (int i = threadIdx.x; i < count; i += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < K; ++k) v[k] += part[size_t(i) * stride + k];
+__device__ __forceinline__ void reduce_records(double (&v)[K], double *buf, double (&out)[K]) {
 #pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll
@@ -78,6 +72,19 @@ __device__ __forceinline__ void sum_records(const double *part, int count, int s
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = buf[k * 17 + 16];
     __syncthreads();
+}
+
+// Fixed-order sums of the first K components of count records (stride doubles apart), one
+// pass and one workgroup round. buf holds 17 * K doubles.
+template <int K>
+__device__ __forceinline__ void sum_records(const double *part, int count, int stride, double *buf, double (&out)[K]) {
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = 0.0;
+    for (int i = threadIdx.x; i < count; i += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] += part[size_t(i) * stride + k];
+    reduce_records<K>(v, buf, out);
 }
 
 // Iteration index of a kernel launch: the launch's own index, plus the device-resident base
@@ -120,16 +127,21 @@ __device__ __forceinline__ void gcn_input(const GcnInput &g, int n, int i, doubl
 __device__ __forceinline__ void cg_vec(const CgParts &c, int j, double *buf, double (&o)[3]) {
     sum_records<3>(c.vp + size_t(j & 1) * 3 * c.nvb, c.nvb, 3, buf, o);
 }
-// CG's stopping test on r_j (r.r <= rtol^2 b.b, or b = 0) from the vector partials, as the
-// operator kernel of iteration j applies it (same partials, same order: bitwise the same
-// answer in every workgroup). On success the first workgroup records the scalars and DONE.
-__device__ __forceinline__ bool cg_stop_test(const CgParts &c, int j, double *scal, double *buf) {
-    double v[3];
-    cg_vec(c, j, buf, v);
-    const bool done = v[1] <= scal[SC_TOL2] * v[2] || v[2] == 0.0;
+// CG's stopping test on r_j, j = iter_of(jdev, j0) (r.r <= rtol^2 b.b, or b = 0), from the
+// vector partials as the operator kernel of iteration j applies it: the r.r and b.b sums are
+// bitwise cg_vec's components 1 and 2 (per-thread order and reduction tree of sum_records),
+// so every workgroup reaches the operator kernel's answer. Returns true (and the first workgroup records the scalars and DONE) if the
+// solve has converged, or if DONE was already set.
+__device__ __forceinline__ bool cg_stop_test(const CgParts &c, const int *jdev, int j0, double *scal, double *buf) {
+    const double flag = scal[SC_DONE];
+    const int j = iter_of(jdev, j0);
+    if (flag != 0.0) return true;
+    double v[2];
+    sum_records<2>(c.vp + size_t(j & 1) * 3 * c.nvb + 1, c.nvb, 3, buf, v);
+    const bool done = v[0] <= scal[SC_TOL2] * v[1] || v[1] == 0.0;
     if (done && blockIdx.x == 0 && threadIdx.x == 0) {
-        scal[SC_RR] = v[1];
-        scal[SC_BB] = v[2];
+        scal[SC_RR] = v[0];
+        scal[SC_BB] = v[1];
         scal[SC_ITERS] = double(j);
         scal[SC_DONE] = 1.0;
     }
