@@ -119,11 +119,15 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
             const int j = iter_of(a.jdev, a.iter);
             if (flag != 0.0) return;
             // Stopping test and beta of iteration j from the partials of the previous kernels;
-            // scal holds r.z and p.q of iteration j-1 (left by the vector kernel).
-            double v[3];
-            cg_vec(a.cg, j, red, v);
-            const double rr = v[1], bb = v[2];
-            const bool done = rr <= a.scal[SC_TOL2] * bb || bb == 0.0;
+            // scal holds r.z and p.q of iteration j-1 (left by the vector kernel). The flexible
+            // solve's test on r_j was applied by the first layer of its preconditioner
+            // (cg_stop_test, the same sums), which set DONE if it held: only beta remains.
+            double v[3] = {0.0, 0.0, 0.0};
+            bool done = false;
+            if (!a.cg.flexible) {
+                cg_vec(a.cg, j, red, v);
+                done = v[1] <= a.scal[SC_TOL2] * v[2] || v[2] == 0.0;
+            }
             if (j > 0) {
                 if (a.cg.flexible) {
                     double zz[2];
@@ -136,8 +140,10 @@ __global__ __launch_bounds__(512) void k_nm_op(OpArgs a) {
                 }
             }
             if (blockIdx.x == 0 && threadIdx.x == 0) {
-                a.scal[SC_RR] = rr;
-                a.scal[SC_BB] = bb;
+                if (!a.cg.flexible) {
+                    a.scal[SC_RR] = v[1];
+                    a.scal[SC_BB] = v[2];
+                }
                 a.scal[SC_ITERS] = double(j);
                 a.scal[SC_BETA] = beta;
                 if (done) a.scal[SC_DONE] = 1.0;

@@ -128,9 +128,10 @@ __device__ __forceinline__ void cg_vec(const CgParts &c, int j, double *buf, dou
     sum_records<3>(c.vp + size_t(j & 1) * 3 * c.nvb, c.nvb, 3, buf, o);
 }
 // CG's stopping test on r_j, j = iter_of(jdev, j0) (r.r <= rtol^2 b.b, or b = 0), from the
-// vector partials as the operator kernel of iteration j applies it: the r.r and b.b sums are
+// vector partials as the Jacobi solve's operator kernel applies it: the r.r and b.b sums are
 // bitwise cg_vec's components 1 and 2 (per-thread order and reduction tree of sum_records),
-// so every workgroup reaches the operator kernel's answer. Returns true (and the first workgroup records the scalars and DONE) if the
+// so every workgroup reaches the same answer. In the flexible solve this is the only test
+// (the operator kernel relies on it). Returns true (and the first workgroup records the scalars and DONE) if the
 // solve has converged, or if DONE was already set.
 __device__ __forceinline__ bool cg_stop_test(const CgParts &c, const int *jdev, int j0, double *scal, double *buf) {
     const double flag = scal[SC_DONE];
