@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from svl_amd import workloads as W
-from svl_amd.integrators import DeviceModel, ExtendedNewmarkBeta, LearnedPreconditioner, NewmarkBeta
+from svl_amd.integrators import DeviceModel, ExtendedNewmarkBeta, LearnedPreconditioner, LinearSystem, NewmarkBeta
 
 pytestmark = pytest.mark.gpu
 
@@ -60,3 +60,38 @@ def test_cg_launch_option_checked():
     m, dt, nt = W.block3d_implicit(4, 4, 4, nt=3)
     with pytest.raises(Exception, match="cg_launch"):
         DeviceModel(m, cg_launch=2)
+
+
+@pytest.mark.parametrize("channels,layers", [(1, 2), (1, 3), (2, 2), (2, 3)])
+def test_learned_stopping_test_in_first_layer(channels, layers):
+    """The flexible solve applies CG's stopping test in the first layer of its preconditioner
+    (with two layers that layer is also the output layer, with two channels it is the general
+    layer kernel). Each solve must reach its tolerance on the true residual ||b - Keff x||,
+    recomputed by svlg_keff_apply; repeated solves use the previous count as the prediction;
+    graph replays must equal eager launches bit for bit."""
+    m, dt, nt = W.block3d_implicit(8, 8, 6, nt=9)
+    rng = np.random.default_rng(7 + 10 * channels + layers)
+    hists = []
+    for cg_launch in (0, 1):
+        dm = DeviceModel(m, cg_launch=cg_launch)
+        nm = ExtendedNewmarkBeta(dm, dt, rtol=1e-10, maxit=3000)
+        nm.Initialize()
+        pc = LearnedPreconditioner(nm, channels=channels, layers=layers, mfma=-1)
+        pc.use(True)
+        if cg_launch == 0:
+            ls = LinearSystem(nm)
+            for trial in range(3):
+                b = rng.standard_normal(m.nfree)
+                ls.SetVector(b)
+                assert not ls.SolveSystem()
+                x = ls.GetSolution()
+                true_rel = np.linalg.norm(b - ls.Apply(x)) / np.linalg.norm(b)
+                print(f"C={channels} L={layers} solve {trial}: {ls.iters} iterations, relres {ls.relres:.2e}, "
+                      f"true {true_rel:.2e}")
+                assert ls.iters > 0 and ls.relres <= 1e-10
+                assert true_rel <= 2e-10
+        rec = list(range(0, m.nfree, 5))
+        hists.append((nm.advance(1, nt - 1, rec), nm.iters.copy()))
+        dm.close()
+    assert np.array_equal(hists[0][1], hists[1][1])
+    assert np.array_equal(hists[0][0], hists[1][0])
