@@ -836,9 +836,17 @@ int cg_solve(Model &m, const double *b, double *x, double rtol, int maxit, int p
         if (pair && launched + 2 <= maxit && !tail) {
             // The graphs start at even iterations: *jdev tracks `launched`. Runs of eight while
             // they stay within the prediction (every graph launch costs a few microseconds of
-            // dispatch), then pairs.
+            // dispatch), then the rest of the prediction as one graph (captured on first use
+            // for its count; after an odd count further iterations are launched eagerly), then
+            // pairs.
             const int lim = det > 0 ? std::min(maxit, det) : maxit;
-            if (run8 && launched + 8 <= lim) {
+            const int rem = lim - launched;
+            hipGraphExec_t rest = det > 0 && rem > 2 && rem < 8 ? cg_run_graph(m, c, a, x, rem) : nullptr;
+            if (rest) {
+                SVLG_HIP(hipGraphLaunch(rest, s));
+                launched += rem;
+                if (rem & 1) pair = nullptr;
+            } else if (run8 && launched + 8 <= lim) {
                 SVLG_HIP(hipGraphLaunch(run8, s));
                 launched += 8;
             } else {
