@@ -1,5 +1,6 @@
 """The integrators' CG solves replayed from captured hipGraphs (runs of eight iterations while
-they stay within the previous solve's count, pairs, one iteration at the predicted end) against
+they stay within the previous solve's count, the rest of that count as one graph, pairs, one
+iteration at the predicted end) against
 the same solves launched eagerly (svlg_exec_opts.cg_launch = 1): the same kernels in the same
 iteration order, so the displacement histories, the iteration counts and the Newton residuals
 must agree bit for bit. Jacobi-PCG solves of the 12x12x10 block take about 40 iterations (runs,
